@@ -1,0 +1,276 @@
+"""Benchmark: single-source BFS GTEPS on a symmetrised RMAT-22 graph (Int32
+CSR resident in HBM), with the step kernel's HBM roofline, plus exact
+betweenness TEPS on RMAT-20 with 1024 explicit sources, each beside the
+CPU oracle timed on this host.
+
+A "step" is one full BFS (distances + parents) from one of 64 seeded roots
+of degree >= 1 (cycled).  value = sum over timed BFS runs of m_comp divided
+by the wall time of the timed region, where m_comp = (sum of degrees of the
+reached vertices) / 2.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-bc]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "lightgraphs.jl_amd"))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "BFS GTEPS (RMAT-22) + exact-BC TEPS; learned vs α/β switch, same GPU"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def step_bytes(st, nv):
+    """Algorithmic HBM bytes of one BFS step (level L -> L+1).
+
+    bottom-up: V/8 visited + V/8 frontier words + V/8 next-frontier write
+               + 8 per examined vertex (row_ptr pair) + 4 per adjacency entry
+               read until the first frontier hit + 8 per new vertex
+               (distance + parent)
+    top-down:  12 per frontier vertex (queue entry + row_ptr pair) + 4 per
+               expanded edge + V/8 visited bitmap + 12 per new vertex
+               (distance, parent, queue entry)
+    """
+    if st.dir == 1:
+        return 3 * nv / 8 + 8 * st.cand + 4 * st.scanned + 8 * st.n_next
+    return 12 * st.n_front + 4 * st.m_front + nv / 8 + 12 * st.n_next
+
+
+def bfs_bench(args, lgmi, _lib, orc):
+    lib = _lib.load()
+    t0 = time.time()
+    g = lgmi.generators.rmat(22, 16, A=0.57, B=0.19, C=0.19, seed=1)
+    rp, col = g.csr()
+    nv, nnz = g.nv(), len(col)
+    deg = np.diff(rp).astype(np.int64)
+    log(f"RMAT-22 built: nv={nv} nnz={nnz} ({time.time() - t0:.1f}s)")
+    h = g.handle()
+    rng = np.random.default_rng(20260101)
+    roots = rng.choice(np.nonzero(deg > 0)[0], size=64, replace=False).astype(np.int32)
+
+    # m_comp per root (untimed): sum of degrees of reached vertices / 2
+    mcomp = np.zeros(len(roots), dtype=np.float64)
+    dist = np.empty(nv, dtype=np.int32)
+    st = _lib.BfsStats()
+    for i, r in enumerate(roots):
+        _lib.check(lib.lgmi_bfs_run(h, int(r), 0, 1, _lib.ctypes.byref(st)))
+        _lib.check(lib.lgmi_bfs_fetch(h, _lib.i32p(dist), None))
+        mcomp[i] = deg[dist >= 0].sum() / 2.0
+    log(f"m_comp per root: min {mcomp.min():.3e} max {mcomp.max():.3e}")
+
+    # warmup
+    for i in range(args.warmup):
+        _lib.check(lib.lgmi_bfs_run(h, int(roots[i % len(roots)]), 0, 1, None))
+    _lib.check(lib.lgmi_synchronize())
+
+    # timed region
+    K = args.steps
+    edges = 0.0
+    _lib.check(lib.lgmi_synchronize())
+    t_start = time.perf_counter()
+    for i in range(K):
+        _lib.check(lib.lgmi_bfs_run(h, int(roots[i % len(roots)]), 0, 1, None))
+        edges += mcomp[i % len(roots)]
+    _lib.check(lib.lgmi_synchronize())
+    elapsed = time.perf_counter() - t_start
+    gteps = edges / elapsed / 1e9
+    log(f"timed {K} BFS: {elapsed * 1e3:.2f} ms total, {gteps:.2f} GTEPS")
+
+    # per-root device times (HIP events) for the harmonic mean and median
+    per_root = []
+    for i, r in enumerate(roots):
+        _lib.check(lib.lgmi_bfs_run(h, int(r), 0, 1, _lib.ctypes.byref(st)))
+        per_root.append(mcomp[i] / (st.device_ms * 1e-3) / 1e9)
+    per_root = np.array(per_root)
+
+    # roofline of the step kernel: HIP events around every step launch
+    _lib.check(lib.lgmi_bfs_time_steps(h, 1))
+    tot_bytes = tot_ms = 0.0
+    bu_bytes = bu_ms = 0.0
+    launches = 0
+    steps_buf = (_lib.BfsStep * 4096)()
+    n_td = n_bu = 0
+    for r in roots:
+        _lib.check(lib.lgmi_bfs_run(h, int(r), 0, 1, _lib.ctypes.byref(st)))
+        n = lib.lgmi_bfs_steps(h, 4096, steps_buf)
+        if n < 0:
+            _lib.check(n)
+        launches += st.launches
+        tot_ms += st.steps_event_ms
+        for k in range(n):
+            s = steps_buf[k]
+            b = step_bytes(s, nv)
+            tot_bytes += b
+            if s.dir == 1:
+                bu_bytes += b
+                bu_ms += s.event_ms
+                n_bu += 1
+            else:
+                n_td += 1
+    _lib.check(lib.lgmi_bfs_time_steps(h, 0))
+    achieved = tot_bytes / (tot_ms * 1e-3) / 1e9 if tot_ms > 0 else 0.0
+    bu_achieved = bu_bytes / (bu_ms * 1e-3) / 1e9 if bu_ms > 0 else 0.0
+    log(f"step kernel: {launches} launches, avg {tot_ms / max(launches, 1) * 1e3:.1f} us, "
+        f"{tot_bytes / max(launches, 1) / 1e6:.2f} MB/launch, {achieved:.0f} GB/s; "
+        f"bottom-up steps {n_bu}: {bu_achieved:.0f} GB/s; top-down steps {n_td}")
+
+    # CPU baseline: serial oracle BFS (distances + parents) on a sample of roots
+    sample_roots = roots[: args.cpu_roots]
+    t_cpu = 0.0
+    e_cpu = 0.0
+    for i, r in enumerate(sample_roots):
+        c0 = time.perf_counter()
+        orc.bfs_dist_parent(rp, col, int(r))
+        t_cpu += time.perf_counter() - c0
+        e_cpu += mcomp[i]
+    cpu_gteps = e_cpu / t_cpu / 1e9
+    log(f"cpu baseline: {len(sample_roots)} roots in {t_cpu:.1f}s -> {cpu_gteps:.4f} GTEPS")
+
+    traffic = read_traffic()
+    result = {
+        "metric": METRIC,
+        "value": round(gteps, 4),
+        "unit": "GTEPS",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed * 1e3 / K, 5),
+        "higher_is_better": True,
+        "scaling": "strong",
+        "vs_baseline": None,
+        "dtype": "int32",
+        "data": "synthetic (project RMAT generator, seed 1)",
+        "config": {
+            "workload": "single-source BFS (distances + parents), symmetrised RMAT scale 22, edge factor 16, "
+                        "a/b/c = 0.57/0.19/0.19, self-loops and duplicates removed, Int32 CSR; "
+                        "64 seeded roots of degree >= 1",
+            "nv": nv, "nnz": nnz, "roots": len(roots), "policy": "alpha=15/beta=18 heuristic",
+            "gteps_harmonic_mean_device": round(float(len(per_root) / np.sum(1.0 / per_root)), 3),
+            "gteps_median_device": round(float(np.median(per_root)), 3),
+        },
+        "roofline": {
+            "kernel": "bfs_step",
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic,
+            "bytes_per_launch": round(tot_bytes / max(launches, 1), 1),
+            "avg_launch_us": round(tot_ms / max(launches, 1) * 1e3, 3),
+            "bottom_up_achieved": round(bu_achieved, 1),
+            "bottom_up_frac": round(bu_achieved / HBM_PEAK_GBS, 4),
+        },
+        "cpu_baseline": {
+            "value": round(cpu_gteps, 5),
+            "unit": "GTEPS",
+            "cores": 1,
+            "kind": "port",
+            "sample": f"{len(sample_roots)} of the 64 roots, serial oracle BFS with distances and parents "
+                      f"({t_cpu:.1f} s)",
+        },
+    }
+    return result
+
+
+def read_traffic():
+    """HBM bytes per launch of bfs_step from the committed rocprofv3 --pmc
+    summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), or
+    None when no summary is present."""
+    p = os.path.join(REPO, "profiles", "bfs_step_rmat22_pmc.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        d = json.load(f)
+    return d.get("hbm_bytes_per_launch")
+
+
+def bc_bench(args, lgmi, _lib, orc):
+    t0 = time.time()
+    g = lgmi.generators.rmat(20, 16, seed=1)
+    rp, col = g.csr()
+    nv, nnz = g.nv(), len(col)
+    deg = np.diff(rp)
+    log(f"RMAT-20 built: nv={nv} nnz={nnz} ({time.time() - t0:.1f}s)")
+    rng = np.random.default_rng(4242)
+    srcs = rng.choice(np.nonzero(deg > 0)[0], size=1024, replace=False).astype(np.int64)
+    g.handle()
+    lgmi.betweenness_centrality(g, srcs[:64] + 1)          # warmup (allocations)
+    from lgmi.centrality import last_stats
+    t1 = time.perf_counter()
+    bc = lgmi.betweenness_centrality(g, srcs + 1)
+    wall = time.perf_counter() - t1
+    stt = last_stats(g)
+    m = nnz / 2
+    teps = 1024 * m / wall
+    log(f"BC 1024 sources: {wall * 1e3:.1f} ms wall, device {stt['device_ms']:.1f} ms "
+        f"(fwd {stt['forward_ms']:.1f}, bwd {stt['backward_ms']:.1f}), {teps / 1e9:.2f} GTEPS_BC")
+    # CPU baseline: threaded oracle on a sample of the sources, scaled
+    nthreads = min(16, os.cpu_count() or 1)
+    ns = args.cpu_bc_sources
+    c0 = time.perf_counter()
+    want = orc.betweenness_threaded(rp, col, rp, False, srcs[:ns], nthreads=nthreads, index_quirk=False)
+    t_cpu = time.perf_counter() - c0
+    cpu_teps = ns * m / t_cpu
+    # parity spot check on the sample: rescale to the 1024-source k
+    got = lgmi.betweenness_centrality(g, srcs[:ns] + 1)
+    floor = 1e-15 * max(1.0, float(np.abs(want).max()))
+    err = float(np.max(np.abs(got - want) / np.maximum(np.abs(want), floor)))
+    log(f"BC cpu baseline: {ns} sources, {nthreads} threads, {t_cpu:.1f}s -> {cpu_teps / 1e9:.4f} GTEPS_BC; "
+        f"sample max rel err {err:.2e}")
+    return {
+        "workload": "exact Brandes betweenness, symmetrised RMAT scale 20 edge factor 16, "
+                    "1024 explicit seeded sources in 64-lane groups, FP64",
+        "value": round(teps / 1e9, 4),
+        "unit": "GTEPS_BC (sources x undirected edges / s)",
+        "wall_ms": round(wall * 1e3, 2),
+        "device_ms": round(stt["device_ms"], 2),
+        "forward_ms": round(stt["forward_ms"], 2),
+        "backward_ms": round(stt["backward_ms"], 2),
+        "sample_max_rel_err": err,
+        "cpu_baseline": {"value": round(cpu_teps / 1e9, 5), "unit": "GTEPS_BC", "cores": nthreads,
+                         "kind": "port",
+                         "sample": f"{ns} of the 1024 sources, threaded oracle Brandes ({t_cpu:.1f} s)"},
+        "speedup_vs_cpu": round(teps / cpu_teps, 1),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=64)
+    ap.add_argument("--warmup", type=int, default=8)
+    ap.add_argument("--cpu-roots", type=int, default=8)
+    ap.add_argument("--cpu-bc-sources", type=int, default=128)
+    ap.add_argument("--no-bc", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
+              file=sys.stderr)
+        sys.exit(2)
+    import lgmi
+    from lgmi import _lib
+    from oracle import oracle as orc
+    if _lib.load().lgmi_device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    result = bfs_bench(args, lgmi, _lib, orc)
+    if not args.no_bc:
+        result["bc"] = bc_bench(args, lgmi, _lib, orc)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,191 @@
+/*
+ * lgmi.h -- C ABI of liblgmi.so: MI355X (gfx950) breadth-first search and
+ * exact Brandes betweenness for LightGraphs.jl's SimpleGraph/SimpleDiGraph.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns
+ * LGMI_OK (0) or a negative LGMI_E_* code, and leaves a message for
+ * lgmi_last_error() on failure.  No C++ exception crosses the boundary.
+ *
+ * Vertex ids are 0-based Int32; a graph is a CSR whose rows are sorted
+ * ascending (the fadjlist/badjlist invariant of the reference,
+ * src/SimpleGraphs/simplegraph.jl:445-461, simpledigraph.jl:414-430).
+ * Unreached distances and parents come back as -1; the Julia shim maps them
+ * to typemax(T) and 0 and adds 1 to every id.
+ *
+ * Host arrays are borrowed for the duration of a call.  Calls on one graph
+ * handle are serialised by the handle; distinct handles may be used from
+ * distinct threads.  Results are complete when a call returns.
+ */
+#ifndef LGMI_H
+#define LGMI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LGMI_OK            0
+#define LGMI_E_ARG        -1  /* ArgumentError in the shim                 */
+#define LGMI_E_BOUNDS     -2  /* BoundsError: a vertex id out of range     */
+#define LGMI_E_TOO_LARGE  -3  /* more than 2^31-1 adjacency entries        */
+#define LGMI_E_NODEV      -4  /* no GPU visible                            */
+#define LGMI_E_ALLOC      -5  /* device allocation failed                  */
+#define LGMI_E_HIP        -6  /* any other HIP runtime error               */
+
+#define LGMI_POLICY_HEURISTIC 0   /* Beamer alpha/beta direction switch    */
+#define LGMI_POLICY_LEARNED   1   /* FP32 MLP 16 -> 64 (ReLU) -> 2 logits  */
+#define LGMI_POLICY_TOPDOWN   2   /* always top-down (reference behaviour) */
+#define LGMI_POLICY_FEATURES  16
+#define LGMI_POLICY_HIDDEN    64
+/* MLP parameter vector layout (FP32): W1[64x16] row-major, b1[64],
+ * W2[2x64] row-major, b2[2], feature mean[16], feature scale[16]. */
+#define LGMI_POLICY_NWEIGHTS  (64 * 16 + 64 + 2 * 64 + 2 + 16 + 16)
+
+typedef struct lgmi_graph* lgmi_graph_t;
+
+/* Statistics of the last BFS run on a handle. */
+typedef struct {
+    int32_t levels;            /* number of BFS steps executed               */
+    int32_t reached;           /* vertices with a finite distance            */
+    int32_t td_steps;          /* steps run top-down                         */
+    int32_t bu_steps;          /* steps run bottom-up                        */
+    double device_ms;          /* HIP-event time, init to last step          */
+    int64_t edges_reached;     /* sum over reached vertices of out-degree    */
+    int32_t launches;          /* step launches (a few after the last level  */
+                               /* exit at once)                              */
+    int32_t pad;
+    double steps_event_ms;     /* sum of per-launch HIP-event times of the   */
+                               /* step kernel (0 unless step timing is on)   */
+} lgmi_bfs_stats;
+
+/* One BFS step (level L -> L+1) of the last run. */
+typedef struct {
+    float feat[16];            /* policy features of the state at step start */
+    int32_t dir;               /* 0 top-down, 1 bottom-up                    */
+    int32_t n_front;           /* |F_L|                                      */
+    int64_t m_front;           /* traversal-direction edges of F_L           */
+    int64_t m_unvisited;       /* scan-direction edges of unvisited vertices */
+    int32_t n_next;            /* |F_{L+1}|                                  */
+    int32_t pad;
+    int64_t scanned;           /* bottom-up: adjacency entries read;         */
+                               /* top-down: edges expanded                   */
+    int64_t cand;              /* bottom-up: unvisited vertices examined     */
+    float kernel_us;           /* first block start to last block end        */
+    float step_us;             /* previous step end to this step end         */
+    float event_ms;            /* HIP-event time of the launch, 0 if off     */
+    int32_t pad2;
+} lgmi_bfs_step;
+
+/* Statistics of the last betweenness run. */
+typedef struct {
+    int32_t groups;            /* 64-source lane groups processed            */
+    int32_t max_levels;        /* deepest BFS level over all groups          */
+    double device_ms;          /* HIP-event time of the whole computation    */
+    double forward_ms;         /* sigma passes                                */
+    double backward_ms;        /* delta passes                                */
+} lgmi_bc_stats;
+
+/* ----------------------------------------------------------------------
+ * Graph handle.  Replaces the reference's fadjlist/badjlist access
+ * (outneighbors/inneighbors, src/SimpleGraphs/SimpleGraphs.jl:78-79,92-93;
+ * badj, simplegraph.jl:376-377, simpledigraph.jl:381-382) with a device CSR.
+ * in_row_ptr/in_col are NULL for an undirected graph (the in-CSR is then the
+ * out-CSR); for a directed graph they are the badjlist as a CSR.
+ * ---------------------------------------------------------------------- */
+int lgmi_graph_create(int32_t nv, int64_t nnz, const int32_t* row_ptr, const int32_t* col,
+                      int64_t in_nnz, const int32_t* in_row_ptr, const int32_t* in_col,
+                      int directed, lgmi_graph_t* out);
+void lgmi_graph_destroy(lgmi_graph_t g);
+
+/* gdistances(g, source) / gdistances!(g, source, vert_level)
+ *   src/traversals/bfs.jl:95-141
+ * Parallel.gdistances(g, sources) / Parallel.gdistances!(...)
+ *   src/Parallel/traversals/gdistances.jl:36-139
+ * Distances along out-edges from all of srcs[0..nsrc) at once; dist[v] = -1
+ * when v is unreached.  nsrc == 0 gives all -1.  Duplicate sources are
+ * harmless.  LGMI_E_BOUNDS when a source is outside [0, nv). */
+int lgmi_gdistances(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int32_t* dist);
+
+/* bfs_parents(g, s; dir)  -- src/traversals/bfs.jl:35-66
+ * bfs_tree / Parallel.bfs_tree(!) use these parents
+ *   (bfs.jl:76, src/Parallel/traversals/bfs.jl:78-104).
+ * dir_in = 0 follows out-edges (dir = :out), 1 follows in-edges.
+ * parent[src] = src; parent[v] = the smallest-id vertex u one level closer
+ * to src with an edge u -> v, which is the vertex the reference's sorted
+ * frontier discovers v from; unreached = -1. */
+int lgmi_bfs_parents(lgmi_graph_t g, int32_t src, int dir_in, int32_t* parent);
+
+/* Distances and parents of one single-source BFS in one pass (either
+ * output may be NULL). */
+int lgmi_bfs(lgmi_graph_t g, int32_t src, int dir_in, int32_t* dist, int32_t* parent);
+
+/* Benchmark form: runs a single-source BFS and leaves dist/parent in the
+ * handle's device buffers (no host copy).  want_parents = 0 skips the
+ * parent array (gdistances).  stats may be NULL. */
+int lgmi_bfs_run(lgmi_graph_t g, int32_t src, int dir_in, int want_parents, lgmi_bfs_stats* stats);
+/* Copies the last run's results to the host; either pointer may be NULL. */
+int lgmi_bfs_fetch(lgmi_graph_t g, int32_t* dist, int32_t* parent);
+
+/* Per-step records of the last BFS run (policy features and labels,
+ * algorithmic-byte counters).  Writes up to max_steps records and returns
+ * the number written (or a negative code). */
+int lgmi_bfs_steps(lgmi_graph_t g, int32_t max_steps, lgmi_bfs_step* out);
+
+/* enable = 1 brackets every step launch of later runs with HIP events
+ * (per-launch durations in lgmi_bfs_step.event_ms and
+ * lgmi_bfs_stats.steps_event_ms); 0 turns it off. */
+int lgmi_bfs_time_steps(lgmi_graph_t g, int enable);
+
+/* Forces every step of the next BFS runs to top-down (0), bottom-up (1) or
+ * lets the policy decide (-1).  Outputs never change; only time does. */
+int lgmi_bfs_force_direction(lgmi_graph_t g, int32_t dir);
+
+/* Direction policy for all later BFS runs (process-wide).
+ * mode: LGMI_POLICY_HEURISTIC (alpha, beta), LGMI_POLICY_LEARNED (mlp must
+ * hold LGMI_POLICY_NWEIGHTS floats; the heuristic is used while none is
+ * loaded) or LGMI_POLICY_TOPDOWN.  No counterpart in the reference, whose
+ * BFS is top-down only (bfs.jl:108-122). */
+int lgmi_bfs_set_policy(int mode, float alpha, float beta, const float* mlp, int32_t nweights);
+
+/* betweenness_centrality(g, vs, DefaultDistance; normalize, endpoints)
+ *   src/centrality/betweenness.jl:46-160 (k-sample form :77-78 with vs
+ *   drawn by the caller)
+ * Parallel.betweenness_centrality(...; parallel = :distributed | :threads)
+ *   src/Parallel/centrality/betweenness.jl:1-75 (the shim passes the
+ *   source list each mode actually uses)
+ * Exact Brandes over srcs[0..nsrc) in FP64.  Sources with degree(g, s) == 0
+ * are skipped as the reference does (betweenness.jl:58); duplicates count
+ * twice.  k_scale is the k of _rescale! (the reference's length(vs)).
+ * Results match the reference within 1e-12 relative error (only the
+ * floating-point summation order differs). */
+int lgmi_betweenness(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int64_t k_scale,
+                     int normalize, int endpoints, double* bc);
+/* Statistics of the last betweenness run on g. */
+int lgmi_betweenness_stats(lgmi_graph_t g, lgmi_bc_stats* stats);
+
+/* All-source betweenness for many independent undirected graphs in one
+ * persistent launch (each graph as betweenness_centrality(g) with
+ * vs = vertices(g)).  Graph i has nv[i] vertices, its row pointer at
+ * row_ptr_cat[row_off[i] .. row_off[i]+nv[i]] (nv[i]+1 entries, 0-based
+ * within the graph) and its adjacency at col_cat[col_off[i] ..]; its
+ * nv[i] results go to bc_cat[bc_off[i] ..].  device_ms may be NULL. */
+int lgmi_betweenness_batch(int32_t ngraphs, const int32_t* nv, const int64_t* row_off,
+                           const int32_t* row_ptr_cat, const int64_t* col_off,
+                           const int32_t* col_cat, const int64_t* bc_off, int normalize,
+                           int endpoints, double* bc_cat, double* device_ms);
+
+/* Thread-local message of the last failed call ("" if none). */
+const char* lgmi_last_error(void);
+
+/* Number of visible GPUs (0 when none; never fails). */
+int lgmi_device_count(void);
+
+/* Blocks until all work the library queued has finished. */
+int lgmi_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LGMI_H */

@@ -1,0 +1,330 @@
+// capi.hip -- graph handle, error reporting and the extern "C" entry points
+// of liblgmi.so (declared in include/lgmi.h).
+#include "lgmi_internal.h"
+
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+namespace lgmi {
+
+namespace {
+thread_local std::string t_last_error;
+
+__global__ void nz_bitmap_kernel(const int32_t* rp, int32_t nv, int32_t nwords, uint32_t* nz) {
+    const int32_t w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nwords) return;
+    uint32_t bits = 0;
+    for (int b = 0; b < 32; b++) {
+        const int64_t v = (int64_t)w * 32 + b;
+        if (v < nv && rp[v + 1] > rp[v]) bits |= 1u << b;
+    }
+    nz[w] = bits;
+}
+
+int g_device_count = -1;
+std::mutex g_dev_mu;
+}  // namespace
+
+void set_last_error(const std::string& msg) { t_last_error = msg; }
+
+void* dev_alloc(size_t bytes) {
+    const char* f = std::getenv("LGMI_FAIL_ALLOC");
+    if (f && f[0] == '1') throw Error(LGMI_E_ALLOC, "device allocation refused (LGMI_FAIL_ALLOC=1)");
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(LGMI_E_ALLOC, std::string("hipMalloc(") + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    }
+    return p;
+}
+
+void dev_free(void* p) {
+    if (p) (void)hipFree(p);
+}
+
+void ensure_device() {
+    std::lock_guard<std::mutex> lk(g_dev_mu);
+    if (g_device_count < 0) {
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+        (void)hipGetLastError();
+        g_device_count = n;
+    }
+    if (g_device_count <= 0) throw Error(LGMI_E_NODEV, "no GPU device visible");
+}
+
+int device_cu_count() {
+    int dev = 0;
+    LGMI_HIP(hipGetDevice(&dev));
+    int cu = 0;
+    LGMI_HIP(hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev));
+    return cu;
+}
+
+PolicyHost& policy_host() {
+    static PolicyHost p;
+    return p;
+}
+std::mutex& policy_mutex() {
+    static std::mutex m;
+    return m;
+}
+
+static void check_csr(int32_t nv, int64_t nnz, const int32_t* rp, const int32_t* col, const char* what) {
+    if (!rp || (nnz > 0 && !col)) throw Error(LGMI_E_ARG, std::string(what) + ": null array");
+    if (rp[0] != 0 || (int64_t)rp[nv] != nnz) throw Error(LGMI_E_ARG, std::string(what) + ": row_ptr[0] must be 0 and row_ptr[nv] == nnz");
+    for (int32_t v = 0; v < nv; v++) {
+        const int32_t b = rp[v], e = rp[v + 1];
+        if (e < b) throw Error(LGMI_E_ARG, std::string(what) + ": row_ptr not monotone");
+        for (int32_t k = b; k < e; k++) {
+            const int32_t u = col[k];
+            if (u < 0 || u >= nv) throw Error(LGMI_E_BOUNDS, std::string(what) + ": neighbour id out of range");
+            if (k > b && col[k - 1] >= u) throw Error(LGMI_E_ARG, std::string(what) + ": rows must be sorted ascending without duplicates");
+        }
+    }
+}
+
+static void upload_csr(lgmi_graph* g, DevCSR& d, const int32_t* rp, const int32_t* col, int64_t nnz) {
+    d.row_ptr = dev_alloc_n<int32_t>((size_t)g->nv + 1);
+    d.col = dev_alloc_n<int32_t>((size_t)std::max<int64_t>(nnz, 1));
+    const int32_t nwords = 2 * ((g->nv + 63) / 64);
+    d.nz_bm = dev_alloc_n<uint32_t>((size_t)std::max(nwords, 1));
+    LGMI_HIP(hipMemcpyAsync(d.row_ptr, rp, sizeof(int32_t) * ((size_t)g->nv + 1), hipMemcpyHostToDevice, g->stream));
+    if (nnz > 0) LGMI_HIP(hipMemcpyAsync(d.col, col, sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice, g->stream));
+    if (nwords > 0) {
+        nz_bitmap_kernel<<<(nwords + 255) / 256, 256, 0, g->stream>>>(d.row_ptr, g->nv, nwords, d.nz_bm);
+        LGMI_HIP(hipGetLastError());
+    }
+}
+
+static void graph_free(lgmi_graph* g) {
+    if (!g) return;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    bfs_free(g->bfs);
+    bc_free(g->bc);
+    if (g->in.row_ptr != g->out.row_ptr) {
+        dev_free(g->in.row_ptr); dev_free(g->in.col); dev_free(g->in.nz_bm);
+    }
+    dev_free(g->out.row_ptr); dev_free(g->out.col); dev_free(g->out.nz_bm);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+}
+
+}  // namespace lgmi
+
+using namespace lgmi;
+
+#define LGMI_API_BEGIN try {
+#define LGMI_API_END                                                  \
+    }                                                                 \
+    catch (const Error& e) {                                          \
+        set_last_error(e.what());                                     \
+        return e.code;                                                \
+    }                                                                 \
+    catch (const std::bad_alloc&) {                                   \
+        set_last_error("host allocation failed");                     \
+        return LGMI_E_ALLOC;                                          \
+    }                                                                 \
+    catch (const std::exception& e) {                                 \
+        set_last_error(e.what());                                     \
+        return LGMI_E_HIP;                                            \
+    }                                                                 \
+    return LGMI_OK;
+
+extern "C" {
+
+const char* lgmi_last_error(void) { return t_last_error.c_str(); }
+
+int lgmi_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) { (void)hipGetLastError(); return 0; }
+    return n;
+}
+
+int lgmi_synchronize(void) {
+    LGMI_API_BEGIN
+    ensure_device();
+    LGMI_HIP(hipDeviceSynchronize());
+    LGMI_API_END
+}
+
+int lgmi_graph_create(int32_t nv, int64_t nnz, const int32_t* row_ptr, const int32_t* col,
+                      int64_t in_nnz, const int32_t* in_row_ptr, const int32_t* in_col,
+                      int directed, lgmi_graph_t* out) {
+    LGMI_API_BEGIN
+    if (!out) throw Error(LGMI_E_ARG, "out is NULL");
+    *out = nullptr;
+    if (nv < 0) throw Error(LGMI_E_ARG, "nv < 0");
+    if (nnz > 0x7fffffffLL || in_nnz > 0x7fffffffLL) throw Error(LGMI_E_TOO_LARGE, "more than 2^31-1 adjacency entries");
+    if (nnz < 0) throw Error(LGMI_E_ARG, "nnz < 0");
+    check_csr(nv, nnz, row_ptr, col, "out-CSR");
+    if (directed) {
+        if (!in_row_ptr) throw Error(LGMI_E_ARG, "a directed graph needs its in-neighbour CSR");
+        if (in_nnz != nnz) throw Error(LGMI_E_ARG, "in_nnz must equal nnz");
+        check_csr(nv, in_nnz, in_row_ptr, in_col, "in-CSR");
+    }
+    ensure_device();
+    lgmi_graph* g = new lgmi_graph();
+    try {
+        g->nv = nv;
+        g->nnz = nnz;
+        g->directed = directed ? 1 : 0;
+        LGMI_HIP(hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking));
+        g->cu_count = device_cu_count();
+        upload_csr(g, g->out, row_ptr, col, nnz);
+        if (directed) upload_csr(g, g->in, in_row_ptr, in_col, in_nnz);
+        else g->in = g->out;
+        LGMI_HIP(hipStreamSynchronize(g->stream));
+    } catch (...) {
+        graph_free(g);
+        throw;
+    }
+    *out = g;
+    LGMI_API_END
+}
+
+void lgmi_graph_destroy(lgmi_graph_t g) { graph_free(g); }
+
+static void check_src(lgmi_graph_t g, int32_t s) {
+    if (s < 0 || s >= g->nv) throw Error(LGMI_E_BOUNDS, "source vertex " + std::to_string(s) + " out of range");
+}
+
+int lgmi_gdistances(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int32_t* dist) {
+    LGMI_API_BEGIN
+    if (!g || !dist || nsrc < 0 || (nsrc > 0 && !srcs)) throw Error(LGMI_E_ARG, "bad arguments");
+    std::lock_guard<std::mutex> lk(g->mu);
+    std::vector<int32_t> s(srcs, srcs + nsrc);
+    for (int32_t x : s) check_src(g, x);
+    std::sort(s.begin(), s.end());
+    s.erase(std::unique(s.begin(), s.end()), s.end());
+    if (s.size() == 1) bfs_run(g, nullptr, 1, s[0], 0, 0, -1, nullptr);
+    else bfs_run(g, s.data(), (int32_t)s.size(), -1, 0, 0, -1, nullptr);
+    bfs_fetch(g, dist, nullptr);
+    LGMI_API_END
+}
+
+int lgmi_bfs_parents(lgmi_graph_t g, int32_t src, int dir_in, int32_t* parent) {
+    LGMI_API_BEGIN
+    if (!g || !parent) throw Error(LGMI_E_ARG, "bad arguments");
+    std::lock_guard<std::mutex> lk(g->mu);
+    check_src(g, src);
+    bfs_run(g, nullptr, 1, src, dir_in ? 1 : 0, 1, -1, nullptr);
+    bfs_fetch(g, nullptr, parent);
+    LGMI_API_END
+}
+
+int lgmi_bfs(lgmi_graph_t g, int32_t src, int dir_in, int32_t* dist, int32_t* parent) {
+    LGMI_API_BEGIN
+    if (!g) throw Error(LGMI_E_ARG, "bad arguments");
+    std::lock_guard<std::mutex> lk(g->mu);
+    check_src(g, src);
+    bfs_run(g, nullptr, 1, src, dir_in ? 1 : 0, parent ? 1 : 0, -1, nullptr);
+    bfs_fetch(g, dist, parent);
+    LGMI_API_END
+}
+
+int lgmi_bfs_run(lgmi_graph_t g, int32_t src, int dir_in, int want_parents, lgmi_bfs_stats* stats) {
+    LGMI_API_BEGIN
+    if (!g) throw Error(LGMI_E_ARG, "bad arguments");
+    std::lock_guard<std::mutex> lk(g->mu);
+    check_src(g, src);
+    const int forced = g->bfs ? g->bfs_forced : -1;
+    bfs_run(g, nullptr, 1, src, dir_in ? 1 : 0, want_parents ? 1 : 0, forced, stats);
+    LGMI_API_END
+}
+
+int lgmi_bfs_fetch(lgmi_graph_t g, int32_t* dist, int32_t* parent) {
+    LGMI_API_BEGIN
+    if (!g) throw Error(LGMI_E_ARG, "bad arguments");
+    std::lock_guard<std::mutex> lk(g->mu);
+    bfs_fetch(g, dist, parent);
+    LGMI_API_END
+}
+
+int lgmi_bfs_force_direction(lgmi_graph_t g, int32_t dir) {
+    LGMI_API_BEGIN
+    if (!g || dir < -1 || dir > 1) throw Error(LGMI_E_ARG, "dir must be -1, 0 or 1");
+    std::lock_guard<std::mutex> lk(g->mu);
+    bfs_prepare(g);
+    g->bfs_forced = dir;
+    LGMI_API_END
+}
+
+int lgmi_bfs_set_policy(int mode, float alpha, float beta, const float* mlp, int32_t nweights) {
+    LGMI_API_BEGIN
+    if (mode != LGMI_POLICY_HEURISTIC && mode != LGMI_POLICY_LEARNED && mode != LGMI_POLICY_TOPDOWN)
+        throw Error(LGMI_E_ARG, "unknown policy mode");
+    if (!(alpha > 0.f) || !(beta > 0.f)) throw Error(LGMI_E_ARG, "alpha and beta must be positive");
+    std::lock_guard<std::mutex> lk(policy_mutex());
+    PolicyHost& p = policy_host();
+    if (mlp) {
+        if (nweights != LGMI_POLICY_NWEIGHTS) throw Error(LGMI_E_ARG, "MLP needs LGMI_POLICY_NWEIGHTS floats");
+        static_assert(sizeof(PolicyWeights) == sizeof(float) * LGMI_POLICY_NWEIGHTS, "layout");
+        std::memcpy(&p.w, mlp, sizeof(PolicyWeights));
+        for (int i = 0; i < kFeat; i++)
+            if (!(p.w.sd[i] > 0.f)) throw Error(LGMI_E_ARG, "feature scales must be positive");
+        p.have_mlp = true;
+    } else if (mode == LGMI_POLICY_LEARNED && !p.have_mlp) {
+        throw Error(LGMI_E_ARG, "learned policy needs MLP weights");
+    }
+    p.mode = mode;
+    p.alpha = alpha;
+    p.beta = beta;
+    p.version++;
+    LGMI_API_END
+}
+
+int lgmi_betweenness(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, int normalize,
+                     int endpoints, double* bc) {
+    LGMI_API_BEGIN
+    if (!g || !bc || nsrc < 0 || (nsrc > 0 && !srcs) || k_scale < 0) throw Error(LGMI_E_ARG, "bad arguments");
+    std::lock_guard<std::mutex> lk(g->mu);
+    for (int32_t i = 0; i < nsrc; i++) check_src(g, srcs[i]);
+    bc_run(g, srcs, nsrc, k_scale, normalize, endpoints, bc, &g->bc_last);
+    LGMI_API_END
+}
+
+int lgmi_betweenness_batch(int32_t ngraphs, const int32_t* nv, const int64_t* row_off, const int32_t* row_ptr_cat,
+                           const int64_t* col_off, const int32_t* col_cat, const int64_t* bc_off, int normalize,
+                           int endpoints, double* bc_cat, double* device_ms) {
+    LGMI_API_BEGIN
+    if (ngraphs < 0 || (ngraphs > 0 && (!nv || !row_off || !row_ptr_cat || !col_off || !col_cat || !bc_off || !bc_cat)))
+        throw Error(LGMI_E_ARG, "bad arguments");
+    ensure_device();
+    double total_ms = 0.0;
+    for (int32_t i = 0; i < ngraphs; i++) {
+        const int32_t n = nv[i];
+        if (n < 0) throw Error(LGMI_E_ARG, "nv < 0");
+        const int32_t* rp = row_ptr_cat + row_off[i];
+        const int64_t nnz = rp[n];
+        lgmi_graph_t g = nullptr;
+        int rc = lgmi_graph_create(n, nnz, rp, col_cat + col_off[i], 0, nullptr, nullptr, 0, &g);
+        if (rc != LGMI_OK) throw Error(rc, t_last_error);
+        std::vector<int32_t> vs((size_t)n);
+        for (int32_t v = 0; v < n; v++) vs[(size_t)v] = v;
+        try {
+            lgmi_bc_stats st{};
+            bc_run(g, vs.data(), n, n, normalize, endpoints, bc_cat + bc_off[i], &st);
+            total_ms += st.device_ms;
+        } catch (...) {
+            lgmi_graph_destroy(g);
+            throw;
+        }
+        lgmi_graph_destroy(g);
+    }
+    if (device_ms) *device_ms = total_ms;
+    LGMI_API_END
+}
+
+int lgmi_betweenness_stats(lgmi_graph_t g, lgmi_bc_stats* stats) {
+    LGMI_API_BEGIN
+    if (!g || !stats) throw Error(LGMI_E_ARG, "bad arguments");
+    std::lock_guard<std::mutex> lk(g->mu);
+    *stats = g->bc_last;
+    LGMI_API_END
+}
+
+}  // extern "C"

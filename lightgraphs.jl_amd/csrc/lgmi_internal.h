@@ -1,0 +1,177 @@
+// lgmi_internal.h -- shared definitions for the liblgmi.so sources.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/lgmi.h"
+
+namespace lgmi {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+#define LGMI_HIP(expr)                                                                        \
+    do {                                                                                      \
+        hipError_t _e = (expr);                                                               \
+        if (_e != hipSuccess)                                                                 \
+            throw ::lgmi::Error(_e == hipErrorOutOfMemory ? LGMI_E_ALLOC : LGMI_E_HIP,        \
+                                std::string(#expr) + ": " + hipGetErrorString(_e));           \
+    } while (0)
+
+// Device allocation with an optional injected failure (LGMI_FAIL_ALLOC=1)
+// so that error paths can be exercised from the host.
+void* dev_alloc(size_t bytes);
+void dev_free(void* p);
+void ensure_device();  // throws LGMI_E_NODEV when no GPU is visible
+int device_cu_count();
+
+template <class T>
+T* dev_alloc_n(size_t n) {
+    return static_cast<T*>(dev_alloc(sizeof(T) * (n ? n : 1)));
+}
+
+// ---------------------------------------------------------------------
+// BFS control block (device resident).  One step kernel advances the BFS
+// by one level; the last work-group to finish a step updates this block
+// and chooses the direction of the next step.
+// ---------------------------------------------------------------------
+constexpr int kFeat = LGMI_POLICY_FEATURES;  // per-level policy features
+constexpr int kHidden = LGMI_POLICY_HIDDEN;
+constexpr int kHistBins = 8;                 // log4-degree histogram of a frontier
+
+struct BfsCtrl {
+    int32_t level;          // L: the frontier holds the vertices at distance L
+    int32_t dir;            // direction of step L -> L+1 (0 top-down, 1 bottom-up)
+    int32_t done;
+    int32_t n_front;        // |F_L|
+    int32_t n_prev;         // |F_{L-1}|
+    int32_t prev_dir;
+    int64_t m_front;        // sum of traversal-direction degrees over F_L
+    int64_t m_unvisited;    // sum of scan-direction degrees over unvisited vertices
+    int64_t n_unvisited;
+    // accumulators of the step in flight
+    int32_t next_count;
+    uint32_t blocks_done;
+    unsigned long long next_out;
+    unsigned long long next_in;
+    unsigned long long next_scanned;  // adjacency entries read by the step
+    unsigned long long next_cand;     // vertices the step examined (bottom-up)
+    uint32_t hist[kHistBins];   // degree histogram of F_{L+1} (being built)
+    uint32_t front_hist[kHistBins];
+    uint64_t t_prev_end;    // wall clock at the end of the previous step
+    uint64_t t_step_start;  // min wall clock over the blocks of the current step
+    int32_t forced;         // -1 policy decides; 0/1 force a direction (label runs)
+    int32_t policy_mode;    // LGMI_POLICY_*
+    float alpha, beta;
+    int32_t nv;
+    int32_t pad;
+    int64_t nnz_scan;
+};
+
+struct LevelRec {
+    float feat[kFeat];      // features of the state at the start of the step
+    int32_t dir;            // direction used by the step
+    int32_t n_front;
+    int64_t m_front;
+    int64_t m_unvisited;
+    int32_t n_next;
+    int32_t pad;
+    int64_t scanned;        // adjacency entries read (bottom-up) / expanded (top-down)
+    int64_t cand;           // bottom-up: unvisited vertices examined
+    uint64_t t_start;       // first block start (wall clock ticks)
+    uint64_t t_end;         // last block end
+    uint64_t t_prev_end;    // end of the previous step (launch gap included)
+};
+
+struct PolicyWeights {      // FP32 MLP 16 -> 64 (ReLU) -> 2
+    float w1[kHidden * kFeat];
+    float b1[kHidden];
+    float w2[2 * kHidden];
+    float b2[2];
+    float mu[kFeat];        // feature standardisation
+    float sd[kFeat];
+};
+
+struct BfsWork {
+    int32_t nv = 0;
+    int32_t nwords = 0;
+    int32_t* dist = nullptr;
+    int32_t* parent = nullptr;
+    int32_t* q[2] = {nullptr, nullptr};
+    uint32_t* bm[3] = {nullptr, nullptr, nullptr};
+    uint32_t* vis = nullptr;
+    BfsCtrl* ctrl = nullptr;
+    BfsCtrl* h_ctrl = nullptr;   // pinned host mirror
+    LevelRec* log = nullptr;
+    int32_t max_log = 0;
+    PolicyWeights* policy = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool time_steps = false;              // HIP events around every step launch
+    std::vector<hipEvent_t> step_ev;      // launch i runs between step_ev[i] and step_ev[i+1]
+    int last_launches = 0;
+    double last_steps_event_ms = 0.0;
+    std::vector<float> last_launch_ms;
+    int last_levels = 0;
+    int last_dir_in = 0;
+    int last_parents = 0;
+    ~BfsWork();
+};
+
+struct BcWork;
+struct DevCSR {
+    int32_t* row_ptr = nullptr;
+    int32_t* col = nullptr;
+    uint32_t* nz_bm = nullptr;   // bit v set when the row of v is non-empty
+};
+
+}  // namespace lgmi
+
+struct lgmi_graph {
+    int32_t nv = 0;
+    int64_t nnz = 0;
+    int directed = 0;
+    lgmi::DevCSR out;        // out-neighbours (fadjlist)
+    lgmi::DevCSR in;         // in-neighbours (badjlist); aliases `out` when undirected
+    int32_t* deg_total = nullptr;  // degree(g, v) as the reference defines it
+    hipStream_t stream = nullptr;
+    std::mutex mu;
+    lgmi::BfsWork* bfs = nullptr;
+    lgmi::BcWork* bc = nullptr;
+    int cu_count = 0;
+    int bfs_forced = -1;           // -1 policy, 0/1 forced direction (label runs)
+    lgmi_bc_stats bc_last{};       // statistics of the last betweenness call
+};
+
+namespace lgmi {
+// bfs.hip
+void bfs_prepare(lgmi_graph* g);
+void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_src, int dir_in,
+             int want_parents, int forced_dir, lgmi_bfs_stats* stats);
+void bfs_fetch(lgmi_graph* g, int32_t* dist, int32_t* parent);
+void bfs_free(BfsWork* w);
+void policy_upload(lgmi_graph* g);
+// bc.hip
+void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, int normalize,
+            int endpoints, double* bc, lgmi_bc_stats* stats);
+void bc_free(BcWork* w);
+// global policy state (host copy), set by lgmi_bfs_set_policy
+struct PolicyHost {
+    int mode = LGMI_POLICY_HEURISTIC;
+    float alpha = 15.0f, beta = 18.0f;
+    bool have_mlp = false;
+    PolicyWeights w{};
+    uint64_t version = 1;
+};
+PolicyHost& policy_host();
+std::mutex& policy_mutex();
+}  // namespace lgmi

@@ -1,0 +1,23 @@
+"""lgmi -- LightGraphs.jl's BFS and exact-betweenness hot path on MI355X.
+
+Host-side mirror of the reference's Julia interface for this path, calling
+hand-written gfx950 HIP kernels in liblgmi.so through its C ABI
+(include/lgmi.h):
+
+* ``gdistances``, ``gdistances_`` (gdistances!), ``bfs_parents``,
+  ``bfs_tree``, ``tree``            -- src/traversals/bfs.jl
+* ``betweenness_centrality``        -- src/centrality/betweenness.jl
+* ``Parallel.*``                    -- src/Parallel/{traversals,centrality}
+* ``SimpleGraph``, ``SimpleDiGraph`` -- src/SimpleGraphs (CSR-backed)
+"""
+from . import Parallel, generators
+from ._lib import LgmiError, NoDeviceError
+from .centrality import betweenness_centrality, sample
+from .graphs import SimpleDiGraph, SimpleGraph
+from .traversals import bfs_parents, bfs_tree, gdistances, gdistances_, tree, typemax
+
+__all__ = [
+    "Parallel", "generators", "LgmiError", "NoDeviceError", "betweenness_centrality", "sample",
+    "SimpleDiGraph", "SimpleGraph", "bfs_parents", "bfs_tree", "gdistances", "gdistances_", "tree",
+    "typemax",
+]

@@ -1,0 +1,70 @@
+"""betweenness_centrality on the GPU (exact Brandes, FP64).
+
+Mirrors src/centrality/betweenness.jl:46-78: ``vs`` defaults to all
+vertices, an integer ``k`` samples k distinct vertices, sources with
+degree 0 are skipped while ``k = length(vs)`` still counts them, and
+``normalize`` / ``endpoints`` follow _rescale! / _accumulate_endpoints!.
+Only the unit distance matrix (``distmx=None``, the reference's
+DefaultDistance) is on the accelerated path; a weighted ``distmx`` raises
+NotImplementedError so that a caller can fall back to the reference method.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib
+from .graphs import _GraphBase
+
+
+def sample(n: int, k: int, rng=None) -> np.ndarray:
+    """sample(vertices(g), k): k distinct vertices (src/utils.jl:43-45),
+    1-based.  The reference draws from Julia's GLOBAL_RNG; this draws from
+    ``rng`` (a numpy Generator or seed)."""
+    if k > n:
+        raise ValueError("cannot sample more vertices than the graph has")
+    gen = rng if isinstance(rng, np.random.Generator) else np.random.default_rng(rng)
+    return gen.choice(n, size=k, replace=False).astype(np.int64) + 1
+
+
+def _vs(g: _GraphBase, vs, rng=None) -> np.ndarray:
+    n = g.nv()
+    if vs is None:
+        return np.arange(1, n + 1, dtype=np.int64)
+    if isinstance(vs, (int, np.integer)):
+        return sample(n, int(vs), rng=rng)
+    arr = np.asarray(list(vs), dtype=np.int64)
+    return arr
+
+
+def _check_distmx(distmx):
+    if distmx is not None:
+        raise NotImplementedError("weighted distmx is not on the accelerated path")
+
+
+def _run(g: _GraphBase, srcs1: np.ndarray, k: int, normalize: bool, endpoints: bool) -> np.ndarray:
+    n = g.nv()
+    if srcs1.size and (srcs1.min() < 1 or srcs1.max() > n):
+        raise IndexError(f"source vertex out of range 1:{n}")
+    bc = np.zeros(max(n, 1), dtype=np.float64)
+    if n == 0:
+        return bc[:0]
+    srcs0 = np.ascontiguousarray(srcs1 - 1, dtype=np.int32)
+    lib = _lib.load()
+    _lib.check(lib.lgmi_betweenness(g.handle(), _lib.i32p(srcs0 if len(srcs0) else np.zeros(1, np.int32)),
+                                    len(srcs0), int(k), int(bool(normalize)), int(bool(endpoints)),
+                                    _lib.f64p(bc)))
+    return bc
+
+
+def betweenness_centrality(g: _GraphBase, vs=None, distmx=None, normalize=True, endpoints=False,
+                           rng=None) -> np.ndarray:
+    """betweenness_centrality(g[, vs | k][, distmx]; normalize, endpoints)."""
+    _check_distmx(distmx)
+    srcs = _vs(g, vs, rng=rng)
+    return _run(g, srcs, len(srcs), normalize, endpoints)
+
+
+def last_stats(g: _GraphBase) -> dict:
+    st = _lib.BcStats()
+    _lib.check(_lib.load().lgmi_betweenness_stats(g.handle(), _lib.ctypes.byref(st)))
+    return {f: getattr(st, f) for f, _ in st._fields_}

@@ -1,0 +1,566 @@
+/*
+ * lgoracle.c -- CPU oracle for the BFS / exact-Brandes path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is a plain-C restatement of the
+ * reference algorithms (LightGraphs.jl v1.3.5, Julia) that the HIP library
+ * replaces.  Only tests/, the smoke() check and the CPU-baseline leg of
+ * bench.py may load it, and only as the checker or as the timed CPU
+ * baseline; the product path never calls it.
+ *
+ * Conventions: vertices are 0-based here (the reference is 1-based); an
+ * "unreached" distance is left untouched (serial gdistances!) or written as
+ * -1 where the reference writes typemax(T); an unreached parent is -1 where
+ * the reference writes 0.  Adjacency arrives as a CSR (row_ptr, col) whose
+ * rows are sorted ascending, which is the reference's fadjlist/badjlist
+ * invariant (src/SimpleGraphs/simplegraph.jl:445-461,
+ * src/SimpleGraphs/simpledigraph.jl:414-430).
+ *
+ * Parity pinning (what checks each function; the tests live in
+ * tests/test_oracle.py):
+ *   orc_gdistances       reference vectors test/traversals/bfs.jl:31-33,
+ *                        test/parallel/traversals/gdistances.jl:7-17; networkx
+ *                        single_source_shortest_path_length on random graphs.
+ *   orc_bfs_parents      test/traversals/bfs.jl:12 ([1,1,1,3]); brute-force
+ *                        min-predecessor closed form on random graphs.
+ *   orc_par_gdistances   equals orc_gdistances with typemax prefill (the
+ *                        reference's own test asserts that equality).
+ *   orc_par_bfs_tree     test/parallel/traversals/bfs.jl:26 ([1,1,1,3]); tree
+ *                        validity (depth = BFS distance) on random graphs.
+ *   orc_dijkstra_allpaths test/shortestpaths/dijkstra.jl:81-97 (pathcounts,
+ *                        predecessors, closest_vertices); sigma vs. brute-force
+ *                        path enumeration on tiny graphs.
+ *   orc_betweenness      fixture test/testdata/graph-50-500-bc.txt; endpoints
+ *                        values test/centrality/betweenness.jl:16-17; self-loop
+ *                        and doctest cases; networkx golden vectors; brute
+ *                        force over all shortest paths on tiny graphs.
+ *   orc_betweenness_threaded  equals orc_betweenness when vs = 1:n (the
+ *                        reference test's only use); index-not-vertex quirk
+ *                        pinned by a hand-derived case.
+ */
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <math.h>
+#include <pthread.h>
+
+/* ------------------------------------------------------------------ */
+/* helpers                                                              */
+/* ------------------------------------------------------------------ */
+
+static int cmp_i32(const void* a, const void* b) {
+    int32_t x = *(const int32_t*)a, y = *(const int32_t*)b;
+    return (x > y) - (x < y);
+}
+
+/* degree(g, v): out-degree for undirected graphs (fadjlist length, a
+ * self-loop counted once as stored), in+out for directed graphs
+ * (src/core.jl:124-128, trait dispatch on IsDirected). */
+static int64_t degree_of(int32_t v, const int32_t* row_ptr, const int32_t* in_row_ptr,
+                         int directed) {
+    int64_t d = (int64_t)row_ptr[v + 1] - row_ptr[v];
+    if (directed) d += (int64_t)in_row_ptr[v + 1] - in_row_ptr[v];
+    return d;
+}
+
+/* ------------------------------------------------------------------ */
+/* gdistances! -- src/traversals/bfs.jl:95-124                          */
+/* ------------------------------------------------------------------ */
+/* Writes level numbers only for reached vertices; vert_level must be
+ * pre-filled by the caller (bfs.jl:141 fills typemax).  Frontier is sorted
+ * after each level (bfs.jl:121); that affects no output. */
+int orc_gdistances(int32_t nv, const int32_t* row_ptr, const int32_t* col,
+                   const int32_t* srcs, int32_t nsrc, int32_t* vert_level) {
+    uint8_t* visited = (uint8_t*)calloc((size_t)nv + 1, 1);
+    int32_t* cur = (int32_t*)malloc(sizeof(int32_t) * ((size_t)nv + (size_t)nsrc + 1));
+    int32_t* nxt = (int32_t*)malloc(sizeof(int32_t) * ((size_t)nv + 1));
+    if (!visited || !cur || !nxt) { free(visited); free(cur); free(nxt); return -1; }
+    int64_t ncur = 0, nnxt = 0;
+    for (int32_t i = 0; i < nsrc; i++) {            /* bfs.jl:103-107 */
+        int32_t s = srcs[i];
+        if (s < 0 || s >= nv) { free(visited); free(cur); free(nxt); return -2; }
+        vert_level[s] = 0;
+        visited[s] = 1;
+        cur[ncur++] = s;
+    }
+    int32_t n_level = 1;
+    while (ncur > 0) {                               /* bfs.jl:108 */
+        for (int64_t k = 0; k < ncur; k++) {
+            int32_t v = cur[k];
+            for (int32_t e = row_ptr[v]; e < row_ptr[v + 1]; e++) {
+                int32_t i = col[e];
+                if (!visited[i]) {                   /* bfs.jl:111-115 */
+                    nxt[nnxt++] = i;
+                    vert_level[i] = n_level;
+                    visited[i] = 1;
+                }
+            }
+        }
+        n_level++;
+        int32_t* t = cur; cur = nxt; nxt = t;
+        ncur = nnxt; nnxt = 0;
+        qsort(cur, (size_t)ncur, sizeof(int32_t), cmp_i32);   /* bfs.jl:121 */
+    }
+    free(visited); free(cur); free(nxt);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* _bfs_parents -- src/traversals/bfs.jl:38-66                          */
+/* ------------------------------------------------------------------ */
+/* (row_ptr, col) is the CSR of neighborfn: out-neighbours for dir=:out,
+ * in-neighbours otherwise (bfs.jl:35-36).  parent[src] = src, unreached
+ * = -1 (the reference's 0 in 1-based ids). */
+int orc_bfs_parents(int32_t nv, const int32_t* row_ptr, const int32_t* col,
+                    int32_t src, int32_t* parents) {
+    if (src < 0 || src >= nv) return -2;
+    uint8_t* visited = (uint8_t*)calloc((size_t)nv, 1);
+    int32_t* cur = (int32_t*)malloc(sizeof(int32_t) * ((size_t)nv + 1));
+    int32_t* nxt = (int32_t*)malloc(sizeof(int32_t) * ((size_t)nv + 1));
+    if (!visited || !cur || !nxt) { free(visited); free(cur); free(nxt); return -1; }
+    for (int32_t i = 0; i < nv; i++) parents[i] = -1;  /* zeros(T, nv) */
+    int64_t ncur = 0, nnxt = 0;
+    visited[src] = 1; cur[ncur++] = src; parents[src] = src;   /* bfs.jl:46-50 */
+    while (ncur > 0) {
+        for (int64_t k = 0; k < ncur; k++) {
+            int32_t v = cur[k];
+            for (int32_t e = row_ptr[v]; e < row_ptr[v + 1]; e++) {
+                int32_t i = col[e];
+                if (!visited[i]) {                   /* bfs.jl:54-58 */
+                    nxt[nnxt++] = i;
+                    parents[i] = v;
+                    visited[i] = 1;
+                }
+            }
+        }
+        int32_t* t = cur; cur = nxt; nxt = t;
+        ncur = nnxt; nnxt = 0;
+        qsort(cur, (size_t)ncur, sizeof(int32_t), cmp_i32);   /* bfs.jl:63 */
+    }
+    free(visited); free(cur); free(nxt);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Parallel.gdistances! -- src/Parallel/traversals/gdistances.jl:36-116  */
+/* ------------------------------------------------------------------ */
+/* Level-synchronous sweep with typemax pre-fill (:46).  The threaded
+ * segment claiming (:71-101) only changes which thread expands a vertex;
+ * the level of every vertex is unique, so a serial sweep over the
+ * partitioned queues gives the same vert_level.  Unreached = -1. */
+int orc_par_gdistances(int32_t nv, const int32_t* row_ptr, const int32_t* col,
+                       const int32_t* srcs, int32_t nsrc, int32_t* vert_level) {
+    for (int32_t i = 0; i < nv; i++) vert_level[i] = -1;   /* fill!(typemax) */
+    uint8_t* visited = (uint8_t*)calloc((size_t)nv + 1, 1);
+    int32_t* cur = (int32_t*)malloc(sizeof(int32_t) * ((size_t)nv + (size_t)nsrc + 1));
+    int32_t* nxt = (int32_t*)malloc(sizeof(int32_t) * ((size_t)nv + 1));
+    if (!visited || !cur || !nxt) { free(visited); free(cur); free(nxt); return -1; }
+    int64_t ncur = 0, nnxt = 0;
+    for (int32_t i = 0; i < nsrc; i++) {            /* :55-58 */
+        int32_t s = srcs[i];
+        if (s < 0 || s >= nv) { free(visited); free(cur); free(nxt); return -2; }
+        visited[s] = 1; vert_level[s] = 0; cur[ncur++] = s;
+    }
+    int32_t n_level = 0;
+    while (ncur > 0) {
+        n_level++;
+        for (int64_t k = 0; k < ncur; k++) {
+            int32_t v = cur[k];
+            if (!(visited[v] && vert_level[v] == n_level - 1)) continue;  /* :88 */
+            for (int32_t e = row_ptr[v]; e < row_ptr[v + 1]; e++) {
+                int32_t i = col[e];
+                if (!visited[i]) { vert_level[i] = n_level; visited[i] = 1; nxt[nnxt++] = i; }
+            }
+        }
+        int32_t* t = cur; cur = nxt; nxt = t;
+        ncur = nnxt; nnxt = 0;
+    }
+    free(visited); free(cur); free(nxt);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Parallel.bfs_tree! -- src/Parallel/traversals/bfs.jl:49-92           */
+/* ------------------------------------------------------------------ */
+/* Serial restatement of the frontier loop: bfskernel visits `level` in
+ * queue order and atomic_cas!(parents[v], 0, src) first-wins (:59).  The
+ * threaded reference is non-deterministic; this serial order is one of its
+ * legal outcomes.  (row_ptr, col) = neighbors(g, .) (:56), i.e.
+ * out-neighbours.  Unset parents stay -1 (the reference's 0). */
+int orc_par_bfs_tree(int32_t nv, const int32_t* row_ptr, const int32_t* col,
+                     int32_t source, int32_t* parents) {
+    if (source < 0 || source >= nv) return -2;
+    int32_t* q = (int32_t*)malloc(sizeof(int32_t) * ((size_t)nv + 1));
+    if (!q) return -1;
+    for (int32_t i = 0; i < nv; i++) parents[i] = -1;
+    int64_t head = 0, tail = 0;
+    parents[source] = source;                        /* :84 */
+    q[tail++] = source;                              /* :85 */
+    while (head != tail) {                           /* :86 */
+        int64_t lo = head, hi = tail;                /* :87-88 */
+        head = tail;
+        for (int64_t k = lo; k < hi; k++) {          /* bfskernel :55-64 */
+            int32_t src = q[k];
+            for (int32_t e = row_ptr[src]; e < row_ptr[src + 1]; e++) {
+                int32_t v = col[e];
+                if (parents[v] == -1) { parents[v] = src; q[tail++] = v; }
+            }
+        }
+    }
+    free(q);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Binary-heap priority queue with a key index.                         */
+/* ------------------------------------------------------------------ */
+/* Restates the published algorithm of DataStructures.jl's PriorityQueue
+ * (compat "0.17, 0.18", /root/reference Project.toml:19; used at
+ * src/shortestpaths/dijkstra.jl:73,80,87,106,116): a 1-based binary
+ * min-heap of (key, priority) pairs, percolate-up on insert, root swap with
+ * the last element and percolate-down on dequeue, comparisons on priority
+ * only with strict less-than.  The heap decides only the dequeue order of
+ * equal-distance vertices, which changes no distance or path count. */
+typedef struct { int32_t key; int64_t pri; } pq_item;
+typedef struct { pq_item* xs; int64_t n; int32_t* index; } pq_t;
+
+static void pq_up(pq_t* pq, int64_t i) {
+    pq_item x = pq->xs[i];
+    while (i > 1) {
+        int64_t j = i >> 1;
+        pq_item xj = pq->xs[j];
+        if (x.pri < xj.pri) { pq->index[xj.key] = (int32_t)i; pq->xs[i] = xj; i = j; }
+        else break;
+    }
+    pq->index[x.key] = (int32_t)i;
+    pq->xs[i] = x;
+}
+
+static void pq_down(pq_t* pq, int64_t i) {
+    pq_item x = pq->xs[i];
+    int64_t l;
+    while ((l = 2 * i) <= pq->n) {
+        int64_t r = l + 1;
+        int64_t j = (r > pq->n || pq->xs[l].pri < pq->xs[r].pri) ? l : r;
+        if (pq->xs[j].pri < x.pri) { pq->index[pq->xs[j].key] = (int32_t)i; pq->xs[i] = pq->xs[j]; i = j; }
+        else break;
+    }
+    pq->index[x.key] = (int32_t)i;
+    pq->xs[i] = x;
+}
+
+static void pq_set(pq_t* pq, int32_t key, int64_t pri) {
+    if (pq->index[key] > 0) {                        /* existing key */
+        int64_t i = pq->index[key];
+        int64_t old = pq->xs[i].pri;
+        pq->xs[i].pri = pri;
+        if (old < pri) pq_down(pq, i); else pq_up(pq, i);
+    } else {
+        pq->n++;
+        pq->xs[pq->n].key = key; pq->xs[pq->n].pri = pri;
+        pq->index[key] = (int32_t)pq->n;
+        pq_up(pq, pq->n);
+    }
+}
+
+static int32_t pq_pop(pq_t* pq) {
+    pq_item x = pq->xs[1];
+    pq_item y = pq->xs[pq->n];
+    pq->n--;
+    if (pq->n > 0) {
+        pq->xs[1] = y;
+        pq->index[y.key] = 1;
+        pq_down(pq, 1);
+    }
+    pq->index[x.key] = 0;
+    return x.key;
+}
+
+/* ------------------------------------------------------------------ */
+/* dijkstra_shortest_paths(g, s, DefaultDistance; allpaths, trackvertices)
+ * -- src/shortestpaths/dijkstra.jl:59-141                               */
+/* ------------------------------------------------------------------ */
+/* Workspace reused across sources.  preds[v] is stored in the slot range
+ * of v's in-neighbour row (in_row_ptr): every predecessor is an
+ * in-neighbour, so the row bounds the count. */
+typedef struct {
+    int32_t nv;
+    int64_t* dists;      /* -1 = typemax */
+    uint8_t* visited;
+    double* pathcounts;
+    int32_t* pred_cnt;
+    int32_t* preds;      /* nnz(in) slots */
+    int32_t* closest;    /* closest_vertices, length nv */
+    int32_t nclosest;
+    pq_t pq;
+    double* delta;
+} dijk_ws;
+
+static int dijk_ws_init(dijk_ws* w, int32_t nv, int64_t in_nnz) {
+    memset(w, 0, sizeof(*w));
+    w->nv = nv;
+    w->dists = (int64_t*)malloc(sizeof(int64_t) * ((size_t)nv + 1));
+    w->visited = (uint8_t*)malloc((size_t)nv + 1);
+    w->pathcounts = (double*)malloc(sizeof(double) * ((size_t)nv + 1));
+    w->pred_cnt = (int32_t*)malloc(sizeof(int32_t) * ((size_t)nv + 1));
+    w->preds = (int32_t*)malloc(sizeof(int32_t) * ((size_t)in_nnz + 1));
+    w->closest = (int32_t*)malloc(sizeof(int32_t) * ((size_t)nv + 1));
+    w->pq.xs = (pq_item*)malloc(sizeof(pq_item) * ((size_t)nv + 2));
+    w->pq.index = (int32_t*)calloc((size_t)nv + 1, sizeof(int32_t));
+    w->delta = (double*)malloc(sizeof(double) * ((size_t)nv + 1));
+    if (!w->dists || !w->visited || !w->pathcounts || !w->pred_cnt || !w->preds ||
+        !w->closest || !w->pq.xs || !w->pq.index || !w->delta) return -1;
+    return 0;
+}
+
+static void dijk_ws_free(dijk_ws* w) {
+    free(w->dists); free(w->visited); free(w->pathcounts); free(w->pred_cnt);
+    free(w->preds); free(w->closest); free(w->pq.xs); free(w->pq.index); free(w->delta);
+}
+
+static void dijkstra_allpaths(dijk_ws* w, const int32_t* row_ptr, const int32_t* col,
+                              const int32_t* in_row_ptr, int32_t s) {
+    int32_t nv = w->nv;
+    for (int32_t i = 0; i < nv; i++) {               /* :67-72 */
+        w->dists[i] = -1; w->visited[i] = 0; w->pathcounts[i] = 0.0; w->pred_cnt[i] = 0;
+    }
+    w->pq.n = 0;
+    w->dists[s] = 0; w->visited[s] = 1; w->pathcounts[s] = 1.0;   /* :76-81 */
+    pq_set(&w->pq, s, 0);
+    w->nclosest = 0;
+    while (w->pq.n > 0) {                            /* :86 */
+        int32_t u = pq_pop(&w->pq);
+        w->closest[w->nclosest++] = u;               /* :89-91 */
+        int64_t d = w->dists[u];
+        for (int32_t e = row_ptr[u]; e < row_ptr[u + 1]; e++) {
+            int32_t v = col[e];
+            int64_t alt = d + 1;                     /* DefaultDistance == 1 */
+            if (!w->visited[v]) {                    /* :97-106 */
+                w->visited[v] = 1;
+                w->dists[v] = alt;
+                w->pathcounts[v] += w->pathcounts[u];
+                w->preds[in_row_ptr[v]] = u; w->pred_cnt[v] = 1;
+                pq_set(&w->pq, v, alt);
+            } else if (alt < w->dists[v]) {          /* :107-116 */
+                w->dists[v] = alt;
+                w->pathcounts[v] = w->pathcounts[u];
+                w->preds[in_row_ptr[v]] = u; w->pred_cnt[v] = 1;
+                pq_set(&w->pq, v, alt);
+            } else if (alt == w->dists[v]) {         /* :117-121 */
+                w->pathcounts[v] += w->pathcounts[u];
+                w->preds[in_row_ptr[v] + w->pred_cnt[v]] = u; w->pred_cnt[v]++;
+            }
+        }
+    }
+    for (int32_t v = 0; v < nv; v++)                 /* :126-132 */
+        if (!w->visited[v]) w->closest[w->nclosest++] = v;
+    w->pathcounts[s] = 1.0;                          /* :134-138 */
+    w->pred_cnt[s] = 0;
+}
+
+/* Exported form for tests (pins pathcounts / predecessors / closest order
+ * against test/shortestpaths/dijkstra.jl:81-97).  preds_out uses the
+ * in_row_ptr slots; pred_cnt_out gives the count per vertex. */
+int orc_dijkstra_allpaths(int32_t nv, const int32_t* row_ptr, const int32_t* col,
+                          const int32_t* in_row_ptr, int32_t s,
+                          int64_t* dists, double* pathcounts, int32_t* pred_cnt,
+                          int32_t* preds, int32_t* closest) {
+    if (s < 0 || s >= nv) return -2;
+    dijk_ws w;
+    if (dijk_ws_init(&w, nv, in_row_ptr[nv]) != 0) { dijk_ws_free(&w); return -1; }
+    dijkstra_allpaths(&w, row_ptr, col, in_row_ptr, s);
+    memcpy(dists, w.dists, sizeof(int64_t) * (size_t)nv);
+    memcpy(pathcounts, w.pathcounts, sizeof(double) * (size_t)nv);
+    memcpy(pred_cnt, w.pred_cnt, sizeof(int32_t) * (size_t)nv);
+    memcpy(preds, w.preds, sizeof(int32_t) * (size_t)in_row_ptr[nv]);
+    memcpy(closest, w.closest, sizeof(int32_t) * (size_t)nv);
+    dijk_ws_free(&w);
+    return 0;
+}
+
+/* _accumulate_basic! -- src/centrality/betweenness.jl:81-107 */
+static void accumulate_basic(double* bc, dijk_ws* w, const int32_t* in_row_ptr, int32_t si) {
+    int32_t nv = w->nv;
+    for (int32_t i = 0; i < nv; i++) w->delta[i] = 0.0;
+    for (int32_t k = w->nclosest - 1; k >= 0; k--) {   /* S = reverse(closest) */
+        int32_t x = w->closest[k];
+        double coeff = (1.0 + w->delta[x]) / w->pathcounts[x];
+        const int32_t* P = w->preds + in_row_ptr[x];
+        int32_t np = (x == si) ? 0 : w->pred_cnt[x];     /* P[si] = [] */
+        for (int32_t j = 0; j < np; j++) {
+            int32_t v = P[j];
+            w->delta[v] += (w->pathcounts[v] * coeff);
+        }
+        if (x != si) bc[x] += w->delta[x];
+    }
+}
+
+/* _accumulate_endpoints! -- src/centrality/betweenness.jl:109-134 */
+static void accumulate_endpoints(double* bc, dijk_ws* w, const int32_t* in_row_ptr, int32_t si) {
+    int32_t nv = w->nv;
+    for (int32_t i = 0; i < nv; i++) w->delta[i] = 0.0;
+    bc[si] += (double)(w->nclosest - 1);              /* length(S) - 1, :122 */
+    for (int32_t k = w->nclosest - 1; k >= 0; k--) {
+        int32_t x = w->closest[k];
+        double coeff = (1.0 + w->delta[x]) / w->pathcounts[x];
+        const int32_t* P = w->preds + in_row_ptr[x];
+        int32_t np = w->pred_cnt[x];
+        for (int32_t j = 0; j < np; j++) {
+            int32_t v = P[j];
+            w->delta[v] += w->pathcounts[v] * coeff;
+        }
+        if (x != si) bc[x] += (w->delta[x] + 1.0);
+    }
+}
+
+/* _rescale! -- src/centrality/betweenness.jl:136-160 */
+void orc_rescale(double* bc, int32_t n, int normalize, int directed, int64_t k) {
+    int do_scale = 0;
+    double scale = 0.0;
+    if (normalize) {
+        if (n <= 2) do_scale = 0;
+        else { do_scale = 1; scale = 1.0 / (double)(((int64_t)n - 1) * ((int64_t)n - 2)); }
+    } else {
+        if (!directed) { do_scale = 1; scale = 1.0 / 2.0; }
+        else do_scale = 0;
+    }
+    if (do_scale) {
+        if (k > 0) scale = scale * (double)n / (double)k;
+        for (int32_t i = 0; i < n; i++) bc[i] *= scale;
+    }
+}
+
+/* betweenness_centrality(g, vs, DefaultDistance; normalize, endpoints)
+ * -- src/centrality/betweenness.jl:46-75.  in_row_ptr is the in-neighbour
+ * CSR (equal to row_ptr for undirected graphs): it provides in-degree for
+ * degree() and the predecessor slots. */
+int orc_betweenness(int32_t nv, const int32_t* row_ptr, const int32_t* col,
+                    const int32_t* in_row_ptr, int directed,
+                    const int32_t* vs, int32_t nvs, int normalize, int endpoints,
+                    double* bc) {
+    for (int32_t i = 0; i < nv; i++) bc[i] = 0.0;
+    for (int32_t i = 0; i < nvs; i++) if (vs[i] < 0 || vs[i] >= nv) return -2;
+    dijk_ws w;
+    if (dijk_ws_init(&w, nv, in_row_ptr[nv]) != 0) { dijk_ws_free(&w); return -1; }
+    for (int32_t i = 0; i < nvs; i++) {
+        int32_t s = vs[i];
+        if (degree_of(s, row_ptr, in_row_ptr, directed) > 0) {    /* :58 */
+            dijkstra_allpaths(&w, row_ptr, col, in_row_ptr, s);
+            if (endpoints) accumulate_endpoints(bc, &w, in_row_ptr, s);
+            else accumulate_basic(bc, &w, in_row_ptr, s);
+        }
+    }
+    dijk_ws_free(&w);
+    orc_rescale(bc, nv, normalize, directed, nvs);   /* k = length(vs) */
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* threaded_betweenness_centrality -- Parallel/centrality/betweenness.jl:42-75
+ * ------------------------------------------------------------------ */
+/* vs_active = findall(x -> degree(g, x) > 0, vs) returns positions in vs
+ * (:53); the loop then uses those positions as source vertices (:55-56).
+ * The restatement keeps that: source = position (0-based) of every entry of
+ * vs whose vertex has degree > 0.  Per-thread accumulators are reduced in
+ * thread order (:63).  With `faithful_index_quirk` = 0 the sources are the
+ * vertices themselves (the :distributed variant, :9-37, and the serial
+ * function's semantics), which is what the CPU baseline times. */
+typedef struct {
+    int32_t nv; const int32_t* row_ptr; const int32_t* col; const int32_t* in_row_ptr;
+    const int32_t* srcs; int64_t nsrc; int endpoints; double* acc;
+    int64_t* next; pthread_mutex_t* mu; int rc;
+} bc_job;
+
+static void* bc_worker(void* arg) {
+    bc_job* j = (bc_job*)arg;
+    dijk_ws w;
+    if (dijk_ws_init(&w, j->nv, j->in_row_ptr[j->nv]) != 0) { dijk_ws_free(&w); j->rc = -1; return NULL; }
+    for (;;) {
+        int64_t i;
+        pthread_mutex_lock(j->mu);
+        i = (*j->next)++;
+        pthread_mutex_unlock(j->mu);
+        if (i >= j->nsrc) break;
+        int32_t s = j->srcs[i];
+        dijkstra_allpaths(&w, j->row_ptr, j->col, j->in_row_ptr, s);
+        if (j->endpoints) accumulate_endpoints(j->acc, &w, j->in_row_ptr, s);
+        else accumulate_basic(j->acc, &w, j->in_row_ptr, s);
+    }
+    dijk_ws_free(&w);
+    j->rc = 0;
+    return NULL;
+}
+
+int orc_betweenness_threaded(int32_t nv, const int32_t* row_ptr, const int32_t* col,
+                             const int32_t* in_row_ptr, int directed,
+                             const int32_t* vs, int32_t nvs, int normalize, int endpoints,
+                             int nthreads, int faithful_index_quirk, double* bc) {
+    if (nthreads < 1) nthreads = 1;
+    for (int32_t i = 0; i < nvs; i++) if (vs[i] < 0 || vs[i] >= nv) return -2;
+    int32_t* act = (int32_t*)malloc(sizeof(int32_t) * ((size_t)nvs + 1));
+    double* accs = (double*)calloc((size_t)nthreads * (size_t)nv + 1, sizeof(double));
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)nthreads);
+    bc_job* jobs = (bc_job*)malloc(sizeof(bc_job) * (size_t)nthreads);
+    if (!act || !accs || !th || !jobs) { free(act); free(accs); free(th); free(jobs); return -1; }
+    int64_t nact = 0;
+    for (int32_t i = 0; i < nvs; i++) {
+        if (degree_of(vs[i], row_ptr, in_row_ptr, directed) > 0) {
+            int32_t s = faithful_index_quirk ? i : vs[i];
+            if (s >= nv) { free(act); free(accs); free(th); free(jobs); return -2; }
+            act[nact++] = s;
+        }
+    }
+    pthread_mutex_t mu;
+    pthread_mutex_init(&mu, NULL);
+    int64_t next = 0;
+    for (int t = 0; t < nthreads; t++) {
+        jobs[t].nv = nv; jobs[t].row_ptr = row_ptr; jobs[t].col = col; jobs[t].in_row_ptr = in_row_ptr;
+        jobs[t].srcs = act; jobs[t].nsrc = nact; jobs[t].endpoints = endpoints;
+        jobs[t].acc = accs + (size_t)t * (size_t)nv; jobs[t].next = &next; jobs[t].mu = &mu; jobs[t].rc = 0;
+        pthread_create(&th[t], NULL, bc_worker, &jobs[t]);
+    }
+    int rc = 0;
+    for (int t = 0; t < nthreads; t++) { pthread_join(th[t], NULL); if (jobs[t].rc) rc = jobs[t].rc; }
+    pthread_mutex_destroy(&mu);
+    for (int32_t v = 0; v < nv; v++) {               /* reduce(+, local_betweenness) */
+        double s = 0.0;
+        for (int t = 0; t < nthreads; t++) s += accs[(size_t)t * (size_t)nv + v];
+        bc[v] = s;
+    }
+    free(act); free(accs); free(th); free(jobs);
+    if (rc) return rc;
+    orc_rescale(bc, nv, normalize, directed, nvs);
+    return 0;
+}
+
+/* Serial single-source BFS used as the CPU-baseline leg for
+ * single-source BFS: distances and parents in one pass, restating
+ * gdistances! (bfs.jl:95-124) with the parent write of _bfs_parents
+ * (bfs.jl:54-58).  Returns the number of edges scanned (for TEPS). */
+int64_t orc_bfs_dist_parent(int32_t nv, const int32_t* row_ptr, const int32_t* col,
+                            int32_t src, int32_t* dist, int32_t* parents) {
+    if (src < 0 || src >= nv) return -2;
+    uint8_t* visited = (uint8_t*)calloc((size_t)nv, 1);
+    int32_t* cur = (int32_t*)malloc(sizeof(int32_t) * ((size_t)nv + 1));
+    int32_t* nxt = (int32_t*)malloc(sizeof(int32_t) * ((size_t)nv + 1));
+    if (!visited || !cur || !nxt) { free(visited); free(cur); free(nxt); return -1; }
+    for (int32_t i = 0; i < nv; i++) { dist[i] = -1; parents[i] = -1; }
+    int64_t ncur = 0, nnxt = 0, scanned = 0;
+    visited[src] = 1; cur[ncur++] = src; parents[src] = src; dist[src] = 0;
+    int32_t lvl = 1;
+    while (ncur > 0) {
+        for (int64_t k = 0; k < ncur; k++) {
+            int32_t v = cur[k];
+            scanned += row_ptr[v + 1] - row_ptr[v];
+            for (int32_t e = row_ptr[v]; e < row_ptr[v + 1]; e++) {
+                int32_t i = col[e];
+                if (!visited[i]) { nxt[nnxt++] = i; parents[i] = v; dist[i] = lvl; visited[i] = 1; }
+            }
+        }
+        lvl++;
+        int32_t* t = cur; cur = nxt; nxt = t;
+        ncur = nnxt; nnxt = 0;
+        qsort(cur, (size_t)ncur, sizeof(int32_t), cmp_i32);
+    }
+    free(visited); free(cur); free(nxt);
+    return scanned;
+}

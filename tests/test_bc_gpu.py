@@ -1,0 +1,187 @@
+"""GPU parity of betweenness_centrality (serial and Parallel) against the
+reference fixture, networkx vectors and the oracle.
+
+Bar: within 1e-12 relative error of the reference's FP64 values (the only
+difference is floating-point summation order); exact where the value is a
+small dyadic/rational the reference tests compare with ==.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_NX, load_golden
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def close(got, want, rtol=RTOL):
+    want = np.asarray(want, dtype=np.float64)
+    scale = np.maximum(np.abs(want), 1e-300)
+    err = np.abs(got - want)
+    ok = (err <= rtol * scale) | (err <= 1e-15 * max(1.0, float(np.abs(want).max(initial=0.0))))
+    return bool(ok.all())
+
+
+def test_reference_fixture_graph_50_500():
+    import lgmi
+    z = load_golden("ref_graph_50_500.npz")
+    n = int(z["n"])
+    for T in (np.int64, np.uint8, np.int16):
+        say(f"graph-50-500 T={np.dtype(T).name}")
+        g = lgmi.SimpleDiGraph.from_edges(n, z["src"], z["dst"], T=T)
+        bc = lgmi.betweenness_centrality(g)
+        # test/centrality/betweenness.jl:13 (Float32 equality) and FP64 1e-12
+        assert (bc.astype(np.float32) == z["bc"].astype(np.float32)).all()
+        assert close(bc, z["bc"])
+        y = lgmi.betweenness_centrality(g, endpoints=True, normalize=False)
+        ref = [122.10760591498584, 159.0072453120582, 176.39547945994505]
+        assert np.round(y[:3], 4).tolist() == np.round(ref, 4).tolist()
+        assert close(y[:3], ref)
+        x = lgmi.betweenness_centrality(g, 3, rng=1)
+        x2 = lgmi.betweenness_centrality(g, list(range(1, 21)))
+        assert len(x) == 50 and len(x2) == 50
+        # Parallel, both modes (test/parallel/centrality/betweenness.jl:10-30)
+        P = lgmi.Parallel
+        assert close(P.betweenness_centrality(g, parallel="threads"), bc)
+        assert close(P.betweenness_centrality(g, parallel="distributed"), bc)
+        assert close(P.betweenness_centrality(g, endpoints=True, normalize=False, parallel="threads"), y)
+        assert len(P.betweenness_centrality(g, 3, parallel="threads", rng=2)) == 50
+        assert len(P.betweenness_centrality(g, list(range(1, 21)), parallel="distributed")) == 50
+
+
+def test_small_reference_cases():
+    import lgmi
+    gen = lgmi.generators
+    s2 = lgmi.SimpleDiGraph(3)
+    s2.add_edge(1, 2); s2.add_edge(2, 3); s2.add_edge(3, 3)
+    s1 = lgmi.SimpleGraph.from_digraph(s2)
+    # test/centrality/betweenness.jl:20-21
+    assert lgmi.betweenness_centrality(s1).tolist() == [0, 1, 0]
+    assert lgmi.betweenness_centrality(s2).tolist() == [0, 0.5, 0]
+    for mode in ("threads", "distributed"):
+        assert lgmi.Parallel.betweenness_centrality(s1, parallel=mode).tolist() == [0, 1, 0]
+        assert lgmi.Parallel.betweenness_centrality(s2, parallel=mode).tolist() == [0, 0.5, 0]
+    # :23-29
+    k2 = gen.path_graph(2)
+    z = lgmi.betweenness_centrality(k2, normalize=True)
+    assert z[0] == z[1] == 0.0
+    assert (lgmi.betweenness_centrality(k2, [1, 2]) == z).all()
+    # :32-33
+    z = lgmi.betweenness_centrality(gen.path_graph(5), normalize=False)
+    assert z[0] == z[4] == 0.0 and z.tolist() == [0.0, 3.0, 4.0, 3.0, 0.0]
+    # doctests betweenness.jl:32-43
+    assert lgmi.betweenness_centrality(gen.star_graph(3)).tolist() == [1.0, 0.0, 0.0]
+    assert close(lgmi.betweenness_centrality(gen.path_graph(4)), [0.0, 2 / 3, 2 / 3, 0.0])
+    # endpoints=true on a disconnected graph: hand-derived closed form
+    g = lgmi.SimpleGraph.from_edges(4, [0, 2], [1, 3])
+    assert lgmi.betweenness_centrality(g, normalize=False, endpoints=True).tolist() == [3.0] * 4
+
+
+def test_grid_50_bound(oracle):
+    import lgmi
+    g = lgmi.generators.grid([50, 50])
+    z = lgmi.betweenness_centrality(g, normalize=False)
+    assert z.max() < 2500 * 2499                  # test/centrality/betweenness.jl:75-77
+    rp, col = g.csr()
+    want = oracle.betweenness_threaded(rp, col, rp, False, np.arange(2500), normalize=False, nthreads=8,
+                                       index_quirk=False)
+    assert close(z, want)
+
+
+@pytest.mark.parametrize("name", GOLDEN_NX)
+def test_networkx_golden(name):
+    import lgmi
+    z = load_golden(name)
+    n, directed = int(z["n"]), bool(z["directed"])
+    say(f"golden {name}")
+    cls = lgmi.SimpleDiGraph if directed else lgmi.SimpleGraph
+    g = cls.from_edges(n, z["src"], z["dst"])
+    assert close(lgmi.betweenness_centrality(g), z["bc_norm"])
+    assert close(lgmi.betweenness_centrality(g, normalize=False), z["bc_raw"])
+    assert close(lgmi.betweenness_centrality(g, z["sub_src"] + 1), z["bc_sub_norm"])
+    if "bc_end_raw" in z:
+        assert close(lgmi.betweenness_centrality(g, normalize=False, endpoints=True), z["bc_end_raw"])
+
+
+def test_sources_quirks(oracle):
+    """Degree-0 and duplicate sources, k = length(vs), the threaded
+    findall-position quirk, and more than one 64-lane group."""
+    import lgmi
+    g = lgmi.generators.rmat(10, 4, seed=3)
+    rp, col = g.csr()
+    n = g.nv()
+    deg = np.diff(rp)
+    iso = np.nonzero(deg == 0)[0]
+    rng = np.random.default_rng(4)
+    vs0 = np.concatenate([rng.integers(0, n, 150), iso[:5], [7, 7, 7]])
+    rng.shuffle(vs0)
+    for endpoints in (False, True):
+        for normalize in (True, False):
+            say(f"quirks endpoints={endpoints} normalize={normalize}")
+            got = lgmi.betweenness_centrality(g, vs0 + 1, normalize=normalize, endpoints=endpoints)
+            want = oracle.betweenness(rp, col, rp, False, vs0, normalize=normalize, endpoints=endpoints)
+            assert close(got, want)
+            got_t = lgmi.Parallel.betweenness_centrality(g, vs0 + 1, normalize=normalize, endpoints=endpoints,
+                                                         parallel="threads")
+            want_t = oracle.betweenness_threaded(rp, col, rp, False, vs0, normalize=normalize,
+                                                 endpoints=endpoints, nthreads=4, index_quirk=True)
+            assert close(got_t, want_t)
+
+
+def test_er10k_all_sources(oracle):
+    """Config: ER(10_000, 50_000, seed=1) all-source BC, normalize=true,
+    endpoints=false; and gdistances from vertex 1."""
+    import lgmi
+    g = lgmi.generators.erdos_renyi(10_000, 50_000, seed=1)
+    rp, col = g.csr()
+    say("er10k bc")
+    bc = lgmi.betweenness_centrality(g)
+    want = oracle.betweenness_threaded(rp, col, rp, False, np.arange(10_000), nthreads=16, index_quirk=False)
+    assert close(bc, want)
+    d = lgmi.gdistances(g, 1)
+    dref = oracle.gdistances(rp, col, [0])
+    assert (np.where(dref < 0, lgmi.typemax(np.int64), dref) == d).all()
+
+
+def test_directed_rmat_subset(oracle):
+    import lgmi
+    g = lgmi.generators.rmat(12, 8, seed=5, directed=True)
+    rp, col = g.csr()
+    irp, _ = g.in_csr()
+    rng = np.random.default_rng(8)
+    vs0 = rng.integers(0, g.nv(), 200)
+    got = lgmi.betweenness_centrality(g, vs0 + 1)
+    want = oracle.betweenness(rp, col, irp, True, vs0)
+    assert close(got, want)
+    got = lgmi.betweenness_centrality(g, vs0 + 1, normalize=False, endpoints=True)
+    want = oracle.betweenness(rp, col, irp, True, vs0, normalize=False, endpoints=True)
+    assert close(got, want)
+
+
+def test_bc_deterministic():
+    import lgmi
+    g = lgmi.generators.barabasi_albert(3000, 4, seed=6)
+    a = lgmi.betweenness_centrality(g)
+    b = lgmi.betweenness_centrality(g)
+    assert (a.view(np.uint64) == b.view(np.uint64)).all()
+
+
+def test_rmat16_sources_vs_oracle(oracle):
+    import lgmi
+    g = lgmi.generators.rmat(16, 16, seed=7)
+    rp, col = g.csr()
+    deg = np.diff(rp)
+    rng = np.random.default_rng(9)
+    vs0 = rng.choice(np.nonzero(deg > 0)[0], size=256, replace=False)
+    say("rmat16 256 sources")
+    got = lgmi.betweenness_centrality(g, vs0 + 1)
+    want = oracle.betweenness_threaded(rp, col, rp, False, vs0, nthreads=16, index_quirk=False)
+    assert close(got, want)

@@ -1,0 +1,209 @@
+"""GPU parity of gdistances / bfs_parents / Parallel BFS against the oracle.
+
+Bar: bit-exact distances and parents.  Every case runs through liblgmi.so on
+the device; the oracle (oracle/liblgoracle.so) is only the checker.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_NX, load_golden
+
+pytestmark = pytest.mark.gpu
+
+T_LIST = [np.int64, np.uint8, np.int16]    # testgraphs triplication, test/runtests.jl:14
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _g5(T=np.int64):
+    import lgmi
+    g = lgmi.SimpleDiGraph(4, T=T)
+    for u, v in [(1, 2), (2, 3), (1, 3), (3, 4)]:
+        g.add_edge(u, v)
+    return g
+
+
+@pytest.mark.parametrize("T", T_LIST)
+def test_reference_vectors(T):
+    import lgmi
+    say(f"reference vectors T={np.dtype(T).name}")
+    house = lgmi.generators.house_graph(T=T)
+    # test/traversals/bfs.jl:31-33
+    assert lgmi.gdistances(house, 2).tolist() == [1, 0, 2, 1, 2]
+    assert lgmi.gdistances(house, 2).dtype == np.dtype(T)
+    assert lgmi.gdistances(house, [1, 2]).tolist() == [0, 0, 1, 1, 2]
+    assert lgmi.gdistances(house, []).tolist() == [lgmi.typemax(T)] * 5
+    # test/traversals/bfs.jl:9-13
+    g5 = _g5(T)
+    assert lgmi.bfs_parents(g5, 1).tolist() == [1, 1, 1, 3]
+    z = lgmi.bfs_tree(g5, 1)
+    assert z.nv() == 4 and z.ne() == 3 and not z.has_edge(2, 3)
+    # :16-26 house tree properties
+    p = lgmi.bfs_parents(house, 1)
+    t1 = lgmi.bfs_tree(house, 1)
+    t2 = lgmi.tree(p)
+    assert len(p) == 5 and t1.csr()[1].tolist() == t2.csr()[1].tolist()
+    assert t2.is_directed() and t2.ne() < t2.nv()
+    assert p.tolist() == [1, 1, 1, 2, 3]
+    # Parallel (test/parallel/traversals/gdistances.jl:7-17, bfs.jl:19-28)
+    P = lgmi.Parallel
+    assert P.gdistances(g5, 1).tolist() == lgmi.gdistances(g5, 1).tolist()
+    assert P.gdistances(g5, [1, 3]).tolist() == lgmi.gdistances(g5, [1, 3]).tolist()
+    assert P.gdistances(house, 2).tolist() == lgmi.gdistances(house, 2).tolist()
+    assert P.gdistances(house, [1, 2]).tolist() == lgmi.gdistances(house, [1, 2]).tolist()
+    parents = np.zeros(4, dtype=T)
+    q = P.ThreadQueue(T, 4)
+    P.bfs_tree_(q, g5, 1, parents)
+    assert parents.tolist() == [1, 1, 1, 3]
+    zt = P.bfs_tree(g5, 1)
+    assert zt.nv() == 4 and zt.ne() == 3 and not zt.has_edge(2, 3)
+
+
+def test_gdistances_inplace_semantics():
+    import lgmi
+    g = lgmi.SimpleGraph.from_edges(5, [0, 3], [1, 4])
+    buf = np.full(5, 99, dtype=np.int64)
+    lgmi.gdistances_(g, 1, buf)               # serial: writes reached entries only
+    assert buf.tolist() == [0, 1, 99, 99, 99]
+    lgmi.Parallel.gdistances_(g, [1], buf)    # Parallel: pre-fills typemax
+    assert buf.tolist() == [0, 1] + [lgmi.typemax(np.int64)] * 3
+
+
+def test_bounds_errors():
+    import lgmi
+    g = lgmi.generators.path_graph(4)
+    with pytest.raises(IndexError):
+        lgmi.gdistances(g, 5)
+    with pytest.raises(IndexError):
+        lgmi.bfs_parents(g, 0)
+    with pytest.raises(IndexError):
+        lgmi.gdistances(g, [1, 9])
+
+
+def _check_graph(oracle, g, roots, tag):
+    rp, col = g.csr()
+    irp, icol = g.in_csr()
+    for r in roots:
+        say(f"{tag}: root {r}")
+        d_or = oracle.gdistances(rp, col, [r])
+        d_gpu = lgmi_dist0(g, r)
+        assert (d_gpu == d_or).all(), f"{tag} dist root {r}"
+        p_or = oracle.bfs_parents(rp, col, r)
+        p_gpu = lgmi_par0(g, r, 0)
+        assert (p_gpu == p_or).all(), f"{tag} parents root {r}"
+        if g.is_directed():
+            p_or_in = oracle.bfs_parents(irp, icol, r)
+            assert (lgmi_par0(g, r, 1) == p_or_in).all(), f"{tag} parents(:in) root {r}"
+
+
+def lgmi_dist0(g, r):
+    from lgmi import traversals
+    return traversals._device_dist(g, np.array([r], dtype=np.int32))
+
+
+def lgmi_par0(g, r, dir_in):
+    from lgmi import traversals
+    return traversals._parents0(g, r + 1, dir_in)
+
+
+@pytest.mark.parametrize("name", GOLDEN_NX)
+def test_golden_graphs(oracle, name):
+    import lgmi
+    z = load_golden(name)
+    n, directed = int(z["n"]), bool(z["directed"])
+    cls = lgmi.SimpleDiGraph if directed else lgmi.SimpleGraph
+    g = cls.from_edges(n, z["src"], z["dst"])
+    for s, dref in zip(z["bfs_src"], z["dist"]):
+        assert (lgmi_dist0(g, int(s)) == dref).all()
+    _check_graph(oracle, g, [int(x) for x in z["bfs_src"]], name)
+
+
+def test_generated_graphs(oracle):
+    import lgmi.generators as gen
+    cases = [
+        ("er10k", gen.erdos_renyi(10_000, 50_000, seed=1)),
+        ("rmat14", gen.rmat(14, 16, seed=2)),
+        ("rmat12dir", gen.rmat(12, 16, seed=3, directed=True)),
+        ("ba5000", gen.barabasi_albert(5000, 4, seed=4)),
+        ("ws5000", gen.watts_strogatz(5000, 8, 0.1, seed=5)),
+        ("grid64", gen.grid([64, 64])),
+        ("path3000", gen.path_graph(3000)),
+        ("star1000", gen.star_graph(1000)),
+    ]
+    rng = np.random.default_rng(0)
+    for tag, g in cases:
+        deg = np.diff(g.csr()[0])
+        roots = [0] + rng.choice(np.nonzero(deg > 0)[0], size=3).tolist() + [int(np.argmax(deg))]
+        _check_graph(oracle, g, roots, tag)
+
+
+def test_multisource_and_isolated(oracle):
+    import lgmi
+    import lgmi.generators as gen
+    g = gen.rmat(13, 8, seed=9)
+    rp, col = g.csr()
+    deg = np.diff(rp)
+    iso = int(np.nonzero(deg == 0)[0][0])
+    for srcs in ([iso], [0, 5, 77, 5], list(range(0, 8192, 97))):
+        say(f"multisource {len(srcs)}")
+        want = oracle.gdistances(rp, col, srcs)
+        got = lgmi.gdistances(g, [s + 1 for s in srcs])
+        tmax = lgmi.typemax(np.int64)
+        assert (np.where(want < 0, tmax, want) == got).all()
+    assert (lgmi_par0(g, iso, 0) == np.where(np.arange(g.nv()) == iso, iso, -1)).all()
+
+
+def test_direction_never_changes_output(oracle):
+    """Forced top-down, forced bottom-up, the alpha/beta heuristic and the
+    reference's top-down-only policy give bit-identical outputs."""
+    from lgmi import _lib
+    import lgmi.generators as gen
+    lib = _lib.load()
+    for tag, g in (("rmat15", gen.rmat(15, 16, seed=21)), ("rmat12dir", gen.rmat(12, 16, seed=22, directed=True)),
+                   ("grid40", gen.grid([40, 40]))):
+        rp, col = g.csr()
+        h = g.handle()
+        root = int(np.argmax(np.diff(rp)))
+        outs = []
+        for forced in (-1, 0, 1):
+            say(f"{tag}: forced direction {forced}")
+            _lib.check(lib.lgmi_bfs_force_direction(h, forced))
+            d = np.empty(g.nv(), np.int32)
+            p = np.empty(g.nv(), np.int32)
+            _lib.check(lib.lgmi_bfs(h, root, 0, _lib.i32p(d), _lib.i32p(p)))
+            outs.append((d, p))
+        _lib.check(lib.lgmi_bfs_force_direction(h, -1))
+        _lib.check(lib.lgmi_bfs_set_policy(_lib.LGMI_POLICY_TOPDOWN, 15.0, 18.0, None, 0))
+        d = np.empty(g.nv(), np.int32)
+        p = np.empty(g.nv(), np.int32)
+        _lib.check(lib.lgmi_bfs(h, root, 0, _lib.i32p(d), _lib.i32p(p)))
+        _lib.check(lib.lgmi_bfs_set_policy(_lib.LGMI_POLICY_HEURISTIC, 15.0, 18.0, None, 0))
+        outs.append((d, p))
+        for d, p in outs[1:]:
+            assert (d == outs[0][0]).all() and (p == outs[0][1]).all()
+        assert (outs[0][0] == oracle.gdistances(rp, col, [root])).all()
+        assert (outs[0][1] == oracle.bfs_parents(rp, col, root)).all()
+
+
+def test_repeat_bit_identical():
+    import lgmi.generators as gen
+    g = gen.rmat(16, 16, seed=31)
+    root = int(np.argmax(np.diff(g.csr()[0])))
+    a = [lgmi_par0(g, root, 0) for _ in range(4)]
+    for x in a[1:]:
+        assert (x == a[0]).all()
+
+
+def test_rmat20_full_compare(oracle):
+    import lgmi.generators as gen
+    say("building rmat20")
+    g = gen.rmat(20, 16, seed=1)
+    rp, col = g.csr()
+    rng = np.random.default_rng(5)
+    roots = rng.choice(np.nonzero(np.diff(rp) > 0)[0], size=3).tolist()
+    _check_graph(oracle, g, roots, "rmat20")

@@ -1,0 +1,117 @@
+"""CPU tests of liblgmi.so's boundary: it loads, exports every symbol the
+headers declare, validates its inputs on the host, and its host-only
+generators / CSR builder behave.  No GPU compute happens here."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import REPO
+
+
+def _declared():
+    names = set()
+    for h in ("lgmi.h", "lgmi_gen.h"):
+        txt = open(os.path.join(REPO, "include", h)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        names |= set(re.findall(r"\b(lgmi_[a-z0-9_]+)\s*\(", txt))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    from lgmi import _lib
+    lib = _lib.load()
+    declared = _declared()
+    assert declared == set(_lib.EXPORTED)
+    for name in declared:
+        assert hasattr(lib, name), name
+
+
+def test_device_count_and_nodev_error():
+    from lgmi import _lib
+    lib = _lib.load()
+    n = lib.lgmi_device_count()
+    assert n >= 0
+    if n == 0:
+        rp = np.array([0, 1, 2], dtype=np.int32)
+        col = np.array([1, 0], dtype=np.int32)
+        h = _lib.ctypes.c_void_p()
+        rc = lib.lgmi_graph_create(2, 2, _lib.i32p(rp), _lib.i32p(col), 0, None, None, 0, _lib.ctypes.byref(h))
+        assert rc == _lib.LGMI_E_NODEV
+        assert "GPU" in _lib.last_error()
+
+
+@pytest.mark.parametrize("rp,col,code", [
+    ([0, 2, 2], [1, 0], -1),        # unsorted row
+    ([0, 1, 2], [5, 0], -2),        # neighbour out of range
+    ([0, 2, 3], [1, 1, 0], -1),     # duplicate in row
+    ([0, 1, 1], [1, 0], -1),        # row_ptr[nv] != nnz
+])
+def test_create_rejects_bad_csr_on_host(rp, col, code):
+    from lgmi import _lib
+    lib = _lib.load()
+    rp = np.array(rp, dtype=np.int32)
+    col = np.array(col, dtype=np.int32)
+    h = _lib.ctypes.c_void_p()
+    rc = lib.lgmi_graph_create(2, len(col), _lib.i32p(rp), _lib.i32p(col), 0, None, None, 0, _lib.ctypes.byref(h))
+    assert rc == code
+    with pytest.raises(IndexError if code == -2 else ValueError):
+        _lib.check(rc)
+
+
+def test_too_large_code():
+    from lgmi import _lib
+    lib = _lib.load()
+    rp = np.zeros(2, dtype=np.int32)
+    h = _lib.ctypes.c_void_p()
+    rc = lib.lgmi_graph_create(1, 1 << 31, _lib.i32p(rp), _lib.i32p(rp), 0, None, None, 0, _lib.ctypes.byref(h))
+    assert rc == _lib.LGMI_E_TOO_LARGE
+
+
+def test_policy_argument_checks():
+    from lgmi import _lib
+    lib = _lib.load()
+    assert lib.lgmi_bfs_set_policy(7, 15.0, 18.0, None, 0) == _lib.LGMI_E_ARG
+    assert lib.lgmi_bfs_set_policy(0, -1.0, 18.0, None, 0) == _lib.LGMI_E_ARG
+    w = np.zeros(10, dtype=np.float32)
+    assert lib.lgmi_bfs_set_policy(1, 15.0, 18.0, _lib.f32p(w), 10) == _lib.LGMI_E_ARG
+    assert lib.lgmi_bfs_set_policy(0, 15.0, 18.0, None, 0) == _lib.LGMI_OK
+
+
+def test_generators_shapes_and_determinism():
+    import lgmi.generators as gen
+    s1, d1 = gen.kronecker_edges(10, 16, seed=3)
+    s2, d2 = gen.kronecker_edges(10, 16, seed=3)
+    s3, _ = gen.kronecker_edges(10, 16, seed=4)
+    assert len(s1) == 16 << 10 and (s1 == s2).all() and (d1 == d2).all() and not (s1 == s3).all()
+    assert s1.min() >= 0 and s1.max() < 1024
+    # RMAT skew: the a-quadrant bias concentrates edges on few vertices
+    deg = np.bincount(np.concatenate([s1, d1]), minlength=1024)
+    assert deg.max() > 20 * deg.mean()
+    g = gen.erdos_renyi(1000, 5000, seed=1)
+    assert g.ne() == 5000 and not g.is_directed()
+    g = gen.barabasi_albert(500, 4, seed=2)
+    assert g.ne() == 4 + (500 - 5) * 4
+    g = gen.watts_strogatz(600, 8, 0.1, seed=3)
+    assert g.ne() == 600 * 4
+    assert (g.degree_array() >= 1).all()
+    g = gen.grid([7, 9])
+    assert g.nv() == 63 and g.ne() == 2 * 63 - 7 - 9
+    # grid ids: (i, j) -> i*r + j (operators.jl:536); 0-based vertex 1 neighbours 0, 2, 1+7
+    assert (g.outneighbors(2) - 1).tolist() == [0, 2, 8]
+
+
+def test_csr_build_sorted_dedup_selfloops():
+    from lgmi.graphs import csr_from_edges
+    src = np.array([3, 0, 0, 2, 2, 1, 1])
+    dst = np.array([0, 3, 3, 2, 1, 2, 0])
+    rp, col = csr_from_edges(4, src, dst, symmetrize=True, drop_self_loops=False)
+    rows = [col[rp[v]:rp[v + 1]].tolist() for v in range(4)]
+    assert rows == [[1, 3], [0, 2], [1, 2], [0]]
+    rp, col = csr_from_edges(4, src, dst, symmetrize=True, drop_self_loops=True)
+    rows = [col[rp[v]:rp[v + 1]].tolist() for v in range(4)]
+    assert rows == [[1, 3], [0, 2], [1], [0]]
+    rp, col = csr_from_edges(4, src, dst, symmetrize=False)
+    rows = [col[rp[v]:rp[v + 1]].tolist() for v in range(4)]
+    assert rows == [[3], [0, 2], [1, 2], [0]]

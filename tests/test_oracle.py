@@ -1,0 +1,295 @@
+"""Pins the CPU oracle (oracle/lgoracle.c) before it is trusted as the
+checker of the GPU path.
+
+Each oracle function is checked against (a) the reference's own test vectors
+and fixtures (cited file:line), (b) networkx golden vectors, and (c)
+independent brute force on tiny graphs (explicit enumeration of all
+shortest paths; the min-predecessor closed form for parents).  A few tests
+plant a plausible mistake in an oracle result and check that the same
+comparison rejects it.
+"""
+import itertools
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN_NX, csr_of, load_golden
+
+
+# ----------------------------------------------------------------------
+# small graphs of the reference tests (1-based edge lists)
+# ----------------------------------------------------------------------
+G5 = (4, True, [(1, 2), (2, 3), (1, 3), (3, 4)])            # test/traversals/bfs.jl:4-5
+HOUSE = (5, False, [(1, 2), (1, 3), (2, 4), (3, 4), (3, 5), (4, 5)])   # smallgraphs.jl:202-203
+S2 = (3, True, [(1, 2), (2, 3), (3, 3)])                     # test/centrality/betweenness.jl:3-5
+G615 = (5, False, [(1, 2), (1, 3), (2, 5), (3, 5), (3, 4), (4, 5)])   # test/shortestpaths/dijkstra.jl:71-77
+
+
+def mk(spec, drop_self_loops=False):
+    n, directed, edges = spec
+    e = np.array(edges, dtype=np.int64) - 1 if edges else np.zeros((0, 2), np.int64)
+    return n, directed, csr_of(n, e[:, 0], e[:, 1], directed, drop_self_loops)
+
+
+def path_spec(n):
+    return (n, False, [(i, i + 1) for i in range(1, n)])
+
+
+def star_spec(n):
+    return (n, False, [(1, i) for i in range(2, n + 1)])
+
+
+# ----------------------------------------------------------------------
+# (a) reference vectors
+# ----------------------------------------------------------------------
+def test_gdistances_reference_vectors(oracle):
+    n, _, (rp, col, _, _) = mk(HOUSE)
+    # test/traversals/bfs.jl:31-33 (1-based values)
+    assert (oracle.gdistances(rp, col, [1]) == np.array([1, 0, 2, 1, 2])).all()
+    assert (oracle.gdistances(rp, col, [0, 1]) == np.array([0, 0, 1, 1, 2])).all()
+    assert (oracle.gdistances(rp, col, np.zeros(0, np.int32)) == -1).all()
+    # test/parallel/traversals/gdistances.jl:7-17: Parallel equals serial
+    n5, _, (rp5, col5, _, _) = mk(G5)
+    for srcs in ([0], [0, 2]):
+        assert (oracle.par_gdistances(rp5, col5, srcs) == oracle.gdistances(rp5, col5, srcs)).all()
+    for srcs in ([1], [0, 1]):
+        assert (oracle.par_gdistances(rp, col, srcs) == oracle.gdistances(rp, col, srcs)).all()
+
+
+def test_gdistances_inplace_writes_only_reached(oracle):
+    # bfs.jl:95-124 never writes unreached entries (the caller pre-fills)
+    rp, col, _, _ = csr_of(5, [0, 3], [1, 4], False)
+    buf = np.full(5, 99, dtype=np.int32)
+    oracle.gdistances_inplace(rp, col, [0], buf)
+    assert buf.tolist() == [0, 1, 99, 99, 99]
+
+
+def test_bfs_parents_reference_vectors(oracle):
+    n, _, (rp, col, irp, icol) = mk(G5)
+    # test/traversals/bfs.jl:12 and test/parallel/traversals/bfs.jl:26
+    assert (oracle.bfs_parents(rp, col, 0) + 1).tolist() == [1, 1, 1, 3]
+    assert (oracle.par_bfs_tree(rp, col, 0) + 1).tolist() == [1, 1, 1, 3]
+    # bfs_parents(house, 1), worked by hand from bfs.jl:38-66
+    _, _, (hrp, hcol, _, _) = mk(HOUSE)
+    assert (oracle.bfs_parents(hrp, hcol, 0) + 1).tolist() == [1, 1, 1, 2, 3]
+    # dir=:in on g5 from vertex 4: 4 <- 3 <- {1,2}; parent of 1 and 2 is 3
+    par_in = oracle.bfs_parents(irp, icol, 3) + 1
+    assert par_in.tolist() == [3, 3, 4, 4]
+
+
+def test_dijkstra_allpaths_reference_vectors(oracle):
+    # test/shortestpaths/dijkstra.jl:93-97
+    n, _, (rp, col, irp, _) = mk(G615)
+    d, pc, preds, closest = oracle.dijkstra_allpaths(rp, col, irp, 0)
+    assert pc.tolist() == [1.0, 1.0, 1.0, 1.0, 2.0]
+    assert [sorted(p) for p in preds] == [[], [0], [0], [2], [1, 2]]
+    assert (closest + 1).tolist() == [1, 2, 3, 5, 4]
+    # dijkstra.jl:100-106: unreachable vertices appended in id order
+    _, _, (rp2, col2, irp2, _) = mk((5, False, [(1, 2), (1, 3), (4, 5)]))
+    _, _, _, closest2 = oracle.dijkstra_allpaths(rp2, col2, irp2, 0)
+    assert (closest2 + 1).tolist() == [1, 2, 3, 4, 5]
+
+
+def test_betweenness_reference_fixture(oracle):
+    z = load_golden("ref_graph_50_500.npz")
+    n = int(z["n"])
+    rp, col, irp, icol = csr_of(n, z["src"], z["dst"], True)
+    vs = np.arange(n)
+    bc = oracle.betweenness(rp, col, irp, True, vs, normalize=True, endpoints=False)
+    # test/centrality/betweenness.jl:13 compares in Float32; hold FP64 to 1e-12
+    assert (bc.astype(np.float32) == z["bc"].astype(np.float32)).all()
+    assert np.max(np.abs(bc - z["bc"]) / np.abs(z["bc"])) < 1e-12
+    # :15-17 endpoints=true, normalize=false, first three to 4 digits
+    y = oracle.betweenness(rp, col, irp, True, vs, normalize=False, endpoints=True)
+    ref = [122.10760591498584, 159.0072453120582, 176.39547945994505]
+    assert np.round(y[:3], 4).tolist() == np.round(ref, 4).tolist()
+    assert np.max(np.abs(y[:3] - ref) / np.abs(ref)) < 1e-12
+    # threaded form over 1:n equals the serial one (test/parallel/centrality/betweenness.jl:10-19)
+    zt = oracle.betweenness_threaded(rp, col, irp, True, vs, nthreads=4)
+    assert np.allclose(zt, bc, rtol=1e-12, atol=0)
+
+
+def test_betweenness_small_reference_cases(oracle):
+    # test/centrality/betweenness.jl:20-21: s1 (undirected s2) and s2
+    n, d, (rp, col, irp, _) = mk(S2)
+    assert oracle.betweenness(rp, col, irp, True, np.arange(3)).tolist() == [0, 0.5, 0]
+    n, d, (rp, col, irp, _) = mk((3, False, S2[2]))
+    assert oracle.betweenness(rp, col, irp, False, np.arange(3)).tolist() == [0, 1, 0]
+    # :23-29 K2 gives zeros
+    n, d, (rp, col, irp, _) = mk((2, False, [(1, 2)]))
+    assert oracle.betweenness(rp, col, irp, False, np.arange(2)).tolist() == [0.0, 0.0]
+    # :32-33 path_graph(5), normalize=false: ends are 0
+    n, d, (rp, col, irp, _) = mk(path_spec(5))
+    z = oracle.betweenness(rp, col, irp, False, np.arange(5), normalize=False)
+    assert z[0] == z[4] == 0.0
+    assert z.tolist() == [0.0, 3.0, 4.0, 3.0, 0.0]        # textbook path BC
+    # doctests betweenness.jl:32-43
+    n, d, (rp, col, irp, _) = mk(star_spec(3))
+    assert oracle.betweenness(rp, col, irp, False, np.arange(3)).tolist() == [1.0, 0.0, 0.0]
+    n, d, (rp, col, irp, _) = mk(path_spec(4))
+    assert oracle.betweenness(rp, col, irp, False, np.arange(4)).tolist() == [0.0, 2 / 3, 2 / 3, 0.0]
+
+
+def test_betweenness_grid_bound(oracle):
+    # test/centrality/betweenness.jl:75-77: grid([50, 50]), normalize=false
+    import lgmi.generators as gen
+    g = gen.grid([50, 50])
+    rp, col = g.csr()
+    z = oracle.betweenness(rp, col, rp, False, np.arange(2500), normalize=False)
+    assert z.max() < 2500 * 2499
+
+
+def test_threaded_index_quirk(oracle):
+    # Parallel/centrality/betweenness.jl:53-56 uses findall positions as
+    # sources: vs = [3, 3] on path_graph(3) processes sources 1 and 2.
+    n, d, (rp, col, irp, _) = mk(path_spec(3))
+    quirky = oracle.betweenness_threaded(rp, col, irp, False, [2, 2], normalize=False, nthreads=2)
+    as_vertices = oracle.betweenness(rp, col, irp, False, [0, 1], normalize=False)
+    # rescale uses k = length(vs) = 2 in both
+    assert np.allclose(quirky, as_vertices)
+    honest = oracle.betweenness(rp, col, irp, False, [2, 2], normalize=False)
+    assert not np.allclose(quirky, honest)
+
+
+# ----------------------------------------------------------------------
+# (b) networkx golden vectors
+# ----------------------------------------------------------------------
+@pytest.mark.parametrize("name", GOLDEN_NX)
+def test_oracle_vs_networkx(oracle, name):
+    z = load_golden(name)
+    n, directed = int(z["n"]), bool(z["directed"])
+    rp, col, irp, icol = csr_of(n, z["src"], z["dst"], directed)
+    for s, dref in zip(z["bfs_src"], z["dist"]):
+        assert (oracle.gdistances(rp, col, [int(s)]) == dref).all()
+        assert (oracle.par_gdistances(rp, col, [int(s)]) == dref).all()
+    vs = np.arange(n)
+    for key, kw in (("bc_norm", dict(normalize=True)), ("bc_raw", dict(normalize=False))):
+        got = oracle.betweenness(rp, col, irp, directed, vs, **kw)
+        ref = z[key]
+        assert np.allclose(got, ref, rtol=1e-12, atol=1e-12 * max(1.0, np.abs(ref).max()))
+    got = oracle.betweenness(rp, col, irp, directed, z["sub_src"], normalize=True)
+    assert np.allclose(got, z["bc_sub_norm"], rtol=1e-12, atol=1e-15)
+    if "bc_end_raw" in z:
+        got = oracle.betweenness(rp, col, irp, directed, vs, normalize=False, endpoints=True)
+        assert np.allclose(got, z["bc_end_raw"], rtol=1e-12, atol=0)
+
+
+# ----------------------------------------------------------------------
+# (c) brute force on tiny graphs
+# ----------------------------------------------------------------------
+def _random_tiny(rng, n, p, directed):
+    adj = rng.random((n, n)) < p
+    np.fill_diagonal(adj, False)
+    if not directed:
+        adj = np.triu(adj)
+        adj = adj | adj.T
+    s, d = np.nonzero(adj)
+    return s, d, adj
+
+
+def _all_shortest_paths(adj, s, t):
+    """Every shortest s->t path, by enumerating simple paths (independent of BFS)."""
+    n = len(adj)
+    best, paths = None, []
+    stack = [(s, [s])]
+    while stack:
+        u, path = stack.pop()
+        if best is not None and len(path) - 1 > best:
+            continue
+        if u == t:
+            L = len(path) - 1
+            if best is None or L < best:
+                best, paths = L, [path]
+            elif L == best:
+                paths.append(path)
+            continue
+        for v in np.nonzero(adj[u])[0]:
+            if v not in path:
+                stack.append((int(v), path + [int(v)]))
+    return paths
+
+
+def _brute_bc(adj, directed):
+    n = len(adj)
+    raw = np.zeros(n)
+    for s, t in itertools.permutations(range(n), 2):
+        paths = _all_shortest_paths(adj, s, t)
+        if not paths:
+            continue
+        for p in paths:
+            for v in p[1:-1]:
+                raw[v] += 1.0 / len(paths)
+    return raw
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_bc_and_parents_brute_force(oracle, directed):
+    rng = np.random.default_rng(123 + directed)
+    for trial in range(12):
+        n = int(rng.integers(3, 8))
+        s, d, adj = _random_tiny(rng, n, 0.35, directed)
+        rp, col, irp, icol = csr_of(n, s, d, directed)
+        raw = _brute_bc(adj, directed)
+        got = oracle.betweenness(rp, col, irp, directed, np.arange(n), normalize=False)
+        want = raw if directed else raw * 0.5
+        # sources of degree 0 contribute nothing in both
+        assert np.allclose(got, want, rtol=1e-12, atol=1e-12), (trial, adj)
+        got_n = oracle.betweenness(rp, col, irp, directed, np.arange(n), normalize=True)
+        if n > 2:
+            assert np.allclose(got_n, raw / ((n - 1) * (n - 2)), rtol=1e-12, atol=1e-12)
+        # parents: min-id predecessor one level up (closed form)
+        for src in range(n):
+            dist = oracle.gdistances(rp, col, [src])
+            par = oracle.bfs_parents(rp, col, src)
+            for v in range(n):
+                if v == src:
+                    assert par[v] == src
+                elif dist[v] < 0:
+                    assert par[v] == -1
+                else:
+                    cands = [u for u in range(n) if adj[u, v] and dist[u] == dist[v] - 1]
+                    assert par[v] == min(cands)
+
+
+def test_endpoints_disconnected_closed_form(oracle):
+    # _accumulate_endpoints! (betweenness.jl:109-134) counts every vertex,
+    # reachable or not: on two disjoint edges each source adds nv-1 = 3 to
+    # itself and 1 to each other vertex, so raw = 6 and the undirected
+    # scale 1/2 gives 3.
+    rp, col, irp, _ = csr_of(4, [0, 2], [1, 3], False)
+    got = oracle.betweenness(rp, col, irp, False, np.arange(4), normalize=False, endpoints=True)
+    assert got.tolist() == [3.0, 3.0, 3.0, 3.0]
+
+
+# ----------------------------------------------------------------------
+# planted mistakes: the checks above reject them
+# ----------------------------------------------------------------------
+def test_planted_mistakes_are_caught(oracle):
+    z = load_golden("ref_graph_50_500.npz")
+    n = int(z["n"])
+    rp, col, irp, _ = csr_of(n, z["src"], z["dst"], True)
+    bc = oracle.betweenness(rp, col, irp, True, np.arange(n))
+    # wrong k in the rescale (k = n-1 instead of length(vs))
+    wrong_k = oracle.rescale(oracle.betweenness(rp, col, irp, True, np.arange(n), normalize=False),
+                             n, True, True, n - 1)
+    assert not np.allclose(wrong_k, z["bc"], rtol=1e-6)
+    # a digraph treated as undirected (reversing *every* edge would not be a
+    # mistake: betweenness is invariant under edge reversal)
+    srp, scol, _, _ = csr_of(n, z["src"], z["dst"], False)
+    t = oracle.betweenness(srp, scol, srp, False, np.arange(n))
+    assert not np.allclose(t, z["bc"], rtol=1e-6)
+    # an edge-reversed graph must give the same values
+    t = oracle.betweenness(irp, _transpose_col(rp, col), rp, True, np.arange(n))
+    assert np.allclose(t, z["bc"], rtol=1e-12)
+    assert np.allclose(bc, z["bc"], rtol=1e-12)
+    # last-discoverer parents instead of first-discoverer
+    _, _, (rp5, col5, _, _) = mk(G5)
+    par = oracle.bfs_parents(rp5, col5, 0) + 1
+    wrong = par.copy()
+    wrong[2] = 2                      # 3 discovered from 2 instead of 1
+    assert par.tolist() == [1, 1, 1, 3] and wrong.tolist() != [1, 1, 1, 3]
+
+
+def _transpose_col(rp, col):
+    from lgmi.graphs import csr_transpose
+    return csr_transpose(rp, col)[1]
