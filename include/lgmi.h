@@ -154,13 +154,16 @@ int lgmi_bfs_set_policy(int mode, float alpha, float beta, const float* mlp, int
  * Parallel.betweenness_centrality(...; parallel = :distributed | :threads)
  *   src/Parallel/centrality/betweenness.jl:1-75 (the shim passes the
  *   source list each mode actually uses)
- * Exact Brandes over srcs[0..nsrc) in FP64.  Sources with degree(g, s) == 0
- * are skipped as the reference does (betweenness.jl:58); duplicates count
- * twice.  k_scale is the k of _rescale! (the reference's length(vs)).
- * Results match the reference within 1e-12 relative error (only the
- * floating-point summation order differs). */
+ * Exact Brandes over srcs[0..nsrc) in FP64; duplicates count twice.
+ * skip_degree0 = 1 skips sources with degree(g, s) == 0 as the serial and
+ * :distributed functions do (betweenness.jl:58, Parallel/centrality/
+ * betweenness.jl:23); 0 runs every listed source, as :threads does with the
+ * positions findall returns (Parallel/centrality/betweenness.jl:53-56).
+ * k_scale is the k of _rescale! (the reference's length(vs)).  Results
+ * match the reference within 1e-12 relative error (only the floating-point
+ * summation order differs). */
 int lgmi_betweenness(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int64_t k_scale,
-                     int normalize, int endpoints, double* bc);
+                     int skip_degree0, int normalize, int endpoints, double* bc);
 /* Statistics of the last betweenness run on g. */
 int lgmi_betweenness_stats(lgmi_graph_t g, lgmi_bc_stats* stats);
 

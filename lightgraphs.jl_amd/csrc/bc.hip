@@ -380,8 +380,8 @@ static void bc_grow_levels(lgmi_graph* g, int32_t new_cap, int32_t keep) {
     w->cap_levels = new_cap;
 }
 
-void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, int normalize, int endpoints,
-            double* bc, lgmi_bc_stats* stats) {
+void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, int skip_degree0, int normalize,
+            int endpoints, double* bc, lgmi_bc_stats* stats) {
     const int32_t nv = g->nv;
     hipStream_t st = g->stream;
     // degree(g, s) > 0 filter (betweenness.jl:58) on host copies of the row pointers
@@ -399,7 +399,7 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
     std::vector<int32_t> act;
     act.reserve((size_t)nsrc);
     for (int32_t i = 0; i < nsrc; i++)
-        if (degree(srcs[i]) > 0) act.push_back(srcs[i]);
+        if (!skip_degree0 || degree(srcs[i]) > 0) act.push_back(srcs[i]);
     const int64_t nact = (int64_t)act.size();
     const int32_t ngroups = (int32_t)((nact + 63) / 64);
 

@@ -277,13 +277,13 @@ int lgmi_bfs_set_policy(int mode, float alpha, float beta, const float* mlp, int
     LGMI_API_END
 }
 
-int lgmi_betweenness(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, int normalize,
-                     int endpoints, double* bc) {
+int lgmi_betweenness(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, int skip_degree0,
+                     int normalize, int endpoints, double* bc) {
     LGMI_API_BEGIN
     if (!g || !bc || nsrc < 0 || (nsrc > 0 && !srcs) || k_scale < 0) throw Error(LGMI_E_ARG, "bad arguments");
     std::lock_guard<std::mutex> lk(g->mu);
     for (int32_t i = 0; i < nsrc; i++) check_src(g, srcs[i]);
-    bc_run(g, srcs, nsrc, k_scale, normalize, endpoints, bc, &g->bc_last);
+    bc_run(g, srcs, nsrc, k_scale, skip_degree0, normalize, endpoints, bc, &g->bc_last);
     LGMI_API_END
 }
 
@@ -307,7 +307,7 @@ int lgmi_betweenness_batch(int32_t ngraphs, const int32_t* nv, const int64_t* ro
         for (int32_t v = 0; v < n; v++) vs[(size_t)v] = v;
         try {
             lgmi_bc_stats st{};
-            bc_run(g, vs.data(), n, n, normalize, endpoints, bc_cat + bc_off[i], &st);
+            bc_run(g, vs.data(), n, n, 1, normalize, endpoints, bc_cat + bc_off[i], &st);
             total_ms += st.device_ms;
         } catch (...) {
             lgmi_graph_destroy(g);

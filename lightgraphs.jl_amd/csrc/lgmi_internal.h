@@ -161,8 +161,8 @@ void bfs_fetch(lgmi_graph* g, int32_t* dist, int32_t* parent);
 void bfs_free(BfsWork* w);
 void policy_upload(lgmi_graph* g);
 // bc.hip
-void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, int normalize,
-            int endpoints, double* bc, lgmi_bc_stats* stats);
+void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, int skip_degree0,
+            int normalize, int endpoints, double* bc, lgmi_bc_stats* stats);
 void bc_free(BcWork* w);
 // global policy state (host copy), set by lgmi_bfs_set_policy
 struct PolicyHost {

@@ -1,0 +1,202 @@
+# LGMI.jl -- LightGraphs.jl drop-in for gdistances / bfs_parents /
+# betweenness_centrality (serial and Parallel) on an MI355X through
+# liblgmi.so (C ABI: include/lgmi.h).
+#
+# Loading this module adds methods that are more specific than the
+# reference's (same names, same arguments, same results) for
+# SimpleGraph{T}/SimpleDiGraph{T} with the default unit distances; every
+# other case (weighted distmx, other graph types, more than 2^31-1 adjacency
+# entries, no GPU) falls through to the reference method.
+#
+# This file is not executed in this repository's test suite (no Julia
+# toolchain is installed); the Python package lgmi/ mirrors it call for call
+# and is what the tests drive.
+module LGMI
+
+using LightGraphs
+using LightGraphs: AbstractGraph, SimpleGraph, SimpleDiGraph, DefaultDistance, nv, ne,
+    is_directed, vertices, degree, sample
+import LightGraphs: gdistances, gdistances!, bfs_parents, bfs_tree, betweenness_centrality
+import LightGraphs.Parallel
+
+const liblgmi = joinpath(@__DIR__, "..", "lgmi", "liblgmi.so")
+
+const LGMI_OK = Cint(0)
+const LGMI_E_ARG = Cint(-1)
+const LGMI_E_BOUNDS = Cint(-2)
+const LGMI_E_TOO_LARGE = Cint(-3)
+const LGMI_E_NODEV = Cint(-4)
+
+const SimpleG{T} = Union{SimpleGraph{T},SimpleDiGraph{T}}
+
+struct Fallback <: Exception end
+
+lasterror() = unsafe_string(ccall((:lgmi_last_error, liblgmi), Cstring, ()))
+
+function check(rc::Cint)
+    rc == LGMI_OK && return nothing
+    (rc == LGMI_E_TOO_LARGE || rc == LGMI_E_NODEV) && throw(Fallback())
+    rc == LGMI_E_BOUNDS && throw(BoundsError(lasterror()))
+    rc == LGMI_E_ARG && throw(ArgumentError(lasterror()))
+    error("liblgmi: ", lasterror())
+end
+
+# fadjlist / badjlist -> 0-based Int32 CSR (rows are already sorted,
+# src/SimpleGraphs/simplegraph.jl:445-461)
+function csr(adj::Vector{Vector{T}}) where T
+    n = length(adj)
+    rp = Vector{Int32}(undef, n + 1)
+    rp[1] = 0
+    @inbounds for v in 1:n
+        rp[v+1] = rp[v] + Int32(length(adj[v]))
+    end
+    col = Vector{Int32}(undef, rp[end])
+    @inbounds for v in 1:n, (k, u) in enumerate(adj[v])
+        col[rp[v]+k] = Int32(u - 1)
+    end
+    return rp, col
+end
+
+mutable struct Handle
+    ptr::Ptr{Cvoid}
+end
+
+function Handle(g::SimpleG)
+    length(g.fadjlist) > 0 && sum(length, g.fadjlist) >= typemax(Int32) && throw(Fallback())
+    rp, col = csr(g.fadjlist)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    if is_directed(g)
+        irp, icol = csr(g.badjlist)
+        rc = ccall((:lgmi_graph_create, liblgmi), Cint,
+                   (Int32, Int64, Ptr{Int32}, Ptr{Int32}, Int64, Ptr{Int32}, Ptr{Int32}, Cint, Ptr{Ptr{Cvoid}}),
+                   nv(g), length(col), rp, col, length(icol), irp, icol, 1, out)
+    else
+        rc = ccall((:lgmi_graph_create, liblgmi), Cint,
+                   (Int32, Int64, Ptr{Int32}, Ptr{Int32}, Int64, Ptr{Int32}, Ptr{Int32}, Cint, Ptr{Ptr{Cvoid}}),
+                   nv(g), length(col), rp, col, 0, C_NULL, C_NULL, 0, out)
+    end
+    check(rc)
+    h = Handle(out[])
+    finalizer(h -> ccall((:lgmi_graph_destroy, liblgmi), Cvoid, (Ptr{Cvoid},), h.ptr), h)
+    return h
+end
+
+# ---------------------------------------------------------------------
+# gdistances! / gdistances  (src/traversals/bfs.jl:95-141)
+# ---------------------------------------------------------------------
+function device_dist(g::SimpleG, source)
+    srcs = Int32[s - 1 for s in source]
+    for s in source
+        1 <= s <= nv(g) || throw(BoundsError(vertices(g), s))
+    end
+    h = Handle(g)
+    dist = Vector{Int32}(undef, nv(g))
+    check(ccall((:lgmi_gdistances, liblgmi), Cint, (Ptr{Cvoid}, Ptr{Int32}, Int32, Ptr{Int32}),
+                h.ptr, srcs, length(srcs), dist))
+    return dist
+end
+
+function gdistances!(g::SimpleG{T}, source, vert_level; sort_alg = Base.Sort.QuickSort) where T
+    dist = try
+        device_dist(g, source)
+    catch e
+        e isa Fallback || rethrow()
+        return invoke(gdistances!, Tuple{AbstractGraph{T},Any,Any}, g, source, vert_level; sort_alg = sort_alg)
+    end
+    @inbounds for v in eachindex(dist)
+        dist[v] >= 0 && (vert_level[v] = T(dist[v]))    # reached entries only
+    end
+    return vert_level
+end
+
+gdistances(g::SimpleG{T}, source; sort_alg = Base.Sort.QuickSort) where T =
+    gdistances!(g, source, fill(typemax(T), nv(g)); sort_alg = sort_alg)
+
+# ---------------------------------------------------------------------
+# bfs_parents / bfs_tree  (src/traversals/bfs.jl:35-76)
+# ---------------------------------------------------------------------
+function bfs_parents(g::SimpleG{T}, s::Integer; dir = :out) where T
+    1 <= s <= nv(g) || throw(BoundsError(vertices(g), s))
+    par = Vector{Int32}(undef, nv(g))
+    try
+        h = Handle(g)
+        check(ccall((:lgmi_bfs_parents, liblgmi), Cint, (Ptr{Cvoid}, Int32, Cint, Ptr{Int32}),
+                    h.ptr, s - 1, dir == :out ? 0 : 1, par))
+    catch e
+        e isa Fallback || rethrow()
+        return invoke(bfs_parents, Tuple{AbstractGraph,Integer}, g, s; dir = dir)
+    end
+    return T[p < 0 ? zero(T) : T(p + 1) for p in par]
+end
+
+# ---------------------------------------------------------------------
+# betweenness_centrality  (src/centrality/betweenness.jl:46-78)
+# ---------------------------------------------------------------------
+function device_bc(g::SimpleG, srcs::AbstractVector, k::Integer, normalize::Bool, endpoints::Bool;
+                   skip_degree0::Bool = true)
+    for s in srcs
+        1 <= s <= nv(g) || throw(BoundsError(vertices(g), s))
+    end
+    h = Handle(g)
+    bc = Vector{Float64}(undef, nv(g))
+    s0 = Int32[s - 1 for s in srcs]
+    check(ccall((:lgmi_betweenness, liblgmi), Cint,
+                (Ptr{Cvoid}, Ptr{Int32}, Int32, Int64, Cint, Cint, Cint, Ptr{Float64}),
+                h.ptr, s0, length(s0), k, skip_degree0, normalize, endpoints, bc))
+    return bc
+end
+
+function betweenness_centrality(g::SimpleG, vs::AbstractVector = vertices(g), distmx::DefaultDistance = weights(g);
+                                normalize = true, endpoints = false)
+    try
+        return device_bc(g, vs, length(vs), normalize, endpoints)
+    catch e
+        e isa Fallback || rethrow()
+        return invoke(betweenness_centrality, Tuple{AbstractGraph,Any,AbstractMatrix}, g, vs, distmx;
+                      normalize = normalize, endpoints = endpoints)
+    end
+end
+
+betweenness_centrality(g::SimpleG, k::Integer, distmx::DefaultDistance = weights(g); normalize = true,
+                       endpoints = false) =
+    betweenness_centrality(g, sample(vertices(g), k), distmx; normalize = normalize, endpoints = endpoints)
+
+# ---------------------------------------------------------------------
+# Parallel.*  (src/Parallel/traversals/{gdistances,bfs}.jl,
+#              src/Parallel/centrality/betweenness.jl)
+# ---------------------------------------------------------------------
+function Parallel.gdistances!(g::SimpleG{T}, sources::Vector{<:Integer}, vert_level::Vector{T};
+                              queue_segment_size::Integer = 20) where T
+    fill!(vert_level, typemax(T))                      # gdistances.jl:46
+    return gdistances!(g, sources, vert_level)
+end
+Parallel.gdistances(g::SimpleG{T}, sources::Vector{<:Integer}; queue_segment_size::Integer = 20) where T =
+    Parallel.gdistances!(g, sources, Vector{T}(undef, nv(g)))
+Parallel.gdistances(g::SimpleG{T}, source::Integer; queue_segment_size::Integer = 20) where T =
+    Parallel.gdistances!(g, [source], Vector{T}(undef, nv(g)))
+
+function Parallel.bfs_tree!(next::Parallel.ThreadQueue, g::SimpleG{T}, source::T,
+                            parents::Array{Threads.Atomic{T}}) where T
+    par = bfs_parents(g, source)
+    for v in eachindex(par)
+        parents[v][] = par[v]
+    end
+    return parents
+end
+Parallel.bfs_tree(g::SimpleG{T}, source::T) where T = LightGraphs.tree(bfs_parents(g, source))
+
+function Parallel.betweenness_centrality(g::SimpleG, vs::AbstractVector = vertices(g),
+                                         distmx::DefaultDistance = weights(g);
+                                         normalize = true, endpoints = false, parallel = :distributed)
+    k = length(vs)
+    # :threads takes findall's positions as sources
+    # (Parallel/centrality/betweenness.jl:53-56)
+    parallel == :distributed && return device_bc(g, vs, k, normalize, endpoints)
+    return device_bc(g, findall(x -> degree(g, x) > 0, vs), k, normalize, endpoints; skip_degree0 = false)
+end
+Parallel.betweenness_centrality(g::SimpleG, k::Integer, distmx::DefaultDistance = weights(g);
+                                normalize = true, endpoints = false, parallel = :distributed) =
+    Parallel.betweenness_centrality(g, sample(vertices(g), k), distmx; normalize = normalize,
+                                    endpoints = endpoints, parallel = parallel)
+
+end # module

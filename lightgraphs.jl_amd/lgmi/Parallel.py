@@ -105,4 +105,4 @@ def betweenness_centrality(g: _GraphBase, vs=None, distmx=None, normalize=True, 
     if srcs.size and (srcs.min() < 1 or srcs.max() > g.nv()):
         raise IndexError("vertex out of range")
     pos = np.nonzero(deg[srcs - 1] > 0)[0].astype(np.int64) + 1 if srcs.size else np.zeros(0, np.int64)
-    return _bc._run(g, pos, k, normalize, endpoints)
+    return _bc._run(g, pos, k, normalize, endpoints, skip_degree0=False)

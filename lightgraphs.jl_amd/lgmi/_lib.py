@@ -101,7 +101,7 @@ def load():
         "lgmi_bfs_time_steps": ([_vp, c.c_int], c.c_int),
         "lgmi_bfs_force_direction": ([_vp, c.c_int32], c.c_int),
         "lgmi_bfs_set_policy": ([c.c_int, c.c_float, c.c_float, _f32p, c.c_int32], c.c_int),
-        "lgmi_betweenness": ([_vp, _i32p, c.c_int32, c.c_int64, c.c_int, c.c_int, _f64p], c.c_int),
+        "lgmi_betweenness": ([_vp, _i32p, c.c_int32, c.c_int64, c.c_int, c.c_int, c.c_int, _f64p], c.c_int),
         "lgmi_betweenness_stats": ([_vp, c.POINTER(BcStats)], c.c_int),
         "lgmi_betweenness_batch": ([c.c_int32, _i32p, _i64p, _i32p, _i64p, _i32p, _i64p, c.c_int, c.c_int,
                                     _f64p, _f64p], c.c_int),

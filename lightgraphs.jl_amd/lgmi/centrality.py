@@ -41,7 +41,8 @@ def _check_distmx(distmx):
         raise NotImplementedError("weighted distmx is not on the accelerated path")
 
 
-def _run(g: _GraphBase, srcs1: np.ndarray, k: int, normalize: bool, endpoints: bool) -> np.ndarray:
+def _run(g: _GraphBase, srcs1: np.ndarray, k: int, normalize: bool, endpoints: bool,
+         skip_degree0: bool = True) -> np.ndarray:
     n = g.nv()
     if srcs1.size and (srcs1.min() < 1 or srcs1.max() > n):
         raise IndexError(f"source vertex out of range 1:{n}")
@@ -51,7 +52,8 @@ def _run(g: _GraphBase, srcs1: np.ndarray, k: int, normalize: bool, endpoints: b
     srcs0 = np.ascontiguousarray(srcs1 - 1, dtype=np.int32)
     lib = _lib.load()
     _lib.check(lib.lgmi_betweenness(g.handle(), _lib.i32p(srcs0 if len(srcs0) else np.zeros(1, np.int32)),
-                                    len(srcs0), int(k), int(bool(normalize)), int(bool(endpoints)),
+                                    len(srcs0), int(k), int(bool(skip_degree0)), int(bool(normalize)),
+                                    int(bool(endpoints)),
                                     _lib.f64p(bc)))
     return bc
 

@@ -148,7 +148,7 @@ def test_er10k_all_sources(oracle):
     assert close(bc, want)
     d = lgmi.gdistances(g, 1)
     dref = oracle.gdistances(rp, col, [0])
-    assert (np.where(dref < 0, lgmi.typemax(np.int64), dref) == d).all()
+    assert (np.where(dref < 0, np.int64(lgmi.typemax(np.int64)), dref.astype(np.int64)) == d).all()
 
 
 def test_directed_rmat_subset(oracle):

@@ -154,7 +154,7 @@ def test_multisource_and_isolated(oracle):
         want = oracle.gdistances(rp, col, srcs)
         got = lgmi.gdistances(g, [s + 1 for s in srcs])
         tmax = lgmi.typemax(np.int64)
-        assert (np.where(want < 0, tmax, want) == got).all()
+        assert (np.where(want < 0, np.int64(tmax), want.astype(np.int64)) == got).all()
     assert (lgmi_par0(g, iso, 0) == np.where(np.arange(g.nv()) == iso, iso, -1)).all()
 
 
