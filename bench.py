@@ -247,8 +247,38 @@ def bc_bench(args, lgmi, _lib, orc):
     }
 
 
+def profile_pass(lgmi, _lib, nbfs):
+    """Workload for rocprofv3 counter runs: one 256 MiB device-to-device
+    copy (the runtime's copy kernel, a known byte count for calibrating
+    FETCH_SIZE / WRITE_SIZE) followed by `nbfs` BFS runs on RMAT-22."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    nbytes = 256 << 20
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, ctypes.c_size_t(nbytes), 3) == 0   # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    lib = _lib.load()
+    g = lgmi.generators.rmat(22, 16, seed=1)
+    deg = np.diff(g.csr()[0])
+    h = g.handle()
+    rng = np.random.default_rng(20260101)
+    roots = rng.choice(np.nonzero(deg > 0)[0], size=64, replace=False).astype(np.int32)
+    for i in range(nbfs):
+        _lib.check(lib.lgmi_bfs_run(h, int(roots[i % 64]), 0, 1, None))
+    _lib.check(lib.lgmi_synchronize())
+    log(f"profile pass done: copy {nbytes} B x3, {nbfs} BFS")
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--profile-pass", type=int, default=0,
+                    help="run only the counter-profiling workload with this many BFS runs")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=8)
@@ -266,6 +296,9 @@ def main():
     if _lib.load().lgmi_device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.profile_pass:
+        profile_pass(lgmi, _lib, args.profile_pass)
+        return
     result = bfs_bench(args, lgmi, _lib, orc)
     if not args.no_bc:
         result["bc"] = bc_bench(args, lgmi, _lib, orc)

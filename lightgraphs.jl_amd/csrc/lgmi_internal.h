@@ -59,17 +59,9 @@ struct BfsCtrl {
     int64_t m_front;        // sum of traversal-direction degrees over F_L
     int64_t m_unvisited;    // sum of scan-direction degrees over unvisited vertices
     int64_t n_unvisited;
-    // accumulators of the step in flight
-    int32_t next_count;
-    uint32_t blocks_done;
-    unsigned long long next_out;
-    unsigned long long next_in;
-    unsigned long long next_scanned;  // adjacency entries read by the step
-    unsigned long long next_cand;     // vertices the step examined (bottom-up)
-    uint32_t hist[kHistBins];   // degree histogram of F_{L+1} (being built)
-    uint32_t front_hist[kHistBins];
+    uint32_t front_hist[kHistBins];   // degree histogram of F_L (policy feature)
     uint64_t t_prev_end;    // wall clock at the end of the previous step
-    uint64_t t_step_start;  // min wall clock over the blocks of the current step
+    uint64_t t_step_start;  // wall clock when block 0 of the current step started
     int32_t forced;         // -1 policy decides; 0/1 force a direction (label runs)
     int32_t policy_mode;    // LGMI_POLICY_*
     float alpha, beta;
@@ -102,12 +94,19 @@ struct PolicyWeights {      // FP32 MLP 16 -> 64 (ReLU) -> 2
     float sd[kFeat];
 };
 
+// Per-block counters of one step, reduced in block order by the finalise
+// kernel (plain stores: no same-address atomics across work-groups).
+constexpr int kPart = 16;
+enum PartSlot { P_NEXT = 0, P_OUT = 1, P_IN = 2, P_SCAN = 3, P_CAND = 4, P_HIST = 5 };  // P_HIST .. +7
+
 struct BfsWork {
     int32_t nv = 0;
     int32_t nwords = 0;
+    int32_t grid = 0;
     int32_t* dist = nullptr;
     int32_t* parent = nullptr;
-    int32_t* q[2] = {nullptr, nullptr};
+    int32_t* srcbuf = nullptr;   // multi-source lists
+    unsigned long long* part = nullptr;  // [grid][kPart]
     uint32_t* bm[3] = {nullptr, nullptr, nullptr};
     uint32_t* vis = nullptr;
     BfsCtrl* ctrl = nullptr;
@@ -128,10 +127,17 @@ struct BfsWork {
 };
 
 struct BcWork;
+// Rows longer than kHeavyRow are split into work items of kHeavyRow
+// entries so that a hub vertex is expanded by many work-groups at once.
+constexpr int32_t kHeavyRow = 1024;
+
 struct DevCSR {
     int32_t* row_ptr = nullptr;
     int32_t* col = nullptr;
     uint32_t* nz_bm = nullptr;   // bit v set when the row of v is non-empty
+    int32_t* item_v = nullptr;   // heavy-row work items: vertex
+    int32_t* item_beg = nullptr; //   and first entry (at most kHeavyRow entries each)
+    int32_t n_items = 0;
 };
 
 }  // namespace lgmi
