@@ -141,6 +141,37 @@ int lgmi_bfs_time_steps(lgmi_graph_t g, int enable);
  * lets the policy decide (-1).  Outputs never change; only time does. */
 int lgmi_bfs_force_direction(lgmi_graph_t g, int32_t dir);
 
+/* One probed BFS level: the policy features of the state and the device
+ * time of the step run top-down and bottom-up on that identical state. */
+typedef struct {
+    float feat[16];
+    float td_us;
+    float bu_us;
+    int32_t level;
+    int32_t n_front;
+} lgmi_probe_rec;
+
+/* Training-label collection for the direction policy (no counterpart in
+ * the reference).  Runs a BFS from src; at every level listed in `levels`
+ * (strictly increasing) it runs the step both ways on the same state
+ * (undoing the first) and records the two HIP-event times.  Levels past the
+ * last one are skipped.  Returns the number of records written. */
+int lgmi_bfs_probe(lgmi_graph_t g, int32_t src, int dir_in, const int32_t* levels, int32_t nlevels,
+                   lgmi_probe_rec* out);
+
+/* Direction-policy MLP on the GPU (FP32, layout LGMI_POLICY_NWEIGHTS).
+ * x: n rows of 16 features, y: labels (0 top-down faster, 1 bottom-up
+ * faster), w: per-sample weights (NULL = 1).  Loss = weighted softmax
+ * cross-entropy / sum of weights.  The feature mean/scale entries of
+ * params are inputs and are never trained (their gradient is 0). */
+int lgmi_mlp_loss_grad(const float* x, const int32_t* y, const float* w, int32_t n, const float* params,
+                       float* loss, float* grad);
+/* Full-batch Adam for `steps` steps at learning rate lr; params in/out. */
+int lgmi_mlp_train(const float* x, const int32_t* y, const float* w, int32_t n, float* params, int32_t steps,
+                   float lr, float* loss_out);
+/* Logits (n x 2) of the policy for n feature rows. */
+int lgmi_mlp_forward(const float* x, int32_t n, const float* params, float* logits);
+
 /* Direction policy for all later BFS runs (process-wide).
  * mode: LGMI_POLICY_HEURISTIC (alpha, beta), LGMI_POLICY_LEARNED (mlp must
  * hold LGMI_POLICY_NWEIGHTS floats; the heuristic is used while none is

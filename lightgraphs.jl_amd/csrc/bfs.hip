@@ -39,6 +39,7 @@
 #include "lgmi_internal.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -51,6 +52,7 @@ constexpr int kWaves = kBlock / 64;
 constexpr int32_t kWaveRow = 32;    // top-down: rows this long or longer are expanded by a wavefront
 constexpr int32_t kBuCoop = 64;     // bottom-up: rows longer than this are scanned by a wavefront
 constexpr int kFinBlock = 1024;     // finalise kernel threads
+constexpr int kGraphLevels = 16;    // step/finalise pairs per captured graph
 
 struct StepArgs {
     const int32_t* td_rp;     // traversal adjacency (top-down expands it)
@@ -386,22 +388,53 @@ __global__ __launch_bounds__(kBlock) void bfs_step(StepArgs a) {
     }
 }
 
+// The learned policy's weights as lane j of wavefront 0 needs them (hidden
+// unit j), loaded at the start of the finalise kernel so that their latency
+// overlaps the partial-sum loads.  The device copy has the feature
+// standardisation folded into W1 and b1 (done on the host when the policy
+// is set), so inference is 16 FMAs per lane and a butterfly sum.
+struct PolicyRegs {
+    float4 w1[kFeat / 4];
+    float b1, w2a, w2b, b2a, b2b;
+};
+
+__device__ inline void load_policy(const PolicyWeights* w, int t, PolicyRegs& r) {
+    const float4* row = reinterpret_cast<const float4*>(w->w1 + t * kFeat);
+    #pragma unroll
+    for (int q = 0; q < kFeat / 4; q++) r.w1[q] = row[q];
+    r.b1 = w->b1[t];
+    r.w2a = w->w2[t];
+    r.w2b = w->w2[kHidden + t];
+    r.b2a = w->b2[0];
+    r.b2b = w->b2[1];
+}
+
 // Direction for the next step (called by every thread of the finalise block).
-__device__ int choose_direction(const StepArgs& a, const BfsCtrl* c, const float* feat, float* h) {
+__device__ int choose_direction(const StepArgs& a, const BfsCtrl* c, const float* feat, const PolicyRegs& pr) {
     if (c->forced >= 0) return c->forced;
     if (c->policy_mode == LGMI_POLICY_TOPDOWN) return 0;
     if (c->policy_mode == LGMI_POLICY_LEARNED && a.policy != nullptr) {
-        const PolicyWeights* w = a.policy;
+        // wavefront 0: lane j = hidden unit j; logits by a butterfly sum
         const int t = threadIdx.x;
+        __shared__ int s_dec;
         if (t < kHidden) {
-            float acc = w->b1[t];
-            for (int i = 0; i < kFeat; i++) acc += w->w1[t * kFeat + i] * ((feat[i] - w->mu[i]) / w->sd[i]);
-            h[t] = acc > 0.f ? acc : 0.f;
+            const float* xs = feat;
+            float acc = pr.b1;
+            #pragma unroll
+            for (int q = 0; q < kFeat / 4; q++) {
+                const float4 r = pr.w1[q];
+                acc += r.x * xs[4 * q] + r.y * xs[4 * q + 1] + r.z * xs[4 * q + 2] + r.w * xs[4 * q + 3];
+            }
+            const float hj = acc > 0.f ? acc : 0.f;
+            float a0 = pr.w2a * hj, a1 = pr.w2b * hj;
+            for (int o = 32; o >= 1; o >>= 1) {
+                a0 += __shfl_xor(a0, o);
+                a1 += __shfl_xor(a1, o);
+            }
+            if (t == 0) s_dec = (a1 + pr.b2b) > (a0 + pr.b2a) ? 1 : 0;
         }
         __syncthreads();
-        float z0 = w->b2[0], z1 = w->b2[1];
-        for (int j = 0; j < kHidden; j++) { z0 += w->w2[j] * h[j]; z1 += w->w2[kHidden + j] * h[j]; }
-        return z1 > z0 ? 1 : 0;
+        return s_dec;
     }
     // Beamer alpha/beta (GAP form): leave top-down when the frontier's edges
     // exceed the unvisited edges / alpha; leave bottom-up when the frontier
@@ -419,13 +452,15 @@ __global__ __launch_bounds__(kFinBlock) void bfs_finalize(StepArgs a, int32_t nb
     __shared__ unsigned long long s_sum[kFinBlock / kPart][kPart];
     __shared__ int32_t s_done;
     __shared__ float s_feat[kFeat];
-    __shared__ float s_h[kHidden];
     BfsCtrl* c = a.ctrl;
     const int tid = threadIdx.x;
     if (tid == 0) s_done = c->done;
     __syncthreads();
     if (s_done) return;
     const uint64_t now = wall_clock64();
+    PolicyRegs pr;
+    if (a.policy != nullptr && tid < kHidden && c->forced < 0 && c->policy_mode == LGMI_POLICY_LEARNED)
+        load_policy(a.policy, tid, pr);
     // thread t reads slot t % kPart of blocks t / kPart, + 64, ... (coalesced rows)
     constexpr int kRows = kFinBlock / kPart;
     const int slot = tid % kPart, row = tid / kPart;
@@ -478,7 +513,7 @@ __global__ __launch_bounds__(kFinBlock) void bfs_finalize(StepArgs a, int32_t nb
     }
     __syncthreads();
     if (s_done) return;
-    const int nd = choose_direction(a, c, s_feat, s_h);
+    const int nd = choose_direction(a, c, s_feat, pr);
     if (tid == 0) {
         c->dir = nd;
         const int32_t L1 = c->level;
@@ -565,6 +600,24 @@ __global__ __launch_bounds__(kBlock) void bfs_seed(StepArgs a, const int32_t* sr
     if (tid == 0) init_ctrl(a, nsrc, s_out, s_in, s_hist, nnz_scan, forced, mode, alpha, beta);
 }
 
+// Undoes one step: every vertex of F_{L+1} (`Fn`) returns to unvisited and
+// Fn is cleared.  Used by the direction probe to run the other direction on
+// the identical state.
+__global__ void bfs_undo(StepArgs a, uint32_t* Fn) {
+    for (int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; w < a.nwords;
+         w += (int64_t)gridDim.x * blockDim.x) {
+        uint32_t bits = Fn[w];
+        while (bits) {
+            const int b = __ffs(bits) - 1;
+            bits &= bits - 1;
+            const int64_t v = w * 32 + b;
+            a.dist[v] = -1;
+            if (a.parent) a.parent[v] = 0xffffffffu;
+        }
+        Fn[w] = 0;
+    }
+}
+
 }  // namespace
 
 BfsWork::~BfsWork() {}
@@ -578,6 +631,8 @@ void bfs_free(BfsWork* w) {
     if (w->ev0) (void)hipEventDestroy(w->ev0);
     if (w->ev1) (void)hipEventDestroy(w->ev1);
     for (auto e : w->step_ev) (void)hipEventDestroy(e);
+    for (auto& x : w->gexec)
+        if (x) (void)hipGraphExecDestroy(x);
     delete w;
 }
 
@@ -595,7 +650,7 @@ void bfs_prepare(lgmi_graph* g) {
         for (int k = 0; k < 3; k++) w->bm[k] = dev_alloc_n<uint32_t>(std::max(w->nwords, 2));
         w->vis = dev_alloc_n<uint32_t>(std::max(w->nwords, 2));
         w->ctrl = dev_alloc_n<BfsCtrl>(1);
-        w->max_log = 4096;
+        w->max_log = 16384;   // covers a 4096 x 4096 grid (up to 8190 levels)
         w->log = dev_alloc_n<LevelRec>(w->max_log);
         w->policy = dev_alloc_n<PolicyWeights>(1);
         LGMI_HIP(hipHostMalloc((void**)&w->h_ctrl, sizeof(BfsCtrl), hipHostMallocDefault));
@@ -680,16 +735,43 @@ void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_
         }
         LGMI_HIP(hipEventRecord(w->step_ev[i], s));
     };
+    static const bool graphs_on = [] {
+        const char* e = std::getenv("LGMI_BFS_GRAPHS");
+        return !(e && e[0] == '0');
+    }();
     for (;;) {
-        for (int i = 0; i < batch; i++) {
-            const size_t li = (size_t)(launched + i);
-            if (w->time_steps) event_at(2 * li);
-            if (want_parents) bfs_step<true><<<grid, kBlock, 0, s>>>(a);
-            else bfs_step<false><<<grid, kBlock, 0, s>>>(a);
-            if (w->time_steps) event_at(2 * li + 1);
-            bfs_finalize<<<1, kFinBlock, 0, s>>>(a, grid);
+        if (graphs_on && !w->time_steps && launched >= kGraphLevels) {
+            // deep search: replay captured step/finalise sequences
+            const int key = (want_parents ? 1 : 0) | (dir_in ? 2 : 0) | (a.policy ? 4 : 0);
+            if (!w->gexec[key]) {
+                hipGraph_t gr = nullptr;
+                LGMI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                for (int i = 0; i < kGraphLevels; i++) {
+                    if (want_parents) bfs_step<true><<<grid, kBlock, 0, s>>>(a);
+                    else bfs_step<false><<<grid, kBlock, 0, s>>>(a);
+                    bfs_finalize<<<1, kFinBlock, 0, s>>>(a, grid);
+                }
+                LGMI_HIP(hipStreamEndCapture(s, &gr));
+                hipGraphExec_t ex = nullptr;
+                const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(gr);
+                LGMI_HIP(e);
+                w->gexec[key] = ex;
+            }
+            const int ng = std::max(1, batch / kGraphLevels);
+            for (int i = 0; i < ng; i++) LGMI_HIP(hipGraphLaunch(w->gexec[key], s));
+            launched += ng * kGraphLevels;
+        } else {
+            for (int i = 0; i < batch; i++) {
+                const size_t li = (size_t)(launched + i);
+                if (w->time_steps) event_at(2 * li);
+                if (want_parents) bfs_step<true><<<grid, kBlock, 0, s>>>(a);
+                else bfs_step<false><<<grid, kBlock, 0, s>>>(a);
+                if (w->time_steps) event_at(2 * li + 1);
+                bfs_finalize<<<1, kFinBlock, 0, s>>>(a, grid);
+            }
+            launched += batch;
         }
-        launched += batch;
         LGMI_HIP(hipGetLastError());
         LGMI_HIP(hipMemcpyAsync(w->h_ctrl, w->ctrl, sizeof(BfsCtrl), hipMemcpyDeviceToHost, s));
         LGMI_HIP(hipEventRecord(w->ev1, s));
@@ -799,6 +881,112 @@ extern "C" int lgmi_bfs_time_steps(lgmi_graph_t g, int enable) {
         bfs_prepare(g);
         g->bfs->time_steps = enable != 0;
         return LGMI_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return LGMI_E_HIP;
+    }
+}
+
+// ----------------------------------------------------------------------
+// direction probe: both directions timed on the identical level state
+// ----------------------------------------------------------------------
+extern "C" int lgmi_bfs_probe(lgmi_graph_t g, int32_t src, int dir_in, const int32_t* levels, int32_t nlevels,
+                              lgmi_probe_rec* out) {
+    using namespace lgmi;
+    try {
+        if (!g || nlevels < 0 || (nlevels > 0 && (!levels || !out))) throw Error(LGMI_E_ARG, "bad arguments");
+        std::lock_guard<std::mutex> lk(g->mu);
+        if (src < 0 || src >= g->nv) throw Error(LGMI_E_BOUNDS, "source vertex out of range");
+        for (int32_t i = 1; i < nlevels; i++)
+            if (levels[i] <= levels[i - 1]) throw Error(LGMI_E_ARG, "levels must be strictly increasing");
+        bfs_prepare(g);
+        BfsWork* w = g->bfs;
+        StepArgs a = make_args(g, dir_in, true);
+        a.policy = nullptr;
+        hipStream_t s = g->stream;
+        const int grid = w->grid;
+        const int init_grid = std::max(1, std::min<int>(grid, (g->nv + 255) / 256));
+        bfs_init<<<init_grid, 256, 0, s>>>(a, src, g->nnz, -1, LGMI_POLICY_HEURISTIC, 15.0f, 18.0f);
+        LGMI_HIP(hipGetLastError());
+        hipEvent_t ev[4];
+        for (auto& e : ev) LGMI_HIP(hipEventCreate(&e));
+        auto sync_ctrl = [&]() {
+            LGMI_HIP(hipMemcpyAsync(w->h_ctrl, w->ctrl, sizeof(BfsCtrl), hipMemcpyDeviceToHost, s));
+            LGMI_HIP(hipStreamSynchronize(s));
+        };
+        auto run_forced = [&](int dir, hipEvent_t e0, hipEvent_t e1) {
+            BfsCtrl c = *w->h_ctrl;
+            c.dir = dir;
+            LGMI_HIP(hipMemcpyAsync(w->ctrl, &c, sizeof(BfsCtrl), hipMemcpyHostToDevice, s));
+            LGMI_HIP(hipEventRecord(e0, s));
+            bfs_step<true><<<grid, kBlock, 0, s>>>(a);
+            LGMI_HIP(hipEventRecord(e1, s));
+            bfs_finalize<<<1, kFinBlock, 0, s>>>(a, grid);
+            LGMI_HIP(hipGetLastError());
+            LGMI_HIP(hipStreamSynchronize(s));
+        };
+        int nout = 0;
+        try {
+            sync_ctrl();
+            for (int32_t li = 0; li < nlevels && !w->h_ctrl->done; li++) {
+                const int32_t target = levels[li];
+                while (!w->h_ctrl->done && w->h_ctrl->level < target) {
+                    const int n = std::min(target - w->h_ctrl->level, 256);
+                    for (int i = 0; i < n; i++) {
+                        bfs_step<true><<<grid, kBlock, 0, s>>>(a);
+                        bfs_finalize<<<1, kFinBlock, 0, s>>>(a, grid);
+                    }
+                    LGMI_HIP(hipGetLastError());
+                    sync_ctrl();
+                }
+                if (w->h_ctrl->done || w->h_ctrl->level != target) break;
+                const BfsCtrl saved = *w->h_ctrl;
+                const int32_t L = saved.level;
+                uint32_t* Fn = w->bm[(L + 1) % 3];
+                // Each direction runs kReps times on the restored state, in
+                // alternating order; the minimum is kept, so one slow launch
+                // (a cold start after the host synchronised) does not decide
+                // the label.
+                constexpr int kReps = 3;
+                float best[2] = {1e30f, 1e30f};
+                const int first = ((L + src) & 1) ? 1 : 0;
+                for (int rep = 0; rep < kReps; rep++) {
+                    for (int q = 0; q < 2; q++) {
+                        const int dir = (first + q + rep) & 1;
+                        *w->h_ctrl = saved;
+                        run_forced(dir, ev[0], ev[1]);
+                        float t = 0.f;
+                        LGMI_HIP(hipEventElapsedTime(&t, ev[0], ev[1]));
+                        best[dir] = std::min(best[dir], t);
+                        if (!(rep == kReps - 1 && q == 1)) {
+                            bfs_undo<<<grid, 256, 0, s>>>(a, Fn);
+                            LGMI_HIP(hipGetLastError());
+                        }
+                    }
+                }
+                const float t_first = best[first], t_second = best[1 - first];
+                LevelRec rec;
+                LGMI_HIP(hipMemcpy(&rec, w->log + std::min(L, w->max_log - 1), sizeof(LevelRec), hipMemcpyDeviceToHost));
+                lgmi_probe_rec& o = out[nout++];
+                std::memset(&o, 0, sizeof(o));
+                for (int q = 0; q < kFeat; q++) o.feat[q] = rec.feat[q];
+                o.td_us = 1000.f * (first == 0 ? t_first : t_second);
+                o.bu_us = 1000.f * (first == 0 ? t_second : t_first);
+                o.level = L;
+                o.n_front = saved.n_front;
+                sync_ctrl();
+            }
+        } catch (...) {
+            for (auto& e : ev) (void)hipEventDestroy(e);
+            throw;
+        }
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        w->last_levels = 0;
+        w->last_parents = 0;
+        return nout;
     } catch (const Error& e) {
         set_last_error(e.what());
         return e.code;

@@ -280,9 +280,23 @@ int lgmi_bfs_set_policy(int mode, float alpha, float beta, const float* mlp, int
     if (mlp) {
         if (nweights != LGMI_POLICY_NWEIGHTS) throw Error(LGMI_E_ARG, "MLP needs LGMI_POLICY_NWEIGHTS floats");
         static_assert(sizeof(PolicyWeights) == sizeof(float) * LGMI_POLICY_NWEIGHTS, "layout");
-        std::memcpy(&p.w, mlp, sizeof(PolicyWeights));
+        PolicyWeights raw;
+        std::memcpy(&raw, mlp, sizeof(PolicyWeights));
         for (int i = 0; i < kFeat; i++)
-            if (!(p.w.sd[i] > 0.f)) throw Error(LGMI_E_ARG, "feature scales must be positive");
+            if (!(raw.sd[i] > 0.f)) throw Error(LGMI_E_ARG, "feature scales must be positive");
+        // Fold the standardisation into the first layer for the device BFS:
+        // W1'[j][i] = W1[j][i] / sd[i], b1'[j] = b1[j] - sum_i W1'[j][i] mu[i].
+        p.w = raw;
+        for (int j = 0; j < kHidden; j++) {
+            double b = raw.b1[j];
+            for (int i = 0; i < kFeat; i++) {
+                const double wf = (double)raw.w1[j * kFeat + i] / (double)raw.sd[i];
+                p.w.w1[j * kFeat + i] = (float)wf;
+                b -= wf * (double)raw.mu[i];
+            }
+            p.w.b1[j] = (float)b;
+        }
+        for (int i = 0; i < kFeat; i++) { p.w.mu[i] = 0.f; p.w.sd[i] = 1.f; }
         p.have_mlp = true;
     } else if (mode == LGMI_POLICY_LEARNED && !p.have_mlp) {
         throw Error(LGMI_E_ARG, "learned policy needs MLP weights");

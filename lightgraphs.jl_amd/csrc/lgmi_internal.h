@@ -116,6 +116,7 @@ struct BfsWork {
     PolicyWeights* policy = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool time_steps = false;              // HIP events around every step launch
+    hipGraphExec_t gexec[8] = {};         // captured step/finalise sequences (parents, dir_in, learned)
     std::vector<hipEvent_t> step_ev;      // launch i runs between step_ev[i] and step_ev[i+1]
     int last_launches = 0;
     double last_steps_event_ms = 0.0;

@@ -32,6 +32,7 @@ LGMI_POLICY_NWEIGHTS = 64 * 16 + 64 + 2 * 64 + 2 + 16 + 16
 EXPORTED = [
     "lgmi_graph_create", "lgmi_graph_destroy", "lgmi_gdistances", "lgmi_bfs_parents", "lgmi_bfs",
     "lgmi_bfs_run", "lgmi_bfs_fetch", "lgmi_bfs_steps", "lgmi_bfs_time_steps", "lgmi_bfs_force_direction",
+    "lgmi_bfs_probe", "lgmi_mlp_loss_grad", "lgmi_mlp_train", "lgmi_mlp_forward",
     "lgmi_bfs_set_policy", "lgmi_betweenness", "lgmi_betweenness_stats", "lgmi_betweenness_batch",
     "lgmi_last_error", "lgmi_device_count", "lgmi_synchronize",
     "lgmi_gen_kronecker", "lgmi_gen_erdos_renyi", "lgmi_gen_barabasi_albert",
@@ -62,6 +63,11 @@ class BfsStep(ctypes.Structure):
                 ("pad", ctypes.c_int32), ("scanned", ctypes.c_int64), ("cand", ctypes.c_int64),
                 ("kernel_us", ctypes.c_float), ("step_us", ctypes.c_float), ("event_ms", ctypes.c_float),
                 ("pad2", ctypes.c_int32)]
+
+
+class ProbeRec(ctypes.Structure):
+    _fields_ = [("feat", ctypes.c_float * 16), ("td_us", ctypes.c_float), ("bu_us", ctypes.c_float),
+                ("level", ctypes.c_int32), ("n_front", ctypes.c_int32)]
 
 
 class BcStats(ctypes.Structure):
@@ -100,6 +106,10 @@ def load():
         "lgmi_bfs_steps": ([_vp, c.c_int32, c.POINTER(BfsStep)], c.c_int),
         "lgmi_bfs_time_steps": ([_vp, c.c_int], c.c_int),
         "lgmi_bfs_force_direction": ([_vp, c.c_int32], c.c_int),
+        "lgmi_bfs_probe": ([_vp, c.c_int32, c.c_int, _i32p, c.c_int32, c.POINTER(ProbeRec)], c.c_int),
+        "lgmi_mlp_loss_grad": ([_f32p, _i32p, _f32p, c.c_int32, _f32p, _f32p, _f32p], c.c_int),
+        "lgmi_mlp_train": ([_f32p, _i32p, _f32p, c.c_int32, _f32p, c.c_int32, c.c_float, _f32p], c.c_int),
+        "lgmi_mlp_forward": ([_f32p, c.c_int32, _f32p, _f32p], c.c_int),
         "lgmi_bfs_set_policy": ([c.c_int, c.c_float, c.c_float, _f32p, c.c_int32], c.c_int),
         "lgmi_betweenness": ([_vp, _i32p, c.c_int32, c.c_int64, c.c_int, c.c_int, c.c_int, _f64p], c.c_int),
         "lgmi_betweenness_stats": ([_vp, c.POINTER(BcStats)], c.c_int),
