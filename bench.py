@@ -247,6 +247,61 @@ def bc_bench(args, lgmi, _lib, orc):
     }
 
 
+def config4_bench(args, lgmi, orc):
+    """4096 graphs, n log-uniform in [200, 20000], half barabasi_albert(n, 4),
+    half watts_strogatz(n, 8, 0.1), seeded per graph: all-source exact
+    betweenness (normalize=true, endpoints=false) for every graph in one
+    persistent launch after autotuning; the CPU oracle runs a seeded
+    subsample and its time is scaled by sum n*m."""
+    from lgmi import batch
+    t0 = time.time()
+    graphs = lgmi.generators.uneven_batch(args.config4_graphs, 200, 20_000, seed=1)
+    nm = [g.nv() * (len(g.csr()[1]) // 2) for g in graphs]
+    nm_tot = float(sum(nm))
+    log(f"batch built: {len(graphs)} graphs, sum n {sum(g.nv() for g in graphs)}, sum n*m {nm_tot:.3e} "
+        f"({time.time() - t0:.1f}s)")
+    tune = batch.autotune(graphs, log=log)
+    if args.save_tuning:
+        batch.save_tuning(tune, args.save_tuning)
+    runs = []
+    for _ in range(2):
+        t1 = time.perf_counter()
+        res, ms = lgmi.betweenness_centrality_batch(graphs, return_ms=True)
+        runs.append((ms, time.perf_counter() - t1))
+    ms = min(r[0] for r in runs)
+    wall = min(r[1] for r in runs)
+    log(f"batch: device {ms:.1f} ms, wall {wall * 1e3:.1f} ms, {nm_tot / (ms * 1e-3) / 1e9:.2f} G n*m/s")
+    rng = np.random.default_rng(7)
+    idx = rng.choice(len(graphs), size=min(args.cpu_batch_graphs, len(graphs)), replace=False)
+    nthreads = min(16, os.cpu_count() or 1)
+    t_cpu = nm_s = 0.0
+    maxerr = 0.0
+    for i in idx:
+        g = graphs[i]
+        rp, col = g.csr()
+        c0 = time.perf_counter()
+        want = orc.betweenness_threaded(rp, col, rp, False, np.arange(g.nv()), nthreads=nthreads, index_quirk=False)
+        t_cpu += time.perf_counter() - c0
+        nm_s += nm[i]
+        floor = 1e-15 * max(1.0, float(np.abs(want).max()))
+        maxerr = max(maxerr, float((np.abs(res[i] - want) / np.maximum(np.abs(want), floor)).max()))
+    cpu_s = t_cpu * nm_tot / nm_s
+    log(f"cpu oracle: {len(idx)} graphs in {t_cpu:.1f}s -> {cpu_s:.0f}s extrapolated; max rel err {maxerr:.2e}")
+    return {
+        "workload": "4096 graphs, n log-uniform in [200, 20000], half barabasi_albert(n, 4), half "
+                    "watts_strogatz(n, 8, 0.1); all-source exact betweenness, FP64, one persistent launch",
+        "value": round(nm_tot / (ms * 1e-3) / 1e9, 4), "unit": "G (sum n*m)/s",
+        "device_ms": round(ms, 2), "wall_ms": round(wall * 1e3, 2), "sum_nm": nm_tot,
+        "tuning": tune["best"], "autotune_candidates": tune["candidates"],
+        "sample_max_rel_err": maxerr,
+        "cpu_baseline": {"value": round(nm_tot / cpu_s / 1e9, 5), "unit": "G (sum n*m)/s", "cores": nthreads,
+                         "kind": "port",
+                         "sample": f"{len(idx)} seeded graphs of the batch, threaded oracle Brandes "
+                                   f"({t_cpu:.1f} s), time scaled by sum n*m"},
+        "speedup_vs_cpu": round(cpu_s / (ms * 1e-3), 1),
+    }
+
+
 def profile_pass(lgmi, _lib, nbfs):
     """Workload for rocprofv3 counter runs: one 256 MiB device-to-device
     copy (the runtime's copy kernel, a known byte count for calibrating
@@ -285,6 +340,11 @@ def main():
     ap.add_argument("--cpu-roots", type=int, default=8)
     ap.add_argument("--cpu-bc-sources", type=int, default=128)
     ap.add_argument("--no-bc", action="store_true")
+    ap.add_argument("--config4", action="store_true",
+                    help="run only the uneven small-graph batch (betweenness) workload")
+    ap.add_argument("--config4-graphs", type=int, default=4096)
+    ap.add_argument("--cpu-batch-graphs", type=int, default=24)
+    ap.add_argument("--save-tuning", default="", help="write the batch autotuner's choice to this JSON file")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
@@ -298,6 +358,9 @@ def main():
         sys.exit(1)
     if args.profile_pass:
         profile_pass(lgmi, _lib, args.profile_pass)
+        return
+    if args.config4:
+        print(json.dumps(config4_bench(args, lgmi, orc)), flush=True)
         return
     result = bfs_bench(args, lgmi, _lib, orc)
     if not args.no_bc:

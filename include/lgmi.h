@@ -208,6 +208,13 @@ int lgmi_betweenness_batch(int32_t ngraphs, const int32_t* nv, const int64_t* ro
                            const int32_t* row_ptr_cat, const int64_t* col_off,
                            const int32_t* col_cat, const int64_t* bc_off, int normalize,
                            int endpoints, double* bc_cat, double* device_ms);
+/* Tuning of lgmi_betweenness_batch (process-wide; no reference counterpart):
+ * graphs with at most wave_max_n vertices run one source per wavefront,
+ * larger ones one source per work-group (-1: the largest n whose 16 sigma
+ * slices fit in LDS, at most 1024); task_work_target is the n * (nnz + n)
+ * work above which a graph's sources are split over several tasks
+ * (0: total work / (6 * number of CUs)). */
+int lgmi_betweenness_batch_tuning(int32_t wave_max_n, double task_work_target);
 
 /* Thread-local message of the last failed call ("" if none). */
 const char* lgmi_last_error(void);

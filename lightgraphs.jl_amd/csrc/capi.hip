@@ -318,38 +318,6 @@ int lgmi_betweenness(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int64_t 
     LGMI_API_END
 }
 
-int lgmi_betweenness_batch(int32_t ngraphs, const int32_t* nv, const int64_t* row_off, const int32_t* row_ptr_cat,
-                           const int64_t* col_off, const int32_t* col_cat, const int64_t* bc_off, int normalize,
-                           int endpoints, double* bc_cat, double* device_ms) {
-    LGMI_API_BEGIN
-    if (ngraphs < 0 || (ngraphs > 0 && (!nv || !row_off || !row_ptr_cat || !col_off || !col_cat || !bc_off || !bc_cat)))
-        throw Error(LGMI_E_ARG, "bad arguments");
-    ensure_device();
-    double total_ms = 0.0;
-    for (int32_t i = 0; i < ngraphs; i++) {
-        const int32_t n = nv[i];
-        if (n < 0) throw Error(LGMI_E_ARG, "nv < 0");
-        const int32_t* rp = row_ptr_cat + row_off[i];
-        const int64_t nnz = rp[n];
-        lgmi_graph_t g = nullptr;
-        int rc = lgmi_graph_create(n, nnz, rp, col_cat + col_off[i], 0, nullptr, nullptr, 0, &g);
-        if (rc != LGMI_OK) throw Error(rc, t_last_error);
-        std::vector<int32_t> vs((size_t)n);
-        for (int32_t v = 0; v < n; v++) vs[(size_t)v] = v;
-        try {
-            lgmi_bc_stats st{};
-            bc_run(g, vs.data(), n, n, 1, normalize, endpoints, bc_cat + bc_off[i], &st);
-            total_ms += st.device_ms;
-        } catch (...) {
-            lgmi_graph_destroy(g);
-            throw;
-        }
-        lgmi_graph_destroy(g);
-    }
-    if (device_ms) *device_ms = total_ms;
-    LGMI_API_END
-}
-
 int lgmi_betweenness_stats(lgmi_graph_t g, lgmi_bc_stats* stats) {
     LGMI_API_BEGIN
     if (!g || !stats) throw Error(LGMI_E_ARG, "bad arguments");

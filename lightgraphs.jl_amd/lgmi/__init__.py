@@ -12,7 +12,7 @@ hand-written gfx950 HIP kernels in liblgmi.so through its C ABI
 """
 from . import Parallel, generators
 from ._lib import LgmiError, NoDeviceError
-from .centrality import betweenness_centrality, sample
+from .centrality import betweenness_centrality, betweenness_centrality_batch, sample
 from .graphs import SimpleDiGraph, SimpleGraph
 from .traversals import bfs_parents, bfs_tree, gdistances, gdistances_, tree, typemax
 

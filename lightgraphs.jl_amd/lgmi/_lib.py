@@ -34,6 +34,7 @@ EXPORTED = [
     "lgmi_bfs_run", "lgmi_bfs_fetch", "lgmi_bfs_steps", "lgmi_bfs_time_steps", "lgmi_bfs_force_direction",
     "lgmi_bfs_probe", "lgmi_mlp_loss_grad", "lgmi_mlp_train", "lgmi_mlp_forward",
     "lgmi_bfs_set_policy", "lgmi_betweenness", "lgmi_betweenness_stats", "lgmi_betweenness_batch",
+    "lgmi_betweenness_batch_tuning",
     "lgmi_last_error", "lgmi_device_count", "lgmi_synchronize",
     "lgmi_gen_kronecker", "lgmi_gen_erdos_renyi", "lgmi_gen_barabasi_albert",
     "lgmi_gen_watts_strogatz", "lgmi_gen_grid2d", "lgmi_csr_build", "lgmi_csr_transpose",
@@ -115,6 +116,7 @@ def load():
         "lgmi_betweenness_stats": ([_vp, c.POINTER(BcStats)], c.c_int),
         "lgmi_betweenness_batch": ([c.c_int32, _i32p, _i64p, _i32p, _i64p, _i32p, _i64p, c.c_int, c.c_int,
                                     _f64p, _f64p], c.c_int),
+        "lgmi_betweenness_batch_tuning": ([c.c_int32, c.c_double], c.c_int),
         "lgmi_last_error": ([], c.c_char_p),
         "lgmi_device_count": ([], c.c_int),
         "lgmi_synchronize": ([], c.c_int),

@@ -66,6 +66,41 @@ def betweenness_centrality(g: _GraphBase, vs=None, distmx=None, normalize=True, 
     return _run(g, srcs, len(srcs), normalize, endpoints)
 
 
+def betweenness_centrality_batch(graphs, normalize=True, endpoints=False, return_ms=False):
+    """betweenness_centrality(g) for each undirected graph of `graphs` (all
+    sources, DefaultDistance), computed in one persistent launch.  Returns a
+    list of FP64 vectors (and the device milliseconds with return_ms)."""
+    lib = _lib.load()
+    ng = len(graphs)
+    nv = np.array([g.nv() for g in graphs], dtype=np.int32)
+    for g in graphs:
+        if g.is_directed():
+            raise ValueError("betweenness_centrality_batch takes undirected graphs")
+    rps, cols = zip(*[g.csr() for g in graphs]) if ng else ((), ())
+    row_off = np.zeros(ng, dtype=np.int64)
+    col_off = np.zeros(ng, dtype=np.int64)
+    bc_off = np.zeros(ng, dtype=np.int64)
+    if ng:
+        row_off[1:] = np.cumsum([len(r) for r in rps])[:-1]
+        col_off[1:] = np.cumsum([len(c) for c in cols])[:-1]
+        bc_off[1:] = np.cumsum(nv.astype(np.int64))[:-1]
+    rp_cat = np.ascontiguousarray(np.concatenate(rps) if ng else np.zeros(1), dtype=np.int32)
+    col_cat = np.ascontiguousarray(np.concatenate(cols) if ng and sum(len(c) for c in cols) else np.zeros(1),
+                                   dtype=np.int32)
+    out = np.zeros(max(1, int(nv.astype(np.int64).sum())), dtype=np.float64)
+    ms = np.zeros(1, dtype=np.float64)
+    _lib.check(lib.lgmi_betweenness_batch(ng, _lib.i32p(nv if ng else np.zeros(1, np.int32)), _lib.i64p(row_off),
+                                          _lib.i32p(rp_cat), _lib.i64p(col_off), _lib.i32p(col_cat),
+                                          _lib.i64p(bc_off), int(bool(normalize)), int(bool(endpoints)),
+                                          _lib.f64p(out), _lib.f64p(ms)))
+    res = [out[bc_off[i]:bc_off[i] + nv[i]].copy() for i in range(ng)]
+    return (res, float(ms[0])) if return_ms else res
+
+
+def batch_tuning(wave_max_n=-1, task_work_target=0.0):
+    _lib.check(_lib.load().lgmi_betweenness_batch_tuning(int(wave_max_n), float(task_work_target)))
+
+
 def last_stats(g: _GraphBase) -> dict:
     st = _lib.BcStats()
     _lib.check(_lib.load().lgmi_betweenness_stats(g.handle(), _lib.ctypes.byref(st)))

@@ -75,6 +75,21 @@ def grid(dims, T=np.int64) -> SimpleGraph:
     return SimpleGraph.from_edges(r * c, src[:got], dst[:got], T=T)
 
 
+def uneven_batch(count: int = 4096, nmin: int = 200, nmax: int = 20_000, seed: int = 1):
+    """The uneven small-graph batch: sizes log-uniform in [nmin, nmax],
+    even positions barabasi_albert(n, 4), odd ones watts_strogatz(n, 8, 0.1),
+    each seeded per graph from `seed`."""
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(nmin), np.log(nmax), count)).astype(np.int64)
+    out = []
+    for i, n in enumerate(sizes):
+        if i % 2 == 0:
+            out.append(barabasi_albert(int(n), 4, seed=seed * 1_000_003 + i))
+        else:
+            out.append(watts_strogatz(int(n), 8, 0.1, seed=seed * 1_000_003 + i))
+    return out
+
+
 def path_graph(n: int, T=np.int64) -> SimpleGraph:
     a = np.arange(max(n - 1, 0))
     return SimpleGraph.from_edges(n, a, a + 1, T=T)

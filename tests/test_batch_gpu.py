@@ -1,0 +1,75 @@
+"""All-source betweenness for a batch of uneven small graphs in one
+persistent launch, against the oracle (bar: 1e-12 relative)."""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def say(m):
+    print(m, flush=True)
+    sys.stdout.flush()
+
+
+def close(got, want, rtol=1e-12):
+    want = np.asarray(want, dtype=np.float64)
+    err = np.abs(got - want)
+    floor = 1e-15 * max(1.0, float(np.abs(want).max(initial=0.0)))
+    return bool(((err <= rtol * np.abs(want)) | (err <= floor)).all())
+
+
+def _oracle_all(oracle, g, normalize=True, endpoints=False):
+    rp, col = g.csr()
+    n = g.nv()
+    return oracle.betweenness_threaded(rp, col, rp, False, np.arange(n), normalize=normalize, endpoints=endpoints,
+                                       nthreads=8, index_quirk=False)
+
+
+@pytest.mark.parametrize("wave_max_n", [-1, 0, 400])
+def test_batch_matches_oracle_modes(oracle, wave_max_n):
+    """Wave mode, block mode and the automatic split, on a mixed batch that
+    includes isolated vertices, tiny graphs and disconnected pieces."""
+    import lgmi
+    from lgmi.centrality import batch_tuning
+    gen = lgmi.generators
+    graphs = gen.uneven_batch(24, 200, 3000, seed=5)
+    graphs += [gen.path_graph(1), gen.path_graph(2), gen.star_graph(3), gen.grid([7, 9]),
+               lgmi.SimpleGraph.from_edges(6, [0, 2], [1, 3]), gen.erdos_renyi(700, 900, seed=3)]
+    batch_tuning(wave_max_n, 0.0)
+    try:
+        say(f"wave_max_n={wave_max_n}: {len(graphs)} graphs")
+        got = lgmi.betweenness_centrality_batch(graphs)
+    finally:
+        batch_tuning(-1, 0.0)
+    for g, bc in zip(graphs, got):
+        assert close(bc, _oracle_all(oracle, g)), g
+
+
+def test_batch_options_and_task_split(oracle):
+    import lgmi
+    from lgmi.centrality import batch_tuning
+    gen = lgmi.generators
+    graphs = [gen.barabasi_albert(2500, 4, seed=1), gen.watts_strogatz(1800, 8, 0.1, seed=2),
+              lgmi.SimpleGraph.from_edges(5, [0, 3], [1, 4])]
+    batch_tuning(300, 1e5)     # force splitting into many source chunks
+    try:
+        for normalize in (True, False):
+            for endpoints in (False, True):
+                say(f"normalize={normalize} endpoints={endpoints}")
+                got = lgmi.betweenness_centrality_batch(graphs, normalize=normalize, endpoints=endpoints)
+                for g, bc in zip(graphs, got):
+                    assert close(bc, _oracle_all(oracle, g, normalize, endpoints))
+    finally:
+        batch_tuning(-1, 0.0)
+
+
+def test_batch_equals_single_graph_path_and_is_deterministic():
+    import lgmi
+    graphs = lgmi.generators.uneven_batch(8, 200, 20_000, seed=9)
+    a = lgmi.betweenness_centrality_batch(graphs)
+    b = lgmi.betweenness_centrality_batch(graphs)
+    for x, y, g in zip(a, b, graphs):
+        assert (x.view(np.uint64) == y.view(np.uint64)).all()
+        assert close(x, lgmi.betweenness_centrality(g))
