@@ -213,8 +213,10 @@ int lgmi_betweenness_batch(int32_t ngraphs, const int32_t* nv, const int64_t* ro
  * larger ones one source per work-group (-1: the largest n whose 16 sigma
  * slices fit in LDS, at most 1024); task_work_target is the n * (nnz + n)
  * work above which a graph's sources are split over several tasks
- * (0: total work / (6 * number of CUs)). */
-int lgmi_betweenness_batch_tuning(int32_t wave_max_n, double task_work_target);
+ * (0: total work / (6 * number of CUs)); graphs with more than lds_max_n
+ * vertices keep the per-vertex search state in global memory instead of LDS
+ * (-1: as many as LDS holds; values above that are clamped). */
+int lgmi_betweenness_batch_tuning(int32_t wave_max_n, double task_work_target, int32_t lds_max_n);
 
 /* Thread-local message of the last failed call ("" if none). */
 const char* lgmi_last_error(void);

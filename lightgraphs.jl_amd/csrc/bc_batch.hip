@@ -8,28 +8,39 @@
 // (src/shortestpaths/dijkstra.jl:59-141).
 //
 // Work decomposition: a task is (graph, contiguous source range); graphs
-// whose n * nnz exceeds the per-task target are split.  Tasks are ordered by
-// decreasing work and claimed by persistent 1024-thread work-groups from an
-// atomic counter.  A task runs in one of two modes:
+// whose n * (nnz + n) exceeds the per-task target are split.  Tasks are
+// ordered by decreasing work and claimed by persistent 1024-thread
+// work-groups from an atomic counter.  A task runs in one of two modes:
 //  * block mode: the whole work-group runs one single-source search at a
-//    time; sigma (FP64) lives in LDS when n fits (160 KB holds 20 000
-//    vertices), else in global scratch;
+//    time, with the per-vertex search state in LDS (160 KB holds 20 000
+//    vertices), or in global scratch for larger graphs;
 //  * wave mode (n <= wave_max_n, an autotuned threshold): each of the 16
 //    wavefronts runs its own source with its own LDS slice, synchronising
 //    only inside the wavefront.
-// Per search:
-//  forward, level by level: the frontier expands into unvisited neighbours,
-//    claimed with a CAS on a per-vertex level stamp (seq << 12 | level, so no
-//    array is cleared between sources); then every new vertex pulls sigma
-//    from its neighbours on the previous level (no atomics; integer-valued
-//    sums are exact in FP64);
-//  backward, level by level from the deepest: every vertex pulls
-//    sum (1 + delta[w]) / sigma[w] over its successors, forms delta, adds it
-//    to the task's partial betweenness and stores its own coefficient in the
-//    sigma slot.
+//
+// The search state is one FP64 word per vertex, read by sign:
+//    0   unvisited;
+//   > 0  on a finished level, holding sigma (the shortest-path count);
+//   < 0  forward: discovered on the level being built, holding -sigma;
+//        backward: finished, holding -(1 + delta) / sigma.
+// In an undirected graph a vertex on level l only has neighbours on levels
+// l-1, l and l+1, so the sign alone tells predecessors from successors.
+//
+// Forward, level by level: every frontier row is expanded (a group of kG
+// lanes per row, a whole wavefront for rows longer than kHeavy) and pushes
+// its sigma into each neighbour whose state is <= 0 with an LDS FP64
+// atomic add; the add that finds 0 appends the neighbour to the queue
+// (wavefront-aggregated counter).  Sigma is integer-valued, so the sums are
+// exact and independent of the order of the adds while sigma < 2^53.  After
+// a barrier the new level's states are negated to positive.
+// Backward, deepest level first: every vertex of level l sums the negative
+// states of its neighbours (exactly its successors' coefficients) in column
+// order per lane and a fixed butterfly across lanes, forms delta = sigma *
+// sum, adds it to the task's partial betweenness, and writes
+// -(1 + delta) / sigma after a barrier.
 // Partial arrays are per task (per wavefront in wave mode, folded in a fixed
 // order), and a second kernel adds the tasks of a graph in task order and
-// applies _rescale!, so results are deterministic.
+// applies _rescale!, so results do not depend on scheduling.
 #include "lgmi_internal.h"
 
 #include <algorithm>
@@ -44,8 +55,9 @@ namespace {
 
 constexpr int kBB = 1024;           // threads per work-group
 constexpr int kBW = kBB / 64;       // wavefronts per work-group
-constexpr int kLevelBits = 12;      // stamp = seq << 12 | level
-constexpr uint32_t kLevelMask = (1u << kLevelBits) - 1;
+constexpr int kG = 4;               // lanes per light row
+constexpr int kRows = 64 / kG;      // rows per wavefront pass
+constexpr int kHeavy = 32;          // rows longer than this use a whole wavefront
 
 struct BatchArgs {
     const int32_t* nv;        // [ngraphs]
@@ -59,18 +71,17 @@ struct BatchArgs {
     const int64_t* task_part; // offset of the task's partial array
     int32_t ntasks;
     int32_t wave_max_n;
-    int32_t lds_max_n;        // sigma fits in LDS for n <= this (block mode)
+    int32_t lds_max_n;        // the state fits in LDS for n <= this (block mode)
     int32_t max_n;
     double* part;             // task partial betweenness
     int32_t* task_ctr;
-    // per-block scratch (global)
-    uint32_t* stamp;          // [blocks][kBW][scr]
-    int32_t* queue;           // [blocks][kBW][scr]
-    int32_t* loff;            // [blocks][kBW][scr + 2]
+    // per-block scratch (global); wave mode slices it per wavefront
+    int32_t* queue;           // [blocks][scr]
+    double* coef;             // [blocks][scr] new backward states, by queue position
+    int32_t* loff;            // [blocks][scr + 2 * kBW] level offsets into the queue
     double* gsig;             // [blocks][max_n] (block mode without LDS room)
     double* wpart;            // [blocks][kBW][wave_max_n]
-    int64_t scr;              // per-wavefront scratch length (>= max_n / kBW and >= wave_max_n)
-    int32_t* err;             // set when a search exceeds the level range
+    int64_t scr;              // >= max(max_n, kBW * wave_max_n)
 };
 
 // Team = one wavefront (TEAM = 64) or the whole work-group (TEAM = kBB).
@@ -85,127 +96,168 @@ __device__ inline void team_sync() {
     }
 }
 
-// One single-source Brandes search by a team.  `t` = thread index in the
-// team.  qcount: LDS counter (block mode) or unused (wave mode uses ballot).
-template <int TEAM>
-__device__ void brandes_source(const int32_t* rp, const int32_t* col, int32_t n, int32_t s, uint32_t seq,
-                               double* sig, uint32_t* stamp, int32_t* queue, int32_t* loff, double* part,
-                               int* qcount, int t, int32_t* err) {
-    const uint32_t base = seq << kLevelBits;
+// State load.  In global scratch the states are also changed by atomics,
+// which are performed in L2, so the load bypasses the CU cache there.
+template <bool LDS>
+__device__ inline double state(const double* sig, int32_t v) {
+    if (LDS) return sig[v];
+    return __hip_atomic_load(sig + v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Push sigma su into neighbour v; true for the one add that discovers v.
+template <bool LDS>
+__device__ inline bool push_edge(double* sig, int32_t v, double su) {
+    if (state<LDS>(sig, v) <= 0.0) return atomicAdd(sig + v, -su) == 0.0;
+    return false;
+}
+
+// Wavefront-aggregated append of the lanes with `claimed` set.  Every lane
+// of the wavefront calls it.
+__device__ inline void append(bool claimed, int32_t v, int* counter, int32_t* queue, int lane) {
+    const uint64_t m = __ballot(claimed);
+    if (m) {
+        const int leader = __ffsll((unsigned long long)m) - 1;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(counter, __popcll(m));
+        base = __shfl(base, leader);
+        if (claimed) queue[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
+    }
+}
+
+template <int W>
+__device__ inline double lane_sum(double x) {
+#pragma unroll
+    for (int o = 1; o < W; o <<= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+// One single-source Brandes search by a team; `t` = thread index in the
+// team, sig = zeroed state array of the graph, counter = the team's LDS
+// queue counter.  Leaves sig zeroed.
+template <int TEAM, bool LDS>
+__device__ void brandes_source(const int32_t* __restrict__ rp, const int32_t* __restrict__ col, int32_t s,
+                               double* sig, int32_t* queue, int32_t* loff, double* coef, double* part,
+                               int* counter, int t) {
+    constexpr int NW = TEAM / 64;
+    const int lane = t & 63, w = t >> 6;
+    const int gl = lane & (kG - 1), grp = lane / kG;
     if (t == 0) {
-        stamp[s] = base;
         sig[s] = 1.0;
         queue[0] = s;
         loff[0] = 0;
         loff[1] = 1;
-        if (TEAM == kBB) *qcount = 1;
+        *counter = 1;
     }
     team_sync<TEAM>();
-    int32_t L = 0;
-    int32_t qn = 1;                  // queue length (team-uniform after sync)
+    int32_t L = 0, lo = 0, hi = 1;
     for (;;) {
-        const int32_t lo = loff[L], hi = qn;
-        if (lo == hi) break;
-        if (L + 1 >= (int32_t)kLevelMask) {
-            if (t == 0) atomicOr(err, 1);
-            break;
-        }
-        const uint32_t next = base | (uint32_t)(L + 1);
-        // discovery: claim unvisited neighbours of the frontier
-        int32_t my_new = 0;
-        if (TEAM == kBB) {
-            for (int32_t i = lo + t; i < hi; i += TEAM) {
-                const int32_t u = queue[i];
-                for (int32_t e = rp[u]; e < rp[u + 1]; e++) {
-                    const int32_t v = col[e];
-                    uint32_t old = stamp[v];
-                    if ((old >> kLevelBits) != seq) {
-                        if (__hip_atomic_compare_exchange_strong(&stamp[v], &old, next, __ATOMIC_RELAXED,
-                                                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP)) {
-                            const int pos = atomicAdd(qcount, 1);
-                            queue[pos] = v;
-                        }
-                    }
-                }
-            }
-            team_sync<TEAM>();
-            qn = *qcount;
-        } else {
-            // wave mode: ballot-aggregated appends, no shared counter
-            const int lane = t;
-            int32_t cnt = hi;
-            for (int32_t i0 = lo; i0 < hi; i0 += 64) {
-                const int32_t i = i0 + lane;
-                const int32_t u = i < hi ? queue[i] : -1;
-                const int32_t eb = u >= 0 ? rp[u] : 0, ee = u >= 0 ? rp[u + 1] : 0;
-                int32_t e = eb;
-                for (;;) {
-                    bool claimed = false;
-                    int32_t v = -1;
-                    if (e < ee) {
+        for (int32_t r0 = lo + w * kRows; r0 < hi; r0 += NW * kRows) {
+            const int32_t i = r0 + grp;
+            const bool valid = i < hi;
+            const int32_t u = valid ? queue[i] : 0;
+            int32_t b = valid ? rp[u] : 0;
+            int32_t e1 = valid ? rp[u + 1] : 0;
+            const double su = valid ? state<LDS>(sig, u) : 0.0;
+            const bool heavy = e1 - b > kHeavy;
+            uint64_t hm = __ballot(heavy && gl == 0);
+            while (hm) {
+                const int j = __ffsll((unsigned long long)hm) - 1;
+                hm &= hm - 1;
+                const int32_t hb = __shfl(b, j), he = __shfl(e1, j);
+                const double hs = __shfl(su, j);
+                for (int32_t e0 = hb; e0 < he; e0 += 64) {
+                    const int32_t e = e0 + lane;
+                    bool c = false;
+                    int32_t v = 0;
+                    if (e < he) {
                         v = col[e];
-                        uint32_t old = stamp[v];
-                        if ((old >> kLevelBits) != seq)
-                            claimed = __hip_atomic_compare_exchange_strong(&stamp[v], &old, next, __ATOMIC_RELAXED,
-                                                                           __ATOMIC_RELAXED,
-                                                                           __HIP_MEMORY_SCOPE_WAVEFRONT);
-                        e++;
+                        c = push_edge<LDS>(sig, v, hs);
                     }
-                    const uint64_t m = __ballot(claimed);
-                    if (m) {
-                        if (claimed) queue[cnt + __popcll(m & ((1ull << lane) - 1ull))] = v;
-                        cnt += __popcll(m);
-                    }
-                    if (__ballot(e < ee) == 0) break;
+                    append(c, v, counter, queue, lane);
                 }
             }
-            qn = cnt;
-            team_sync<TEAM>();
+            if (heavy) e1 = b;
+            for (int32_t e = b + gl; __ballot(e < e1); e += kG) {
+                bool c = false;
+                int32_t v = 0;
+                if (e < e1) {
+                    v = col[e];
+                    c = push_edge<LDS>(sig, v, su);
+                }
+                append(c, v, counter, queue, lane);
+            }
         }
-        (void)my_new;
-        // sigma of the new level: pull from neighbours on level L
-        const uint32_t cur = base | (uint32_t)L;
+        team_sync<TEAM>();
+        const int32_t qn = *counter;
+        if (qn == hi) break;
         for (int32_t i = hi + t; i < qn; i += TEAM) {
             const int32_t v = queue[i];
-            double sv = 0.0;
-            for (int32_t e = rp[v]; e < rp[v + 1]; e++) {
-                const int32_t u = col[e];
-                if (stamp[u] == cur) sv += sig[u];
-            }
-            sig[v] = sv;
+            sig[v] = -state<LDS>(sig, v);
         }
         if (t == 0) loff[L + 2] = qn;
         L++;
+        lo = hi;
+        hi = qn;
         team_sync<TEAM>();
     }
     // backward, deepest level first; level 0 (the source) adds nothing
     for (int32_t l = L; l >= 1; l--) {
-        const int32_t lo = loff[l], hi = loff[l + 1];
-        const uint32_t succ = base | (uint32_t)(l + 1);
-        for (int32_t i = lo + t; i < hi; i += TEAM) {
-            const int32_t v = queue[i];
-            double acc = 0.0;
-            for (int32_t e = rp[v]; e < rp[v + 1]; e++) {
-                const int32_t w = col[e];
-                if (stamp[w] == succ) acc += sig[w];
+        const int32_t a0 = loff[l], a1 = loff[l + 1];
+        for (int32_t r0 = a0 + w * kRows; r0 < a1; r0 += NW * kRows) {
+            const int32_t i = r0 + grp;
+            const bool valid = i < a1;
+            const int32_t u = valid ? queue[i] : 0;
+            int32_t b = valid ? rp[u] : 0;
+            int32_t e1 = valid ? rp[u + 1] : 0;
+            const bool heavy = e1 - b > kHeavy;
+            uint64_t hm = __ballot(heavy && gl == 0);
+            while (hm) {
+                const int j = __ffsll((unsigned long long)hm) - 1;
+                hm &= hm - 1;
+                const int32_t hb = __shfl(b, j), he = __shfl(e1, j);
+                double acc = 0.0;
+                for (int32_t e = hb + lane; e < he; e += 64) {
+                    const double c = state<LDS>(sig, col[e]);
+                    if (c < 0.0) acc -= c;
+                }
+                acc = lane_sum<64>(acc);
+                if (lane == j) {
+                    const double su = state<LDS>(sig, u);
+                    const double d = su * acc;
+                    part[u] += d;
+                    coef[i] = (1.0 + d) / su;
+                }
             }
-            const double sv = sig[v];
-            const double d = sv * acc;
-            part[v] += d;
-            sig[v] = (1.0 + d) / sv;
+            if (heavy) e1 = b;
+            double acc = 0.0;
+            for (int32_t e = b + gl; e < e1; e += kG) {
+                const double c = state<LDS>(sig, col[e]);
+                if (c < 0.0) acc -= c;
+            }
+            acc = lane_sum<kG>(acc);
+            if (valid && !heavy && gl == 0) {
+                const double su = state<LDS>(sig, u);
+                const double d = su * acc;
+                part[u] += d;
+                coef[i] = (1.0 + d) / su;
+            }
         }
         team_sync<TEAM>();
+        for (int32_t i = a0 + t; i < a1; i += TEAM) sig[queue[i]] = -coef[i];
+        team_sync<TEAM>();
     }
+    for (int32_t i = t; i < hi; i += TEAM) sig[queue[i]] = 0.0;
+    team_sync<TEAM>();
 }
 
 __global__ __launch_bounds__(kBB) void bc_batch_kernel(BatchArgs a) {
     extern __shared__ double s_sig[];
     __shared__ int s_task;
-    __shared__ int s_qcount;
+    __shared__ int s_qc[kBW];
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    uint32_t* stamp_b = a.stamp + (size_t)blockIdx.x * kBW * a.scr;
-    int32_t* queue_b = a.queue + (size_t)blockIdx.x * kBW * a.scr;
-    int32_t* loff_b = a.loff + (size_t)blockIdx.x * kBW * (a.scr + 2);
+    int32_t* queue_b = a.queue + (size_t)blockIdx.x * a.scr;
+    double* coef_b = a.coef + (size_t)blockIdx.x * a.scr;
+    int32_t* loff_b = a.loff + (size_t)blockIdx.x * (a.scr + 2 * kBW);
     for (;;) {
         if (tid == 0) s_task = atomicAdd(a.task_ctr, 1);
         __syncthreads();
@@ -220,46 +272,58 @@ __global__ __launch_bounds__(kBB) void bc_batch_kernel(BatchArgs a) {
         double* part = a.part + a.task_part[task];
         if (n <= a.wave_max_n) {
             // ---------------- wave mode ----------------
-            double* sig = s_sig + (size_t)wid * a.wave_max_n;
-            uint32_t* stamp = stamp_b + (size_t)wid * a.scr;
-            int32_t* queue = queue_b + (size_t)wid * a.scr;
-            int32_t* loff = loff_b + (size_t)wid * (a.scr + 2);
-            double* wp = a.wpart + ((size_t)blockIdx.x * kBW + wid) * a.wave_max_n;
-            for (int32_t v = lane; v < n; v += 64) { stamp[v] = 0; wp[v] = 0.0; }
-            team_sync<64>();
-            uint32_t seq = 0;
-            for (int32_t s = s0 + wid; s < s1; s += kBW) {
-                seq++;
-                brandes_source<64>(rp, col, n, s, seq, sig, stamp, queue, loff, wp, nullptr, lane, a.err);
+            const int32_t wn = a.wave_max_n;
+            double* sig = s_sig + (size_t)wid * wn;
+            int32_t* queue = queue_b + (size_t)wid * wn;
+            double* coef = coef_b + (size_t)wid * wn;
+            int32_t* loff = loff_b + (size_t)wid * (wn + 2);
+            double* wp = a.wpart + ((size_t)blockIdx.x * kBW + wid) * wn;
+            for (int32_t v = lane; v < n; v += 64) {
+                sig[v] = 0.0;
+                wp[v] = 0.0;
             }
+            team_sync<64>();
+            for (int32_t s = s0 + wid; s < s1; s += kBW)
+                brandes_source<64, true>(rp, col, s, sig, queue, loff, coef, wp, &s_qc[wid], lane);
             __syncthreads();
             // fold the wavefront partials into the task partial, fixed order
             for (int32_t v = tid; v < n; v += kBB) {
                 double acc = 0.0;
-                for (int q = 0; q < kBW; q++) acc += a.wpart[((size_t)blockIdx.x * kBW + q) * a.wave_max_n + v];
+                for (int q = 0; q < kBW; q++) acc += a.wpart[((size_t)blockIdx.x * kBW + q) * wn + v];
                 part[v] = acc;
             }
             __syncthreads();
-        } else {
-            // ---------------- block mode ----------------
-            double* sig = n <= a.lds_max_n ? s_sig : a.gsig + (size_t)blockIdx.x * a.max_n;
-            for (int32_t v = tid; v < n; v += kBB) { stamp_b[v] = 0; part[v] = 0.0; }
-            __syncthreads();
-            uint32_t seq = 0;
-            for (int32_t s = s0; s < s1; s++) {
-                seq++;
-                brandes_source<kBB>(rp, col, n, s, seq, sig, stamp_b, queue_b, loff_b, part, &s_qcount, tid, a.err);
+        } else if (n <= a.lds_max_n) {
+            // ---------------- block mode, state in LDS ----------------
+            for (int32_t v = tid; v < n; v += kBB) {
+                s_sig[v] = 0.0;
+                part[v] = 0.0;
             }
+            __syncthreads();
+            for (int32_t s = s0; s < s1; s++)
+                brandes_source<kBB, true>(rp, col, s, s_sig, queue_b, loff_b, coef_b, part, &s_qc[0], tid);
+            __syncthreads();
+        } else {
+            // ---------------- block mode, state in global scratch ----------------
+            double* gs = a.gsig + (size_t)blockIdx.x * a.max_n;
+            for (int32_t v = tid; v < n; v += kBB) {
+                gs[v] = 0.0;
+                part[v] = 0.0;
+            }
+            __syncthreads();
+            for (int32_t s = s0; s < s1; s++)
+                brandes_source<kBB, false>(rp, col, s, gs, queue_b, loff_b, coef_b, part, &s_qc[0], tid);
             __syncthreads();
         }
     }
 }
 
-// bc of every graph: sum of its tasks' partials in task order, endpoints
-// term, _rescale! scale.  One thread per output vertex.
+// bc of every graph: sum of its tasks' partials in task order times the
+// _rescale! scale.  One thread per output slot; slots in gaps between the
+// graphs' ranges are left alone.
 __global__ void bc_batch_finish(int32_t ngraphs, const int32_t* nv, const int64_t* bc_off, const int32_t* gtask0,
                                 const int32_t* gtask_n, const int32_t* gtasks, const int64_t* task_part,
-                                const double* part, const double* scale, int endpoints, double* out, int64_t total) {
+                                const double* part, const double* scale, double* out, int64_t total) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
         // find the graph of output i (binary search over bc_off)
         int lo = 0, hi = ngraphs - 1;
@@ -269,13 +333,9 @@ __global__ void bc_batch_finish(int32_t ngraphs, const int32_t* nv, const int64_
         }
         const int g = lo;
         const int64_t v = i - bc_off[g];
+        if (v >= nv[g]) continue;
         double s = 0.0;
         for (int k = 0; k < gtask_n[g]; k++) s += part[task_part[gtasks[gtask0[g] + k]] + v];
-        if (endpoints) {
-            // every vertex of a graph with an edge is a processed source:
-            // it adds n-1 to itself and 1 to each other vertex
-            s += 0.0;
-        }
         out[i] = s * scale[g];
     }
 }
@@ -283,6 +343,7 @@ __global__ void bc_batch_finish(int32_t ngraphs, const int32_t* nv, const int64_
 struct BatchTuning {
     int32_t wave_max_n = -1;        // -1: pick from LDS capacity
     double task_work_target = 0.0;  // 0: automatic
+    int32_t lds_max_n = -1;         // -1: LDS capacity
 };
 
 BatchTuning& tuning() {
@@ -295,11 +356,13 @@ BatchTuning& tuning() {
 
 using namespace lgmi;
 
-extern "C" int lgmi_betweenness_batch_tuning(int32_t wave_max_n, double task_work_target) {
+extern "C" int lgmi_betweenness_batch_tuning(int32_t wave_max_n, double task_work_target, int32_t lds_max_n) {
     try {
-        if (wave_max_n < -1 || task_work_target < 0) throw Error(LGMI_E_ARG, "bad tuning values");
+        if (wave_max_n < -1 || !(task_work_target >= 0) || lds_max_n < -1)
+            throw Error(LGMI_E_ARG, "bad tuning values");
         tuning().wave_max_n = wave_max_n;
         tuning().task_work_target = task_work_target;
+        tuning().lds_max_n = lds_max_n;
         return LGMI_OK;
     } catch (const Error& e) {
         set_last_error(e.what());
@@ -347,10 +410,11 @@ extern "C" int lgmi_betweenness_batch(int32_t ngraphs, const int32_t* nv, const 
         LGMI_HIP(hipDeviceGetAttribute(&lds_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
         const size_t static_lds = 64;
         const size_t dyn_lds = (size_t)std::max(0, lds_block - 1024);
-        const int32_t lds_max_n = (int32_t)(dyn_lds / sizeof(double));
+        const int32_t lds_cap = (int32_t)(dyn_lds / sizeof(double));
+        const int32_t lds_max_n = tuning().lds_max_n >= 0 ? std::min(tuning().lds_max_n, lds_cap) : lds_cap;
         int32_t wave_max_n = tuning().wave_max_n >= 0 ? tuning().wave_max_n : (int32_t)std::min<int64_t>(
-            lds_max_n / kBW, 1024);
-        wave_max_n = std::min<int32_t>(wave_max_n, lds_max_n / kBW);
+            lds_cap / kBW, 1024);
+        wave_max_n = std::min<int32_t>(wave_max_n, lds_cap / kBW);
         (void)static_lds;
 
         // ---- tasks: split graphs by work, largest first ----
@@ -450,15 +514,13 @@ extern "C" int lgmi_betweenness_batch(int32_t ngraphs, const int32_t* nv, const 
             a.lds_max_n = lds_max_n;
             a.max_n = max_n;
             a.part = (double*)dalloc(sizeof(double) * (size_t)std::max<int64_t>(part_total, 1));
-            a.task_ctr = (int32_t*)dalloc(sizeof(int32_t) * 2);
-            a.err = a.task_ctr + 1;
-            LGMI_HIP(hipMemsetAsync(a.task_ctr, 0, sizeof(int32_t) * 2, st));
-            a.scr = std::max<int64_t>(max_n, 1);
-            // block mode uses wavefront 0's slice for the whole block
-            a.stamp = (uint32_t*)dalloc(sizeof(uint32_t) * (size_t)nblocks * kBW * a.scr);
-            a.queue = (int32_t*)dalloc(sizeof(int32_t) * (size_t)nblocks * kBW * a.scr);
-            a.loff = (int32_t*)dalloc(sizeof(int32_t) * (size_t)nblocks * kBW * (a.scr + 2));
-            a.gsig = (double*)dalloc(sizeof(double) * (size_t)nblocks * std::max(max_n, 1));
+            a.task_ctr = (int32_t*)dalloc(sizeof(int32_t));
+            LGMI_HIP(hipMemsetAsync(a.task_ctr, 0, sizeof(int32_t), st));
+            a.scr = std::max<int64_t>({(int64_t)max_n, (int64_t)kBW * wave_max_n, 1});
+            a.queue = (int32_t*)dalloc(sizeof(int32_t) * (size_t)nblocks * a.scr);
+            a.coef = (double*)dalloc(sizeof(double) * (size_t)nblocks * a.scr);
+            a.loff = (int32_t*)dalloc(sizeof(int32_t) * (size_t)nblocks * (a.scr + 2 * kBW));
+            a.gsig = (double*)dalloc(sizeof(double) * (size_t)nblocks * (max_n > lds_max_n ? max_n : 1));
             a.wpart = (double*)dalloc(sizeof(double) * (size_t)nblocks * kBW * std::max(wave_max_n, 1));
             double* d_out = (double*)dalloc(sizeof(double) * (size_t)std::max<int64_t>(out_total, 1));
             const int64_t* d_bcoff = (const int64_t*)up(bc_off, sizeof(int64_t) * ngraphs);
@@ -476,15 +538,16 @@ extern "C" int lgmi_betweenness_batch(int32_t ngraphs, const int32_t* nv, const 
             LGMI_HIP(hipGetLastError());
             int32_t total_out = (int32_t)std::min<int64_t>(out_total, 1 << 30);
             bc_batch_finish<<<std::max(1, std::min(cu * 8, (int)((out_total + 255) / 256))), 256, 0, st>>>(
-                ngraphs, a.nv, d_bcoff, d_gt0, d_gtn, d_gts, a.task_part, a.part, d_scale, 0, d_out, out_total);
+                ngraphs, a.nv, d_bcoff, d_gt0, d_gtn, d_gts, a.task_part, a.part, d_scale, d_out, out_total);
             (void)total_out;
             LGMI_HIP(hipGetLastError());
             LGMI_HIP(hipEventRecord(e1, st));
-            int32_t herr = 0;
-            LGMI_HIP(hipMemcpyAsync(&herr, a.err, sizeof(int32_t), hipMemcpyDeviceToHost, st));
-            LGMI_HIP(hipMemcpyAsync(bc_cat, d_out, sizeof(double) * (size_t)out_total, hipMemcpyDeviceToHost, st));
+            std::vector<double> hout((size_t)out_total);
+            LGMI_HIP(hipMemcpyAsync(hout.data(), d_out, sizeof(double) * (size_t)out_total, hipMemcpyDeviceToHost,
+                                    st));
             LGMI_HIP(hipStreamSynchronize(st));
-            if (herr) throw Error(LGMI_E_HIP, "a search exceeded 4095 levels");
+            for (int32_t g = 0; g < ngraphs; g++)
+                std::copy(hout.begin() + bc_off[g], hout.begin() + bc_off[g] + nv[g], bc_cat + bc_off[g]);
             float ms = 0.f;
             LGMI_HIP(hipEventElapsedTime(&ms, e0, e1));
             if (device_ms) *device_ms = ms;

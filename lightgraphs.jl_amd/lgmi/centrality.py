@@ -97,8 +97,14 @@ def betweenness_centrality_batch(graphs, normalize=True, endpoints=False, return
     return (res, float(ms[0])) if return_ms else res
 
 
-def batch_tuning(wave_max_n=-1, task_work_target=0.0):
-    _lib.check(_lib.load().lgmi_betweenness_batch_tuning(int(wave_max_n), float(task_work_target)))
+def batch_tuning(wave_max_n=-1, task_work_target=0.0, lds_max_n=-1):
+    """Launch thresholds of betweenness_centrality_batch (process-wide):
+    wave_max_n -- graphs up to this size run one source per wavefront;
+    task_work_target -- n * (nnz + n) above which a graph's sources are split;
+    lds_max_n -- graphs above this size keep their search state in global
+    memory.  wave_max_n = -1, task_work_target = 0 and lds_max_n = -1
+    select the defaults."""
+    _lib.check(_lib.load().lgmi_betweenness_batch_tuning(int(wave_max_n), float(task_work_target), int(lds_max_n)))
 
 
 def last_stats(g: _GraphBase) -> dict:

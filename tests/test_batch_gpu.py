@@ -27,22 +27,23 @@ def _oracle_all(oracle, g, normalize=True, endpoints=False):
                                        nthreads=8, index_quirk=False)
 
 
-@pytest.mark.parametrize("wave_max_n", [-1, 0, 400])
-def test_batch_matches_oracle_modes(oracle, wave_max_n):
-    """Wave mode, block mode and the automatic split, on a mixed batch that
-    includes isolated vertices, tiny graphs and disconnected pieces."""
+@pytest.mark.parametrize("wave_max_n,lds_max_n", [(-1, -1), (0, -1), (400, -1), (0, 0), (100, 1500)])
+def test_batch_matches_oracle_modes(oracle, wave_max_n, lds_max_n):
+    """Wave mode, block mode with the search state in LDS or in global
+    memory, and the automatic split, on a mixed batch that includes isolated
+    vertices, tiny graphs and disconnected pieces."""
     import lgmi
     from lgmi.centrality import batch_tuning
     gen = lgmi.generators
     graphs = gen.uneven_batch(24, 200, 3000, seed=5)
     graphs += [gen.path_graph(1), gen.path_graph(2), gen.star_graph(3), gen.grid([7, 9]),
                lgmi.SimpleGraph.from_edges(6, [0, 2], [1, 3]), gen.erdos_renyi(700, 900, seed=3)]
-    batch_tuning(wave_max_n, 0.0)
+    batch_tuning(wave_max_n, 0.0, lds_max_n)
     try:
-        say(f"wave_max_n={wave_max_n}: {len(graphs)} graphs")
+        say(f"wave_max_n={wave_max_n} lds_max_n={lds_max_n}: {len(graphs)} graphs")
         got = lgmi.betweenness_centrality_batch(graphs)
     finally:
-        batch_tuning(-1, 0.0)
+        batch_tuning(-1, 0.0, -1)
     for g, bc in zip(graphs, got):
         assert close(bc, _oracle_all(oracle, g)), g
 
@@ -73,3 +74,26 @@ def test_batch_equals_single_graph_path_and_is_deterministic():
     for x, y, g in zip(a, b, graphs):
         assert (x.view(np.uint64) == y.view(np.uint64)).all()
         assert close(x, lgmi.betweenness_centrality(g))
+
+
+def test_batch_raw_abi_gaps_untouched(oracle):
+    """Result ranges may leave gaps in bc_cat; the call writes only the
+    graphs' own ranges."""
+    import lgmi
+    from lgmi import _lib
+    gen = lgmi.generators
+    graphs = [gen.barabasi_albert(300, 3, seed=4), gen.cycle_graph(9)]
+    nv = np.array([g.nv() for g in graphs], dtype=np.int32)
+    rps, cols = zip(*[g.csr() for g in graphs])
+    row_off = np.array([0, len(rps[0])], dtype=np.int64)
+    col_off = np.array([0, len(cols[0])], dtype=np.int64)
+    bc_off = np.array([5, 5 + 300 + 7], dtype=np.int64)
+    out = np.full(5 + 300 + 7 + 9 + 3, -7.0)
+    rc = _lib.load().lgmi_betweenness_batch(2, _lib.i32p(nv), _lib.i64p(row_off),
+                                            _lib.i32p(np.concatenate(rps).astype(np.int32)), _lib.i64p(col_off),
+                                            _lib.i32p(np.concatenate(cols).astype(np.int32)), _lib.i64p(bc_off),
+                                            1, 0, _lib.f64p(out), None)
+    assert rc == 0, _lib.load().lgmi_last_error()
+    assert (out[:5] == -7.0).all() and (out[305:312] == -7.0).all() and (out[321:] == -7.0).all()
+    assert close(out[5:305], _oracle_all(oracle, graphs[0]))
+    assert close(out[312:321], _oracle_all(oracle, graphs[1]))

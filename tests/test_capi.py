@@ -79,6 +79,16 @@ def test_policy_argument_checks():
     assert lib.lgmi_bfs_set_policy(0, 15.0, 18.0, None, 0) == _lib.LGMI_OK
 
 
+def test_batch_tuning_argument_checks():
+    from lgmi import _lib
+    lib = _lib.load()
+    assert lib.lgmi_betweenness_batch_tuning(-2, 0.0, -1) == _lib.LGMI_E_ARG
+    assert lib.lgmi_betweenness_batch_tuning(-1, -1.0, -1) == _lib.LGMI_E_ARG
+    assert lib.lgmi_betweenness_batch_tuning(-1, float("nan"), -1) == _lib.LGMI_E_ARG
+    assert lib.lgmi_betweenness_batch_tuning(-1, 0.0, -5) == _lib.LGMI_E_ARG
+    assert lib.lgmi_betweenness_batch_tuning(-1, 0.0, -1) == _lib.LGMI_OK
+
+
 def test_generators_shapes_and_determinism():
     import lgmi.generators as gen
     s1, d1 = gen.kronecker_edges(10, 16, seed=3)
