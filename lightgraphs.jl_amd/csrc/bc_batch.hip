@@ -26,8 +26,9 @@
 // In an undirected graph a vertex on level l only has neighbours on levels
 // l-1, l and l+1, so the sign alone tells predecessors from successors.
 //
-// Forward, level by level: every frontier row is expanded (a group of kG
-// lanes per row, a whole wavefront for rows longer than kHeavy) and pushes
+// Forward, level by level: every frontier row is expanded (a group of KG
+// lanes per row, KG = 4, 8 or 16 as tuned; a whole wavefront for rows
+// longer than 8 * KG) and pushes
 // its sigma into each neighbour whose state is <= 0 with an LDS FP64
 // atomic add; the add that finds 0 appends the neighbour to the queue
 // (wavefront-aggregated counter).  Sigma is integer-valued, so the sums are
@@ -55,9 +56,6 @@ namespace {
 
 constexpr int kBB = 1024;           // threads per work-group
 constexpr int kBW = kBB / 64;       // wavefronts per work-group
-constexpr int kG = 4;               // lanes per light row
-constexpr int kRows = 64 / kG;      // rows per wavefront pass
-constexpr int kHeavy = 32;          // rows longer than this use a whole wavefront
 
 struct BatchArgs {
     const int32_t* nv;        // [ngraphs]
@@ -111,16 +109,27 @@ __device__ inline bool push_edge(double* sig, int32_t v, double su) {
     return false;
 }
 
-// Wavefront-aggregated append of the lanes with `claimed` set.  Every lane
-// of the wavefront calls it.
-__device__ inline void append(bool claimed, int32_t v, int* counter, int32_t* queue, int lane) {
-    const uint64_t m = __ballot(claimed);
-    if (m) {
-        const int leader = __ffsll((unsigned long long)m) - 1;
+// Wavefront-aggregated queue append of the lanes' claimed candidates (SL
+// slots per lane): one counter add per wavefront.  Every lane calls it.
+template <int SL>
+__device__ inline void append_n(const bool* c, const int32_t* v, int* counter, int32_t* queue, int lane) {
+    uint64_t m[SL];
+    int tot = 0;
+#pragma unroll
+    for (int k = 0; k < SL; k++) {
+        m[k] = __ballot(c[k]);
+        tot += __popcll(m[k]);
+    }
+    if (tot) {
         int base = 0;
-        if (lane == leader) base = atomicAdd(counter, __popcll(m));
-        base = __shfl(base, leader);
-        if (claimed) queue[base + __popcll(m & ((1ull << lane) - 1ull))] = v;
+        if (lane == 0) base = atomicAdd(counter, tot);
+        base = __shfl(base, 0);
+        const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+        for (int k = 0; k < SL; k++) {
+            if (c[k]) queue[base + __popcll(m[k] & lt)] = v[k];
+            base += __popcll(m[k]);
+        }
     }
 }
 
@@ -131,16 +140,37 @@ __device__ inline double lane_sum(double x) {
     return x;
 }
 
+// Rows of one pass are taken kRows = 64 / KG at a time per wavefront.  The
+// loads are software-pipelined over three row blocks: while block k is
+// expanded, the row-pointer pair (and, backward, the partial betweenness)
+// of block k+1 and the queue entries of block k+2 are already in flight, so
+// one memory round trip is exposed per block instead of three.
+struct RowBlock {
+    int32_t u;       // vertex (-1: none)
+    int32_t b, e1;   // adjacency range
+    double pu;       // backward: part[u] before this source's delta
+};
+
+template <bool BWD>
+__device__ inline void row_fetch(RowBlock& rb, const int32_t* __restrict__ rp, const double* part) {
+    rb.b = rb.u >= 0 ? rp[rb.u] : 0;
+    rb.e1 = rb.u >= 0 ? rp[rb.u + 1] : 0;
+    if (BWD) rb.pu = rb.u >= 0 ? part[rb.u] : 0.0;
+}
+
 // One single-source Brandes search by a team; `t` = thread index in the
 // team, sig = zeroed state array of the graph, counter = the team's LDS
 // queue counter.  Leaves sig zeroed.
-template <int TEAM, bool LDS>
+template <int TEAM, bool LDS, int KG, int SL>
 __device__ void brandes_source(const int32_t* __restrict__ rp, const int32_t* __restrict__ col, int32_t s,
                                double* sig, int32_t* queue, int32_t* loff, double* coef, double* part,
                                int* counter, int t) {
     constexpr int NW = TEAM / 64;
+    constexpr int kRows = 64 / KG;
+    constexpr int kHeavy = 2 * KG * SL; // longer rows use a whole wavefront
     const int lane = t & 63, w = t >> 6;
-    const int gl = lane & (kG - 1), grp = lane / kG;
+    const int gl = lane & (KG - 1), grp = lane / KG;
+    const int32_t step = NW * kRows;
     if (t == 0) {
         sig[s] = 1.0;
         queue[0] = s;
@@ -151,40 +181,47 @@ __device__ void brandes_source(const int32_t* __restrict__ rp, const int32_t* __
     team_sync<TEAM>();
     int32_t L = 0, lo = 0, hi = 1;
     for (;;) {
-        for (int32_t r0 = lo + w * kRows; r0 < hi; r0 += NW * kRows) {
-            const int32_t i = r0 + grp;
-            const bool valid = i < hi;
-            const int32_t u = valid ? queue[i] : 0;
-            int32_t b = valid ? rp[u] : 0;
-            int32_t e1 = valid ? rp[u + 1] : 0;
-            const double su = valid ? state<LDS>(sig, u) : 0.0;
-            const bool heavy = e1 - b > kHeavy;
-            uint64_t hm = __ballot(heavy && gl == 0);
-            while (hm) {
-                const int j = __ffsll((unsigned long long)hm) - 1;
-                hm &= hm - 1;
-                const int32_t hb = __shfl(b, j), he = __shfl(e1, j);
-                const double hs = __shfl(su, j);
-                for (int32_t e0 = hb; e0 < he; e0 += 64) {
-                    const int32_t e = e0 + lane;
-                    bool c = false;
-                    int32_t v = 0;
-                    if (e < he) {
-                        v = col[e];
-                        c = push_edge<LDS>(sig, v, hs);
+        {
+            int32_t r0 = lo + w * kRows;
+            RowBlock cur, nxt;
+            cur.u = r0 + grp < hi ? queue[r0 + grp] : -1;
+            nxt.u = r0 + step + grp < hi ? queue[r0 + step + grp] : -1;
+            row_fetch<false>(cur, rp, part);
+            for (; r0 < hi; r0 += step) {
+                const int32_t i2 = r0 + 2 * step + grp;
+                const int32_t u2 = i2 < hi ? queue[i2] : -1;
+                row_fetch<false>(nxt, rp, part);
+                const int32_t u = cur.u;
+                int32_t b = cur.b, e1 = cur.e1;
+                const double su = u >= 0 ? state<LDS>(sig, u) : 0.0;
+                const bool heavy = e1 - b > kHeavy;
+                uint64_t hm = __ballot(heavy && gl == 0);
+                while (hm) {
+                    const int j = __ffsll((unsigned long long)hm) - 1;
+                    hm &= hm - 1;
+                    const int32_t hb = __shfl(b, j), he = __shfl(e1, j);
+                    const double hs = __shfl(su, j);
+                    for (int32_t e = hb + lane; __ballot(e < he); e += 128) {
+                        const bool a0 = e < he, a1 = e + 64 < he;
+                        const int32_t v0 = a0 ? col[e] : 0;
+                        const int32_t v1 = a1 ? col[e + 64] : 0;
+                        const bool cc[2] = {a0 && push_edge<LDS>(sig, v0, hs), a1 && push_edge<LDS>(sig, v1, hs)};
+                        const int32_t vv[2] = {v0, v1};
+                        append_n<2>(cc, vv, counter, queue, lane);
                     }
-                    append(c, v, counter, queue, lane);
                 }
-            }
-            if (heavy) e1 = b;
-            for (int32_t e = b + gl; __ballot(e < e1); e += kG) {
-                bool c = false;
-                int32_t v = 0;
-                if (e < e1) {
-                    v = col[e];
-                    c = push_edge<LDS>(sig, v, su);
+                if (heavy) e1 = b;
+                for (int32_t e = b + gl; __ballot(e < e1); e += SL * KG) {
+                    int32_t vv[SL];
+                    bool cc[SL];
+#pragma unroll
+                    for (int k = 0; k < SL; k++) vv[k] = e + k * KG < e1 ? col[e + k * KG] : 0;
+#pragma unroll
+                    for (int k = 0; k < SL; k++) cc[k] = e + k * KG < e1 && push_edge<LDS>(sig, vv[k], su);
+                    append_n<SL>(cc, vv, counter, queue, lane);
                 }
-                append(c, v, counter, queue, lane);
+                cur = nxt;
+                nxt.u = u2;
             }
         }
         team_sync<TEAM>();
@@ -203,12 +240,18 @@ __device__ void brandes_source(const int32_t* __restrict__ rp, const int32_t* __
     // backward, deepest level first; level 0 (the source) adds nothing
     for (int32_t l = L; l >= 1; l--) {
         const int32_t a0 = loff[l], a1 = loff[l + 1];
-        for (int32_t r0 = a0 + w * kRows; r0 < a1; r0 += NW * kRows) {
+        int32_t r0 = a0 + w * kRows;
+        RowBlock cur, nxt;
+        cur.u = r0 + grp < a1 ? queue[r0 + grp] : -1;
+        nxt.u = r0 + step + grp < a1 ? queue[r0 + step + grp] : -1;
+        row_fetch<true>(cur, rp, part);
+        for (; r0 < a1; r0 += step) {
+            const int32_t i2 = r0 + 2 * step + grp;
+            const int32_t u2 = i2 < a1 ? queue[i2] : -1;
+            row_fetch<true>(nxt, rp, part);
             const int32_t i = r0 + grp;
-            const bool valid = i < a1;
-            const int32_t u = valid ? queue[i] : 0;
-            int32_t b = valid ? rp[u] : 0;
-            int32_t e1 = valid ? rp[u + 1] : 0;
+            const int32_t u = cur.u;
+            int32_t b = cur.b, e1 = cur.e1;
             const bool heavy = e1 - b > kHeavy;
             uint64_t hm = __ballot(heavy && gl == 0);
             while (hm) {
@@ -216,31 +259,42 @@ __device__ void brandes_source(const int32_t* __restrict__ rp, const int32_t* __
                 hm &= hm - 1;
                 const int32_t hb = __shfl(b, j), he = __shfl(e1, j);
                 double acc = 0.0;
-                for (int32_t e = hb + lane; e < he; e += 64) {
-                    const double c = state<LDS>(sig, col[e]);
-                    if (c < 0.0) acc -= c;
+                for (int32_t e = hb + lane; e < he; e += 128) {
+                    const int32_t w1 = e + 64 < he ? col[e + 64] : -1;
+                    const double c0 = state<LDS>(sig, col[e]);
+                    const double c1 = w1 >= 0 ? state<LDS>(sig, w1) : 0.0;
+                    if (c0 < 0.0) acc -= c0;
+                    if (c1 < 0.0) acc -= c1;
                 }
                 acc = lane_sum<64>(acc);
                 if (lane == j) {
                     const double su = state<LDS>(sig, u);
                     const double d = su * acc;
-                    part[u] += d;
+                    part[u] = cur.pu + d;
                     coef[i] = (1.0 + d) / su;
                 }
             }
             if (heavy) e1 = b;
             double acc = 0.0;
-            for (int32_t e = b + gl; e < e1; e += kG) {
-                const double c = state<LDS>(sig, col[e]);
-                if (c < 0.0) acc -= c;
+            for (int32_t e = b + gl; e < e1; e += SL * KG) {
+                int32_t ww[SL];
+#pragma unroll
+                for (int k = 0; k < SL; k++) ww[k] = e + k * KG < e1 ? col[e + k * KG] : -1;
+#pragma unroll
+                for (int k = 0; k < SL; k++) {
+                    const double c = ww[k] >= 0 ? state<LDS>(sig, ww[k]) : 0.0;
+                    if (c < 0.0) acc -= c;
+                }
             }
-            acc = lane_sum<kG>(acc);
-            if (valid && !heavy && gl == 0) {
+            acc = lane_sum<KG>(acc);
+            if (u >= 0 && !heavy && gl == 0) {
                 const double su = state<LDS>(sig, u);
                 const double d = su * acc;
-                part[u] += d;
+                part[u] = cur.pu + d;
                 coef[i] = (1.0 + d) / su;
             }
+            cur = nxt;
+            nxt.u = u2;
         }
         team_sync<TEAM>();
         for (int32_t i = a0 + t; i < a1; i += TEAM) sig[queue[i]] = -coef[i];
@@ -250,6 +304,7 @@ __device__ void brandes_source(const int32_t* __restrict__ rp, const int32_t* __
     team_sync<TEAM>();
 }
 
+template <int KG, int SL>
 __global__ __launch_bounds__(kBB) void bc_batch_kernel(BatchArgs a) {
     extern __shared__ double s_sig[];
     __shared__ int s_task;
@@ -284,7 +339,7 @@ __global__ __launch_bounds__(kBB) void bc_batch_kernel(BatchArgs a) {
             }
             team_sync<64>();
             for (int32_t s = s0 + wid; s < s1; s += kBW)
-                brandes_source<64, true>(rp, col, s, sig, queue, loff, coef, wp, &s_qc[wid], lane);
+                brandes_source<64, true, KG, SL>(rp, col, s, sig, queue, loff, coef, wp, &s_qc[wid], lane);
             __syncthreads();
             // fold the wavefront partials into the task partial, fixed order
             for (int32_t v = tid; v < n; v += kBB) {
@@ -301,7 +356,7 @@ __global__ __launch_bounds__(kBB) void bc_batch_kernel(BatchArgs a) {
             }
             __syncthreads();
             for (int32_t s = s0; s < s1; s++)
-                brandes_source<kBB, true>(rp, col, s, s_sig, queue_b, loff_b, coef_b, part, &s_qc[0], tid);
+                brandes_source<kBB, true, KG, SL>(rp, col, s, s_sig, queue_b, loff_b, coef_b, part, &s_qc[0], tid);
             __syncthreads();
         } else {
             // ---------------- block mode, state in global scratch ----------------
@@ -312,7 +367,7 @@ __global__ __launch_bounds__(kBB) void bc_batch_kernel(BatchArgs a) {
             }
             __syncthreads();
             for (int32_t s = s0; s < s1; s++)
-                brandes_source<kBB, false>(rp, col, s, gs, queue_b, loff_b, coef_b, part, &s_qc[0], tid);
+                brandes_source<kBB, false, KG, SL>(rp, col, s, gs, queue_b, loff_b, coef_b, part, &s_qc[0], tid);
             __syncthreads();
         }
     }
@@ -344,7 +399,22 @@ struct BatchTuning {
     int32_t wave_max_n = -1;        // -1: pick from LDS capacity
     double task_work_target = 0.0;  // 0: automatic
     int32_t lds_max_n = -1;         // -1: LDS capacity
+    int32_t lane_group = 4;         // lanes per light row: 2, 4 or 8
+    int32_t slots = 4;              // adjacency entries per lane per round: 2, 4 or 8
 };
+
+template <int KG>
+const void* batch_kernel_sl(int sl) {
+    if (sl == 8) return (const void*)bc_batch_kernel<KG, 8>;
+    if (sl == 2) return (const void*)bc_batch_kernel<KG, 2>;
+    return (const void*)bc_batch_kernel<KG, 4>;
+}
+
+const void* batch_kernel(int kg, int sl) {
+    if (kg == 2) return batch_kernel_sl<2>(sl);
+    if (kg == 8) return batch_kernel_sl<8>(sl);
+    return batch_kernel_sl<4>(sl);
+}
 
 BatchTuning& tuning() {
     static BatchTuning t;
@@ -356,10 +426,15 @@ BatchTuning& tuning() {
 
 using namespace lgmi;
 
-extern "C" int lgmi_betweenness_batch_tuning(int32_t wave_max_n, double task_work_target, int32_t lds_max_n) {
+extern "C" int lgmi_betweenness_batch_tuning(int32_t wave_max_n, double task_work_target, int32_t lds_max_n,
+                                             int32_t lane_group, int32_t slots) {
     try {
-        if (wave_max_n < -1 || !(task_work_target >= 0) || lds_max_n < -1)
+        if (wave_max_n < -1 || !(task_work_target >= 0) || lds_max_n < -1 ||
+            (lane_group != 0 && lane_group != 2 && lane_group != 4 && lane_group != 8) ||
+            (slots != 0 && slots != 2 && slots != 4 && slots != 8))
             throw Error(LGMI_E_ARG, "bad tuning values");
+        tuning().lane_group = lane_group ? lane_group : 4;
+        tuning().slots = slots ? slots : 4;
         tuning().wave_max_n = wave_max_n;
         tuning().task_work_target = task_work_target;
         tuning().lds_max_n = lds_max_n;
@@ -529,12 +604,13 @@ extern "C" int lgmi_betweenness_batch(int32_t ngraphs, const int32_t* nv, const 
             const int32_t* d_gts = (const int32_t*)up(gtasks.data(), sizeof(int32_t) * std::max(ntasks, 1));
             const double* d_scale = (const double*)up(scale.data(), sizeof(double) * ngraphs);
             LGMI_HIP(hipStreamSynchronize(st));
-            LGMI_HIP(hipFuncSetAttribute((const void*)bc_batch_kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                         (int)dyn_lds));
+            const void* kern = batch_kernel(tuning().lane_group, tuning().slots);
+            LGMI_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn_lds));
             LGMI_HIP(hipEventCreate(&e0));
             LGMI_HIP(hipEventCreate(&e1));
             LGMI_HIP(hipEventRecord(e0, st));
-            bc_batch_kernel<<<nblocks, kBB, dyn_lds, st>>>(a);
+            void* kargs[] = {&a};
+            LGMI_HIP(hipLaunchKernel(kern, dim3(nblocks), dim3(kBB), kargs, dyn_lds, st));
             LGMI_HIP(hipGetLastError());
             int32_t total_out = (int32_t)std::min<int64_t>(out_total, 1 << 30);
             bc_batch_finish<<<std::max(1, std::min(cu * 8, (int)((out_total + 255) / 256))), 256, 0, st>>>(

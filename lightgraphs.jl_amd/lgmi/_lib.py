@@ -116,7 +116,7 @@ def load():
         "lgmi_betweenness_stats": ([_vp, c.POINTER(BcStats)], c.c_int),
         "lgmi_betweenness_batch": ([c.c_int32, _i32p, _i64p, _i32p, _i64p, _i32p, _i64p, c.c_int, c.c_int,
                                     _f64p, _f64p], c.c_int),
-        "lgmi_betweenness_batch_tuning": ([c.c_int32, c.c_double, c.c_int32], c.c_int),
+        "lgmi_betweenness_batch_tuning": ([c.c_int32, c.c_double, c.c_int32, c.c_int32, c.c_int32], c.c_int),
         "lgmi_last_error": ([], c.c_char_p),
         "lgmi_device_count": ([], c.c_int),
         "lgmi_synchronize": ([], c.c_int),

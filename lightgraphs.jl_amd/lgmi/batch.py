@@ -1,17 +1,19 @@
 """Uneven small-graph betweenness batch: launch-threshold autotuner.
 
 The batch kernel (lgmi_betweenness_batch) runs a graph either one source
-per wavefront (n <= wave_max_n) or one source per work-group, and splits
-graphs with n * (nnz + n) above a work target into several source ranges.
-autotune() times candidate (wave_max_n, work target) pairs on a seeded
-sample of the batch and keeps the fastest; the choice is written to a JSON
+per wavefront (n <= wave_max_n) or one source per work-group, expands
+adjacency rows with lane groups of 2 to 8 lanes taking 2 to 8 entries
+each per round (a whole wavefront for longer rows), and splits graphs with
+n * (nnz + n) above a work target into several source ranges.  autotune()
+times candidate (wave_max_n, work target, lane group, slots) settings on a
+seeded sample of the
+batch and keeps the fastest; the choice is written to a JSON
 file next to this module and applied with lgmi_betweenness_batch_tuning.
 """
 from __future__ import annotations
 
 import json
 import os
-import time
 
 import numpy as np
 
@@ -28,17 +30,19 @@ def autotune(graphs, sample: int = 512, seed: int = 0, candidates=None, log=None
     rng = np.random.default_rng(seed)
     idx = rng.choice(len(graphs), size=min(sample, len(graphs)), replace=False)
     sub = [graphs[i] for i in idx]
-    candidates = candidates or [(0, 0.0), (128, 0.0), (256, 0.0), (512, 0.0), (-1, 0.0)]
+    candidates = candidates or [(wmax, 0.0, kg, sl) for kg, sl in ((2, 8), (4, 4), (4, 8), (8, 4))
+                                for wmax in (0, -1)]
     results = []
-    for wmax, target in candidates:
-        batch_tuning(wmax, target)
+    for wmax, target, kg, sl in candidates:
+        batch_tuning(wmax, target, -1, kg, sl)
         betweenness_centrality_batch(sub[:16])                 # warm up
         _, ms = betweenness_centrality_batch(sub, return_ms=True)
-        results.append({"wave_max_n": wmax, "task_work_target": target, "device_ms": ms})
+        results.append({"wave_max_n": wmax, "task_work_target": target, "lane_group": kg, "slots": sl,
+                        "device_ms": ms})
         if log:
-            log(f"autotune: wave_max_n={wmax} target={target:g}: {ms:.1f} ms")
+            log(f"autotune: wave_max_n={wmax} target={target:g} lane_group={kg} slots={sl}: {ms:.1f} ms")
     best = min(results, key=lambda r: r["device_ms"])
-    batch_tuning(best["wave_max_n"], best["task_work_target"])
+    batch_tuning(best["wave_max_n"], best["task_work_target"], -1, best["lane_group"], best["slots"])
     out = {"best": best, "candidates": results, "sample_graphs": len(sub)}
     return out
 
@@ -53,5 +57,5 @@ def load_tuning(path: str = TUNING_PATH) -> bool:
         return False
     with open(path) as f:
         best = json.load(f)["best"]
-    batch_tuning(best["wave_max_n"], best["task_work_target"])
+    batch_tuning(best["wave_max_n"], best["task_work_target"], -1, best.get("lane_group", 0), best.get("slots", 0))
     return True

@@ -97,14 +97,17 @@ def betweenness_centrality_batch(graphs, normalize=True, endpoints=False, return
     return (res, float(ms[0])) if return_ms else res
 
 
-def batch_tuning(wave_max_n=-1, task_work_target=0.0, lds_max_n=-1):
+def batch_tuning(wave_max_n=-1, task_work_target=0.0, lds_max_n=-1, lane_group=0, slots=0):
     """Launch thresholds of betweenness_centrality_batch (process-wide):
     wave_max_n -- graphs up to this size run one source per wavefront;
     task_work_target -- n * (nnz + n) above which a graph's sources are split;
     lds_max_n -- graphs above this size keep their search state in global
-    memory.  wave_max_n = -1, task_work_target = 0 and lds_max_n = -1
-    select the defaults."""
-    _lib.check(_lib.load().lgmi_betweenness_batch_tuning(int(wave_max_n), float(task_work_target), int(lds_max_n)))
+    memory; lane_group -- lanes per adjacency row (2, 4 or 8); slots --
+    entries per lane per round (2, 4 or 8).  wave_max_n = -1,
+    task_work_target = 0, lds_max_n = -1, lane_group = 0 and slots = 0 select
+    the defaults."""
+    _lib.check(_lib.load().lgmi_betweenness_batch_tuning(int(wave_max_n), float(task_work_target), int(lds_max_n),
+                                                         int(lane_group), int(slots)))
 
 
 def last_stats(g: _GraphBase) -> dict:

@@ -27,8 +27,10 @@ def _oracle_all(oracle, g, normalize=True, endpoints=False):
                                        nthreads=8, index_quirk=False)
 
 
-@pytest.mark.parametrize("wave_max_n,lds_max_n", [(-1, -1), (0, -1), (400, -1), (0, 0), (100, 1500)])
-def test_batch_matches_oracle_modes(oracle, wave_max_n, lds_max_n):
+@pytest.mark.parametrize("wave_max_n,lds_max_n,lane_group,slots",
+                         [(-1, -1, 0, 0), (0, -1, 4, 2), (400, -1, 8, 8), (0, 0, 8, 4), (100, 1500, 4, 4),
+                          (0, 0, 4, 8), (0, -1, 2, 4), (300, 0, 2, 8)])
+def test_batch_matches_oracle_modes(oracle, wave_max_n, lds_max_n, lane_group, slots):
     """Wave mode, block mode with the search state in LDS or in global
     memory, and the automatic split, on a mixed batch that includes isolated
     vertices, tiny graphs and disconnected pieces."""
@@ -38,12 +40,13 @@ def test_batch_matches_oracle_modes(oracle, wave_max_n, lds_max_n):
     graphs = gen.uneven_batch(24, 200, 3000, seed=5)
     graphs += [gen.path_graph(1), gen.path_graph(2), gen.star_graph(3), gen.grid([7, 9]),
                lgmi.SimpleGraph.from_edges(6, [0, 2], [1, 3]), gen.erdos_renyi(700, 900, seed=3)]
-    batch_tuning(wave_max_n, 0.0, lds_max_n)
+    batch_tuning(wave_max_n, 0.0, lds_max_n, lane_group, slots)
     try:
-        say(f"wave_max_n={wave_max_n} lds_max_n={lds_max_n}: {len(graphs)} graphs")
+        say(f"wave_max_n={wave_max_n} lds_max_n={lds_max_n} lane_group={lane_group} slots={slots}: "
+            f"{len(graphs)} graphs")
         got = lgmi.betweenness_centrality_batch(graphs)
     finally:
-        batch_tuning(-1, 0.0, -1)
+        batch_tuning(-1, 0.0, -1, 0, 0)
     for g, bc in zip(graphs, got):
         assert close(bc, _oracle_all(oracle, g)), g
 
@@ -63,7 +66,7 @@ def test_batch_options_and_task_split(oracle):
                 for g, bc in zip(graphs, got):
                     assert close(bc, _oracle_all(oracle, g, normalize, endpoints))
     finally:
-        batch_tuning(-1, 0.0)
+        batch_tuning(-1, 0.0, -1, 0, 0)
 
 
 def test_batch_equals_single_graph_path_and_is_deterministic():

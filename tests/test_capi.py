@@ -82,11 +82,14 @@ def test_policy_argument_checks():
 def test_batch_tuning_argument_checks():
     from lgmi import _lib
     lib = _lib.load()
-    assert lib.lgmi_betweenness_batch_tuning(-2, 0.0, -1) == _lib.LGMI_E_ARG
-    assert lib.lgmi_betweenness_batch_tuning(-1, -1.0, -1) == _lib.LGMI_E_ARG
-    assert lib.lgmi_betweenness_batch_tuning(-1, float("nan"), -1) == _lib.LGMI_E_ARG
-    assert lib.lgmi_betweenness_batch_tuning(-1, 0.0, -5) == _lib.LGMI_E_ARG
-    assert lib.lgmi_betweenness_batch_tuning(-1, 0.0, -1) == _lib.LGMI_OK
+    assert lib.lgmi_betweenness_batch_tuning(-2, 0.0, -1, 0, 0) == _lib.LGMI_E_ARG
+    assert lib.lgmi_betweenness_batch_tuning(-1, -1.0, -1, 0, 0) == _lib.LGMI_E_ARG
+    assert lib.lgmi_betweenness_batch_tuning(-1, float("nan"), -1, 0, 0) == _lib.LGMI_E_ARG
+    assert lib.lgmi_betweenness_batch_tuning(-1, 0.0, -5, 0, 0) == _lib.LGMI_E_ARG
+    assert lib.lgmi_betweenness_batch_tuning(-1, 0.0, -1, 5, 0) == _lib.LGMI_E_ARG
+    assert lib.lgmi_betweenness_batch_tuning(-1, 0.0, -1, 4, 3) == _lib.LGMI_E_ARG
+    assert lib.lgmi_betweenness_batch_tuning(-1, 0.0, -1, 16, 4) == _lib.LGMI_E_ARG
+    assert lib.lgmi_betweenness_batch_tuning(-1, 0.0, -1, 0, 0) == _lib.LGMI_OK
 
 
 def test_generators_shapes_and_determinism():
