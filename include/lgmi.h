@@ -216,9 +216,10 @@ int lgmi_betweenness_batch(int32_t ngraphs, const int32_t* nv, const int64_t* ro
  * (0: total work / (6 * number of CUs)); graphs with more than lds_max_n
  * vertices keep the per-vertex search state in global memory instead of LDS
  * (-1: as many as LDS holds; values above that are clamped); lane_group
- * (2, 4 or 8; 0: 4) is the number of lanes that expand one adjacency row,
- * each taking `slots` entries per round (2, 4 or 8; 0: 4), rows longer
- * than 2 * lane_group * slots being expanded by a whole wavefront. */
+ * (1, 2, 4 or 8; 0: 2) is the number of lanes that expand one adjacency
+ * row, each taking `slots` entries per round (2, 4 or 8, or 16 with one
+ * lane per row; 0: 8), rows
+ * longer than 2 * lane_group * slots being expanded by a whole wavefront. */
 int lgmi_betweenness_batch_tuning(int32_t wave_max_n, double task_work_target, int32_t lds_max_n,
                                   int32_t lane_group, int32_t slots);
 

@@ -399,18 +399,22 @@ struct BatchTuning {
     int32_t wave_max_n = -1;        // -1: pick from LDS capacity
     double task_work_target = 0.0;  // 0: automatic
     int32_t lds_max_n = -1;         // -1: LDS capacity
-    int32_t lane_group = 4;         // lanes per light row: 2, 4 or 8
-    int32_t slots = 4;              // adjacency entries per lane per round: 2, 4 or 8
+    int32_t lane_group = 2;         // lanes per light row: 1, 2, 4 or 8
+    int32_t slots = 8;              // adjacency entries per lane per round: 2, 4, 8 or 16
 };
 
 template <int KG>
 const void* batch_kernel_sl(int sl) {
+    if constexpr (KG == 1) {
+        if (sl == 16) return (const void*)bc_batch_kernel<KG, 16>;
+    }
     if (sl == 8) return (const void*)bc_batch_kernel<KG, 8>;
     if (sl == 2) return (const void*)bc_batch_kernel<KG, 2>;
     return (const void*)bc_batch_kernel<KG, 4>;
 }
 
 const void* batch_kernel(int kg, int sl) {
+    if (kg == 1) return batch_kernel_sl<1>(sl);
     if (kg == 2) return batch_kernel_sl<2>(sl);
     if (kg == 8) return batch_kernel_sl<8>(sl);
     return batch_kernel_sl<4>(sl);
@@ -430,11 +434,11 @@ extern "C" int lgmi_betweenness_batch_tuning(int32_t wave_max_n, double task_wor
                                              int32_t lane_group, int32_t slots) {
     try {
         if (wave_max_n < -1 || !(task_work_target >= 0) || lds_max_n < -1 ||
-            (lane_group != 0 && lane_group != 2 && lane_group != 4 && lane_group != 8) ||
-            (slots != 0 && slots != 2 && slots != 4 && slots != 8))
+            (lane_group != 0 && lane_group != 1 && lane_group != 2 && lane_group != 4 && lane_group != 8) ||
+            (slots != 0 && slots != 2 && slots != 4 && slots != 8 && !(slots == 16 && lane_group == 1)))
             throw Error(LGMI_E_ARG, "bad tuning values");
-        tuning().lane_group = lane_group ? lane_group : 4;
-        tuning().slots = slots ? slots : 4;
+        tuning().lane_group = lane_group ? lane_group : 2;
+        tuning().slots = slots ? slots : 8;
         tuning().wave_max_n = wave_max_n;
         tuning().task_work_target = task_work_target;
         tuning().lds_max_n = lds_max_n;

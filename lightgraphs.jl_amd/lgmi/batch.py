@@ -2,7 +2,7 @@
 
 The batch kernel (lgmi_betweenness_batch) runs a graph either one source
 per wavefront (n <= wave_max_n) or one source per work-group, expands
-adjacency rows with lane groups of 2 to 8 lanes taking 2 to 8 entries
+adjacency rows with lane groups of 1 to 8 lanes taking 2 to 16 entries
 each per round (a whole wavefront for longer rows), and splits graphs with
 n * (nnz + n) above a work target into several source ranges.  autotune()
 times candidate (wave_max_n, work target, lane group, slots) settings on a
@@ -30,7 +30,7 @@ def autotune(graphs, sample: int = 512, seed: int = 0, candidates=None, log=None
     rng = np.random.default_rng(seed)
     idx = rng.choice(len(graphs), size=min(sample, len(graphs)), replace=False)
     sub = [graphs[i] for i in idx]
-    candidates = candidates or [(wmax, 0.0, kg, sl) for kg, sl in ((2, 8), (4, 4), (4, 8), (8, 4))
+    candidates = candidates or [(wmax, 0.0, kg, sl) for kg, sl in ((1, 8), (1, 16), (2, 4), (2, 8), (4, 4))
                                 for wmax in (0, -1)]
     results = []
     for wmax, target, kg, sl in candidates:

@@ -102,8 +102,8 @@ def batch_tuning(wave_max_n=-1, task_work_target=0.0, lds_max_n=-1, lane_group=0
     wave_max_n -- graphs up to this size run one source per wavefront;
     task_work_target -- n * (nnz + n) above which a graph's sources are split;
     lds_max_n -- graphs above this size keep their search state in global
-    memory; lane_group -- lanes per adjacency row (2, 4 or 8); slots --
-    entries per lane per round (2, 4 or 8).  wave_max_n = -1,
+    memory; lane_group -- lanes per adjacency row (1, 2, 4 or 8); slots --
+    entries per lane per round (2, 4, 8 or 16).  wave_max_n = -1,
     task_work_target = 0, lds_max_n = -1, lane_group = 0 and slots = 0 select
     the defaults."""
     _lib.check(_lib.load().lgmi_betweenness_batch_tuning(int(wave_max_n), float(task_work_target), int(lds_max_n),

@@ -29,7 +29,7 @@ def _oracle_all(oracle, g, normalize=True, endpoints=False):
 
 @pytest.mark.parametrize("wave_max_n,lds_max_n,lane_group,slots",
                          [(-1, -1, 0, 0), (0, -1, 4, 2), (400, -1, 8, 8), (0, 0, 8, 4), (100, 1500, 4, 4),
-                          (0, 0, 4, 8), (0, -1, 2, 4), (300, 0, 2, 8)])
+                          (0, 0, 4, 8), (0, -1, 2, 4), (300, 0, 2, 8), (0, -1, 1, 16), (200, 0, 1, 8)])
 def test_batch_matches_oracle_modes(oracle, wave_max_n, lds_max_n, lane_group, slots):
     """Wave mode, block mode with the search state in LDS or in global
     memory, and the automatic split, on a mixed batch that includes isolated

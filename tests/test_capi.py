@@ -89,6 +89,8 @@ def test_batch_tuning_argument_checks():
     assert lib.lgmi_betweenness_batch_tuning(-1, 0.0, -1, 5, 0) == _lib.LGMI_E_ARG
     assert lib.lgmi_betweenness_batch_tuning(-1, 0.0, -1, 4, 3) == _lib.LGMI_E_ARG
     assert lib.lgmi_betweenness_batch_tuning(-1, 0.0, -1, 16, 4) == _lib.LGMI_E_ARG
+    assert lib.lgmi_betweenness_batch_tuning(-1, 0.0, -1, 4, 32) == _lib.LGMI_E_ARG
+    assert lib.lgmi_betweenness_batch_tuning(-1, 0.0, -1, 2, 16) == _lib.LGMI_E_ARG
     assert lib.lgmi_betweenness_batch_tuning(-1, 0.0, -1, 0, 0) == _lib.LGMI_OK
 
 
