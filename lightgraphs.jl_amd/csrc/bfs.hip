@@ -5,40 +5,52 @@
 // Parallel.bfs_tree! (src/Parallel/traversals/gdistances.jl:36-116,
 // src/Parallel/traversals/bfs.jl:49-92).
 //
-// State in HBM: dist[nv], parent[nv] (Int32), three frontier bitmaps F_L
-// rotating by level (read F_L, write F_{L+1}, clear F_{L+2}) and the visited
-// bitmap vis = F_0 | ... | F_{L-1} (F_L is folded in during step L).  No
-// vertex queues: every step sweeps bitmap tiles of 64 words (2048 vertices),
-// one wavefront per tile; the set bits of interest (frontier vertices for
-// top-down, unvisited vertices with a non-empty row for bottom-up) are
-// compacted into batches of 64 vertices with a prefix popcount over the
-// lanes and a rank-select inside the word, so every lane of a batch works.
+// State in HBM: dist[nv], parent[nv] (Int32), three frontier bitmaps F_k
+// rotating by level (read F_k, write F_{k+1}, clear F_{k+2}) and the visited
+// bitmap vis = F_0 | ... | F_{k-1} (F_k is folded in during step k).  No
+// vertex queues: every step sweeps bitmap tiles of 512 vertices, one
+// wavefront per tile and 8 bits per lane; the set bits of interest (frontier
+// vertices for top-down, unvisited vertices with a non-empty row for
+// bottom-up) are compacted into batches of 64 vertices with a prefix
+// popcount over the lanes and a rank-select inside the byte, so every lane
+// of a batch works.
 //
-// One launch of bfs_step advances the search by one level L -> L+1:
-//  * top-down: rows of 32..1024 entries are expanded by the whole
-//    wavefront, shorter rows by a scan-based gather that packs several rows
-//    into the 64 lanes, and rows longer than 1024 entries through
-//    precomputed 1024-entry work items spread over all work-groups.  A new
-//    vertex is claimed with an atomicOr on F_{L+1}; its parent is the
-//    minimum over its frontier neighbours (atomicMin after a plain-load
-//    filter).
-//  * bottom-up: every candidate scans its reverse adjacency in ascending
-//    order, eight entries per round trip, and stops at the first neighbour
-//    in F_L, which is the minimum-id parent; the tile's F_{L+1} words are
-//    gathered in LDS and written whole.
+// One launch of bfs_level per level k.  Its prologue replaces a separate
+// reduction kernel and any cross-work-group atomics: every work-group loads
+// the per-block counters the launch for level k-1 wrote (one round of
+// independent loads), sums them in LDS, derives the level state S_k from
+// S_{k-1} and picks the direction of step k (Beamer alpha/beta, the learned
+// MLP, or a forced direction).  All work-groups compute the same S_k;
+// work-group 0 also stores it, with the level log.  Level parity selects
+// the counter and state buffers, so no launch overwrites what a work-group
+// of the same launch still reads.  The launch that finds F_k empty stores
+// k as done_level; later launches of the batch exit at once.
+//
+// Step k -> k+1:
+//  * top-down: rows of 32..256 entries are expanded by the whole wavefront,
+//    shorter rows by a scan-based gather that packs several rows into the
+//    64 lanes, rows longer than 256 entries through precomputed 256-entry
+//    work items dealt round-robin over all wavefronts; each lane handles 4
+//    entries per round with all their loads in flight together.  A vertex's
+//    parent is the minimum over its frontier neighbours (atomicMin after a
+//    plain-load filter); the atomicMin that finds the parent unset claims
+//    the vertex.  Device-scope atomics run at about 26 G operations/s on this
+//    GPU, several times below plain loads and stores, so the step issues at
+//    most one per fresh edge plus one per new vertex.
+//  * bottom-up: every candidate scans its sorted reverse row for the first
+//    neighbour in F_k, which is its minimum-id parent.  Stage one tests the
+//    first entry of every candidate (it is the parent for most vertices on
+//    the large levels); the rest wait in a per-wavefront LDS queue and scan
+//    four entries per round trip (a whole wavefront for rows with more than
+//    64 entries left).  The tile's F_{k+1} words are gathered in LDS and
+//    written whole.
 // Both directions give identical distances and parents: the reference's
 // sorted frontier makes parent[v] the smallest u with dist[u] = dist[v]-1
 // and an edge u -> v, which is what either direction computes.
-//
-// Every block writes its counters (new vertices, their degrees, entries
-// scanned, a degree histogram) to its own slot of a partials array; the
-// one-block bfs_finalize kernel that follows each step sums them in block
-// order, updates the control block and picks the next direction (Beamer
-// alpha/beta, the learned MLP, or a forced direction).  The host enqueues
-// step/finalise pairs and synchronises once per batch of levels.
 #include "lgmi_internal.h"
 
 #include <algorithm>
+#include <climits>
 #include <cstdlib>
 #include <cstring>
 #include <vector>
@@ -47,18 +59,23 @@ namespace lgmi {
 
 namespace {
 
-constexpr int kBlock = 256;
+constexpr int kBlock = 1024;
 constexpr int kWaves = kBlock / 64;
+constexpr int kLaneBits = 8;                    // frontier bits per lane in a tile
+constexpr int kTileVerts = 64 * kLaneBits;      // vertices per wavefront tile
+constexpr int kTileWords = kTileVerts / 32;     // bitmap words per tile
 constexpr int32_t kWaveRow = 32;    // top-down: rows this long or longer are expanded by a wavefront
 constexpr int32_t kBuCoop = 64;     // bottom-up: rows longer than this are scanned by a wavefront
-constexpr int kFinBlock = 1024;     // finalise kernel threads
-constexpr int kGraphLevels = 16;    // step/finalise pairs per captured graph
+constexpr int kUnroll = 4;          // top-down: adjacency entries per lane per round
+constexpr int kGraphLevels = 16;    // level launches per captured graph (even: parity is static)
+constexpr int kPartRows = kBlock / kPart;
 
 struct StepArgs {
     const int32_t* td_rp;     // traversal adjacency (top-down expands it)
     const int32_t* td_col;
     const int32_t* td_item_v; // heavy-row items of the traversal adjacency
     const int32_t* td_item_beg;
+    const int32_t* td_item_end;
     int32_t td_items;
     const int32_t* bu_rp;     // reverse adjacency (bottom-up scans it)
     const int32_t* bu_col;
@@ -71,8 +88,9 @@ struct StepArgs {
     uint32_t* bm1;
     uint32_t* bm2;
     uint32_t* vis;
-    BfsCtrl* ctrl;
-    unsigned long long* part; // [gridDim.x][kPart]
+    BfsShared* sh;
+    unsigned long long* part; // [2][grid][kPart] by launch parity
+    int32_t grid;
     LevelRec* log;
     int32_t max_log;
     const PolicyWeights* policy;
@@ -89,26 +107,31 @@ __device__ inline int hist_bin(int32_t deg) {
     return b > 6 ? 6 : b;
 }
 
-__device__ inline float flog2p(double x) { return (float)log2(1.0 + (x > 0 ? x : 0.0)); }
+__device__ inline float flog2p(float x) { return __log2f(1.0f + (x > 0.f ? x : 0.f)); }
 
-// Position of the r-th (0-based) set bit of x (r < popcount(x)).
-__device__ inline int select_bit(uint32_t x, int r) {
+// Position of the r-th (0-based) set bit of the byte x (r < popcount(x)).
+__device__ inline int select_bit8(uint32_t x, int r) {
     int pos = 0, c;
-    c = __popc(x & 0xffffu); if (r >= c) { r -= c; x >>= 16; pos += 16; }
-    c = __popc(x & 0xffu);   if (r >= c) { r -= c; x >>= 8;  pos += 8; }
-    c = __popc(x & 0xfu);    if (r >= c) { r -= c; x >>= 4;  pos += 4; }
-    c = __popc(x & 0x3u);    if (r >= c) { r -= c; x >>= 2;  pos += 2; }
-    c = (int)(x & 1u);       if (r >= c) { pos += 1; }
+    c = __popc(x & 0xfu); if (r >= c) { r -= c; x >>= 4; pos += 4; }
+    c = __popc(x & 0x3u); if (r >= c) { r -= c; x >>= 2; pos += 2; }
+    c = (int)(x & 1u);    if (r >= c) { pos += 1; }
     return pos;
 }
 
-// Lane `lane` of a batch starting at set-bit index `base` of the tile's
-// words (lane l holds word cw with exclusive popcount prefix excl) gets
-// the vertex of set bit base+lane, or -1 past the end.  Must be called by
-// all 64 lanes.  wi/bi return the word index in the tile and the bit.
-__device__ inline int32_t batch_vertex(uint32_t cw, int32_t excl, int32_t total, int32_t base, int32_t word0,
-                                       int lane, int& wi, int& bi) {
-    const int32_t g = base + lane;
+// Exclusive prefix sum over the 64 lanes; total in `total`.
+__device__ inline int32_t wave_excl_scan(int32_t x, int lane, int32_t& total) {
+    int32_t incl = x;
+    #pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int32_t y = __shfl_up(incl, d);
+        if (lane >= d) incl += y;
+    }
+    total = __shfl(incl, 63);
+    return incl - x;
+}
+
+// Largest lane lo with excl[lo] <= g (excl non-decreasing over lanes, excl[0] = 0).
+__device__ inline int owner_lane(int32_t excl, int32_t g) {
     int lo = 0;
     #pragma unroll
     for (int step = 32; step >= 1; step >>= 1) {
@@ -116,283 +139,378 @@ __device__ inline int32_t batch_vertex(uint32_t cw, int32_t excl, int32_t total,
         const int32_t pv = __shfl(excl, probe & 63);
         if (probe < 64 && pv <= g) lo = probe;
     }
-    const uint32_t word = __shfl(cw, lo);
-    const int32_t r = g - __shfl(excl, lo);
-    wi = lo;
-    bi = 0;
-    if (g >= total) return -1;
-    bi = select_bit(word, r);
-    return (word0 + lo) * 32 + bi;
+    return lo;
 }
 
-// Features of the state at the start of a step (LGMI_POLICY_FEATURES).
+// Lane `lane` of a batch starting at set-bit index `base` of a tile (lane l
+// holds the byte cw of vertices v0 + 8l .. v0 + 8l + 7, with exclusive
+// popcount prefix excl) gets the vertex of set bit base + lane, or -1 past
+// the end.  Must be called by all 64 lanes.  lo/bit return the owning lane
+// and the bit inside its byte.
+__device__ inline int32_t batch_vertex(uint32_t cw, int32_t excl, int32_t total, int32_t base, int32_t v0,
+                                       int lane, int& lo, int& bit) {
+    const int32_t g = base + lane;
+    lo = owner_lane(excl, g);
+    const uint32_t byte = __shfl(cw, lo);
+    const int32_t r = g - __shfl(excl, lo);
+    bit = 0;
+    if (g >= total) return -1;
+    bit = select_bit8(byte, r);
+    return v0 + lo * kLaneBits + bit;
+}
+
+// Features of the level state (LGMI_POLICY_FEATURES), in FP32 (the
+// hardware log2).
 __device__ void compute_features(const BfsCtrl* c, float* f) {
-    const double nf = c->n_front, mf = (double)c->m_front, mu = (double)c->m_unvisited;
-    const double nu = (double)c->n_unvisited;
-    f[0] = flog2p(nf);
-    f[1] = flog2p(mf);
-    f[2] = flog2p(mu);
+    const float nf = (float)c->n_front, mf = (float)c->m_front, mu = (float)c->m_unvisited;
+    const float nu = (float)c->n_unvisited;
+    const float lmf = flog2p(mf), lmu = flog2p(mu), lnf = flog2p(nf);
+    f[0] = lnf;
+    f[1] = lmf;
+    f[2] = lmu;
     f[3] = flog2p(nu);
-    f[4] = flog2p((double)c->level);
-    f[5] = (float)(nf / (double)(c->nv > 0 ? c->nv : 1));
-    f[6] = flog2p(mf) - flog2p(mu);
-    f[7] = flog2p(nf) - flog2p((double)c->n_prev);
+    f[4] = flog2p((float)c->level);
+    f[5] = nf / (float)(c->nv > 0 ? c->nv : 1);
+    f[6] = lmf - lmu;
+    f[7] = lnf - flog2p((float)c->n_prev);
     f[8] = (float)c->prev_dir;
     uint32_t tot = 0;
     for (int b = 0; b < 7; b++) tot += c->front_hist[b];
-    for (int b = 0; b < 7; b++) f[9 + b] = tot ? (float)c->front_hist[b] / (float)tot : 0.f;
-}
-
-template <bool PARENTS>
-__device__ inline bool td_visit(const StepArgs& a, int32_t u, int32_t v, int32_t L, const uint32_t* FL,
-                                uint32_t* Fn) {
-    const int w = v >> 5;
-    const uint32_t bit = 1u << (v & 31);
-    if ((a.vis[w] | FL[w]) & bit) return false;       // distance <= L already
-    if (PARENTS) {
-        // parent only decreases, so a stale value <= u already rules u out
-        if (a.parent[v] > (uint32_t)u) atomicMin(&a.parent[v], (uint32_t)u);
-    }
-    if (Fn[w] & bit) return false;
-    const uint32_t old = atomicOr(&Fn[w], bit);
-    if (old & bit) return false;
-    a.dist[v] = L + 1;
-    return true;
+    const float inv = tot ? 1.f / (float)tot : 0.f;
+    for (int b = 0; b < 7; b++) f[9 + b] = (float)c->front_hist[b] * inv;
 }
 
 struct Counters {
     unsigned long long n = 0, out = 0, in = 0, scan = 0, cand = 0;
 };
 
-template <bool PARENTS>
-__global__ __launch_bounds__(kBlock) void bfs_step(StepArgs a) {
-    __shared__ int32_t s_level, s_dir, s_done;
-    __shared__ uint32_t s_hist[kHistBins];
-    __shared__ unsigned long long s_red[kWaves][5];
-    __shared__ uint32_t s_fn[kWaves][64];
-
-    const BfsCtrl* c = a.ctrl;
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
-    if (tid == 0) {
-        s_done = c->done;
-        s_level = c->level;
-        s_dir = c->dir;
-        if (blockIdx.x == 0 && !s_done) a.ctrl->t_step_start = wall_clock64();
+// A new vertex v of F_{k+1}: its degrees feed the next level's state.
+// din_known >= 0 is v's reverse-adjacency degree when the caller has it.
+__device__ inline void account_new(const StepArgs& a, int32_t v, int32_t din_known, Counters& k, uint32_t* s_hist) {
+    int32_t dout, din;
+    if (a.td_rp == a.bu_rp && din_known >= 0) {
+        dout = din = din_known;
+    } else {
+        dout = a.td_rp[v + 1] - a.td_rp[v];
+        din = (a.td_rp == a.bu_rp) ? dout : (din_known >= 0 ? din_known : a.bu_rp[v + 1] - a.bu_rp[v]);
     }
-    if (tid < kHistBins) s_hist[tid] = 0;
-    __syncthreads();
-    if (s_done) return;
+    k.n++;
+    k.out += (unsigned long long)dout;
+    k.in += (unsigned long long)din;
+    atomicAdd(&s_hist[hist_bin(dout)], 1u);
+}
 
-    const int32_t L = s_level;
-    const uint32_t* FL = bm_at(a, L % 3);
-    uint32_t* Fn = bm_at(a, (L + 1) % 3);
-    uint32_t* Fc = bm_at(a, (L + 2) % 3);
-    const bool same_adj = (a.td_rp == a.bu_rp);
-    const int32_t ntiles = (a.nwords + 63) >> 6;
-    const int gw = blockIdx.x * kWaves + wid, nw = gridDim.x * kWaves;
-    Counters k;
-
-    auto account = [&](bool claimed, int32_t v) {
-        if (claimed) {
-            const int32_t dout = a.td_rp[v + 1] - a.td_rp[v];
-            const int32_t din = same_adj ? dout : (a.bu_rp[v + 1] - a.bu_rp[v]);
-            k.n++;
-            k.out += (unsigned long long)dout;
-            k.in += (unsigned long long)din;
-            atomicAdd(&s_hist[hist_bin(dout)], 1u);
+// Top-down visit of N edges u[q] -> v[q] (v[q] < 0: none).  All loads of the
+// N edges are issued before any result is used.  With parents, the atomicMin
+// that finds a vertex's parent unset is its claim (one atomic per fresh edge
+// that passes the plain-load filter, plus one atomicOr per new vertex);
+// without parents the claim is an atomicOr on F_{k+1}.
+template <bool PARENTS, int N>
+__device__ inline void visit_n(const StepArgs& a, const int32_t* u, const int32_t* v, int32_t L,
+                               const uint32_t* FL, uint32_t* Fn, Counters& k, uint32_t* s_hist) {
+    uint32_t seen[N], pre[N];
+    #pragma unroll
+    for (int q = 0; q < N; q++) {
+        seen[q] = 0xffffffffu;
+        pre[q] = 0;
+        if (v[q] >= 0) {
+            const int32_t w = v[q] >> 5;
+            seen[q] = a.vis[w] | FL[w];
+            pre[q] = PARENTS ? a.parent[v[q]] : Fn[w];
         }
-    };
+    }
+    bool claim[N];
+    #pragma unroll
+    for (int q = 0; q < N; q++) {
+        const bool fresh = v[q] >= 0 && !((seen[q] >> (v[q] & 31)) & 1u);   // distance > L
+        // parent only decreases, so a stale value <= u already rules u out
+        claim[q] = fresh && (PARENTS ? pre[q] > (uint32_t)u[q] : !((pre[q] >> (v[q] & 31)) & 1u));
+    }
+    uint32_t old[N];
+    #pragma unroll
+    for (int q = 0; q < N; q++) {
+        old[q] = 0;
+        if (claim[q]) {
+            if (PARENTS) old[q] = atomicMin(&a.parent[v[q]], (uint32_t)u[q]);
+            else old[q] = atomicOr(&Fn[v[q] >> 5], 1u << (v[q] & 31));
+        }
+    }
+    #pragma unroll
+    for (int q = 0; q < N; q++) {
+        const bool won = claim[q] && (PARENTS ? old[q] == 0xffffffffu : !((old[q] >> (v[q] & 31)) & 1u));
+        if (won) {
+            if (PARENTS) atomicOr(&Fn[v[q] >> 5], 1u << (v[q] & 31));
+            a.dist[v[q]] = L + 1;
+            account_new(a, v[q], -1, k, s_hist);
+        }
+    }
+}
 
-    if (s_dir == 0) {
-        // ---------------- top-down ----------------
-        // heavy rows: 1024-entry items; a wavefront tests 64 items at once
-        // (strided by the number of wavefronts, so the items of one hub
-        // land on different wavefronts) and expands those on the frontier
+// Top-down expansion of row entries [b, e) of frontier vertex u by a whole
+// wavefront, kUnroll entries per lane per round.
+template <bool PARENTS>
+__device__ inline void expand_row(const StepArgs& a, int32_t u, int32_t b, int32_t e, int32_t L,
+                                  const uint32_t* FL, uint32_t* Fn, Counters& k, uint32_t* s_hist, int lane) {
+    for (int32_t x0 = b; x0 < e; x0 += 64 * kUnroll) {
+        int32_t uu[kUnroll], vv[kUnroll];
+        #pragma unroll
+        for (int q = 0; q < kUnroll; q++) {
+            const int32_t x = x0 + q * 64 + lane;
+            uu[q] = u;
+            vv[q] = x < e ? a.td_col[x] : -1;
+        }
+        visit_n<PARENTS, kUnroll>(a, uu, vv, L, FL, Fn, k, s_hist);
+    }
+}
+
+// Top-down step L -> L+1 of one wavefront (gw of nw).
+template <bool PARENTS>
+__device__ void td_step(const StepArgs& a, const BfsCtrl& c, int32_t L, const uint32_t* FL, uint32_t* Fn,
+                        uint32_t* Fc, Counters& k, uint32_t* s_hist, int gw, int nw, int lane) {
+    const int32_t ntiles = (a.nwords + kTileWords - 1) / kTileWords;
+    const int sh8 = (lane & 3) * kLaneBits;
+    // Heavy rows: 256-entry items; lane j of wavefront gw tests item
+    // gw + (base + j) * nw, so consecutive items of one hub go to
+    // consecutive wavefronts, and expands those of frontier vertices.
+    // Skipped when no frontier vertex has 256 or more entries.
+    if (c.front_hist[5] + c.front_hist[6] > 0) {
         for (int64_t base = 0; (int64_t)gw + base * nw < a.td_items; base += 64) {
             const int64_t it = (int64_t)gw + (base + lane) * nw;
-            int32_t iu = -1;
+            int32_t iu = -1, ib = 0, ie = 0;
             if (it < a.td_items) {
                 iu = a.td_item_v[it];
-                if (!((FL[iu >> 5] >> (iu & 31)) & 1u)) iu = -1;
+                ib = a.td_item_beg[it];
+                ie = a.td_item_end[it];
             }
+            if (iu >= 0 && !((FL[iu >> 5] >> (iu & 31)) & 1u)) iu = -1;
             uint64_t m = __ballot(iu >= 0);
             while (m) {
                 const int j = __ffsll((unsigned long long)m) - 1;
                 m &= m - 1;
-                const int32_t u = __shfl(iu, j);
-                const int32_t b = a.td_item_beg[(int64_t)gw + (base + j) * nw];
-                const int32_t e = min(b + kHeavyRow, a.td_rp[u + 1]);
-                k.scan += (lane == 0) ? (unsigned long long)(e - b) : 0ull;
-                for (int32_t x = b + lane; x < e; x += 64) {
-                    const int32_t v = a.td_col[x];
-                    account(td_visit<PARENTS>(a, u, v, L, FL, Fn), v);
-                }
+                const int32_t u = __shfl(iu, j), b = __shfl(ib, j), e = __shfl(ie, j);
+                if (lane == 0) k.scan += (unsigned long long)(e - b);
+                expand_row<PARENTS>(a, u, b, e, L, FL, Fn, k, s_hist, lane);
             }
         }
-        // frontier tiles
-        for (int32_t t = gw; t < ntiles; t += nw) {
-            const int32_t word0 = t * 64;
-            const int32_t w = word0 + lane;
-            uint32_t fw = 0;
-            if (w < a.nwords) {
-                fw = FL[w];
-                a.vis[w] |= fw;           // fold F_L into vis
-                Fc[w] = 0;                // clear F_{L+2}
-            }
-            const int32_t pc = __popc(fw);
-            int32_t incl = pc;
-            for (int d = 1; d < 64; d <<= 1) {
-                const int32_t y = __shfl_up(incl, d);
-                if (lane >= d) incl += y;
-            }
-            const int32_t excl = incl - pc;
-            const int32_t total = __shfl(incl, 63);
-            for (int32_t base = 0; base < total; base += 64) {
-                int wi, bi;
-                const int32_t u = batch_vertex(fw, excl, total, base, word0, lane, wi, bi);
-                int32_t beg = 0, deg = 0;
-                if (u >= 0) {
-                    beg = a.td_rp[u];
-                    deg = a.td_rp[u + 1] - beg;
-                    if (deg > kHeavyRow) deg = 0;     // expanded through its items
-                }
-                k.scan += (unsigned long long)deg;
-                // wavefront-wide rows
-                uint64_t m = __ballot(deg >= kWaveRow);
-                while (m) {
-                    const int j = __ffsll((unsigned long long)m) - 1;
-                    m &= m - 1;
-                    const int32_t wu = __shfl(u, j), wb = __shfl(beg, j), wd = __shfl(deg, j);
-                    for (int32_t x = lane; x < wd; x += 64) {
-                        const int32_t v = a.td_col[wb + x];
-                        account(td_visit<PARENTS>(a, wu, v, L, FL, Fn), v);
-                    }
-                }
-                if (deg >= kWaveRow) deg = 0;
-                // short rows: scan-based gather over the batch
-                int32_t rin = deg;
-                for (int d = 1; d < 64; d <<= 1) {
-                    const int32_t y = __shfl_up(rin, d);
-                    if (lane >= d) rin += y;
-                }
-                const int32_t rex = rin - deg;
-                const int32_t rtot = __shfl(rin, 63);
-                for (int32_t off = 0; off < rtot; off += 64) {
-                    const int32_t e = off + lane;
-                    int lo = 0;
-                    #pragma unroll
-                    for (int step = 32; step >= 1; step >>= 1) {
-                        const int probe = lo + step;
-                        const int32_t pv = __shfl(rex, probe & 63);
-                        if (probe < 64 && pv <= e) lo = probe;
-                    }
-                    const int32_t ob = __shfl(beg, lo), oe = __shfl(rex, lo), ou = __shfl(u, lo);
-                    if (e < rtot) {
-                        const int32_t v = a.td_col[ob + (e - oe)];
-                        account(td_visit<PARENTS>(a, ou, v, L, FL, Fn), v);
-                    }
-                }
+    }
+    // frontier tiles; the next tile's frontier word is loaded before the
+    // current tile is expanded
+    int32_t t = gw;
+    uint32_t fl_next = 0;
+    {
+        const int32_t w = t * kTileWords + (lane >> 2);
+        if (t < ntiles && w < a.nwords) fl_next = FL[w];
+    }
+    while (t < ntiles) {
+        const int32_t word0 = t * kTileWords;
+        const int32_t w = word0 + (lane >> 2);
+        const uint32_t fl = fl_next;
+        t += nw;
+        {
+            const int32_t wn = t * kTileWords + (lane >> 2);
+            fl_next = (t < ntiles && wn < a.nwords) ? FL[wn] : 0u;
+        }
+        uint32_t cw = 0;
+        if (w < a.nwords) {
+            cw = (fl >> sh8) & 0xffu;
+            if ((lane & 3) == 0) {
+                if (fl) a.vis[w] |= fl;   // fold F_k into vis
+                Fc[w] = 0;                // clear F_{k+2}
             }
         }
-    } else {
-        // ---------------- bottom-up ----------------
-        for (int32_t t = gw; t < ntiles; t += nw) {
-            const int32_t word0 = t * 64;
-            const int32_t w = word0 + lane;
-            uint32_t cw = 0;
-            if (w < a.nwords) {
-                const uint32_t visw = a.vis[w] | FL[w];
-                cw = ~visw & a.bu_nz[w];  // unvisited with a non-empty reverse row
-                a.vis[w] = visw;          // fold F_L into vis
-                Fc[w] = 0;                // clear F_{L+2}
+        int32_t total;
+        const int32_t excl = wave_excl_scan(__popc(cw), lane, total);
+        const int32_t v0 = word0 * 32;
+        for (int32_t base = 0; base < total; base += 64) {
+            int lo, bit;
+            const int32_t u = batch_vertex(cw, excl, total, base, v0, lane, lo, bit);
+            int32_t beg = 0, deg = 0;
+            if (u >= 0) {
+                beg = a.td_rp[u];
+                deg = a.td_rp[u + 1] - beg;
+                if (deg > kHeavyRow) deg = 0;     // expanded through its items
             }
-            s_fn[wid][lane] = 0;
-            const int32_t pc = __popc(cw);
-            int32_t incl = pc;
-            for (int d = 1; d < 64; d <<= 1) {
-                const int32_t y = __shfl_up(incl, d);
-                if (lane >= d) incl += y;
+            k.scan += (unsigned long long)deg;
+            // wavefront-wide rows
+            uint64_t m = __ballot(deg >= kWaveRow);
+            while (m) {
+                const int j = __ffsll((unsigned long long)m) - 1;
+                m &= m - 1;
+                const int32_t wu = __shfl(u, j), wb = __shfl(beg, j), wd = __shfl(deg, j);
+                expand_row<PARENTS>(a, wu, wb, wb + wd, L, FL, Fn, k, s_hist, lane);
             }
-            const int32_t excl = incl - pc;
-            const int32_t total = __shfl(incl, 63);
-            k.cand += (lane == 0) ? (unsigned long long)total : 0ull;
-            for (int32_t base = 0; base < total; base += 64) {
-                int wi, bi;
-                const int32_t v = batch_vertex(cw, excl, total, base, word0, lane, wi, bi);
-                int32_t beg = 0, end = 0;
-                if (v >= 0) { beg = a.bu_rp[v]; end = a.bu_rp[v + 1]; }
-                int32_t par = -1;
-                const bool big = v >= 0 && (end - beg) > kBuCoop;
-                if (v >= 0 && !big) {
-                    for (int32_t e = beg; e < end; e += 8) {
-                        k.scan += (unsigned long long)min(8, end - e);
-                        int32_t uu[8];
-                        uint32_t bw[8];
-                        #pragma unroll
-                        for (int q = 0; q < 8; q++) uu[q] = (e + q < end) ? a.bu_col[e + q] : -1;
-                        #pragma unroll
-                        for (int q = 0; q < 8; q++) bw[q] = uu[q] >= 0 ? FL[uu[q] >> 5] : 0u;
-                        #pragma unroll
-                        for (int q = 0; q < 8; q++) {
-                            if (par < 0 && uu[q] >= 0 && ((bw[q] >> (uu[q] & 31)) & 1u)) par = uu[q];
-                        }
-                        if (par >= 0) break;
-                    }
+            if (deg >= kWaveRow) deg = 0;
+            // short rows: scan-based gather over the batch, kUnroll entries per lane
+            int32_t rtot;
+            const int32_t rex = wave_excl_scan(deg, lane, rtot);
+            for (int32_t off = 0; off < rtot; off += 64 * kUnroll) {
+                int32_t uu[kUnroll], vv[kUnroll];
+                #pragma unroll
+                for (int q = 0; q < kUnroll; q++) {
+                    const int32_t e = off + q * 64 + lane;
+                    const int o = owner_lane(rex, e);
+                    const int32_t ob = __shfl(beg, o), oe = __shfl(rex, o), ou = __shfl(u, o);
+                    uu[q] = ou;
+                    vv[q] = e < rtot ? a.td_col[ob + (e - oe)] : -1;
                 }
-                uint64_t mb = __ballot(big);
-                while (mb) {
-                    const int j = __ffsll((unsigned long long)mb) - 1;
-                    mb &= mb - 1;
-                    const int32_t jb = __shfl(beg, j), je = __shfl(end, j);
-                    int32_t found_u = -1;
-                    for (int32_t off = jb; off < je; off += 64) {
-                        const int32_t e = off + lane;
-                        bool hit = false;
-                        int32_t uu = 0;
-                        if (e < je) { uu = a.bu_col[e]; hit = (FL[uu >> 5] >> (uu & 31)) & 1u; }
-                        const uint64_t hm = __ballot(hit);
-                        if (lane == j) k.scan += (unsigned long long)min(64, je - off);
-                        if (hm) { found_u = __shfl(uu, __ffsll((unsigned long long)hm) - 1); break; }
-                    }
-                    if (lane == j) par = found_u;
-                }
-                if (par >= 0) {
-                    a.dist[v] = L + 1;
-                    if (PARENTS) a.parent[v] = (uint32_t)par;
-                    atomicOr(&s_fn[wid][wi], 1u << bi);
-                }
-                account(par >= 0, v);
+                visit_n<PARENTS, kUnroll>(a, uu, vv, L, FL, Fn, k, s_hist);
             }
+        }
+    }
+}
+
+// A candidate of the parent scan waiting in a wavefront's LDS queue.
+struct Pending {
+    int32_t v, b, e, b0;   // vertex, next entry, end of row, start of row
+};
+
+// Records a found vertex v (parent par, tile offset off, row [b0, e)).
+template <bool PARENTS>
+__device__ inline void bu_found(const StepArgs& a, int32_t v, int32_t par, int32_t off, int32_t b0, int32_t e,
+                                int32_t L, Counters& k, uint32_t* s_hist, uint32_t* s_fn) {
+    a.dist[v] = L + 1;
+    if (PARENTS) a.parent[v] = (uint32_t)par;
+    atomicOr(&s_fn[off >> 5], 1u << (off & 31));
+    account_new(a, v, e - b0, k, s_hist);
+}
+
+// Second stage of the parent scan: up to 64 queued candidates, one per lane,
+// scanning four entries per round trip from entry b onwards (the whole
+// wavefront scans rows with more than kBuCoop entries left).
+template <bool PARENTS>
+__device__ void bu_drain(const StepArgs& a, const Pending* q, int n, int32_t v0, int32_t L, const uint32_t* FL,
+                         Counters& k, uint32_t* s_hist, uint32_t* s_fn, int lane) {
+    int32_t v = -1, b = 0, e = 0, b0 = 0;
+    if (lane < n) { const Pending p = q[lane]; v = p.v; b = p.b; e = p.e; b0 = p.b0; }
+    int32_t par = -1;
+    const bool big = v >= 0 && (e - b) > kBuCoop;
+    if (v >= 0 && !big) {
+        for (int32_t x = b; x < e; x += 4) {
+            k.scan += (unsigned long long)min(4, e - x);
+            int32_t uu[4];
+            uint32_t bw[4];
+            #pragma unroll
+            for (int r = 0; r < 4; r++) uu[r] = (x + r < e) ? a.bu_col[x + r] : -1;
+            #pragma unroll
+            for (int r = 0; r < 4; r++) bw[r] = uu[r] >= 0 ? FL[uu[r] >> 5] : 0u;
+            #pragma unroll
+            for (int r = 0; r < 4; r++)
+                if (par < 0 && uu[r] >= 0 && ((bw[r] >> (uu[r] & 31)) & 1u)) par = uu[r];
+            if (par >= 0) break;
+        }
+    }
+    uint64_t mb = __ballot(big);
+    while (mb) {
+        const int j = __ffsll((unsigned long long)mb) - 1;
+        mb &= mb - 1;
+        const int32_t jb = __shfl(b, j), je = __shfl(e, j);
+        int32_t found_u = -1;
+        for (int32_t off = jb; off < je; off += 64) {
+            const int32_t x = off + lane;
+            bool hit = false;
+            int32_t uu = 0;
+            if (x < je) { uu = a.bu_col[x]; hit = (FL[uu >> 5] >> (uu & 31)) & 1u; }
+            const uint64_t hm = __ballot(hit);
+            if (lane == j) k.scan += (unsigned long long)min(64, je - off);
+            if (hm) { found_u = __shfl(uu, __ffsll((unsigned long long)hm) - 1); break; }
+        }
+        if (lane == j) par = found_u;
+    }
+    if (par >= 0) bu_found<PARENTS>(a, v, par, v - v0, b0, e, L, k, s_hist, s_fn);
+}
+
+// Bottom-up step for one wavefront.  Candidates are the unvisited vertices
+// with a non-empty reverse row; each scans its sorted reverse row for the
+// first neighbour in F_k, which is its minimum-id parent.  Stage one tests
+// every candidate's first entry; candidates left open go to a per-wavefront
+// LDS queue that bu_drain empties 64 at a time.
+template <bool PARENTS>
+__device__ void bu_step(const StepArgs& a, int32_t L, const uint32_t* FL, uint32_t* Fn, uint32_t* Fc,
+                        Counters& k, uint32_t* s_hist, uint32_t* s_fn, Pending* s_q, int gw, int nw, int lane) {
+    const int32_t ntiles = (a.nwords + kTileWords - 1) / kTileWords;
+    const int sh8 = (lane & 3) * kLaneBits;
+    // the next tile's words are loaded before the current tile is scanned
+    int32_t t = gw;
+    uint32_t fl_n = 0, vs_n = 0, nz_n = 0;
+    auto load_tile = [&](int32_t tt) {
+        const int32_t wn = tt * kTileWords + (lane >> 2);
+        const bool ok = tt < ntiles && wn < a.nwords;
+        fl_n = ok ? FL[wn] : 0u;
+        vs_n = ok ? a.vis[wn] : 0u;
+        nz_n = ok ? a.bu_nz[wn] : 0u;
+    };
+    load_tile(t);
+    while (t < ntiles) {
+        const int32_t word0 = t * kTileWords;
+        const int32_t w = word0 + (lane >> 2);
+        uint32_t cw = 0;
+        if (w < a.nwords) {
+            const uint32_t visw = vs_n | fl_n;
+            cw = ((~visw & nz_n) >> sh8) & 0xffu;   // unvisited with a non-empty reverse row
+            if ((lane & 3) == 0) {
+                if (fl_n) a.vis[w] = visw;          // fold F_k into vis
+                Fc[w] = 0;                          // clear F_{k+2}
+            }
+        }
+        t += nw;
+        load_tile(t);
+        if (lane < kTileWords) s_fn[lane] = 0;
+        int32_t total;
+        const int32_t excl = wave_excl_scan(__popc(cw), lane, total);
+        k.cand += (lane == 0) ? (unsigned long long)total : 0ull;
+        const int32_t v0 = word0 * 32;
+        int qn = 0;   // queued candidates (uniform over the wavefront)
+        for (int32_t base = 0; base < total; base += 64) {
+            int lo, bit;
+            const int32_t v = batch_vertex(cw, excl, total, base, v0, lane, lo, bit);
+            int32_t beg = 0, end = 0;
+            if (v >= 0) { beg = a.bu_rp[v]; end = a.bu_rp[v + 1]; }
+            // stage one: the first entry
+            int32_t par = -1;
+            bool open = false;
+            if (v >= 0 && end > beg) {
+                const int32_t u0 = a.bu_col[beg];
+                k.scan += 1ull;
+                if ((FL[u0 >> 5] >> (u0 & 31)) & 1u) par = u0;
+                else open = end - beg > 1;
+            }
+            if (par >= 0) bu_found<PARENTS>(a, v, par, v - v0, beg, end, L, k, s_hist, s_fn);
+            const uint64_t om = __ballot(open);
+            if (open) {
+                const int pos = qn + (int)__popcll(om & ((1ull << lane) - 1ull));
+                s_q[pos] = Pending{v, beg + 1, end, beg};
+            }
+            qn += (int)__popcll(om);
+            if (qn >= 64) {
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                bu_drain<PARENTS>(a, s_q, 64, v0, L, FL, k, s_hist, s_fn, lane);
+                qn -= 64;
+                Pending keep{-1, 0, 0, 0};
+                if (lane < qn) keep = s_q[64 + lane];
+                __builtin_amdgcn_wave_barrier();
+                if (lane < qn) s_q[lane] = keep;
+            }
+        }
+        if (qn > 0) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            if (w < a.nwords) Fn[w] = s_fn[wid][lane];
-            __builtin_amdgcn_wave_barrier();
+            bu_drain<PARENTS>(a, s_q, qn, v0, L, FL, k, s_hist, s_fn, lane);
         }
-    }
-
-    // block reduction -> this block's partials slot (plain stores)
-    unsigned long long r[5] = {k.n, k.out, k.in, k.scan, k.cand};
-    #pragma unroll
-    for (int i = 0; i < 5; i++)
-        for (int d = 32; d >= 1; d >>= 1) r[i] += __shfl_down(r[i], d);
-    if (lane == 0)
-        for (int i = 0; i < 5; i++) s_red[wid][i] = r[i];
-    __syncthreads();
-    if (tid < 5) {
-        unsigned long long s = 0;
-        for (int q = 0; q < kWaves; q++) s += s_red[q][tid];
-        a.part[(size_t)blockIdx.x * kPart + tid] = s;
-    } else if (tid >= P_HIST && tid < P_HIST + kHistBins) {
-        a.part[(size_t)blockIdx.x * kPart + tid] = s_hist[tid - P_HIST];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < kTileWords && word0 + lane < a.nwords) Fn[word0 + lane] = s_fn[lane];
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
 // The learned policy's weights as lane j of wavefront 0 needs them (hidden
-// unit j), loaded at the start of the finalise kernel so that their latency
-// overlaps the partial-sum loads.  The device copy has the feature
-// standardisation folded into W1 and b1 (done on the host when the policy
-// is set), so inference is 16 FMAs per lane and a butterfly sum.
+// unit j), loaded with the prologue's other loads.  The device copy has the
+// feature standardisation folded into W1 and b1 (done on the host when the
+// policy is set), so inference is 16 FMAs per lane and a butterfly sum.
 struct PolicyRegs {
     float4 w1[kFeat / 4];
     float b1, w2a, w2b, b2a, b2b;
@@ -409,150 +527,223 @@ __device__ inline void load_policy(const PolicyWeights* w, int t, PolicyRegs& r)
     r.b2b = w->b2[1];
 }
 
-// Direction for the next step (called by every thread of the finalise block).
-__device__ int choose_direction(const StepArgs& a, const BfsCtrl* c, const float* feat, const PolicyRegs& pr) {
-    if (c->forced >= 0) return c->forced;
-    if (c->policy_mode == LGMI_POLICY_TOPDOWN) return 0;
-    if (c->policy_mode == LGMI_POLICY_LEARNED && a.policy != nullptr) {
-        // wavefront 0: lane j = hidden unit j; logits by a butterfly sum
-        const int t = threadIdx.x;
-        __shared__ int s_dec;
-        if (t < kHidden) {
-            const float* xs = feat;
-            float acc = pr.b1;
-            #pragma unroll
-            for (int q = 0; q < kFeat / 4; q++) {
-                const float4 r = pr.w1[q];
-                acc += r.x * xs[4 * q] + r.y * xs[4 * q + 1] + r.z * xs[4 * q + 2] + r.w * xs[4 * q + 3];
-            }
-            const float hj = acc > 0.f ? acc : 0.f;
-            float a0 = pr.w2a * hj, a1 = pr.w2b * hj;
-            for (int o = 32; o >= 1; o >>= 1) {
-                a0 += __shfl_xor(a0, o);
-                a1 += __shfl_xor(a1, o);
-            }
-            if (t == 0) s_dec = (a1 + pr.b2b) > (a0 + pr.b2a) ? 1 : 0;
-        }
-        __syncthreads();
-        return s_dec;
-    }
-    // Beamer alpha/beta (GAP form): leave top-down when the frontier's edges
-    // exceed the unvisited edges / alpha; leave bottom-up when the frontier
-    // shrinks and holds at most nv / beta vertices.
-    if (c->prev_dir == 0)
-        return ((double)c->m_front > (double)c->m_unvisited / (double)c->alpha) ? 1 : 0;
-    const bool shrinking = c->n_front < c->n_prev;
-    const bool small = (double)c->n_front <= (double)c->nv / (double)c->beta;
+// Beamer alpha/beta (GAP form): leave top-down when the frontier's edges
+// exceed the unvisited edges / alpha; leave bottom-up when the frontier
+// shrinks and holds at most nv / beta vertices.
+__device__ inline int heuristic_direction(const BfsCtrl& c) {
+    if (c.policy_mode == LGMI_POLICY_TOPDOWN) return 0;
+    if (c.prev_dir == 0)
+        return ((double)c.m_front > (double)c.m_unvisited / (double)c.alpha) ? 1 : 0;
+    const bool shrinking = c.n_front < c.n_prev;
+    const bool small = (double)c.n_front <= (double)c.nv / (double)c.beta;
     return (shrinking && small) ? 0 : 1;
 }
 
-// One block: sums the step's partials in a fixed order, updates the control
-// block and the step log, and picks the next direction.
-__global__ __launch_bounds__(kFinBlock) void bfs_finalize(StepArgs a, int32_t nblocks) {
-    __shared__ unsigned long long s_sum[kFinBlock / kPart][kPart];
-    __shared__ int32_t s_done;
-    __shared__ float s_feat[kFeat];
-    BfsCtrl* c = a.ctrl;
-    const int tid = threadIdx.x;
-    if (tid == 0) s_done = c->done;
-    __syncthreads();
-    if (s_done) return;
-    const uint64_t now = wall_clock64();
-    PolicyRegs pr;
-    if (a.policy != nullptr && tid < kHidden && c->forced < 0 && c->policy_mode == LGMI_POLICY_LEARNED)
-        load_policy(a.policy, tid, pr);
-    // thread t reads slot t % kPart of blocks t / kPart, + 64, ... (coalesced rows)
-    constexpr int kRows = kFinBlock / kPart;
-    const int slot = tid % kPart, row = tid / kPart;
-    // integer counters: eight independent loads in flight per thread
-    unsigned long long acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int b = row;
-    for (; b + 7 * kRows < nblocks; b += 8 * kRows) {
-        #pragma unroll
-        for (int q = 0; q < 8; q++) acc[q] += a.part[(size_t)(b + q * kRows) * kPart + slot];
-    }
-    for (; b < nblocks; b += kRows) acc[0] += a.part[(size_t)b * kPart + slot];
-    unsigned long long s = 0;
+// MLP logits by wavefront 0 (lane j = hidden unit j); returns the
+// direction in every lane.
+__device__ inline int mlp_direction(const float* xs, const PolicyRegs& pr) {
+    float acc = pr.b1;
     #pragma unroll
-    for (int q = 0; q < 8; q++) s += acc[q];
-    s_sum[row][slot] = s;
-    __syncthreads();
-    for (int h = kRows / 2; h >= 1; h >>= 1) {
-        if (row < h) s_sum[row][slot] += s_sum[row + h][slot];
-        __syncthreads();
+    for (int q = 0; q < kFeat / 4; q++) {
+        const float4 r = pr.w1[q];
+        acc += r.x * xs[4 * q] + r.y * xs[4 * q + 1] + r.z * xs[4 * q + 2] + r.w * xs[4 * q + 3];
     }
-    if (tid == 0) {
-        const unsigned long long* tot = s_sum[0];
-        const int32_t L = c->level;
-        const int32_t n_next = (int32_t)tot[P_NEXT];
-        if (L < a.max_log) {
-            LevelRec* r = &a.log[L];
-            r->dir = c->dir;
-            r->n_front = c->n_front;
-            r->m_front = c->m_front;
-            r->m_unvisited = c->m_unvisited;
-            r->n_next = n_next;
-            r->scanned = (int64_t)tot[P_SCAN];
-            r->cand = (int64_t)tot[P_CAND];
-            r->t_start = c->t_step_start;
-            r->t_end = now;
-            r->t_prev_end = c->t_prev_end;
+    const float hj = acc > 0.f ? acc : 0.f;
+    float a0 = pr.w2a * hj, a1 = pr.w2b * hj;
+    for (int o = 32; o >= 1; o >>= 1) {
+        a0 += __shfl_xor(a0, o);
+        a1 += __shfl_xor(a1, o);
+    }
+    return (a1 + pr.b2b) > (a0 + pr.b2a) ? 1 : 0;
+}
+
+// One level of the search.  karg >= 0: the level p itself; karg < 0
+// (captured graphs): p is unknown to the host but has parity -1 - karg, and
+// p is read from the state the launch before stored.  forced >= 0 forces
+// the direction.
+template <bool PARENTS>
+__global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, int32_t forced) {
+    __shared__ BfsCtrl s_c;
+    __shared__ unsigned long long s_red[kWaves][kPart];
+    __shared__ unsigned long long s_sum[kPart];
+    __shared__ float s_feat[kFeat];
+    __shared__ int32_t s_p, s_dir, s_exit, s_dl;
+    __shared__ uint32_t s_hist[kHistBins];
+    __shared__ uint32_t s_fn[kWaves][kTileWords];
+    __shared__ Pending s_q[kWaves][128];
+
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const uint64_t t_now = wall_clock64();
+    const int lpar = karg >= 0 ? (karg & 1) : (-1 - karg);   // parity of p
+    const bool first = karg == 0;
+    BfsShared* sh = a.sh;
+
+    // ---- prologue: the loads below are independent of each other ----
+    unsigned long long acc = 0;
+    if (!first) {
+        const unsigned long long* P = a.part + (size_t)(lpar ^ 1) * a.grid * kPart;
+        const int slot = tid % kPart, row = tid / kPart;
+        unsigned long long x0 = 0, x1 = 0, x2 = 0, x3 = 0;
+        int b = row;
+        for (; b + 3 * kPartRows < a.grid; b += 4 * kPartRows) {
+            x0 += P[(size_t)b * kPart + slot];
+            x1 += P[(size_t)(b + kPartRows) * kPart + slot];
+            x2 += P[(size_t)(b + 2 * kPartRows) * kPart + slot];
+            x3 += P[(size_t)(b + 3 * kPartRows) * kPart + slot];
         }
-        c->t_prev_end = now;
-        c->n_prev = c->n_front;
-        c->n_front = n_next;
-        c->m_front = (int64_t)tot[P_OUT];
-        c->m_unvisited -= (int64_t)tot[P_IN];
-        c->n_unvisited -= n_next;
-        c->prev_dir = c->dir;
-        c->level = L + 1;
-        for (int b = 0; b < kHistBins; b++) c->front_hist[b] = (uint32_t)tot[P_HIST + b];
-        if (n_next == 0) c->done = 1;
-        else compute_features(c, s_feat);
-        s_done = c->done;
+        for (; b < a.grid; b += kPartRows) x0 += P[(size_t)b * kPart + slot];
+        acc = (x0 + x1) + (x2 + x3);
+    }
+    PolicyRegs pr;
+    const bool may_learn = a.policy != nullptr && forced < 0 && !first;
+    if (may_learn && tid < kHidden) load_policy(a.policy, tid, pr);
+    if (tid == 0) {
+        s_c = sh->state[lpar ^ 1];   // the first launch reads S_0 from state[1]
+        s_dl = sh->done_level;
+    }
+    if (tid < kHistBins) s_hist[tid] = 0;
+    // lanes l, l+16, l+32, l+48 of a wavefront hold the same slot
+    acc += __shfl_xor(acc, 16);
+    acc += __shfl_xor(acc, 32);
+    if (lane < kPart) s_red[wid][lane] = acc;
+    __syncthreads();
+    if (tid < kPart) {
+        unsigned long long s = 0;
+        for (int q = 0; q < kWaves; q++) s += s_red[q][tid];
+        s_sum[tid] = s;
     }
     __syncthreads();
-    if (s_done) return;
-    const int nd = choose_direction(a, c, s_feat, pr);
     if (tid == 0) {
-        c->dir = nd;
-        const int32_t L1 = c->level;
-        if (L1 < a.max_log)
-            for (int i = 0; i < kFeat; i++) a.log[L1].feat[i] = s_feat[i];
+        const int32_t p = first ? 0 : (karg > 0 ? karg : s_c.phase + 1);
+        int ex = s_dl <= p ? 1 : 0;
+        if (!ex && !first) {
+            // S_k from S_{k-1} and the counters of level k-1
+            const BfsCtrl& q = s_c;
+            BfsCtrl c = q;
+            const int32_t k = q.level + 1;
+            const int32_t n_next = (int32_t)s_sum[P_NEXT];
+            c.level = k;
+            c.n_prev = q.n_front;
+            c.n_front = n_next;
+            c.prev_dir = q.dir;
+            c.m_front = (int64_t)s_sum[P_OUT];
+            c.m_unvisited = q.m_unvisited - (int64_t)s_sum[P_IN];
+            c.n_unvisited = q.n_unvisited - n_next;
+            c.phase = p;
+            c.t_main = t_now;
+            for (int b = 0; b < kHistBins; b++) c.front_hist[b] = (uint32_t)s_sum[P_HIST + b];
+            if (blockIdx.x == 0 && k - 1 < a.max_log) {
+                LevelRec* r = &a.log[k - 1];
+                r->dir = q.dir;
+                r->n_front = q.n_front;
+                r->m_front = q.m_front;
+                r->m_unvisited = q.m_unvisited;
+                r->n_next = n_next;
+                r->scanned = (int64_t)s_sum[P_SCAN];
+                r->cand = (int64_t)s_sum[P_CAND];
+                r->t_start = q.t_main;
+                r->t_end = t_now;
+                r->t_prev_end = q.t_main;
+            }
+            if (n_next == 0) {
+                ex = 1;
+                if (blockIdx.x == 0) {
+                    sh->state[p & 1] = c;
+                    sh->done_level = p;
+                }
+            } else {
+                compute_features(&c, s_feat);
+            }
+            s_c = c;
+        }
+        s_p = p;
+        s_exit = ex;
+    }
+    __syncthreads();
+    if (s_exit) return;
+    const int32_t p = s_p;
+    const int32_t L = s_c.level;
+    if (may_learn) {
+        if (wid == 0) {
+            const int d = mlp_direction(s_feat, pr);
+            if (lane == 0) s_dir = d;
+        }
+    } else if (tid == 0) {
+        s_dir = forced >= 0 ? forced : (first ? s_c.dir : heuristic_direction(s_c));
+    }
+    __syncthreads();
+    const int dir = s_dir;
+    if (blockIdx.x == 0 && tid == 0) {
+        BfsCtrl c = s_c;
+        c.phase = p;
+        c.dir = dir;
+        if (first) c.t_main = t_now;
+        if (!first && L < a.max_log)
+            for (int i = 0; i < kFeat; i++) a.log[L].feat[i] = s_feat[i];
+        sh->state[p & 1] = c;
+    }
+
+    const uint32_t* FL = bm_at(a, L % 3);
+    uint32_t* Fn = bm_at(a, (L + 1) % 3);
+    uint32_t* Fc = bm_at(a, (L + 2) % 3);
+    const int gw = blockIdx.x * kWaves + wid, nw = gridDim.x * kWaves;
+    Counters k;
+    if (dir == 1) bu_step<PARENTS>(a, L, FL, Fn, Fc, k, s_hist, s_fn[wid], s_q[wid], gw, nw, lane);
+    else td_step<PARENTS>(a, s_c, L, FL, Fn, Fc, k, s_hist, gw, nw, lane);
+
+    // block reduction -> this block's counters (plain stores)
+    unsigned long long r[5] = {k.n, k.out, k.in, k.scan, k.cand};
+    #pragma unroll
+    for (int i = 0; i < 5; i++)
+        for (int d = 32; d >= 1; d >>= 1) r[i] += __shfl_down(r[i], d);
+    if (lane == 0)
+        for (int i = 0; i < 5; i++) s_red[wid][i] = r[i];
+    __syncthreads();
+    unsigned long long* out = a.part + ((size_t)lpar * a.grid + blockIdx.x) * kPart;
+    if (tid < 5) {
+        unsigned long long s = 0;
+        for (int q = 0; q < kWaves; q++) s += s_red[q][tid];
+        out[tid] = s;
+    } else if (tid >= P_HIST && tid < P_HIST + kHistBins) {
+        out[tid] = s_hist[tid - P_HIST];
     }
 }
 
-__device__ void init_ctrl(const StepArgs& a, int32_t nsrc, unsigned long long m_out, unsigned long long m_in,
-                          const uint32_t* hist, int64_t nnz_scan, int32_t forced, int32_t mode, float alpha,
-                          float beta) {
-    BfsCtrl* c = a.ctrl;
-    c->level = 0;
-    c->dir = forced >= 0 ? forced : 0;
-    c->done = nsrc == 0 ? 1 : 0;
-    c->n_front = nsrc;
-    c->n_prev = 0;
-    c->prev_dir = 0;
-    c->m_front = (int64_t)m_out;
-    c->m_unvisited = nnz_scan - (int64_t)m_in;
-    c->n_unvisited = (int64_t)a.nv - nsrc;
-    for (int b = 0; b < kHistBins; b++) c->front_hist[b] = hist[b];
-    c->t_prev_end = wall_clock64();
-    c->t_step_start = c->t_prev_end;
-    c->forced = forced;
-    c->policy_mode = mode;
-    c->alpha = alpha;
-    c->beta = beta;
-    c->nv = a.nv;
-    c->nnz_scan = nnz_scan;
+// S_0 of a search from nsrc sources (level-0 counters given).
+__device__ void init_state(const StepArgs& a, int32_t nsrc, unsigned long long m_out, unsigned long long m_in,
+                           const uint32_t* hist, int64_t nnz_scan, int32_t forced, int32_t mode, float alpha,
+                           float beta) {
+    BfsCtrl c;
+    c.level = 0;
+    c.dir = forced >= 0 ? forced : 0;
+    c.n_front = nsrc;
+    c.n_prev = 0;
+    c.prev_dir = 0;
+    c.policy_mode = mode;
+    c.m_front = (int64_t)m_out;
+    c.m_unvisited = nnz_scan - (int64_t)m_in;
+    c.n_unvisited = (int64_t)a.nv - nsrc;
+    for (int b = 0; b < kHistBins; b++) c.front_hist[b] = hist[b];
+    c.alpha = alpha;
+    c.beta = beta;
+    c.nv = a.nv;
+    c.phase = -1;
+    c.nnz_scan = nnz_scan;
+    c.pad = 0;
+    c.t_main = wall_clock64();
+    a.sh->state[0] = c;
+    a.sh->state[1] = c;      // what the first launch reads
+    a.sh->done_level = nsrc == 0 ? 0 : INT_MAX;
     if (a.max_log > 0) {
         float f[kFeat];
-        compute_features(c, f);
+        compute_features(&c, f);
         for (int i = 0; i < kFeat; i++) a.log[0].feat[i] = f[i];
     }
 }
 
+
 // Clears the search state.  With src >= 0 it also seeds that single source
-// and fills the control block (no separate seed launch).
+// and stores S_0 (no separate seed launch).
 __global__ void bfs_init(StepArgs a, int32_t src, int64_t nnz_scan, int32_t forced, int32_t mode, float alpha,
                          float beta) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -569,14 +760,14 @@ __global__ void bfs_init(StepArgs a, int32_t src, int64_t nnz_scan, int32_t forc
         const int32_t din = a.bu_rp[src + 1] - a.bu_rp[src];
         uint32_t hist[kHistBins] = {0, 0, 0, 0, 0, 0, 0, 0};
         hist[hist_bin(dout)] = 1;
-        init_ctrl(a, 1, (unsigned long long)dout, (unsigned long long)din, hist, nnz_scan, forced, mode, alpha, beta);
+        init_state(a, 1, (unsigned long long)dout, (unsigned long long)din, hist, nnz_scan, forced, mode, alpha, beta);
     }
 }
 
-// Marks a list of distinct sources and fills the control block.  One block.
-__global__ __launch_bounds__(kBlock) void bfs_seed(StepArgs a, const int32_t* srcs, int32_t nsrc,
-                                                   int64_t nnz_scan, int32_t forced, int32_t mode, float alpha,
-                                                   float beta) {
+// Marks a list of distinct sources and stores S_0.  One block.
+__global__ __launch_bounds__(256) void bfs_seed(StepArgs a, const int32_t* srcs, int32_t nsrc,
+                                                int64_t nnz_scan, int32_t forced, int32_t mode, float alpha,
+                                                float beta) {
     __shared__ unsigned long long s_out, s_in;
     __shared__ uint32_t s_hist[kHistBins];
     const int tid = threadIdx.x;
@@ -584,7 +775,7 @@ __global__ __launch_bounds__(kBlock) void bfs_seed(StepArgs a, const int32_t* sr
     if (tid < kHistBins) s_hist[tid] = 0;
     __syncthreads();
     unsigned long long mo = 0, mi = 0;
-    for (int i = tid; i < nsrc; i += kBlock) {
+    for (int i = tid; i < nsrc; i += blockDim.x) {
         const int32_t s = srcs[i];
         a.dist[s] = 0;
         if (a.parent) a.parent[s] = (uint32_t)s;
@@ -597,10 +788,10 @@ __global__ __launch_bounds__(kBlock) void bfs_seed(StepArgs a, const int32_t* sr
     atomicAdd(&s_out, mo);
     atomicAdd(&s_in, mi);
     __syncthreads();
-    if (tid == 0) init_ctrl(a, nsrc, s_out, s_in, s_hist, nnz_scan, forced, mode, alpha, beta);
+    if (tid == 0) init_state(a, nsrc, s_out, s_in, s_hist, nnz_scan, forced, mode, alpha, beta);
 }
 
-// Undoes one step: every vertex of F_{L+1} (`Fn`) returns to unvisited and
+// Undoes one step: every vertex of F_{k+1} (`Fn`) returns to unvisited and
 // Fn is cleared.  Used by the direction probe to run the other direction on
 // the identical state.
 __global__ void bfs_undo(StepArgs a, uint32_t* Fn) {
@@ -626,8 +817,8 @@ void bfs_free(BfsWork* w) {
     if (!w) return;
     dev_free(w->dist); dev_free(w->parent); dev_free(w->srcbuf); dev_free(w->part);
     dev_free(w->bm[0]); dev_free(w->bm[1]); dev_free(w->bm[2]); dev_free(w->vis);
-    dev_free(w->ctrl); dev_free(w->log); dev_free(w->policy);
-    if (w->h_ctrl) (void)hipHostFree(w->h_ctrl);
+    dev_free(w->sh); dev_free(w->log); dev_free(w->policy);
+    if (w->h_sh) (void)hipHostFree(w->h_sh);
     if (w->ev0) (void)hipEventDestroy(w->ev0);
     if (w->ev1) (void)hipEventDestroy(w->ev1);
     for (auto e : w->step_ev) (void)hipEventDestroy(e);
@@ -642,23 +833,23 @@ void bfs_prepare(lgmi_graph* g) {
     try {
         w->nv = g->nv;
         w->nwords = 2 * ((g->nv + 63) / 64);
-        w->grid = std::max(1, g->cu_count) * 8;
+        w->grid = std::max(1, g->cu_count);
         w->dist = dev_alloc_n<int32_t>(g->nv);
         w->parent = dev_alloc_n<int32_t>(g->nv);
         w->srcbuf = dev_alloc_n<int32_t>((size_t)g->nv + 1);
-        w->part = dev_alloc_n<unsigned long long>((size_t)w->grid * kPart);
+        w->part = dev_alloc_n<unsigned long long>((size_t)2 * w->grid * kPart);
         for (int k = 0; k < 3; k++) w->bm[k] = dev_alloc_n<uint32_t>(std::max(w->nwords, 2));
         w->vis = dev_alloc_n<uint32_t>(std::max(w->nwords, 2));
-        w->ctrl = dev_alloc_n<BfsCtrl>(1);
+        w->sh = dev_alloc_n<BfsShared>(1);
         w->max_log = 16384;   // covers a 4096 x 4096 grid (up to 8190 levels)
         w->log = dev_alloc_n<LevelRec>(w->max_log);
         w->policy = dev_alloc_n<PolicyWeights>(1);
-        LGMI_HIP(hipHostMalloc((void**)&w->h_ctrl, sizeof(BfsCtrl), hipHostMallocDefault));
+        LGMI_HIP(hipHostMalloc((void**)&w->h_sh, sizeof(BfsShared), hipHostMallocDefault));
         LGMI_HIP(hipEventCreate(&w->ev0));
         LGMI_HIP(hipEventCreate(&w->ev1));
-        LGMI_HIP(hipMemsetAsync(w->ctrl, 0, sizeof(BfsCtrl), g->stream));
+        LGMI_HIP(hipMemsetAsync(w->sh, 0, sizeof(BfsShared), g->stream));
         LGMI_HIP(hipMemsetAsync(w->log, 0, sizeof(LevelRec) * w->max_log, g->stream));
-        LGMI_HIP(hipMemsetAsync(w->part, 0, sizeof(unsigned long long) * (size_t)w->grid * kPart, g->stream));
+        LGMI_HIP(hipMemsetAsync(w->part, 0, sizeof(unsigned long long) * 2 * (size_t)w->grid * kPart, g->stream));
     } catch (...) {
         bfs_free(w);
         throw;
@@ -672,7 +863,8 @@ static StepArgs make_args(lgmi_graph* g, int dir_in, bool parents) {
     const DevCSR& td = dir_in ? g->in : g->out;
     const DevCSR& bu = dir_in ? g->out : g->in;
     a.td_rp = td.row_ptr; a.td_col = td.col;
-    a.td_item_v = td.item_v; a.td_item_beg = td.item_beg; a.td_items = td.n_items;
+    a.td_item_v = td.item_v; a.td_item_beg = td.item_beg; a.td_item_end = td.item_end;
+    a.td_items = td.n_items;
     a.bu_rp = bu.row_ptr; a.bu_col = bu.col; a.bu_nz = bu.nz_bm;
     a.nv = g->nv;
     a.nwords = w->nwords;
@@ -680,12 +872,18 @@ static StepArgs make_args(lgmi_graph* g, int dir_in, bool parents) {
     a.parent = parents ? reinterpret_cast<uint32_t*>(w->parent) : nullptr;
     a.bm0 = w->bm[0]; a.bm1 = w->bm[1]; a.bm2 = w->bm[2];
     a.vis = w->vis;
-    a.ctrl = w->ctrl;
+    a.sh = w->sh;
     a.part = w->part;
+    a.grid = w->grid;
     a.log = w->log;
     a.max_log = w->max_log;
     a.policy = w->policy;
     return a;
+}
+
+static void launch_level(const StepArgs& a, bool parents, int32_t karg, int32_t forced, hipStream_t s) {
+    if (parents) bfs_level<true><<<a.grid, kBlock, 0, s>>>(a, karg, forced);
+    else bfs_level<false><<<a.grid, kBlock, 0, s>>>(a, karg, forced);
 }
 
 void policy_upload(lgmi_graph* g) {
@@ -715,17 +913,19 @@ void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_
     if (mode != LGMI_POLICY_LEARNED) a.policy = nullptr;
     const int64_t nnz_scan = g->nnz;  // the reverse CSR has as many entries as the forward one
     hipStream_t s = g->stream;
-    const int grid = w->grid;
     if (single_src < 0 && nsrc > 0)
         LGMI_HIP(hipMemcpyAsync(w->srcbuf, h_srcs, sizeof(int32_t) * (size_t)nsrc, hipMemcpyHostToDevice, s));
     LGMI_HIP(hipEventRecord(w->ev0, s));
-    const int init_grid = std::max(1, std::min<int>(grid, (g->nv + 255) / 256));
+    const int init_grid = std::max(1, std::min<int>(w->grid * 8, (g->nv + 255) / 256));
     bfs_init<<<init_grid, 256, 0, s>>>(a, single_src, nnz_scan, forced_dir, mode, alpha, beta);
     if (single_src < 0)
-        bfs_seed<<<1, kBlock, 0, s>>>(a, w->srcbuf, nsrc, nnz_scan, forced_dir, mode, alpha, beta);
+        bfs_seed<<<1, 256, 0, s>>>(a, w->srcbuf, nsrc, nnz_scan, forced_dir, mode, alpha, beta);
     LGMI_HIP(hipGetLastError());
-    // Enqueue steps in batches; a step after the last level exits at once.
-    int batch = std::max(4, std::min(w->last_levels + 1, 64));
+    // Enqueue level launches in batches (even-sized, so that a captured
+    // graph always starts on an even level); a launch after the last level
+    // exits at once.
+    int batch = std::max(4, std::min(w->last_launches_used + 1, 64));
+    batch = (batch + 1) & ~1;
     int launched = 0;
     auto event_at = [&](size_t i) {
         while (w->step_ev.size() <= i) {
@@ -739,18 +939,15 @@ void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_
         const char* e = std::getenv("LGMI_BFS_GRAPHS");
         return !(e && e[0] == '0');
     }();
+    const bool parents = want_parents != 0;
     for (;;) {
-        if (graphs_on && !w->time_steps && launched >= kGraphLevels) {
-            // deep search: replay captured step/finalise sequences
-            const int key = (want_parents ? 1 : 0) | (dir_in ? 2 : 0) | (a.policy ? 4 : 0);
+        if (graphs_on && !w->time_steps && launched >= kGraphLevels && (launched & 1) == 0) {
+            // deep search: replay captured level sequences
+            const int key = (parents ? 1 : 0) | (dir_in ? 2 : 0) | (a.policy ? 4 : 0) | ((forced_dir + 1) << 3);
             if (!w->gexec[key]) {
                 hipGraph_t gr = nullptr;
                 LGMI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                for (int i = 0; i < kGraphLevels; i++) {
-                    if (want_parents) bfs_step<true><<<grid, kBlock, 0, s>>>(a);
-                    else bfs_step<false><<<grid, kBlock, 0, s>>>(a);
-                    bfs_finalize<<<1, kFinBlock, 0, s>>>(a, grid);
-                }
+                for (int i = 0; i < kGraphLevels; i++) launch_level(a, parents, -1 - (i & 1), forced_dir, s);
                 LGMI_HIP(hipStreamEndCapture(s, &gr));
                 hipGraphExec_t ex = nullptr;
                 const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
@@ -765,22 +962,22 @@ void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_
             for (int i = 0; i < batch; i++) {
                 const size_t li = (size_t)(launched + i);
                 if (w->time_steps) event_at(2 * li);
-                if (want_parents) bfs_step<true><<<grid, kBlock, 0, s>>>(a);
-                else bfs_step<false><<<grid, kBlock, 0, s>>>(a);
+                launch_level(a, parents, launched + i, forced_dir, s);
                 if (w->time_steps) event_at(2 * li + 1);
-                bfs_finalize<<<1, kFinBlock, 0, s>>>(a, grid);
             }
             launched += batch;
         }
         LGMI_HIP(hipGetLastError());
-        LGMI_HIP(hipMemcpyAsync(w->h_ctrl, w->ctrl, sizeof(BfsCtrl), hipMemcpyDeviceToHost, s));
+        LGMI_HIP(hipMemcpyAsync(w->h_sh, w->sh, sizeof(BfsShared), hipMemcpyDeviceToHost, s));
         LGMI_HIP(hipEventRecord(w->ev1, s));
         LGMI_HIP(hipEventSynchronize(w->ev1));
-        if (w->h_ctrl->done) break;
+        if (w->h_sh->done_level < launched) break;
         if (launched > g->nv + 8) throw Error(LGMI_E_HIP, "BFS did not terminate");
         batch = std::min(batch * 2, 1024);
     }
-    w->last_levels = w->h_ctrl->level;
+    const int levels = w->h_sh->state[w->h_sh->done_level & 1].level;
+    w->last_levels = levels;
+    w->last_launches_used = w->h_sh->done_level;
     w->last_dir_in = dir_in;
     w->last_parents = want_parents;
     w->last_launches = launched;
@@ -798,11 +995,11 @@ void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_
         float ms = 0.f;
         LGMI_HIP(hipEventElapsedTime(&ms, w->ev0, w->ev1));
         std::memset(stats, 0, sizeof(*stats));
-        stats->levels = w->h_ctrl->level;
+        stats->levels = levels;
         stats->device_ms = ms;
         stats->launches = launched;
         stats->steps_event_ms = w->last_steps_event_ms;
-        const int nrec = std::min(w->h_ctrl->level, w->max_log);
+        const int nrec = std::min(levels, w->max_log);
         if (nrec > 0) {
             std::vector<LevelRec> recs((size_t)nrec);
             LGMI_HIP(hipMemcpy(recs.data(), w->log, sizeof(LevelRec) * (size_t)nrec, hipMemcpyDeviceToHost));
@@ -893,6 +1090,10 @@ extern "C" int lgmi_bfs_time_steps(lgmi_graph_t g, int enable) {
 // ----------------------------------------------------------------------
 // direction probe: both directions timed on the identical level state
 // ----------------------------------------------------------------------
+
+// ----------------------------------------------------------------------
+// direction probe: both directions timed on the identical level state
+// ----------------------------------------------------------------------
 extern "C" int lgmi_bfs_probe(lgmi_graph_t g, int32_t src, int dir_in, const int32_t* levels, int32_t nlevels,
                               lgmi_probe_rec* out) {
     using namespace lgmi;
@@ -907,77 +1108,72 @@ extern "C" int lgmi_bfs_probe(lgmi_graph_t g, int32_t src, int dir_in, const int
         StepArgs a = make_args(g, dir_in, true);
         a.policy = nullptr;
         hipStream_t s = g->stream;
-        const int grid = w->grid;
-        const int init_grid = std::max(1, std::min<int>(grid, (g->nv + 255) / 256));
+        const int init_grid = std::max(1, std::min<int>(w->grid * 8, (g->nv + 255) / 256));
         bfs_init<<<init_grid, 256, 0, s>>>(a, src, g->nnz, -1, LGMI_POLICY_HEURISTIC, 15.0f, 18.0f);
         LGMI_HIP(hipGetLastError());
-        hipEvent_t ev[4];
+        hipEvent_t ev[2];
         for (auto& e : ev) LGMI_HIP(hipEventCreate(&e));
-        auto sync_ctrl = [&]() {
-            LGMI_HIP(hipMemcpyAsync(w->h_ctrl, w->ctrl, sizeof(BfsCtrl), hipMemcpyDeviceToHost, s));
+        auto sync_sh = [&]() {
+            LGMI_HIP(hipMemcpyAsync(w->h_sh, w->sh, sizeof(BfsShared), hipMemcpyDeviceToHost, s));
             LGMI_HIP(hipStreamSynchronize(s));
         };
-        auto run_forced = [&](int dir, hipEvent_t e0, hipEvent_t e1) {
-            BfsCtrl c = *w->h_ctrl;
-            c.dir = dir;
-            LGMI_HIP(hipMemcpyAsync(w->ctrl, &c, sizeof(BfsCtrl), hipMemcpyHostToDevice, s));
-            LGMI_HIP(hipEventRecord(e0, s));
-            bfs_step<true><<<grid, kBlock, 0, s>>>(a);
-            LGMI_HIP(hipEventRecord(e1, s));
-            bfs_finalize<<<1, kFinBlock, 0, s>>>(a, grid);
-            LGMI_HIP(hipGetLastError());
-            LGMI_HIP(hipStreamSynchronize(s));
-        };
+        int32_t launched = 0;   // level launches so far: the next one is level `launched`
+        auto finished = [&]() { return w->h_sh->done_level < launched; };
         int nout = 0;
         try {
-            sync_ctrl();
-            for (int32_t li = 0; li < nlevels && !w->h_ctrl->done; li++) {
+            sync_sh();
+            for (int32_t li = 0; li < nlevels; li++) {
                 const int32_t target = levels[li];
-                while (!w->h_ctrl->done && w->h_ctrl->level < target) {
-                    const int n = std::min(target - w->h_ctrl->level, 256);
-                    for (int i = 0; i < n; i++) {
-                        bfs_step<true><<<grid, kBlock, 0, s>>>(a);
-                        bfs_finalize<<<1, kFinBlock, 0, s>>>(a, grid);
-                    }
+                while (!finished() && launched < target) {
+                    const int n = std::min(target - launched, 256);
+                    for (int i = 0; i < n; i++) launch_level(a, true, launched + i, -1, s);
+                    launched += n;
                     LGMI_HIP(hipGetLastError());
-                    sync_ctrl();
+                    sync_sh();
                 }
-                if (w->h_ctrl->done || w->h_ctrl->level != target) break;
-                const BfsCtrl saved = *w->h_ctrl;
-                const int32_t L = saved.level;
+                if (finished()) break;
+                // Level `target` is next.  A launch of level L reads only what
+                // level L-1 wrote, so it can run again on the same inputs after
+                // the new vertices are undone.  Each direction runs kReps
+                // times, in alternating order; the minimum is kept, so one
+                // slow launch (a cold start after the host synchronised) does
+                // not decide the label.  The launch finds the frontier empty
+                // when the search ends at this level.
+                const int32_t L = target;
                 uint32_t* Fn = w->bm[(L + 1) % 3];
-                // Each direction runs kReps times on the restored state, in
-                // alternating order; the minimum is kept, so one slow launch
-                // (a cold start after the host synchronised) does not decide
-                // the label.
                 constexpr int kReps = 3;
                 float best[2] = {1e30f, 1e30f};
                 const int first = ((L + src) & 1) ? 1 : 0;
-                for (int rep = 0; rep < kReps; rep++) {
+                bool ended = false;
+                for (int rep = 0; rep < kReps && !ended; rep++) {
                     for (int q = 0; q < 2; q++) {
                         const int dir = (first + q + rep) & 1;
-                        *w->h_ctrl = saved;
-                        run_forced(dir, ev[0], ev[1]);
+                        LGMI_HIP(hipEventRecord(ev[0], s));
+                        launch_level(a, true, L, dir, s);
+                        LGMI_HIP(hipEventRecord(ev[1], s));
+                        LGMI_HIP(hipGetLastError());
+                        sync_sh();
+                        if (w->h_sh->done_level <= L) { ended = true; break; }
                         float t = 0.f;
                         LGMI_HIP(hipEventElapsedTime(&t, ev[0], ev[1]));
                         best[dir] = std::min(best[dir], t);
                         if (!(rep == kReps - 1 && q == 1)) {
-                            bfs_undo<<<grid, 256, 0, s>>>(a, Fn);
+                            bfs_undo<<<init_grid, 256, 0, s>>>(a, Fn);
                             LGMI_HIP(hipGetLastError());
                         }
                     }
                 }
-                const float t_first = best[first], t_second = best[1 - first];
+                launched = L + 1;
+                if (ended) break;
                 LevelRec rec;
                 LGMI_HIP(hipMemcpy(&rec, w->log + std::min(L, w->max_log - 1), sizeof(LevelRec), hipMemcpyDeviceToHost));
                 lgmi_probe_rec& o = out[nout++];
                 std::memset(&o, 0, sizeof(o));
                 for (int q = 0; q < kFeat; q++) o.feat[q] = rec.feat[q];
-                o.td_us = 1000.f * (first == 0 ? t_first : t_second);
-                o.bu_us = 1000.f * (first == 0 ? t_second : t_first);
+                o.td_us = 1000.f * best[0];
+                o.bu_us = 1000.f * best[1];
                 o.level = L;
-                o.n_front = saved.n_front;
-                sync_ctrl();
+                o.n_front = w->h_sh->state[L & 1].n_front;
             }
         } catch (...) {
             for (auto& e : ev) (void)hipEventDestroy(e);

@@ -98,19 +98,26 @@ static void upload_csr(lgmi_graph* g, DevCSR& d, const int32_t* rp, const int32_
         nz_bitmap_kernel<<<(nwords + 255) / 256, 256, 0, g->stream>>>(d.row_ptr, g->nv, nwords, d.nz_bm);
         LGMI_HIP(hipGetLastError());
     }
-    // rows longer than kHeavyRow -> items of kHeavyRow entries
-    std::vector<int32_t> iv, ib;
+    // rows longer than kHeavyRow -> items of at most kHeavyRow entries
+    std::vector<int32_t> iv, ib, ie;
     for (int32_t v = 0; v < g->nv; v++) {
         const int32_t b = rp[v], e = rp[v + 1];
         if (e - b > kHeavyRow)
-            for (int32_t x = b; x < e; x += kHeavyRow) { iv.push_back(v); ib.push_back(x); }
+            for (int32_t x = b; x < e; x += kHeavyRow) {
+                iv.push_back(v);
+                ib.push_back(x);
+                ie.push_back(std::min(e, x + kHeavyRow));
+            }
     }
+    if (iv.size() > (size_t)0x7fffffff) throw Error(LGMI_E_TOO_LARGE, "too many heavy-row items");
     d.n_items = (int32_t)iv.size();
     if (d.n_items > 0) {
         d.item_v = dev_alloc_n<int32_t>(iv.size());
         d.item_beg = dev_alloc_n<int32_t>(ib.size());
+        d.item_end = dev_alloc_n<int32_t>(ie.size());
         LGMI_HIP(hipMemcpyAsync(d.item_v, iv.data(), sizeof(int32_t) * iv.size(), hipMemcpyHostToDevice, g->stream));
         LGMI_HIP(hipMemcpyAsync(d.item_beg, ib.data(), sizeof(int32_t) * ib.size(), hipMemcpyHostToDevice, g->stream));
+        LGMI_HIP(hipMemcpyAsync(d.item_end, ie.data(), sizeof(int32_t) * ie.size(), hipMemcpyHostToDevice, g->stream));
         LGMI_HIP(hipStreamSynchronize(g->stream));   // the host vectors go out of scope
     }
 }
@@ -122,10 +129,10 @@ static void graph_free(lgmi_graph* g) {
     bc_free(g->bc);
     if (g->in.row_ptr != g->out.row_ptr) {
         dev_free(g->in.row_ptr); dev_free(g->in.col); dev_free(g->in.nz_bm);
-        dev_free(g->in.item_v); dev_free(g->in.item_beg);
+        dev_free(g->in.item_v); dev_free(g->in.item_beg); dev_free(g->in.item_end);
     }
     dev_free(g->out.row_ptr); dev_free(g->out.col); dev_free(g->out.nz_bm);
-    dev_free(g->out.item_v); dev_free(g->out.item_beg);
+    dev_free(g->out.item_v); dev_free(g->out.item_beg); dev_free(g->out.item_end);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
 }

@@ -41,33 +41,40 @@ T* dev_alloc_n(size_t n) {
 }
 
 // ---------------------------------------------------------------------
-// BFS control block (device resident).  One step kernel advances the BFS
-// by one level; the last work-group to finish a step updates this block
-// and chooses the direction of the next step.
+// BFS level state (device resident).  One launch of the level kernel per
+// BFS level k: every work-group sums the per-block counters of step k-1,
+// derives the state S_k from S_{k-1}, picks the direction of step k and
+// runs it; work-group 0 also stores S_k (state[k & 1]) and the level log.
 // ---------------------------------------------------------------------
 constexpr int kFeat = LGMI_POLICY_FEATURES;  // per-level policy features
 constexpr int kHidden = LGMI_POLICY_HIDDEN;
 constexpr int kHistBins = 8;                 // log4-degree histogram of a frontier
 
 struct BfsCtrl {
-    int32_t level;          // L: the frontier holds the vertices at distance L
-    int32_t dir;            // direction of step L -> L+1 (0 top-down, 1 bottom-up)
-    int32_t done;
-    int32_t n_front;        // |F_L|
-    int32_t n_prev;         // |F_{L-1}|
+    int32_t level;          // k: the frontier holds the vertices at distance k
+    int32_t dir;            // direction of step k -> k+1 (0 top-down, 1 bottom-up)
+    int32_t n_front;        // |F_k|
+    int32_t n_prev;         // |F_{k-1}|
     int32_t prev_dir;
-    int64_t m_front;        // sum of traversal-direction degrees over F_L
+    int32_t policy_mode;    // LGMI_POLICY_*
+    int64_t m_front;        // sum of traversal-direction degrees over F_k
     int64_t m_unvisited;    // sum of scan-direction degrees over unvisited vertices
     int64_t n_unvisited;
-    uint32_t front_hist[kHistBins];   // degree histogram of F_L (policy feature)
-    uint64_t t_prev_end;    // wall clock at the end of the previous step
-    uint64_t t_step_start;  // wall clock when block 0 of the current step started
-    int32_t forced;         // -1 policy decides; 0/1 force a direction (label runs)
-    int32_t policy_mode;    // LGMI_POLICY_*
+    uint32_t front_hist[kHistBins];   // degree histogram of F_k (policy feature)
     float alpha, beta;
     int32_t nv;
-    int32_t pad;
+    int32_t phase;          // launch (level) index that stored this state
     int64_t nnz_scan;
+    int32_t pad;
+    int32_t pad2;
+    uint64_t t_main;        // wall clock at the start of level k's launch
+};
+
+// Words shared between the launches of one search and the host.
+struct BfsShared {
+    int32_t done_level;     // level whose launch found F_k empty (INT32_MAX while running)
+    int32_t pad[3];
+    BfsCtrl state[2];       // S_k by k & 1
 };
 
 struct LevelRec {
@@ -94,8 +101,9 @@ struct PolicyWeights {      // FP32 MLP 16 -> 64 (ReLU) -> 2
     float sd[kFeat];
 };
 
-// Per-block counters of one step, reduced in block order by the finalise
-// kernel (plain stores: no same-address atomics across work-groups).
+// Per-block counters of one step (plain stores: no same-address atomics
+// across work-groups); every work-group of the next level's launch sums
+// them.
 constexpr int kPart = 16;
 enum PartSlot { P_NEXT = 0, P_OUT = 1, P_IN = 2, P_SCAN = 3, P_CAND = 4, P_HIST = 5 };  // P_HIST .. +7
 
@@ -106,22 +114,23 @@ struct BfsWork {
     int32_t* dist = nullptr;
     int32_t* parent = nullptr;
     int32_t* srcbuf = nullptr;   // multi-source lists
-    unsigned long long* part = nullptr;  // [grid][kPart]
+    unsigned long long* part = nullptr;  // [2][grid][kPart], by level parity
     uint32_t* bm[3] = {nullptr, nullptr, nullptr};
     uint32_t* vis = nullptr;
-    BfsCtrl* ctrl = nullptr;
-    BfsCtrl* h_ctrl = nullptr;   // pinned host mirror
+    BfsShared* sh = nullptr;
+    BfsShared* h_sh = nullptr;   // pinned host mirror
     LevelRec* log = nullptr;
     int32_t max_log = 0;
     PolicyWeights* policy = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool time_steps = false;              // HIP events around every step launch
-    hipGraphExec_t gexec[8] = {};         // captured step/finalise sequences (parents, dir_in, learned)
+    hipGraphExec_t gexec[32] = {};        // captured level sequences (parents, dir_in, learned, forced)
     std::vector<hipEvent_t> step_ev;      // launch i runs between step_ev[i] and step_ev[i+1]
     int last_launches = 0;
     double last_steps_event_ms = 0.0;
     std::vector<float> last_launch_ms;
     int last_levels = 0;
+    int last_launches_used = 0;        // launch index that finished the last search
     int last_dir_in = 0;
     int last_parents = 0;
     ~BfsWork();
@@ -129,15 +138,16 @@ struct BfsWork {
 
 struct BcWork;
 // Rows longer than kHeavyRow are split into work items of kHeavyRow
-// entries so that a hub vertex is expanded by many work-groups at once.
-constexpr int32_t kHeavyRow = 1024;
+// entries so that a hub vertex is expanded by many wavefronts at once.
+constexpr int32_t kHeavyRow = 256;
 
 struct DevCSR {
     int32_t* row_ptr = nullptr;
     int32_t* col = nullptr;
     uint32_t* nz_bm = nullptr;   // bit v set when the row of v is non-empty
     int32_t* item_v = nullptr;   // heavy-row work items: vertex
-    int32_t* item_beg = nullptr; //   and first entry (at most kHeavyRow entries each)
+    int32_t* item_beg = nullptr; //   first entry
+    int32_t* item_end = nullptr; //   and one past the last (at most kHeavyRow entries)
     int32_t n_items = 0;
 };
 
