@@ -67,7 +67,8 @@ typedef struct {
     int64_t m_front;           /* traversal-direction edges of F_L           */
     int64_t m_unvisited;       /* scan-direction edges of unvisited vertices */
     int32_t n_next;            /* |F_{L+1}|                                  */
-    int32_t pad;
+    int32_t variant;           /* 1: top-down in range mode (each work-group */
+                               /* owns a vertex range), 0: otherwise         */
     int64_t scanned;           /* bottom-up: adjacency entries read;         */
                                /* top-down: edges expanded                   */
     int64_t cand;              /* bottom-up: unvisited vertices examined     */

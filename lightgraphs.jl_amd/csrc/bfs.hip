@@ -69,6 +69,8 @@ constexpr int32_t kBuCoop = 64;     // bottom-up: rows longer than this are scan
 constexpr int kUnroll = 4;          // top-down: adjacency entries per lane per round
 constexpr int kGraphLevels = 16;    // level launches per captured graph (even: parity is static)
 constexpr int kPartRows = kBlock / kPart;
+constexpr int32_t kRangeFront = 4096;    // range mode: frontier vertices at most
+constexpr int64_t kRangeLight = 32768;   // range mode: light-row frontier edges at most
 
 struct StepArgs {
     const int32_t* td_rp;     // traversal adjacency (top-down expands it)
@@ -77,6 +79,9 @@ struct StepArgs {
     const int32_t* td_item_beg;
     const int32_t* td_item_end;
     int32_t td_items;
+    const int32_t* td_heavy_id; // range table of the traversal adjacency (range_size 0: none)
+    const int32_t* td_rtab;
+    int32_t range_size;
     const int32_t* bu_rp;     // reverse adjacency (bottom-up scans it)
     const int32_t* bu_col;
     const uint32_t* bu_nz;    // bit v: row v of the reverse adjacency is non-empty
@@ -88,6 +93,7 @@ struct StepArgs {
     uint32_t* bm1;
     uint32_t* bm2;
     uint32_t* vis;
+    int32_t* qbuf;            // [2][grid][kQCap] new vertices per work-group, by level parity
     BfsShared* sh;
     unsigned long long* part; // [2][grid][kPart] by launch parity
     int32_t grid;
@@ -181,7 +187,7 @@ __device__ void compute_features(const BfsCtrl* c, float* f) {
 }
 
 struct Counters {
-    unsigned long long n = 0, out = 0, in = 0, scan = 0, cand = 0;
+    unsigned long long n = 0, out = 0, in = 0, scan = 0, cand = 0, light = 0;
 };
 
 // A new vertex v of F_{k+1}: its degrees feed the next level's state.
@@ -197,7 +203,21 @@ __device__ inline void account_new(const StepArgs& a, int32_t v, int32_t din_kno
     k.n++;
     k.out += (unsigned long long)dout;
     k.in += (unsigned long long)din;
+    if (dout <= kHeavyRow) k.light += (unsigned long long)dout;
     atomicAdd(&s_hist[hist_bin(dout)], 1u);
+}
+
+// Appends the lanes' new vertices (flag set) to the work-group's queue, the
+// first kQCap of them; s_hist[kHistBins] counts the places taken.  One LDS
+// atomic per wavefront; every lane of the wavefront calls it.
+__device__ inline void q_push(bool flag, int32_t v, uint32_t* s_hist, int32_t* qout, int lane) {
+    const uint64_t m = __ballot(flag);
+    if (!m) return;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&s_hist[kHistBins], (uint32_t)__popcll(m));
+    base = __shfl(base, 0);
+    const uint32_t pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    if (flag && pos < (uint32_t)kQCap) qout[pos] = v;
 }
 
 // Top-down visit of N edges u[q] -> v[q] (v[q] < 0: none).  All loads of the
@@ -207,7 +227,8 @@ __device__ inline void account_new(const StepArgs& a, int32_t v, int32_t din_kno
 // without parents the claim is an atomicOr on F_{k+1}.
 template <bool PARENTS, int N>
 __device__ inline void visit_n(const StepArgs& a, const int32_t* u, const int32_t* v, int32_t L,
-                               const uint32_t* FL, uint32_t* Fn, Counters& k, uint32_t* s_hist) {
+                               const uint32_t* FL, uint32_t* Fn, Counters& k, uint32_t* s_hist, int32_t* qout,
+                               int lane) {
     uint32_t seen[N], pre[N];
     #pragma unroll
     for (int q = 0; q < N; q++) {
@@ -243,6 +264,7 @@ __device__ inline void visit_n(const StepArgs& a, const int32_t* u, const int32_
             a.dist[v[q]] = L + 1;
             account_new(a, v[q], -1, k, s_hist);
         }
+        q_push(won, v[q], s_hist, qout, lane);
     }
 }
 
@@ -250,7 +272,8 @@ __device__ inline void visit_n(const StepArgs& a, const int32_t* u, const int32_
 // wavefront, kUnroll entries per lane per round.
 template <bool PARENTS>
 __device__ inline void expand_row(const StepArgs& a, int32_t u, int32_t b, int32_t e, int32_t L,
-                                  const uint32_t* FL, uint32_t* Fn, Counters& k, uint32_t* s_hist, int lane) {
+                                  const uint32_t* FL, uint32_t* Fn, Counters& k, uint32_t* s_hist, int32_t* qout,
+                                  int lane) {
     for (int32_t x0 = b; x0 < e; x0 += 64 * kUnroll) {
         int32_t uu[kUnroll], vv[kUnroll];
         #pragma unroll
@@ -259,14 +282,14 @@ __device__ inline void expand_row(const StepArgs& a, int32_t u, int32_t b, int32
             uu[q] = u;
             vv[q] = x < e ? a.td_col[x] : -1;
         }
-        visit_n<PARENTS, kUnroll>(a, uu, vv, L, FL, Fn, k, s_hist);
+        visit_n<PARENTS, kUnroll>(a, uu, vv, L, FL, Fn, k, s_hist, qout, lane);
     }
 }
 
 // Top-down step L -> L+1 of one wavefront (gw of nw).
 template <bool PARENTS>
 __device__ void td_step(const StepArgs& a, const BfsCtrl& c, int32_t L, const uint32_t* FL, uint32_t* Fn,
-                        uint32_t* Fc, Counters& k, uint32_t* s_hist, int gw, int nw, int lane) {
+                        uint32_t* Fc, Counters& k, uint32_t* s_hist, int32_t* qout, int gw, int nw, int lane) {
     const int32_t ntiles = (a.nwords + kTileWords - 1) / kTileWords;
     const int sh8 = (lane & 3) * kLaneBits;
     // Heavy rows: 256-entry items; lane j of wavefront gw tests item
@@ -289,7 +312,7 @@ __device__ void td_step(const StepArgs& a, const BfsCtrl& c, int32_t L, const ui
                 m &= m - 1;
                 const int32_t u = __shfl(iu, j), b = __shfl(ib, j), e = __shfl(ie, j);
                 if (lane == 0) k.scan += (unsigned long long)(e - b);
-                expand_row<PARENTS>(a, u, b, e, L, FL, Fn, k, s_hist, lane);
+                expand_row<PARENTS>(a, u, b, e, L, FL, Fn, k, s_hist, qout, lane);
             }
         }
     }
@@ -337,7 +360,7 @@ __device__ void td_step(const StepArgs& a, const BfsCtrl& c, int32_t L, const ui
                 const int j = __ffsll((unsigned long long)m) - 1;
                 m &= m - 1;
                 const int32_t wu = __shfl(u, j), wb = __shfl(beg, j), wd = __shfl(deg, j);
-                expand_row<PARENTS>(a, wu, wb, wb + wd, L, FL, Fn, k, s_hist, lane);
+                expand_row<PARENTS>(a, wu, wb, wb + wd, L, FL, Fn, k, s_hist, qout, lane);
             }
             if (deg >= kWaveRow) deg = 0;
             // short rows: scan-based gather over the batch, kUnroll entries per lane
@@ -353,7 +376,7 @@ __device__ void td_step(const StepArgs& a, const BfsCtrl& c, int32_t L, const ui
                     uu[q] = ou;
                     vv[q] = e < rtot ? a.td_col[ob + (e - oe)] : -1;
                 }
-                visit_n<PARENTS, kUnroll>(a, uu, vv, L, FL, Fn, k, s_hist);
+                visit_n<PARENTS, kUnroll>(a, uu, vv, L, FL, Fn, k, s_hist, qout, lane);
             }
         }
     }
@@ -379,7 +402,7 @@ __device__ inline void bu_found(const StepArgs& a, int32_t v, int32_t par, int32
 // wavefront scans rows with more than kBuCoop entries left).
 template <bool PARENTS>
 __device__ void bu_drain(const StepArgs& a, const Pending* q, int n, int32_t v0, int32_t L, const uint32_t* FL,
-                         Counters& k, uint32_t* s_hist, uint32_t* s_fn, int lane) {
+                         Counters& k, uint32_t* s_hist, int32_t* qout, uint32_t* s_fn, int lane) {
     int32_t v = -1, b = 0, e = 0, b0 = 0;
     if (lane < n) { const Pending p = q[lane]; v = p.v; b = p.b; e = p.e; b0 = p.b0; }
     int32_t par = -1;
@@ -417,6 +440,7 @@ __device__ void bu_drain(const StepArgs& a, const Pending* q, int n, int32_t v0,
         if (lane == j) par = found_u;
     }
     if (par >= 0) bu_found<PARENTS>(a, v, par, v - v0, b0, e, L, k, s_hist, s_fn);
+    q_push(par >= 0, v, s_hist, qout, lane);
 }
 
 // Bottom-up step for one wavefront.  Candidates are the unvisited vertices
@@ -426,7 +450,8 @@ __device__ void bu_drain(const StepArgs& a, const Pending* q, int n, int32_t v0,
 // LDS queue that bu_drain empties 64 at a time.
 template <bool PARENTS>
 __device__ void bu_step(const StepArgs& a, int32_t L, const uint32_t* FL, uint32_t* Fn, uint32_t* Fc,
-                        Counters& k, uint32_t* s_hist, uint32_t* s_fn, Pending* s_q, int gw, int nw, int lane) {
+                        Counters& k, uint32_t* s_hist, int32_t* qout, uint32_t* s_fn, Pending* s_q, int gw, int nw,
+                        int lane) {
     const int32_t ntiles = (a.nwords + kTileWords - 1) / kTileWords;
     const int sh8 = (lane & 3) * kLaneBits;
     // the next tile's words are loaded before the current tile is scanned
@@ -475,6 +500,7 @@ __device__ void bu_step(const StepArgs& a, int32_t L, const uint32_t* FL, uint32
                 else open = end - beg > 1;
             }
             if (par >= 0) bu_found<PARENTS>(a, v, par, v - v0, beg, end, L, k, s_hist, s_fn);
+            q_push(par >= 0, v, s_hist, qout, lane);
             const uint64_t om = __ballot(open);
             if (open) {
                 const int pos = qn + (int)__popcll(om & ((1ull << lane) - 1ull));
@@ -485,7 +511,7 @@ __device__ void bu_step(const StepArgs& a, int32_t L, const uint32_t* FL, uint32
                 __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
                 __builtin_amdgcn_wave_barrier();
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                bu_drain<PARENTS>(a, s_q, 64, v0, L, FL, k, s_hist, s_fn, lane);
+                bu_drain<PARENTS>(a, s_q, 64, v0, L, FL, k, s_hist, qout, s_fn, lane);
                 qn -= 64;
                 Pending keep{-1, 0, 0, 0};
                 if (lane < qn) keep = s_q[64 + lane];
@@ -497,13 +523,105 @@ __device__ void bu_step(const StepArgs& a, int32_t L, const uint32_t* FL, uint32
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            bu_drain<PARENTS>(a, s_q, qn, v0, L, FL, k, s_hist, s_fn, lane);
+            bu_drain<PARENTS>(a, s_q, qn, v0, L, FL, k, s_hist, qout, s_fn, lane);
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         if (lane < kTileWords && word0 + lane < a.nwords) Fn[word0 + lane] = s_fn[lane];
         __builtin_amdgcn_wave_barrier();
+    }
+}
+
+// Top-down step in range mode, for a small frontier with heavy rows.  Work-
+// group b owns the vertex ids [b * range_size, (b + 1) * range_size): it
+// reads the whole frontier list (the work-group queues of the previous
+// level, s_qoff[] their offsets), takes from every frontier row the entries
+// in its range (heavy rows through the range table, light rows filtered),
+// forms the minimum frontier parent of each unvisited vertex of its range
+// with LDS atomics, and writes distances, parents and its F_{k+1} words
+// with plain stores.  No device-scope atomics are issued.
+template <bool PARENTS>
+__device__ void range_step(const StepArgs& a, int32_t L, const uint32_t* FL, uint32_t* Fn, uint32_t* Fc,
+                           Counters& k, uint32_t* s_hist, int32_t* qout, uint32_t* s_lds, const int32_t* s_qoff,
+                           int32_t nf, int lpar, int tid, int lane, int wid) {
+    const int32_t RS = a.range_size;              // a multiple of 64
+    const int32_t R0 = (int32_t)blockIdx.x * RS;
+    const int32_t nwr = RS / 32;
+    const int32_t w0 = R0 / 32;
+    uint32_t* s_par = s_lds;                      // [RS] minimum frontier parent, 0xffffffff = none
+    uint32_t* s_seen = s_lds + RS;                // [nwr] visited | F_k words of the range
+    for (int32_t i = tid; i < nwr; i += kBlock) {
+        const int32_t w = w0 + i;
+        uint32_t sw = 0xffffffffu;
+        if (w < a.nwords) {
+            const uint32_t fl = FL[w], vs = a.vis[w];
+            sw = vs | fl;
+            if (fl) a.vis[w] = sw;                // fold F_k into vis
+            Fc[w] = 0;                            // clear F_{k+2}
+        }
+        s_seen[i] = sw;
+    }
+    for (int32_t i = tid; i < RS; i += kBlock) s_par[i] = 0xffffffffu;
+    __syncthreads();
+    const int32_t* qin = a.qbuf + (size_t)(lpar ^ 1) * a.grid * kQCap;
+    for (int32_t j0 = wid * 64; j0 < nf; j0 += kWaves * 64) {
+        const int32_t j = j0 + lane;
+        int32_t u = -1, sb = 0, se = 0;
+        if (j < nf) {
+            int lo = 0, hi = a.grid;                  // largest b with s_qoff[b] <= j
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_qoff[mid] <= j) lo = mid;
+                else hi = mid;
+            }
+            u = qin[(size_t)lo * kQCap + (j - s_qoff[lo])];
+            sb = a.td_rp[u];
+            se = a.td_rp[u + 1];
+            if (se - sb > kHeavyRow) {
+                const int32_t* t = a.td_rtab + (size_t)a.td_heavy_id[u] * (a.grid + 1) + blockIdx.x;
+                sb = t[0];
+                se = t[1];
+            }
+        }
+        int32_t tot;
+        const int32_t ex = wave_excl_scan(se - sb, lane, tot);
+        if (lane == 0) k.scan += (unsigned long long)tot;
+        for (int32_t off = 0; off < tot; off += 64 * kUnroll) {
+            int32_t uu[kUnroll], vv[kUnroll];
+            #pragma unroll
+            for (int q = 0; q < kUnroll; q++) {
+                const int32_t e = off + q * 64 + lane;
+                const int o = owner_lane(ex, e);
+                const int32_t ob = __shfl(sb, o), oe = __shfl(ex, o), ou = __shfl(u, o);
+                uu[q] = ou;
+                vv[q] = e < tot ? a.td_col[ob + (e - oe)] : -1;
+            }
+            #pragma unroll
+            for (int q = 0; q < kUnroll; q++) {
+                const int32_t loc = vv[q] - R0;
+                if (vv[q] >= 0 && loc >= 0 && loc < RS && !((s_seen[loc >> 5] >> (loc & 31)) & 1u))
+                    atomicMin(&s_par[loc], (uint32_t)uu[q]);
+            }
+        }
+    }
+    __syncthreads();
+    for (int32_t base = wid * 64; base < RS; base += kWaves * 64) {
+        const int32_t loc = base + lane;
+        const uint32_t par = s_par[loc];
+        const int32_t v = R0 + loc;
+        const bool fresh = par != 0xffffffffu && v < a.nv;
+        const uint64_t m = __ballot(fresh);
+        if (lane < 2) {
+            const int32_t w = w0 + (base >> 5) + lane;
+            if (w < a.nwords) Fn[w] = (uint32_t)(m >> (32 * lane));
+        }
+        if (fresh) {
+            a.dist[v] = L + 1;
+            if (PARENTS) a.parent[v] = par;
+            account_new(a, v, -1, k, s_hist);
+        }
+        q_push(fresh, v, s_hist, qout, lane);
     }
 }
 
@@ -568,9 +686,15 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
     __shared__ unsigned long long s_sum[kPart];
     __shared__ float s_feat[kFeat];
     __shared__ int32_t s_p, s_dir, s_exit, s_dl;
-    __shared__ uint32_t s_hist[kHistBins];
+    __shared__ uint32_t s_hist[kHistBins + 1];   // + the queue counter
     __shared__ uint32_t s_fn[kWaves][kTileWords];
-    __shared__ Pending s_q[kWaves][128];
+    __shared__ int32_t s_qoff[kMaxGrid + 1];      // per-work-group new-vertex counts, then queue offsets
+    // bottom-up: the wavefronts' pending queues; range mode: parents and
+    // visited words of the work-group's range
+    constexpr int kLdsWords = (kRangeMax + kRangeMax / 32) > (kWaves * 128 * 4) ? (kRangeMax + kRangeMax / 32)
+                                                                                : (kWaves * 128 * 4);
+    __shared__ uint32_t s_lds[kLdsWords];
+    Pending* s_q = reinterpret_cast<Pending*>(s_lds);
 
     const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
     const uint64_t t_now = wall_clock64();
@@ -586,12 +710,23 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
         unsigned long long x0 = 0, x1 = 0, x2 = 0, x3 = 0;
         int b = row;
         for (; b + 3 * kPartRows < a.grid; b += 4 * kPartRows) {
-            x0 += P[(size_t)b * kPart + slot];
-            x1 += P[(size_t)(b + kPartRows) * kPart + slot];
-            x2 += P[(size_t)(b + 2 * kPartRows) * kPart + slot];
-            x3 += P[(size_t)(b + 3 * kPartRows) * kPart + slot];
+            const unsigned long long y0 = P[(size_t)b * kPart + slot];
+            const unsigned long long y1 = P[(size_t)(b + kPartRows) * kPart + slot];
+            const unsigned long long y2 = P[(size_t)(b + 2 * kPartRows) * kPart + slot];
+            const unsigned long long y3 = P[(size_t)(b + 3 * kPartRows) * kPart + slot];
+            if (slot == P_NEXT) {
+                s_qoff[b] = (int32_t)y0;
+                s_qoff[b + kPartRows] = (int32_t)y1;
+                s_qoff[b + 2 * kPartRows] = (int32_t)y2;
+                s_qoff[b + 3 * kPartRows] = (int32_t)y3;
+            }
+            x0 += y0; x1 += y1; x2 += y2; x3 += y3;
         }
-        for (; b < a.grid; b += kPartRows) x0 += P[(size_t)b * kPart + slot];
+        for (; b < a.grid; b += kPartRows) {
+            const unsigned long long y0 = P[(size_t)b * kPart + slot];
+            if (slot == P_NEXT) s_qoff[b] = (int32_t)y0;
+            x0 += y0;
+        }
         acc = (x0 + x1) + (x2 + x3);
     }
     PolicyRegs pr;
@@ -601,7 +736,7 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
         s_c = sh->state[lpar ^ 1];   // the first launch reads S_0 from state[1]
         s_dl = sh->done_level;
     }
-    if (tid < kHistBins) s_hist[tid] = 0;
+    if (tid <= kHistBins) s_hist[tid] = 0;
     // lanes l, l+16, l+32, l+48 of a wavefront hold the same slot
     acc += __shfl_xor(acc, 16);
     acc += __shfl_xor(acc, 32);
@@ -629,12 +764,15 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
             c.m_front = (int64_t)s_sum[P_OUT];
             c.m_unvisited = q.m_unvisited - (int64_t)s_sum[P_IN];
             c.n_unvisited = q.n_unvisited - n_next;
+            c.m_light = (int64_t)s_sum[P_LIGHT];
+            c.qovf = s_sum[P_QOVF] != 0 ? 1 : 0;
             c.phase = p;
             c.t_main = t_now;
             for (int b = 0; b < kHistBins; b++) c.front_hist[b] = (uint32_t)s_sum[P_HIST + b];
             if (blockIdx.x == 0 && k - 1 < a.max_log) {
                 LevelRec* r = &a.log[k - 1];
                 r->dir = q.dir;
+                r->variant = q.variant;
                 r->n_front = q.n_front;
                 r->m_front = q.m_front;
                 r->m_unvisited = q.m_unvisited;
@@ -673,10 +811,33 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
     }
     __syncthreads();
     const int dir = s_dir;
+    // range mode: top-down over a small frontier that holds heavy rows and
+    // few light-row edges, when the range table exists and every work-group
+    // queue of the frontier is complete
+    const bool range = dir == 0 && !first && a.range_size > 0 && s_c.qovf == 0 && s_c.n_front <= kRangeFront &&
+                       s_c.m_light <= kRangeLight && s_c.front_hist[5] + s_c.front_hist[6] > 0;
+    if (range) {
+        if (wid == 0) {
+            // queue offsets: exclusive scan of the per-work-group counts
+            int32_t carry = 0;
+            for (int32_t b0 = 0; b0 < a.grid; b0 += 64) {
+                const int32_t b = b0 + lane;
+                const int32_t c = b < a.grid ? s_qoff[b] : 0;
+                int32_t tot;
+                const int32_t ex = wave_excl_scan(c, lane, tot);
+                if (b < a.grid) s_qoff[b] = carry + ex;
+                carry += tot;
+            }
+            if (lane == 0) s_qoff[a.grid] = carry;
+        }
+        __syncthreads();
+    }
+
     if (blockIdx.x == 0 && tid == 0) {
         BfsCtrl c = s_c;
         c.phase = p;
         c.dir = dir;
+        c.variant = range ? 1 : 0;
         if (first) c.t_main = t_now;
         if (!first && L < a.max_log)
             for (int i = 0; i < kFeat; i++) a.log[L].feat[i] = s_feat[i];
@@ -687,25 +848,28 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
     uint32_t* Fn = bm_at(a, (L + 1) % 3);
     uint32_t* Fc = bm_at(a, (L + 2) % 3);
     const int gw = blockIdx.x * kWaves + wid, nw = gridDim.x * kWaves;
+    int32_t* qout = a.qbuf + ((size_t)lpar * a.grid + blockIdx.x) * kQCap;
     Counters k;
-    if (dir == 1) bu_step<PARENTS>(a, L, FL, Fn, Fc, k, s_hist, s_fn[wid], s_q[wid], gw, nw, lane);
-    else td_step<PARENTS>(a, s_c, L, FL, Fn, Fc, k, s_hist, gw, nw, lane);
+    if (dir == 1) bu_step<PARENTS>(a, L, FL, Fn, Fc, k, s_hist, qout, s_fn[wid], s_q + wid * 128, gw, nw, lane);
+    else if (range) range_step<PARENTS>(a, L, FL, Fn, Fc, k, s_hist, qout, s_lds, s_qoff, s_qoff[a.grid], lpar, tid, lane, wid);
+    else td_step<PARENTS>(a, s_c, L, FL, Fn, Fc, k, s_hist, qout, gw, nw, lane);
 
     // block reduction -> this block's counters (plain stores)
-    unsigned long long r[5] = {k.n, k.out, k.in, k.scan, k.cand};
+    unsigned long long r[6] = {k.n, k.out, k.in, k.scan, k.cand, k.light};
     #pragma unroll
-    for (int i = 0; i < 5; i++)
+    for (int i = 0; i < 6; i++)
         for (int d = 32; d >= 1; d >>= 1) r[i] += __shfl_down(r[i], d);
     if (lane == 0)
-        for (int i = 0; i < 5; i++) s_red[wid][i] = r[i];
+        for (int i = 0; i < 6; i++) s_red[wid][i] = r[i];
     __syncthreads();
     unsigned long long* out = a.part + ((size_t)lpar * a.grid + blockIdx.x) * kPart;
-    if (tid < 5) {
+    if (tid < 6) {
         unsigned long long s = 0;
         for (int q = 0; q < kWaves; q++) s += s_red[q][tid];
-        out[tid] = s;
-    } else if (tid >= P_HIST && tid < P_HIST + kHistBins) {
-        out[tid] = s_hist[tid - P_HIST];
+        out[tid < 5 ? tid : P_LIGHT] = s;
+        if (tid == P_NEXT) out[P_QOVF] = s > (unsigned long long)kQCap ? 1ull : 0ull;
+    } else if (tid >= 16 && tid < 16 + kHistBins) {
+        out[P_HIST + tid - 16] = s_hist[tid - 16];
     }
 }
 
@@ -729,7 +893,9 @@ __device__ void init_state(const StepArgs& a, int32_t nsrc, unsigned long long m
     c.nv = a.nv;
     c.phase = -1;
     c.nnz_scan = nnz_scan;
-    c.pad = 0;
+    c.m_light = 0;
+    c.qovf = 1;              // level 0 has no work-group queues
+    c.variant = 0;
     c.t_main = wall_clock64();
     a.sh->state[0] = c;
     a.sh->state[1] = c;      // what the first launch reads
@@ -816,7 +982,7 @@ BfsWork::~BfsWork() {}
 void bfs_free(BfsWork* w) {
     if (!w) return;
     dev_free(w->dist); dev_free(w->parent); dev_free(w->srcbuf); dev_free(w->part);
-    dev_free(w->bm[0]); dev_free(w->bm[1]); dev_free(w->bm[2]); dev_free(w->vis);
+    dev_free(w->bm[0]); dev_free(w->bm[1]); dev_free(w->bm[2]); dev_free(w->vis); dev_free(w->qbuf);
     dev_free(w->sh); dev_free(w->log); dev_free(w->policy);
     if (w->h_sh) (void)hipHostFree(w->h_sh);
     if (w->ev0) (void)hipEventDestroy(w->ev0);
@@ -833,13 +999,14 @@ void bfs_prepare(lgmi_graph* g) {
     try {
         w->nv = g->nv;
         w->nwords = 2 * ((g->nv + 63) / 64);
-        w->grid = std::max(1, g->cu_count);
+        w->grid = std::min(std::max(1, g->cu_count), kMaxGrid);
         w->dist = dev_alloc_n<int32_t>(g->nv);
         w->parent = dev_alloc_n<int32_t>(g->nv);
         w->srcbuf = dev_alloc_n<int32_t>((size_t)g->nv + 1);
         w->part = dev_alloc_n<unsigned long long>((size_t)2 * w->grid * kPart);
         for (int k = 0; k < 3; k++) w->bm[k] = dev_alloc_n<uint32_t>(std::max(w->nwords, 2));
         w->vis = dev_alloc_n<uint32_t>(std::max(w->nwords, 2));
+        w->qbuf = dev_alloc_n<int32_t>((size_t)2 * w->grid * kQCap);
         w->sh = dev_alloc_n<BfsShared>(1);
         w->max_log = 16384;   // covers a 4096 x 4096 grid (up to 8190 levels)
         w->log = dev_alloc_n<LevelRec>(w->max_log);
@@ -865,6 +1032,9 @@ static StepArgs make_args(lgmi_graph* g, int dir_in, bool parents) {
     a.td_rp = td.row_ptr; a.td_col = td.col;
     a.td_item_v = td.item_v; a.td_item_beg = td.item_beg; a.td_item_end = td.item_end;
     a.td_items = td.n_items;
+    a.td_heavy_id = td.heavy_id;
+    a.td_rtab = td.rtab;
+    a.range_size = (td.range_size > 0 && td.range_grid == w->grid) ? td.range_size : 0;
     a.bu_rp = bu.row_ptr; a.bu_col = bu.col; a.bu_nz = bu.nz_bm;
     a.nv = g->nv;
     a.nwords = w->nwords;
@@ -872,6 +1042,7 @@ static StepArgs make_args(lgmi_graph* g, int dir_in, bool parents) {
     a.parent = parents ? reinterpret_cast<uint32_t*>(w->parent) : nullptr;
     a.bm0 = w->bm[0]; a.bm1 = w->bm[1]; a.bm2 = w->bm[2];
     a.vis = w->vis;
+    a.qbuf = w->qbuf;
     a.sh = w->sh;
     a.part = w->part;
     a.grid = w->grid;
@@ -1054,6 +1225,7 @@ extern "C" int lgmi_bfs_steps(lgmi_graph_t g, int32_t max_steps, lgmi_bfs_step* 
             o.m_front = r.m_front;
             o.m_unvisited = r.m_unvisited;
             o.n_next = r.n_next;
+            o.variant = r.variant;
             o.scanned = r.scanned;
             o.cand = r.cand;
             o.kernel_us = (float)((double)(r.t_end - r.t_start) * us_per_tick);

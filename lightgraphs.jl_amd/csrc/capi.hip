@@ -23,6 +23,24 @@ __global__ void nz_bitmap_kernel(const int32_t* rp, int32_t nv, int32_t nwords, 
     nz[w] = bits;
 }
 
+// rtab[h][r] = first entry of heavy row h with neighbour id >= r * rs (a
+// binary search over the sorted row); one thread per (row, boundary).
+__global__ void range_table_kernel(const int32_t* rp, const int32_t* col, const int32_t* heavy_v, int32_t n_heavy,
+                                   int32_t ngrid, int32_t rs, int32_t* tab) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)n_heavy * (ngrid + 1)) return;
+    const int32_t h = (int32_t)(i / (ngrid + 1)), r = (int32_t)(i % (ngrid + 1));
+    const int32_t v = heavy_v[h];
+    int32_t lo = rp[v], hi = rp[v + 1];
+    const int64_t key = (int64_t)r * rs;
+    while (lo < hi) {
+        const int32_t mid = lo + (hi - lo) / 2;
+        if ((int64_t)col[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    tab[i] = lo;
+}
+
 int g_device_count = -1;
 std::mutex g_dev_mu;
 }  // namespace
@@ -110,6 +128,37 @@ static void upload_csr(lgmi_graph* g, DevCSR& d, const int32_t* rp, const int32_
             }
     }
     if (iv.size() > (size_t)0x7fffffff) throw Error(LGMI_E_TOO_LARGE, "too many heavy-row items");
+    // range table for the BFS range mode: one range of vertex ids per
+    // work-group (the BFS grid is one work-group per CU), when a range fits LDS
+    {
+        const int32_t ngrid = std::min(std::max(1, g->cu_count), kMaxGrid);
+        int64_t rs = ((int64_t)g->nv + ngrid - 1) / ngrid;
+        rs = (rs + 63) / 64 * 64;
+        std::vector<int32_t> hv, hid((size_t)g->nv, -1);
+        for (int32_t v = 0; v < g->nv; v++)
+            if (rp[v + 1] - rp[v] > kHeavyRow) { hid[(size_t)v] = (int32_t)hv.size(); hv.push_back(v); }
+        if (g->nv > 0 && rs <= kRangeMax && !hv.empty()) {
+            d.range_grid = ngrid;
+            d.range_size = (int32_t)rs;
+            d.n_heavy = (int32_t)hv.size();
+            d.heavy_id = dev_alloc_n<int32_t>((size_t)g->nv);
+            d.rtab = dev_alloc_n<int32_t>((size_t)d.n_heavy * (ngrid + 1));
+            int32_t* dhv = dev_alloc_n<int32_t>(hv.size());
+            try {
+                LGMI_HIP(hipMemcpyAsync(d.heavy_id, hid.data(), sizeof(int32_t) * hid.size(), hipMemcpyHostToDevice, g->stream));
+                LGMI_HIP(hipMemcpyAsync(dhv, hv.data(), sizeof(int32_t) * hv.size(), hipMemcpyHostToDevice, g->stream));
+                const int64_t n = (int64_t)d.n_heavy * (ngrid + 1);
+                range_table_kernel<<<(unsigned)((n + 255) / 256), 256, 0, g->stream>>>(d.row_ptr, d.col, dhv, d.n_heavy,
+                                                                                      ngrid, d.range_size, d.rtab);
+                LGMI_HIP(hipGetLastError());
+                LGMI_HIP(hipStreamSynchronize(g->stream));
+            } catch (...) {
+                dev_free(dhv);
+                throw;
+            }
+            dev_free(dhv);
+        }
+    }
     d.n_items = (int32_t)iv.size();
     if (d.n_items > 0) {
         d.item_v = dev_alloc_n<int32_t>(iv.size());
@@ -130,9 +179,11 @@ static void graph_free(lgmi_graph* g) {
     if (g->in.row_ptr != g->out.row_ptr) {
         dev_free(g->in.row_ptr); dev_free(g->in.col); dev_free(g->in.nz_bm);
         dev_free(g->in.item_v); dev_free(g->in.item_beg); dev_free(g->in.item_end);
+        dev_free(g->in.heavy_id); dev_free(g->in.rtab);
     }
     dev_free(g->out.row_ptr); dev_free(g->out.col); dev_free(g->out.nz_bm);
     dev_free(g->out.item_v); dev_free(g->out.item_beg); dev_free(g->out.item_end);
+    dev_free(g->out.heavy_id); dev_free(g->out.rtab);
     if (g->stream) (void)hipStreamDestroy(g->stream);
     delete g;
 }

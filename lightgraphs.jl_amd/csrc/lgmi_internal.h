@@ -65,8 +65,9 @@ struct BfsCtrl {
     int32_t nv;
     int32_t phase;          // launch (level) index that stored this state
     int64_t nnz_scan;
-    int32_t pad;
-    int32_t pad2;
+    int64_t m_light;        // traversal-direction edges of F_k's rows of at most kHeavyRow entries
+    int32_t qovf;           // 1: some work-group's queue of F_k overflowed (or there is none)
+    int32_t variant;        // step k: 1 range-mode top-down, 0 otherwise
     uint64_t t_main;        // wall clock at the start of level k's launch
 };
 
@@ -84,7 +85,7 @@ struct LevelRec {
     int64_t m_front;
     int64_t m_unvisited;
     int32_t n_next;
-    int32_t pad;
+    int32_t variant;        // 1: range-mode top-down
     int64_t scanned;        // adjacency entries read (bottom-up) / expanded (top-down)
     int64_t cand;           // bottom-up: unvisited vertices examined
     uint64_t t_start;       // first block start (wall clock ticks)
@@ -105,7 +106,8 @@ struct PolicyWeights {      // FP32 MLP 16 -> 64 (ReLU) -> 2
 // across work-groups); every work-group of the next level's launch sums
 // them.
 constexpr int kPart = 16;
-enum PartSlot { P_NEXT = 0, P_OUT = 1, P_IN = 2, P_SCAN = 3, P_CAND = 4, P_HIST = 5 };  // P_HIST .. +7
+enum PartSlot { P_NEXT = 0, P_OUT = 1, P_IN = 2, P_SCAN = 3, P_CAND = 4, P_HIST = 5,  // P_HIST .. +7
+                P_LIGHT = 13, P_QOVF = 14 };
 
 struct BfsWork {
     int32_t nv = 0;
@@ -117,6 +119,7 @@ struct BfsWork {
     unsigned long long* part = nullptr;  // [2][grid][kPart], by level parity
     uint32_t* bm[3] = {nullptr, nullptr, nullptr};
     uint32_t* vis = nullptr;
+    int32_t* qbuf = nullptr;     // [2][grid][kQCap]: each work-group's new vertices, by level parity
     BfsShared* sh = nullptr;
     BfsShared* h_sh = nullptr;   // pinned host mirror
     LevelRec* log = nullptr;
@@ -149,7 +152,20 @@ struct DevCSR {
     int32_t* item_beg = nullptr; //   first entry
     int32_t* item_end = nullptr; //   and one past the last (at most kHeavyRow entries)
     int32_t n_items = 0;
+    // Range table of the rows longer than kHeavyRow: the vertices are cut
+    // into `range_grid` ranges of range_size ids, and rtab[h][r] is the
+    // first entry of heavy row h whose neighbour id is >= r * range_size.
+    int32_t* heavy_id = nullptr; // [nv]: heavy row index of v, or -1
+    int32_t* rtab = nullptr;     // [n_heavy][range_grid + 1]
+    int32_t n_heavy = 0;
+    int32_t range_grid = 0;
+    int32_t range_size = 0;      // 0: no range table
 };
+
+// Work-group queues and ranges of the BFS range mode.
+constexpr int kQCap = 4096;         // new vertices a work-group records per level (the range-mode frontier limit)
+constexpr int kMaxGrid = 1024;      // BFS work-groups at most
+constexpr int32_t kRangeMax = 32768; // vertices per work-group range at most (LDS)
 
 }  // namespace lgmi
 

@@ -61,7 +61,7 @@ class BfsStats(ctypes.Structure):
 class BfsStep(ctypes.Structure):
     _fields_ = [("feat", ctypes.c_float * 16), ("dir", ctypes.c_int32), ("n_front", ctypes.c_int32),
                 ("m_front", ctypes.c_int64), ("m_unvisited", ctypes.c_int64), ("n_next", ctypes.c_int32),
-                ("pad", ctypes.c_int32), ("scanned", ctypes.c_int64), ("cand", ctypes.c_int64),
+                ("variant", ctypes.c_int32), ("scanned", ctypes.c_int64), ("cand", ctypes.c_int64),
                 ("kernel_us", ctypes.c_float), ("step_us", ctypes.c_float), ("event_ms", ctypes.c_float),
                 ("pad2", ctypes.c_int32)]
 

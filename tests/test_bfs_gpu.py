@@ -207,3 +207,34 @@ def test_rmat20_full_compare(oracle):
     rng = np.random.default_rng(5)
     roots = rng.choice(np.nonzero(np.diff(rp) > 0)[0], size=3).tolist()
     _check_graph(oracle, g, roots, "rmat20")
+
+
+def test_range_mode_levels_exact(oracle):
+    """Top-down levels over a frontier of hub rows run in range mode (each
+    work-group owns a vertex range and forms parents in LDS); distances and
+    parents stay bit-exact, with and without parents, along out- and
+    in-edges."""
+    import ctypes
+
+    import lgmi.generators as gen
+    from lgmi import _lib
+    lib = _lib.load()
+    seen = 0
+    for g, tag in ((gen.rmat(20, 16, seed=2), "rmat20"), (gen.rmat(17, 16, seed=3, directed=True), "rmat17-dir")):
+        rp, col = g.csr()
+        irp, icol = g.in_csr()
+        deg = np.diff(rp)
+        hub = int(np.argmax(deg))
+        # low-degree vertices with an edge to the hub: the hub is in F_1
+        nb = icol[irp[hub]:irp[hub + 1]]
+        nb = nb[deg[nb] > 0]
+        roots = [int(x) for x in nb[np.argsort(deg[nb], kind="stable")[:3]]]
+        _check_graph(oracle, g, roots, tag)
+        buf = (_lib.BfsStep * 256)()
+        st = _lib.BfsStats()
+        for r in roots:
+            _lib.check(lib.lgmi_bfs_run(g.handle(), r, 0, 1, ctypes.byref(st)))
+            n = lib.lgmi_bfs_steps(g.handle(), 256, buf)
+            seen += sum(int(buf[i].variant) for i in range(n))
+    say(f"range-mode levels: {seen}")
+    assert seen > 0, "no level ran in range mode"
