@@ -41,9 +41,9 @@
 //    neighbour in F_k, which is its minimum-id parent.  Stage one tests the
 //    first entry of every candidate (it is the parent for most vertices on
 //    the large levels); the rest wait in a per-wavefront LDS queue and scan
-//    four entries per round trip (a whole wavefront for rows with more than
-//    64 entries left).  The tile's F_{k+1} words are gathered in LDS and
-//    written whole.
+//    four entries per round trip for up to four round trips, after which a
+//    whole wavefront scans each row still open, 64 entries per round trip.
+//    The tile's F_{k+1} words are gathered in LDS and written whole.
 // Both directions give identical distances and parents: the reference's
 // sorted frontier makes parent[v] the smallest u with dist[u] = dist[v]-1
 // and an edge u -> v, which is what either direction computes.
@@ -65,7 +65,6 @@ constexpr int kLaneBits = 8;                    // frontier bits per lane in a t
 constexpr int kTileVerts = 64 * kLaneBits;      // vertices per wavefront tile
 constexpr int kTileWords = kTileVerts / 32;     // bitmap words per tile
 constexpr int32_t kWaveRow = 32;    // top-down: rows this long or longer are expanded by a wavefront
-constexpr int32_t kBuCoop = 64;     // bottom-up: rows longer than this are scanned by a wavefront
 constexpr int kUnroll = 4;          // top-down: adjacency entries per lane per round
 constexpr int kGraphLevels = 16;    // level launches per captured graph (even: parity is static)
 constexpr int kPartRows = kBlock / kPart;
@@ -398,41 +397,44 @@ __device__ inline void bu_found(const StepArgs& a, int32_t v, int32_t par, int32
 }
 
 // Second stage of the parent scan: up to 64 queued candidates, one per lane,
-// scanning four entries per round trip from entry b onwards (the whole
-// wavefront scans rows with more than kBuCoop entries left).
+// scanning four entries per round trip from entry b onwards, for at most
+// kLaneRounds rounds; rows still open after that are scanned by the whole
+// wavefront, 64 entries per round trip.
 template <bool PARENTS>
 __device__ void bu_drain(const StepArgs& a, const Pending* q, int n, int32_t v0, int32_t L, const uint32_t* FL,
                          Counters& k, uint32_t* s_hist, int32_t* qout, uint32_t* s_fn, int lane) {
+    constexpr int kLaneRounds = 4;
     int32_t v = -1, b = 0, e = 0, b0 = 0;
     if (lane < n) { const Pending p = q[lane]; v = p.v; b = p.b; e = p.e; b0 = p.b0; }
     int32_t par = -1;
-    const bool big = v >= 0 && (e - b) > kBuCoop;
-    if (v >= 0 && !big) {
-        for (int32_t x = b; x < e; x += 4) {
+    int32_t x = b;
+    if (v >= 0) {
+        for (int r = 0; r < kLaneRounds && x < e; r++, x += 4) {
             k.scan += (unsigned long long)min(4, e - x);
             int32_t uu[4];
             uint32_t bw[4];
             #pragma unroll
-            for (int r = 0; r < 4; r++) uu[r] = (x + r < e) ? a.bu_col[x + r] : -1;
+            for (int t = 0; t < 4; t++) uu[t] = (x + t < e) ? a.bu_col[x + t] : -1;
             #pragma unroll
-            for (int r = 0; r < 4; r++) bw[r] = uu[r] >= 0 ? FL[uu[r] >> 5] : 0u;
+            for (int t = 0; t < 4; t++) bw[t] = uu[t] >= 0 ? FL[uu[t] >> 5] : 0u;
             #pragma unroll
-            for (int r = 0; r < 4; r++)
-                if (par < 0 && uu[r] >= 0 && ((bw[r] >> (uu[r] & 31)) & 1u)) par = uu[r];
+            for (int t = 0; t < 4; t++)
+                if (par < 0 && uu[t] >= 0 && ((bw[t] >> (uu[t] & 31)) & 1u)) par = uu[t];
             if (par >= 0) break;
         }
     }
-    uint64_t mb = __ballot(big);
+    const bool open = v >= 0 && par < 0 && x < e;
+    uint64_t mb = __ballot(open);
     while (mb) {
         const int j = __ffsll((unsigned long long)mb) - 1;
         mb &= mb - 1;
-        const int32_t jb = __shfl(b, j), je = __shfl(e, j);
+        const int32_t jb = __shfl(x, j), je = __shfl(e, j);
         int32_t found_u = -1;
         for (int32_t off = jb; off < je; off += 64) {
-            const int32_t x = off + lane;
+            const int32_t y = off + lane;
             bool hit = false;
             int32_t uu = 0;
-            if (x < je) { uu = a.bu_col[x]; hit = (FL[uu >> 5] >> (uu & 31)) & 1u; }
+            if (y < je) { uu = a.bu_col[y]; hit = (FL[uu >> 5] >> (uu & 31)) & 1u; }
             const uint64_t hm = __ballot(hit);
             if (lane == j) k.scan += (unsigned long long)min(64, je - off);
             if (hm) { found_u = __shfl(uu, __ffsll((unsigned long long)hm) - 1); break; }
@@ -606,22 +608,42 @@ __device__ void range_step(const StepArgs& a, int32_t L, const uint32_t* FL, uin
         }
     }
     __syncthreads();
-    for (int32_t base = wid * 64; base < RS; base += kWaves * 64) {
-        const int32_t loc = base + lane;
-        const uint32_t par = s_par[loc];
-        const int32_t v = R0 + loc;
-        const bool fresh = par != 0xffffffffu && v < a.nv;
-        const uint64_t m = __ballot(fresh);
-        if (lane < 2) {
-            const int32_t w = w0 + (base >> 5) + lane;
-            if (w < a.nwords) Fn[w] = (uint32_t)(m >> (32 * lane));
+    // emit: four 1024-vertex slices per round, so that the degree loads of
+    // their new vertices are in flight together
+    for (int32_t c0 = 0; c0 < RS; c0 += 4 * kBlock) {
+        int32_t v[4], d0[4], d1[4];
+        uint32_t par[4];
+        bool fresh[4];
+        #pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int32_t loc = c0 + q * kBlock + tid;
+            par[q] = loc < RS ? s_par[loc] : 0xffffffffu;
+            v[q] = R0 + loc;
+            fresh[q] = par[q] != 0xffffffffu && v[q] < a.nv;
+            d0[q] = d1[q] = 0;
+            if (fresh[q]) { d0[q] = a.td_rp[v[q]]; d1[q] = a.td_rp[v[q] + 1]; }
         }
-        if (fresh) {
-            a.dist[v] = L + 1;
-            if (PARENTS) a.parent[v] = par;
-            account_new(a, v, -1, k, s_hist);
+        #pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int32_t base = c0 + q * kBlock + wid * 64;
+            const uint64_t m = __ballot(fresh[q]);
+            if (lane < 2 && base < RS) {
+                const int32_t w = w0 + (base >> 5) + lane;
+                if (w < a.nwords) Fn[w] = (uint32_t)(m >> (32 * lane));
+            }
+            if (fresh[q]) {
+                a.dist[v[q]] = L + 1;
+                if (PARENTS) a.parent[v[q]] = par[q];
+                const int32_t dout = d1[q] - d0[q];
+                const int32_t din = (a.td_rp == a.bu_rp) ? dout : (a.bu_rp[v[q] + 1] - a.bu_rp[v[q]]);
+                k.n++;
+                k.out += (unsigned long long)dout;
+                k.in += (unsigned long long)din;
+                if (dout <= kHeavyRow) k.light += (unsigned long long)dout;
+                atomicAdd(&s_hist[hist_bin(dout)], 1u);
+            }
+            q_push(fresh[q], v[q], s_hist, qout, lane);
         }
-        q_push(fresh, v, s_hist, qout, lane);
     }
 }
 
