@@ -68,7 +68,7 @@ constexpr int32_t kWaveRow = 32;    // top-down: rows this long or longer are ex
 constexpr int kUnroll = 4;          // top-down: adjacency entries per lane per round
 constexpr int kGraphLevels = 16;    // level launches per captured graph (even: parity is static)
 constexpr int kPartRows = kBlock / kPart;
-constexpr int32_t kRangeFront = 4096;    // range mode: frontier vertices at most
+constexpr int32_t kRangeFront = 2048;    // range mode: frontier vertices at most
 constexpr int64_t kRangeLight = 32768;   // range mode: light-row frontier edges at most
 
 struct StepArgs {
@@ -135,6 +135,22 @@ __device__ inline int32_t wave_excl_scan(int32_t x, int lane, int32_t& total) {
     return incl - x;
 }
 
+// Sum over the 64 lanes with DPP row shifts and row broadcasts (VALU
+// cross-lane operations; no LDS traffic).  Lane 63 collects the total,
+// which readlane returns to every lane.
+__device__ inline uint32_t wave_sum(uint32_t x) {
+    // inclusive sums within rows of 16 lanes: row_shr:1, 2, 4, 8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xf, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xf, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xf, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xf, 0xf, false);
+    // row_bcast:15 adds a row's total into the next row (rows 1 and 3),
+    // row_bcast:31 adds lane 31's total into rows 2 and 3
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xa, 0xf, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xc, 0xf, false);
+    return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
+}
+
 // Largest lane lo with excl[lo] <= g (excl non-decreasing over lanes, excl[0] = 0).
 __device__ inline int owner_lane(int32_t excl, int32_t g) {
     int lo = 0;
@@ -191,7 +207,9 @@ struct Counters {
 
 // A new vertex v of F_{k+1}: its degrees feed the next level's state.
 // din_known >= 0 is v's reverse-adjacency degree when the caller has it.
-__device__ inline void account_new(const StepArgs& a, int32_t v, int32_t din_known, Counters& k, uint32_t* s_hist) {
+// Returns the histogram bin of v's traversal-direction degree (push_new
+// records it).
+__device__ inline int account_new(const StepArgs& a, int32_t v, int32_t din_known, Counters& k) {
     int32_t dout, din;
     if (a.td_rp == a.bu_rp && din_known >= 0) {
         dout = din = din_known;
@@ -203,20 +221,33 @@ __device__ inline void account_new(const StepArgs& a, int32_t v, int32_t din_kno
     k.out += (unsigned long long)dout;
     k.in += (unsigned long long)din;
     if (dout <= kHeavyRow) k.light += (unsigned long long)dout;
-    atomicAdd(&s_hist[hist_bin(dout)], 1u);
+    return hist_bin(dout);
 }
 
-// Appends the lanes' new vertices (flag set) to the work-group's queue, the
-// first kQCap of them; s_hist[kHistBins] counts the places taken.  One LDS
-// atomic per wavefront; every lane of the wavefront calls it.
-__device__ inline void q_push(bool flag, int32_t v, uint32_t* s_hist, int32_t* qout, int lane) {
+// Records the lanes' new vertices (flag set): their histogram bins, with one
+// LDS add per non-empty bin, and the work-group's queue, the first kQCap of
+// them (s_hist[kHistBins] counts the places taken), with one LDS atomic per
+// wavefront.  Every lane of the wavefront calls it.
+__device__ inline void push_new(bool flag, int32_t v, int bin, uint32_t* s_hist, int32_t* qout, int lane) {
     const uint64_t m = __ballot(flag);
     if (!m) return;
+    #pragma unroll
+    for (int b = 0; b < 7; b++) {
+        const uint64_t mb = __ballot(flag && bin == b);
+        if (lane == 0 && mb) atomicAdd(&s_hist[b], (uint32_t)__popcll(mb));
+    }
     uint32_t base = 0;
     if (lane == 0) base = atomicAdd(&s_hist[kHistBins], (uint32_t)__popcll(m));
     base = __shfl(base, 0);
     const uint32_t pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
     if (flag && pos < (uint32_t)kQCap) qout[pos] = v;
+}
+
+// Work-group barrier that orders LDS accesses only: the global stores a
+// wavefront has in flight are not waited for (they complete before the
+// kernel ends, and nothing in the work-group reads them back).
+__device__ inline void lds_barrier() {
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
 // Top-down visit of N edges u[q] -> v[q] (v[q] < 0: none).  All loads of the
@@ -258,12 +289,13 @@ __device__ inline void visit_n(const StepArgs& a, const int32_t* u, const int32_
     #pragma unroll
     for (int q = 0; q < N; q++) {
         const bool won = claim[q] && (PARENTS ? old[q] == 0xffffffffu : !((old[q] >> (v[q] & 31)) & 1u));
+        int bin = 0;
         if (won) {
             if (PARENTS) atomicOr(&Fn[v[q] >> 5], 1u << (v[q] & 31));
             a.dist[v[q]] = L + 1;
-            account_new(a, v[q], -1, k, s_hist);
+            bin = account_new(a, v[q], -1, k);
         }
-        q_push(won, v[q], s_hist, qout, lane);
+        push_new(won, v[q], bin, s_hist, qout, lane);
     }
 }
 
@@ -388,12 +420,12 @@ struct Pending {
 
 // Records a found vertex v (parent par, tile offset off, row [b0, e)).
 template <bool PARENTS>
-__device__ inline void bu_found(const StepArgs& a, int32_t v, int32_t par, int32_t off, int32_t b0, int32_t e,
-                                int32_t L, Counters& k, uint32_t* s_hist, uint32_t* s_fn) {
+__device__ inline int bu_found(const StepArgs& a, int32_t v, int32_t par, int32_t off, int32_t b0, int32_t e,
+                               int32_t L, Counters& k, uint32_t* s_fn) {
     a.dist[v] = L + 1;
     if (PARENTS) a.parent[v] = (uint32_t)par;
     atomicOr(&s_fn[off >> 5], 1u << (off & 31));
-    account_new(a, v, e - b0, k, s_hist);
+    return account_new(a, v, e - b0, k);
 }
 
 // Second stage of the parent scan: up to 64 queued candidates, one per lane,
@@ -441,8 +473,9 @@ __device__ void bu_drain(const StepArgs& a, const Pending* q, int n, int32_t v0,
         }
         if (lane == j) par = found_u;
     }
-    if (par >= 0) bu_found<PARENTS>(a, v, par, v - v0, b0, e, L, k, s_hist, s_fn);
-    q_push(par >= 0, v, s_hist, qout, lane);
+    int bin = 0;
+    if (par >= 0) bin = bu_found<PARENTS>(a, v, par, v - v0, b0, e, L, k, s_fn);
+    push_new(par >= 0, v, bin, s_hist, qout, lane);
 }
 
 // Bottom-up step for one wavefront.  Candidates are the unvisited vertices
@@ -501,8 +534,9 @@ __device__ void bu_step(const StepArgs& a, int32_t L, const uint32_t* FL, uint32
                 if ((FL[u0 >> 5] >> (u0 & 31)) & 1u) par = u0;
                 else open = end - beg > 1;
             }
-            if (par >= 0) bu_found<PARENTS>(a, v, par, v - v0, beg, end, L, k, s_hist, s_fn);
-            q_push(par >= 0, v, s_hist, qout, lane);
+            int bin = 0;
+            if (par >= 0) bin = bu_found<PARENTS>(a, v, par, v - v0, beg, end, L, k, s_fn);
+            push_new(par >= 0, v, bin, s_hist, qout, lane);
             const uint64_t om = __ballot(open);
             if (open) {
                 const int pos = qn + (int)__popcll(om & ((1ull << lane) - 1ull));
@@ -536,13 +570,15 @@ __device__ void bu_step(const StepArgs& a, int32_t L, const uint32_t* FL, uint32
 }
 
 // Top-down step in range mode, for a small frontier with heavy rows.  Work-
-// group b owns the vertex ids [b * range_size, (b + 1) * range_size): it
+// group b owns the vertex ids [b * range_size, (b + 1) * range_size).  It
 // reads the whole frontier list (the work-group queues of the previous
-// level, s_qoff[] their offsets), takes from every frontier row the entries
-// in its range (heavy rows through the range table, light rows filtered),
-// forms the minimum frontier parent of each unvisited vertex of its range
-// with LDS atomics, and writes distances, parents and its F_{k+1} words
-// with plain stores.  No device-scope atomics are issued.
+// level, s_qoff[] their offsets) and takes from every frontier row the
+// segment in its range: heavy rows through the range table, light rows
+// whole (entries outside the range are dropped).  The segments are laid out
+// in LDS with a prefix sum, so that all wavefronts share the entries
+// evenly; the minimum frontier parent of each unvisited vertex of the range
+// is formed with LDS atomics, and distances, parents and the range's
+// F_{k+1} words are written with plain stores.  No device-scope atomics.
 template <bool PARENTS>
 __device__ void range_step(const StepArgs& a, int32_t L, const uint32_t* FL, uint32_t* Fn, uint32_t* Fc,
                            Counters& k, uint32_t* s_hist, int32_t* qout, uint32_t* s_lds, const int32_t* s_qoff,
@@ -552,7 +588,11 @@ __device__ void range_step(const StepArgs& a, int32_t L, const uint32_t* FL, uin
     const int32_t nwr = RS / 32;
     const int32_t w0 = R0 / 32;
     uint32_t* s_par = s_lds;                      // [RS] minimum frontier parent, 0xffffffff = none
-    uint32_t* s_seen = s_lds + RS;                // [nwr] visited | F_k words of the range
+    uint32_t* s_seen = s_par + RS;                // [nwr] visited | F_k words of the range
+    int32_t* s_su = reinterpret_cast<int32_t*>(s_seen + nwr);   // [kRangeFront] segment vertex
+    int32_t* s_sb = s_su + kRangeFront;           // [kRangeFront] segment first entry
+    int32_t* s_sx = s_sb + kRangeFront;           // [kRangeFront + 1] segment offsets (prefix)
+    int32_t* s_ws = s_sx + kRangeFront + 1;       // [kWaves] wavefront totals
     for (int32_t i = tid; i < nwr; i += kBlock) {
         const int32_t w = w0 + i;
         uint32_t sw = 0xffffffffu;
@@ -565,11 +605,12 @@ __device__ void range_step(const StepArgs& a, int32_t L, const uint32_t* FL, uin
         s_seen[i] = sw;
     }
     for (int32_t i = tid; i < RS; i += kBlock) s_par[i] = 0xffffffffu;
-    __syncthreads();
+    // segments: thread t takes frontier entries 2t and 2t + 1
     const int32_t* qin = a.qbuf + (size_t)(lpar ^ 1) * a.grid * kQCap;
-    for (int32_t j0 = wid * 64; j0 < nf; j0 += kWaves * 64) {
-        const int32_t j = j0 + lane;
-        int32_t u = -1, sb = 0, se = 0;
+    int32_t len[2] = {0, 0};
+    #pragma unroll
+    for (int q = 0; q < 2; q++) {
+        const int32_t j = 2 * tid + q;
         if (j < nf) {
             int lo = 0, hi = a.grid;                  // largest b with s_qoff[b] <= j
             while (hi - lo > 1) {
@@ -577,37 +618,71 @@ __device__ void range_step(const StepArgs& a, int32_t L, const uint32_t* FL, uin
                 if (s_qoff[mid] <= j) lo = mid;
                 else hi = mid;
             }
-            u = qin[(size_t)lo * kQCap + (j - s_qoff[lo])];
-            sb = a.td_rp[u];
-            se = a.td_rp[u + 1];
+            const int32_t u = qin[(size_t)lo * kQCap + (j - s_qoff[lo])];
+            int32_t sb = a.td_rp[u], se = a.td_rp[u + 1];
             if (se - sb > kHeavyRow) {
                 const int32_t* t = a.td_rtab + (size_t)a.td_heavy_id[u] * (a.grid + 1) + blockIdx.x;
                 sb = t[0];
                 se = t[1];
             }
-        }
-        int32_t tot;
-        const int32_t ex = wave_excl_scan(se - sb, lane, tot);
-        if (lane == 0) k.scan += (unsigned long long)tot;
-        for (int32_t off = 0; off < tot; off += 64 * kUnroll) {
-            int32_t uu[kUnroll], vv[kUnroll];
-            #pragma unroll
-            for (int q = 0; q < kUnroll; q++) {
-                const int32_t e = off + q * 64 + lane;
-                const int o = owner_lane(ex, e);
-                const int32_t ob = __shfl(sb, o), oe = __shfl(ex, o), ou = __shfl(u, o);
-                uu[q] = ou;
-                vv[q] = e < tot ? a.td_col[ob + (e - oe)] : -1;
-            }
-            #pragma unroll
-            for (int q = 0; q < kUnroll; q++) {
-                const int32_t loc = vv[q] - R0;
-                if (vv[q] >= 0 && loc >= 0 && loc < RS && !((s_seen[loc >> 5] >> (loc & 31)) & 1u))
-                    atomicMin(&s_par[loc], (uint32_t)uu[q]);
-            }
+            s_su[j] = u;
+            s_sb[j] = sb;
+            len[q] = se - sb;
         }
     }
-    __syncthreads();
+    int32_t wtot;
+    const int32_t ex = wave_excl_scan(len[0] + len[1], lane, wtot);
+    if (lane == 0) s_ws[wid] = wtot;
+    lds_barrier();
+    if (wid == 0) {
+        const int32_t x = lane < kWaves ? s_ws[lane] : 0;
+        int32_t t2;
+        const int32_t e2 = wave_excl_scan(x, lane, t2);
+        if (lane < kWaves) s_ws[lane] = e2;
+        if (lane == 0) s_sx[nf] = t2;
+    }
+    lds_barrier();
+    {
+        const int32_t off = s_ws[wid] + ex;
+        if (2 * tid < nf) s_sx[2 * tid] = off;
+        if (2 * tid + 1 < nf) s_sx[2 * tid + 1] = off + len[0];
+    }
+    lds_barrier();
+    // entries: lane l of a round takes 4 consecutive entries
+    const int32_t T = s_sx[nf];
+    if (lane == 0 && wid == 0) k.scan += (unsigned long long)T;
+    for (int32_t e0 = wid * 256; e0 < T; e0 += kWaves * 256) {
+        const int32_t e = e0 + lane * 4;
+        int32_t uu[4], vv[4];
+        int seg = 0;
+        if (e < T) {
+            int lo = 0, hi = nf;                      // largest segment with s_sx[seg] <= e
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_sx[mid] <= e) lo = mid;
+                else hi = mid;
+            }
+            seg = lo;
+        }
+        #pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int32_t eq = e + q;
+            uu[q] = 0;
+            vv[q] = -1;
+            if (eq < T) {
+                while (s_sx[seg + 1] <= eq) seg++;
+                uu[q] = s_su[seg];
+                vv[q] = a.td_col[s_sb[seg] + (eq - s_sx[seg])];
+            }
+        }
+        #pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int32_t loc = vv[q] - R0;
+            if (vv[q] >= 0 && loc >= 0 && loc < RS && !((s_seen[loc >> 5] >> (loc & 31)) & 1u))
+                atomicMin(&s_par[loc], (uint32_t)uu[q]);
+        }
+    }
+    lds_barrier();
     // emit: four 1024-vertex slices per round, so that the degree loads of
     // their new vertices are in flight together
     for (int32_t c0 = 0; c0 < RS; c0 += 4 * kBlock) {
@@ -631,6 +706,7 @@ __device__ void range_step(const StepArgs& a, int32_t L, const uint32_t* FL, uin
                 const int32_t w = w0 + (base >> 5) + lane;
                 if (w < a.nwords) Fn[w] = (uint32_t)(m >> (32 * lane));
             }
+            int bin = 0;
             if (fresh[q]) {
                 a.dist[v[q]] = L + 1;
                 if (PARENTS) a.parent[v[q]] = par[q];
@@ -640,9 +716,9 @@ __device__ void range_step(const StepArgs& a, int32_t L, const uint32_t* FL, uin
                 k.out += (unsigned long long)dout;
                 k.in += (unsigned long long)din;
                 if (dout <= kHeavyRow) k.light += (unsigned long long)dout;
-                atomicAdd(&s_hist[hist_bin(dout)], 1u);
+                bin = hist_bin(dout);
             }
-            q_push(fresh[q], v[q], s_hist, qout, lane);
+            push_new(fresh[q], v[q], bin, s_hist, qout, lane);
         }
     }
 }
@@ -713,8 +789,8 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
     __shared__ int32_t s_qoff[kMaxGrid + 1];      // per-work-group new-vertex counts, then queue offsets
     // bottom-up: the wavefronts' pending queues; range mode: parents and
     // visited words of the work-group's range
-    constexpr int kLdsWords = (kRangeMax + kRangeMax / 32) > (kWaves * 128 * 4) ? (kRangeMax + kRangeMax / 32)
-                                                                                : (kWaves * 128 * 4);
+    constexpr int kRangeWords = kRangeMax + kRangeMax / 32 + 3 * kRangeFront + 1 + kWaves;
+    constexpr int kLdsWords = kRangeWords > (kWaves * 128 * 4) ? kRangeWords : (kWaves * 128 * 4);
     __shared__ uint32_t s_lds[kLdsWords];
     Pending* s_q = reinterpret_cast<Pending*>(s_lds);
 
@@ -877,13 +953,15 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
     else td_step<PARENTS>(a, s_c, L, FL, Fn, Fc, k, s_hist, qout, gw, nw, lane);
 
     // block reduction -> this block's counters (plain stores)
-    unsigned long long r[6] = {k.n, k.out, k.in, k.scan, k.cand, k.light};
+    // A work-group's counters of one step fit in 32 bits (at most nnz < 2^31
+    // adjacency entries, nv vertices).
+    uint32_t r[6] = {(uint32_t)k.n, (uint32_t)k.out, (uint32_t)k.in, (uint32_t)k.scan, (uint32_t)k.cand,
+                     (uint32_t)k.light};
     #pragma unroll
-    for (int i = 0; i < 6; i++)
-        for (int d = 32; d >= 1; d >>= 1) r[i] += __shfl_down(r[i], d);
+    for (int i = 0; i < 6; i++) r[i] = wave_sum(r[i]);
     if (lane == 0)
         for (int i = 0; i < 6; i++) s_red[wid][i] = r[i];
-    __syncthreads();
+    lds_barrier();
     unsigned long long* out = a.part + ((size_t)lpar * a.grid + blockIdx.x) * kPart;
     if (tid < 6) {
         unsigned long long s = 0;

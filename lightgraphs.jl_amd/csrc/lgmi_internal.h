@@ -165,7 +165,7 @@ struct DevCSR {
 // Work-group queues and ranges of the BFS range mode.
 constexpr int kQCap = 4096;         // new vertices a work-group records per level (the range-mode frontier limit)
 constexpr int kMaxGrid = 1024;      // BFS work-groups at most
-constexpr int32_t kRangeMax = 32768; // vertices per work-group range at most (LDS)
+constexpr int32_t kRangeMax = 24576; // vertices per work-group range at most (LDS)
 
 }  // namespace lgmi
 

@@ -238,3 +238,35 @@ def test_range_mode_levels_exact(oracle):
             seen += sum(int(buf[i].variant) for i in range(n))
     say(f"range-mode levels: {seen}")
     assert seen > 0, "no level ran in range mode"
+
+
+def test_level_counters_match_oracle(oracle):
+    """The per-level records the kernels reduce on the device (frontier size,
+    frontier edges, new vertices) equal the counts from the oracle's
+    distances, level by level, for both directions and for range mode."""
+    import ctypes
+
+    import lgmi.generators as gen
+    from lgmi import _lib
+    lib = _lib.load()
+    for g, tag in ((gen.rmat(18, 16, seed=4), "rmat18"), (gen.rmat(15, 16, seed=5, directed=True), "rmat15-dir")):
+        rp, col = g.csr()
+        deg = np.diff(rp)
+        hub = int(np.argmax(deg))
+        roots = [hub, int(np.nonzero(deg > 0)[0][0])]
+        buf = (_lib.BfsStep * 512)()
+        st = _lib.BfsStats()
+        for forced in (-1, 0, 1):
+            _lib.check(lib.lgmi_bfs_force_direction(g.handle(), forced))
+            for r in roots:
+                say(f"{tag}: root {r} forced {forced}")
+                d = oracle.gdistances(rp, col, [r])
+                _lib.check(lib.lgmi_bfs_run(g.handle(), r, 0, 1, ctypes.byref(st)))
+                n = lib.lgmi_bfs_steps(g.handle(), 512, buf)
+                assert n == int(d.max()) + 1 == st.levels
+                for L in range(n):
+                    f = d == L
+                    assert buf[L].n_front == int(f.sum()), (tag, r, L)
+                    assert buf[L].m_front == int(deg[f].sum()), (tag, r, L)
+                    assert buf[L].n_next == int((d == L + 1).sum()), (tag, r, L)
+        _lib.check(lib.lgmi_bfs_force_direction(g.handle(), -1))
