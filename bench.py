@@ -122,7 +122,7 @@ def bfs_bench(args, lgmi, _lib, orc):
     _lib.check(lib.lgmi_bfs_time_steps(h, 0))
     achieved = tot_bytes / (tot_ms * 1e-3) / 1e9 if tot_ms > 0 else 0.0
     bu_achieved = bu_bytes / (bu_ms * 1e-3) / 1e9 if bu_ms > 0 else 0.0
-    log(f"step kernel: {launches} launches, avg {tot_ms / max(launches, 1) * 1e3:.1f} us, "
+    log(f"level kernel: {launches} launches, avg {tot_ms / max(launches, 1) * 1e3:.1f} us, "
         f"{tot_bytes / max(launches, 1) / 1e6:.2f} MB/launch, {achieved:.0f} GB/s; "
         f"bottom-up steps {n_bu}: {bu_achieved:.0f} GB/s; top-down steps {n_td}")
 
@@ -161,7 +161,7 @@ def bfs_bench(args, lgmi, _lib, orc):
             "gteps_median_device": round(float(np.median(per_root)), 3),
         },
         "roofline": {
-            "kernel": "bfs_step",
+            "kernel": "bfs_level",
             "bound": "hbm",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
@@ -186,10 +186,10 @@ def bfs_bench(args, lgmi, _lib, orc):
 
 
 def read_traffic():
-    """HBM bytes per launch of bfs_step from the committed rocprofv3 --pmc
+    """HBM bytes per launch of bfs_level from the committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), or
     None when no summary is present."""
-    p = os.path.join(REPO, "profiles", "bfs_step_rmat22_pmc.json")
+    p = os.path.join(REPO, "profiles", "bfs_level_rmat22_pmc.json")
     if not os.path.exists(p):
         return None
     with open(p) as f:
