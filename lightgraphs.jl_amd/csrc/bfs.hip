@@ -151,6 +151,17 @@ __device__ inline uint32_t wave_sum(uint32_t x) {
     return (uint32_t)__builtin_amdgcn_readlane((int)x, 63);
 }
 
+// wave_sum for FP32 (the same DPP sequence, FP32 adds).
+__device__ inline float wave_sum_f(float x) {
+    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x111, 0xf, 0xf, false));
+    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x112, 0xf, 0xf, false));
+    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x114, 0xf, 0xf, false));
+    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x118, 0xf, 0xf, false));
+    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x142, 0xa, 0xf, false));
+    x += __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), 0x143, 0xc, 0xf, false));
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), 63));
+}
+
 // Largest lane lo with excl[lo] <= g (excl non-decreasing over lanes, excl[0] = 0).
 __device__ inline int owner_lane(int32_t excl, int32_t g) {
     int lo = 0;
@@ -765,11 +776,7 @@ __device__ inline int mlp_direction(const float* xs, const PolicyRegs& pr) {
         acc += r.x * xs[4 * q] + r.y * xs[4 * q + 1] + r.z * xs[4 * q + 2] + r.w * xs[4 * q + 3];
     }
     const float hj = acc > 0.f ? acc : 0.f;
-    float a0 = pr.w2a * hj, a1 = pr.w2b * hj;
-    for (int o = 32; o >= 1; o >>= 1) {
-        a0 += __shfl_xor(a0, o);
-        a1 += __shfl_xor(a1, o);
-    }
+    const float a0 = wave_sum_f(pr.w2a * hj), a1 = wave_sum_f(pr.w2b * hj);
     return (a1 + pr.b2b) > (a0 + pr.b2a) ? 1 : 0;
 }
 
@@ -783,7 +790,7 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
     __shared__ unsigned long long s_red[kWaves][kPart];
     __shared__ unsigned long long s_sum[kPart];
     __shared__ float s_feat[kFeat];
-    __shared__ int32_t s_p, s_dir, s_exit, s_dl;
+    __shared__ int32_t s_p, s_dir, s_exit, s_dl, s_range;
     __shared__ uint32_t s_hist[kHistBins + 1];   // + the queue counter
     __shared__ uint32_t s_fn[kWaves][kTileWords];
     __shared__ int32_t s_qoff[kMaxGrid + 1];      // per-work-group new-vertex counts, then queue offsets
@@ -840,96 +847,107 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
     acc += __shfl_xor(acc, 32);
     if (lane < kPart) s_red[wid][lane] = acc;
     __syncthreads();
-    if (tid < kPart) {
-        unsigned long long s = 0;
-        for (int q = 0; q < kWaves; q++) s += s_red[q][tid];
-        s_sum[tid] = s;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        const int32_t p = first ? 0 : (karg > 0 ? karg : s_c.phase + 1);
-        int ex = s_dl <= p ? 1 : 0;
-        if (!ex && !first) {
-            // S_k from S_{k-1} and the counters of level k-1
-            const BfsCtrl& q = s_c;
-            BfsCtrl c = q;
-            const int32_t k = q.level + 1;
-            const int32_t n_next = (int32_t)s_sum[P_NEXT];
-            c.level = k;
-            c.n_prev = q.n_front;
-            c.n_front = n_next;
-            c.prev_dir = q.dir;
-            c.m_front = (int64_t)s_sum[P_OUT];
-            c.m_unvisited = q.m_unvisited - (int64_t)s_sum[P_IN];
-            c.n_unvisited = q.n_unvisited - n_next;
-            c.m_light = (int64_t)s_sum[P_LIGHT];
-            c.qovf = s_sum[P_QOVF] != 0 ? 1 : 0;
-            c.phase = p;
-            c.t_main = t_now;
-            for (int b = 0; b < kHistBins; b++) c.front_hist[b] = (uint32_t)s_sum[P_HIST + b];
-            if (blockIdx.x == 0 && k - 1 < a.max_log) {
-                LevelRec* r = &a.log[k - 1];
-                r->dir = q.dir;
-                r->variant = q.variant;
-                r->n_front = q.n_front;
-                r->m_front = q.m_front;
-                r->m_unvisited = q.m_unvisited;
-                r->n_next = n_next;
-                r->scanned = (int64_t)s_sum[P_SCAN];
-                r->cand = (int64_t)s_sum[P_CAND];
-                r->t_start = q.t_main;
-                r->t_end = t_now;
-                r->t_prev_end = q.t_main;
-            }
-            if (n_next == 0) {
-                ex = 1;
-                if (blockIdx.x == 0) {
-                    sh->state[p & 1] = c;
-                    sh->done_level = p;
-                }
-            } else {
-                compute_features(&c, s_feat);
-            }
-            s_c = c;
+    // Wavefront 0 derives the level state (lane 0) and the direction (lane 0
+    // for the heuristic, all 64 lanes for the MLP); wavefront-level ordering
+    // suffices inside it, and one work-group barrier publishes the result.
+    if (wid == 0) {
+        if (lane < kPart) {
+            unsigned long long s = 0;
+            for (int q = 0; q < kWaves; q++) s += s_red[q][lane];
+            s_sum[lane] = s;
         }
-        s_p = p;
-        s_exit = ex;
-    }
-    __syncthreads();
-    if (s_exit) return;
-    const int32_t p = s_p;
-    const int32_t L = s_c.level;
-    if (may_learn) {
-        if (wid == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane == 0) {
+            const int32_t p = first ? 0 : (karg > 0 ? karg : s_c.phase + 1);
+            int ex = s_dl <= p ? 1 : 0;
+            if (!ex && !first) {
+                // S_k from S_{k-1} and the counters of level k-1
+                const BfsCtrl& q = s_c;
+                BfsCtrl c = q;
+                const int32_t k = q.level + 1;
+                const int32_t n_next = (int32_t)s_sum[P_NEXT];
+                c.level = k;
+                c.n_prev = q.n_front;
+                c.n_front = n_next;
+                c.prev_dir = q.dir;
+                c.m_front = (int64_t)s_sum[P_OUT];
+                c.m_unvisited = q.m_unvisited - (int64_t)s_sum[P_IN];
+                c.n_unvisited = q.n_unvisited - n_next;
+                c.m_light = (int64_t)s_sum[P_LIGHT];
+                c.qovf = s_sum[P_QOVF] != 0 ? 1 : 0;
+                c.phase = p;
+                c.t_main = t_now;
+                for (int b = 0; b < kHistBins; b++) c.front_hist[b] = (uint32_t)s_sum[P_HIST + b];
+                if (blockIdx.x == 0 && k - 1 < a.max_log) {
+                    LevelRec* r = &a.log[k - 1];
+                    r->dir = q.dir;
+                    r->variant = q.variant;
+                    r->n_front = q.n_front;
+                    r->m_front = q.m_front;
+                    r->m_unvisited = q.m_unvisited;
+                    r->n_next = n_next;
+                    r->scanned = (int64_t)s_sum[P_SCAN];
+                    r->cand = (int64_t)s_sum[P_CAND];
+                    r->t_start = q.t_main;
+                    r->t_end = t_now;
+                    r->t_prev_end = q.t_main;
+                }
+                if (n_next == 0) {
+                    ex = 1;
+                    if (blockIdx.x == 0) {
+                        sh->state[p & 1] = c;
+                        sh->done_level = p;
+                    }
+                } else {
+                    compute_features(&c, s_feat);
+                }
+                s_c = c;
+            }
+            s_p = p;
+            s_exit = ex;
+            if (!ex && !may_learn)
+                s_dir = forced >= 0 ? forced : (first ? s_c.dir : heuristic_direction(s_c));
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (may_learn && !s_exit) {
             const int d = mlp_direction(s_feat, pr);
             if (lane == 0) s_dir = d;
         }
-    } else if (tid == 0) {
-        s_dir = forced >= 0 ? forced : (first ? s_c.dir : heuristic_direction(s_c));
-    }
-    __syncthreads();
-    const int dir = s_dir;
-    // range mode: top-down over a small frontier that holds heavy rows and
-    // few light-row edges, when the range table exists and every work-group
-    // queue of the frontier is complete
-    const bool range = dir == 0 && !first && a.range_size > 0 && s_c.qovf == 0 && s_c.n_front <= kRangeFront &&
-                       s_c.m_light <= kRangeLight && s_c.front_hist[5] + s_c.front_hist[6] > 0;
-    if (range) {
-        if (wid == 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // range mode: top-down over a small frontier that holds heavy rows
+        // and few light-row edges, when the range table exists and every
+        // work-group queue of the frontier is complete
+        const BfsCtrl& c = s_c;
+        const bool range = !s_exit && s_dir == 0 && !first && a.range_size > 0 && c.qovf == 0 &&
+                           c.n_front <= kRangeFront && c.m_light <= kRangeLight &&
+                           c.front_hist[5] + c.front_hist[6] > 0;
+        if (lane == 0) s_range = range ? 1 : 0;
+        if (range) {
             // queue offsets: exclusive scan of the per-work-group counts
             int32_t carry = 0;
             for (int32_t b0 = 0; b0 < a.grid; b0 += 64) {
                 const int32_t b = b0 + lane;
-                const int32_t c = b < a.grid ? s_qoff[b] : 0;
+                const int32_t cnt = b < a.grid ? s_qoff[b] : 0;
                 int32_t tot;
-                const int32_t ex = wave_excl_scan(c, lane, tot);
+                const int32_t ex = wave_excl_scan(cnt, lane, tot);
                 if (b < a.grid) s_qoff[b] = carry + ex;
                 carry += tot;
             }
             if (lane == 0) s_qoff[a.grid] = carry;
         }
-        __syncthreads();
     }
+    __syncthreads();
+    if (s_exit) return;
+    const int32_t p = s_p;
+    const int32_t L = s_c.level;
+    const int dir = s_dir;
+    const bool range = s_range != 0;
 
     if (blockIdx.x == 0 && tid == 0) {
         BfsCtrl c = s_c;
