@@ -207,15 +207,22 @@ def bc_bench(args, lgmi, _lib, orc):
     rng = np.random.default_rng(4242)
     srcs = rng.choice(np.nonzero(deg > 0)[0], size=1024, replace=False).astype(np.int64)
     g.handle()
-    lgmi.betweenness_centrality(g, srcs[:64] + 1)          # warmup (allocations)
     from lgmi.centrality import last_stats
-    t1 = time.perf_counter()
-    bc = lgmi.betweenness_centrality(g, srcs + 1)
-    wall = time.perf_counter() - t1
-    stt = last_stats(g)
+    lgmi.betweenness_centrality(g, srcs + 1)               # warmup: allocations, graph capture
+    reps = 3
+    walls, stats = [], []
+    for _ in range(reps):
+        t1 = time.perf_counter()
+        bc = lgmi.betweenness_centrality(g, srcs + 1)
+        walls.append(time.perf_counter() - t1)
+        stats.append(last_stats(g))
+    wall = float(np.mean(walls))
+    stt = {k: float(np.mean([s[k] for s in stats])) for k in ("device_ms", "forward_ms", "backward_ms")}
+    launch = {0: "one launch per level", 1: "one persistent cooperative launch",
+              2: "one persistent launch replayed from a captured hipGraph"}[int(stats[-1]["launch"])]
     m = nnz / 2
     teps = 1024 * m / wall
-    log(f"BC 1024 sources: {wall * 1e3:.1f} ms wall, device {stt['device_ms']:.1f} ms "
+    log(f"BC 1024 sources ({launch}): {wall * 1e3:.1f} ms wall (mean of {reps}), device {stt['device_ms']:.1f} ms "
         f"(fwd {stt['forward_ms']:.1f}, bwd {stt['backward_ms']:.1f}), {teps / 1e9:.2f} GTEPS_BC")
     # CPU baseline: threaded oracle on a sample of the sources, scaled
     nthreads = min(16, os.cpu_count() or 1)
@@ -239,6 +246,7 @@ def bc_bench(args, lgmi, _lib, orc):
         "device_ms": round(stt["device_ms"], 2),
         "forward_ms": round(stt["forward_ms"], 2),
         "backward_ms": round(stt["backward_ms"], 2),
+        "launch": launch,
         "sample_max_rel_err": err,
         "cpu_baseline": {"value": round(cpu_teps / 1e9, 5), "unit": "GTEPS_BC", "cores": nthreads,
                          "kind": "port",

@@ -82,6 +82,10 @@ typedef struct {
 typedef struct {
     int32_t groups;            /* 64-source lane groups processed            */
     int32_t max_levels;        /* deepest BFS level over all groups          */
+    int32_t launch;            /* 2: one persistent launch replayed from a   */
+                               /* captured hipGraph, 1: one persistent       */
+                               /* cooperative launch, 0: one launch per level */
+    int32_t pad;
     double device_ms;          /* HIP-event time of the whole computation    */
     double forward_ms;         /* sigma passes                                */
     double backward_ms;        /* delta passes                                */
@@ -198,6 +202,17 @@ int lgmi_betweenness(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int64_t 
                      int skip_degree0, int normalize, int endpoints, double* bc);
 /* Statistics of the last betweenness run on g. */
 int lgmi_betweenness_stats(lgmi_graph_t g, lgmi_bc_stats* stats);
+
+/* How lgmi_betweenness launches its work (process-wide; no counterpart in
+ * the reference, whose source loop runs on the host):
+ *   0 (default) -- all lane groups in one persistent launch, captured in a
+ *                  hipGraph and replayed while the arguments stay the same,
+ *                  whenever the groups fit the slots at once; one launch
+ *                  per level otherwise, and as the fallback when the
+ *                  persistent launch cannot finish;
+ *   1           -- one launch per level always.
+ * Both give bit-identical results. */
+int lgmi_betweenness_set_launch(int mode);
 
 /* All-source betweenness for many independent undirected graphs in one
  * persistent launch (each graph as betweenness_centrality(g) with

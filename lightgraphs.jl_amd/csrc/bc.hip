@@ -34,6 +34,7 @@
 #include "lgmi_internal.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -48,6 +49,26 @@ struct BcCtrl {
     uint32_t blocks_done;
     int32_t overflow;
     int32_t pad[2];
+};
+
+// Barrier and result words of the persistent launch.  arrive and closed are
+// cleared before every launch, bar_count is zero between launches; bar_gen
+// only grows.  abort is set when a barrier wait
+// times out (the blocks of a launch were not all resident), overflow when a
+// slot needs more levels than F holds; either makes the host redo the
+// computation level by level.
+struct BcPersistCtl {
+    uint32_t arrive;      // blocks that have started (reset before each launch)
+    uint32_t closed;      // participants + 1 once fixed, 0 before
+    uint32_t bar_count;
+    uint32_t bar_gen;
+    int32_t abort;
+    int32_t overflow;
+    int32_t levels;
+    int32_t barriers;
+    uint64_t t_fwd;
+    uint64_t t_bwd;
+    uint64_t t_end;
 };
 
 struct BcWork {
@@ -65,12 +86,54 @@ struct BcWork {
     BcCtrl* ctrl = nullptr;      // [slot]
     BcCtrl* h_ctrl = nullptr;    // pinned mirror
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    double* hpart = nullptr;      // [slot][seg_stride][64]
+    uint64_t* hgot = nullptr;     // [slot][seg_stride]
+    int32_t seg_stride = 0;
+    // persistent launch
+    BcPersistCtl* pctl = nullptr;    // barrier and result words
+    BcPersistCtl* h_pctl = nullptr;  // pinned mirror
+    uint64_t* found = nullptr;              // [2][grid][words] per-block slot masks
+    uint32_t* rel = nullptr;                // [grid][32] per-block barrier flags
+    hipGraphExec_t pexec = nullptr;         // captured launch, replayed while its key matches
+    std::vector<unsigned char> pkey;
 };
+
+// Per-graph host row pointers and the heavy-row segment lists (device).
+struct BcGraph {
+    std::vector<int32_t> rp_out, rp_in;
+    int32_t* dev = nullptr;           // all segment arrays, one allocation
+    int32_t nseg_in = 0, nh_in = 0, nseg_out = 0, nh_out = 0;
+    int32_t* seg_in[6] = {};          // seg_v, seg_b, seg_e, seg_h, h_first, h_nseg
+    int32_t* seg_out[6] = {};
+};
+
+void bc_graph_free(BcGraph* b) {
+    if (!b) return;
+    dev_free(b->dev);
+    delete b;
+}
 
 namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
+
+// Rows longer than kBcHeavyRow entries are cut into segments of kBcSegLen
+// entries, each a work item of its own; after all segments of a level step,
+// one wavefront per row adds the segments' partial sums in segment order.
+constexpr int32_t kBcHeavyRow = 1024;
+constexpr int32_t kBcSegLen = 1024;
+
+struct BcSegs {
+    const int32_t* seg_v;    // [nseg] the row's vertex
+    const int32_t* seg_b;    // [nseg] first entry
+    const int32_t* seg_e;    // [nseg] end entry
+    const int32_t* seg_h;    // [nseg] heavy-row index
+    const int32_t* h_first;  // [nh] first segment of the row
+    const int32_t* h_nseg;   // [nh] segments of the row
+    int32_t nseg;
+    int32_t nh;
+};
 
 struct BcArgs {
     const int32_t* in_rp;
@@ -86,7 +149,46 @@ struct BcArgs {
     double* bc_acc;
     const uint64_t* active;
     BcCtrl* ctrl;
+    BcSegs hin;              // heavy in-rows (forward)
+    BcSegs hout;             // heavy out-rows (backward)
+    double* hpart;           // [slot][seg_stride][64] per-segment lane sums
+    uint64_t* hgot;          // [slot][seg_stride] per-segment new-lane masks
+    int32_t seg_stride;
 };
+
+// Up to eight of the hits in `hits` (lanes whose adjacency entry uk[t] has
+// a non-empty lane mask mk[t]): lane i adds to acc the FP64 rows' entry i of
+// the hits whose mask has bit i, one after another in ascending hit order
+// (the order of the row), with the eight loads in flight together.  Removes
+// the hits it took and ORs their masks into `got`.  Every lane calls it with
+// the same `hits`.
+__device__ inline void gather_hits(uint64_t& hits, int32_t uk, uint64_t mk, const double* sig, int lane,
+                                   double& acc, uint64_t& got) {
+    constexpr int kH = 8;
+    int32_t uu[kH];
+    uint64_t mm[kH];
+    #pragma unroll
+    for (int q = 0; q < kH; q++) {
+        uu[q] = 0;
+        mm[q] = 0;
+        if (hits) {
+            // t is the same in every lane, so the reads are scalar
+            const int t = __builtin_amdgcn_readfirstlane(__ffsll((unsigned long long)hits) - 1);
+            hits &= hits - 1;
+            uu[q] = __builtin_amdgcn_readlane(uk, t);
+            mm[q] = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(mk >> 32), t) << 32) |
+                    (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)mk, t);
+        }
+    }
+    double x[kH];
+    #pragma unroll
+    for (int q = 0; q < kH; q++) x[q] = ((mm[q] >> lane) & 1) ? sig[(size_t)uu[q] * 64 + lane] : 0.0;
+    #pragma unroll
+    for (int q = 0; q < kH; q++) {
+        acc += x[q];
+        got |= mm[q];
+    }
+}
 
 __global__ void bc_clear_acc(double* acc, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -128,12 +230,127 @@ __global__ void bc_seed(BcArgs a, const int32_t* lane_src) {
     }
 }
 
+// Forward step L -> L+1 of one slot over the 64 vertices of chunk ch (one
+// wavefront): every vertex with an unseen lane ORs its in-neighbours'
+// frontier masks and adds their sigma per lane.  Returns, in every lane,
+// whether any vertex of the chunk got a new lane.
+__device__ bool fwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int lane) {
+    const int32_t nv = a.nv;
+    const uint64_t act = a.active[slot];
+    const uint64_t* FL = a.F + ((size_t)slot * a.cap + L) * nv;
+    uint64_t* Fn = a.F + ((size_t)slot * a.cap + L + 1) * nv;
+    uint64_t* seen = a.seen + (size_t)slot * nv;
+    double* sig = a.sig + (size_t)slot * nv * 64;
+    const int32_t vj = ch * 64 + lane;
+    uint64_t sj = 0;
+    int32_t bj = 0, ej = 0;
+    bool cand = false, heavy = false;
+    if (vj < nv) {
+        sj = seen[vj];
+        bj = a.in_rp[vj];
+        ej = a.in_rp[vj + 1];
+        heavy = ej - bj > kBcHeavyRow;
+        cand = (~sj & act) && ej > bj && !heavy;
+    }
+    uint64_t newF = 0;
+    uint64_t cm = __ballot(cand);
+    while (cm) {
+        const int j = __ffsll((unsigned long long)cm) - 1;
+        cm &= cm - 1;
+        const int32_t v = ch * 64 + j;
+        const uint64_t unseen = ~__shfl(sj, j) & act;
+        const int32_t b = __shfl(bj, j), e = __shfl(ej, j);
+        double acc = 0.0;
+        uint64_t got = 0;
+        for (int32_t off = b; off < e; off += 64) {
+            const int32_t k = off + lane;
+            uint64_t mk = 0;
+            int32_t uk = 0;
+            if (k < e) {
+                uk = a.in_col[k];
+                mk = FL[uk] & unseen;
+            }
+            uint64_t hits = __ballot(mk != 0);
+            while (hits) gather_hits(hits, uk, mk, sig, lane, acc, got);
+        }
+        if ((got >> lane) & 1) sig[(size_t)v * 64 + lane] = acc;
+        if (lane == j) newF = got;
+    }
+    if (vj < nv && !heavy) {
+        Fn[vj] = newF;
+        if (newF) seen[vj] = sj | newF;
+    }
+    return __ballot(newF != 0) != 0;
+}
+
+// Forward step of one slot over heavy-row segment sg (one wavefront): the
+// lane sums of sigma over the segment's in-neighbours on the frontier, and
+// the lanes they reach, go to the segment's partial.  fwd_combine finishes
+// the row once every segment is done.  Returns, in every lane, whether the
+// segment reached a new lane.
+__device__ bool fwd_seg(const BcArgs& a, int slot, int32_t sg, int32_t L, int lane) {
+    const int32_t nv = a.nv;
+    const int32_t v = a.hin.seg_v[sg];
+    const uint64_t act = a.active[slot];
+    const uint64_t* FL = a.F + ((size_t)slot * a.cap + L) * nv;
+    const double* sig = a.sig + (size_t)slot * nv * 64;
+    const uint64_t unseen = ~a.seen[(size_t)slot * nv + v] & act;
+    double acc = 0.0;
+    uint64_t got = 0;
+    if (unseen) {
+        const int32_t b = a.hin.seg_b[sg], e = a.hin.seg_e[sg];
+        for (int32_t off = b; off < e; off += 64) {
+            const int32_t k = off + lane;
+            uint64_t mk = 0;
+            int32_t uk = 0;
+            if (k < e) {
+                uk = a.in_col[k];
+                mk = FL[uk] & unseen;
+            }
+            uint64_t hits = __ballot(mk != 0);
+            while (hits) gather_hits(hits, uk, mk, sig, lane, acc, got);
+        }
+    }
+    a.hpart[((size_t)slot * a.seg_stride + sg) * 64 + lane] = acc;
+    if (lane == 0) a.hgot[(size_t)slot * a.seg_stride + sg] = got;
+    return got != 0;
+}
+
+// Finishes heavy in-row h of one slot after its segments: the partials in
+// segment order (sigma is integer-valued, so the sum equals the row-order
+// one exactly) give the vertex's sigma; its F_{L+1} and seen follow.
+__device__ void fwd_combine(const BcArgs& a, int slot, int32_t h, int32_t L, int lane) {
+    const int32_t nv = a.nv;
+    const int32_t f0 = a.hin.h_first[h], nsg = a.hin.h_nseg[h];
+    const int32_t v = a.hin.seg_v[f0];
+    uint64_t* Fn = a.F + ((size_t)slot * a.cap + L + 1) * nv;
+    const uint64_t sv = a.seen[(size_t)slot * nv + v];
+    if (!(~sv & a.active[slot])) {   // every lane seen: the segments did nothing
+        if (lane == 0) Fn[v] = 0;
+        return;
+    }
+    const double* part = a.hpart + ((size_t)slot * a.seg_stride + f0) * 64;
+    const uint64_t* pgot = a.hgot + (size_t)slot * a.seg_stride + f0;
+    double tot = 0.0;
+    uint64_t gm = 0;
+    for (int32_t q = 0; q < nsg; q++) {
+        tot += part[(size_t)q * 64 + lane];
+        gm |= pgot[q];
+    }
+    if ((gm >> lane) & 1) a.sig[((size_t)slot * nv + v) * 64 + lane] = tot;
+    if (lane == 0) {
+        Fn[v] = gm;
+        if (gm) a.seen[(size_t)slot * nv + v] = sv | gm;
+    }
+}
+
+// Forward level step, part 1: every light vertex of every chunk and every
+// heavy-row segment; sets the slot's found flag when anything new is reached.
 __global__ __launch_bounds__(kBlock) void bc_forward(BcArgs a) {
     const int slot = blockIdx.y;
     BcCtrl* c = &a.ctrl[slot];
     __shared__ int32_t s_L, s_done, s_found;
-    __shared__ uint32_t s_ticket;
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     if (tid == 0) {
         s_done = c->done;
         s_L = c->level;
@@ -146,79 +363,35 @@ __global__ __launch_bounds__(kBlock) void bc_forward(BcArgs a) {
     __syncthreads();
     if (s_done) return;
     const int32_t L = s_L;
-    const int32_t nv = a.nv;
-    const uint64_t act = a.active[slot];
-    const uint64_t* FL = a.F + ((size_t)slot * a.cap + L) * nv;
-    uint64_t* Fn = a.F + ((size_t)slot * a.cap + L + 1) * nv;
-    uint64_t* seen = a.seen + (size_t)slot * nv;
-    double* sig = a.sig + (size_t)slot * nv * 64;
     bool any = false;
-    for (int32_t ch = blockIdx.x * kWaves + wid; ch < a.nchunks; ch += gridDim.x * kWaves) {
-        const int32_t vj = ch * 64 + lane;
-        uint64_t sj = 0;
-        int32_t bj = 0, ej = 0;
-        bool cand = false;
-        if (vj < nv) {
-            sj = seen[vj];
-            if (~sj & act) {
-                bj = a.in_rp[vj];
-                ej = a.in_rp[vj + 1];
-                cand = ej > bj;
-            }
-        }
-        uint64_t newF = 0;
-        uint64_t cm = __ballot(cand);
-        while (cm) {
-            const int j = __ffsll((unsigned long long)cm) - 1;
-            cm &= cm - 1;
-            const int32_t v = ch * 64 + j;
-            const uint64_t unseen = ~__shfl(sj, j) & act;
-            const int32_t b = __shfl(bj, j), e = __shfl(ej, j);
-            double acc = 0.0;
-            uint64_t got = 0;
-            for (int32_t off = b; off < e; off += 64) {
-                const int32_t k = off + lane;
-                uint64_t mk = 0;
-                int32_t uk = 0;
-                if (k < e) {
-                    uk = a.in_col[k];
-                    mk = FL[uk] & unseen;
-                }
-                uint64_t hits = __ballot(mk != 0);
-                while (hits) {
-                    const int t0 = __ffsll((unsigned long long)hits) - 1;
-                    hits &= hits - 1;
-                    const int32_t u0 = __shfl(uk, t0);
-                    const uint64_t m0 = __shfl(mk, t0);
-                    got |= m0;
-                    if (hits) {
-                        const int t1 = __ffsll((unsigned long long)hits) - 1;
-                        hits &= hits - 1;
-                        const int32_t u1 = __shfl(uk, t1);
-                        const uint64_t m1 = __shfl(mk, t1);
-                        got |= m1;
-                        const double x0 = ((m0 >> lane) & 1) ? sig[(size_t)u0 * 64 + lane] : 0.0;
-                        const double x1 = ((m1 >> lane) & 1) ? sig[(size_t)u1 * 64 + lane] : 0.0;
-                        acc += x0;
-                        acc += x1;
-                    } else {
-                        if ((m0 >> lane) & 1) acc += sig[(size_t)u0 * 64 + lane];
-                    }
-                }
-            }
-            if ((got >> lane) & 1) sig[(size_t)v * 64 + lane] = acc;
-            if (lane == j) newF = got;
-        }
-        if (vj < nv) {
-            Fn[vj] = newF;
-            if (newF) seen[vj] = sj | newF;
-        }
-        any |= newF != 0;
+    const int32_t nseg = a.hin.nseg;
+    for (int32_t i = blockIdx.x * kWaves + wid; i < nseg + a.nchunks; i += gridDim.x * kWaves)
+        any |= i < nseg ? fwd_seg(a, slot, i, L, lane) : fwd_chunk(a, slot, i - nseg, L, lane);
+    if (any && lane == 0) s_found = 1;
+    __syncthreads();
+    if (tid == 0 && s_found) atomicOr(&c->found, 1);
+}
+
+// Forward level step, part 2: finishes the heavy rows; the last block of
+// the launch advances the slot's level, or marks the slot done when the
+// step found nothing (maxlevel = L).
+__global__ __launch_bounds__(kBlock) void bc_forward_fin(BcArgs a) {
+    const int slot = blockIdx.y;
+    BcCtrl* c = &a.ctrl[slot];
+    __shared__ int32_t s_L, s_done;
+    __shared__ uint32_t s_ticket;
+    const int tid = threadIdx.x, lane = tid & 63, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+    if (tid == 0) {
+        s_done = c->done;
+        s_L = c->level;
+        if (!s_done && s_L + 1 >= a.cap) s_done = 1;
     }
-    if (__ballot(any) && lane == 0) s_found = 1;
+    __syncthreads();
+    if (s_done) return;
+    const int32_t L = s_L;
+    for (int32_t h = blockIdx.x * kWaves + wid; h < a.hin.nh; h += gridDim.x * kWaves) fwd_combine(a, slot, h, L, lane);
     __syncthreads();
     if (tid == 0) {
-        if (s_found) atomicOr(&c->found, 1);
         __threadfence();
         s_ticket = atomicAdd(&c->blocks_done, 1u);
     }
@@ -238,73 +411,435 @@ __global__ __launch_bounds__(kBlock) void bc_forward(BcArgs a) {
     }
 }
 
-__global__ __launch_bounds__(kBlock) void bc_backward(BcArgs a, int32_t k) {
-    const int slot = blockIdx.y;
-    const int32_t L = a.ctrl[slot].maxlevel - k;
-    if (L < 1) return;
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+// Backward step of one slot at level L (>= 1) over chunk ch (one
+// wavefront): delta of every vertex of the chunk with lanes on level L from
+// its successors' coefficients, the vertex's coefficient (1 + delta) / sigma
+// into its sigma slot, and the lane sum of delta into the slot's partial
+// betweenness.
+__device__ void bwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int lane) {
     const int32_t nv = a.nv;
     const uint64_t* FL = a.F + ((size_t)slot * a.cap + L) * nv;
     const uint64_t* Fn = a.F + ((size_t)slot * a.cap + L + 1) * nv;
     double* sig = a.sig + (size_t)slot * nv * 64;
     double* acc_bc = a.bc_acc + (size_t)slot * nv;
-    for (int32_t ch = blockIdx.x * kWaves + wid; ch < a.nchunks; ch += gridDim.x * kWaves) {
-        const int32_t vj = ch * 64 + lane;
-        uint64_t Aj = 0;
-        int32_t bj = 0, ej = 0;
-        if (vj < nv) {
-            Aj = FL[vj];
-            if (Aj) { bj = a.out_rp[vj]; ej = a.out_rp[vj + 1]; }
+    const int32_t vj = ch * 64 + lane;
+    uint64_t Aj = 0;
+    int32_t bj = 0, ej = 0;
+    if (vj < nv) {
+        Aj = FL[vj];
+        if (Aj) {
+            bj = a.out_rp[vj];
+            ej = a.out_rp[vj + 1];
+            if (ej - bj > kBcHeavyRow) Aj = 0;   // a heavy row: its segments do it
         }
-        double bcj = 0.0;
-        uint64_t am = __ballot(Aj != 0);
-        while (am) {
-            const int j = __ffsll((unsigned long long)am) - 1;
-            am &= am - 1;
-            const int32_t v = ch * 64 + j;
-            const uint64_t A = __shfl(Aj, j);
-            const int32_t b = __shfl(bj, j), e = __shfl(ej, j);
-            double acc = 0.0;
-            for (int32_t off = b; off < e; off += 64) {
-                const int32_t kk = off + lane;
-                uint64_t mk = 0;
-                int32_t wk = 0;
-                if (kk < e) {
-                    wk = a.out_col[kk];
-                    mk = Fn[wk] & A;
-                }
-                uint64_t hits = __ballot(mk != 0);
-                while (hits) {
-                    const int t0 = __ffsll((unsigned long long)hits) - 1;
-                    hits &= hits - 1;
-                    const int32_t w0 = __shfl(wk, t0);
-                    const uint64_t m0 = __shfl(mk, t0);
-                    if (hits) {
-                        const int t1 = __ffsll((unsigned long long)hits) - 1;
-                        hits &= hits - 1;
-                        const int32_t w1 = __shfl(wk, t1);
-                        const uint64_t m1 = __shfl(mk, t1);
-                        const double x0 = ((m0 >> lane) & 1) ? sig[(size_t)w0 * 64 + lane] : 0.0;
-                        const double x1 = ((m1 >> lane) & 1) ? sig[(size_t)w1 * 64 + lane] : 0.0;
-                        acc += x0;
-                        acc += x1;
-                    } else {
-                        if ((m0 >> lane) & 1) acc += sig[(size_t)w0 * 64 + lane];
-                    }
-                }
-            }
-            double contrib = 0.0;
-            if ((A >> lane) & 1) {
-                const double s = sig[(size_t)v * 64 + lane];
-                const double d = s * acc;
-                sig[(size_t)v * 64 + lane] = (1.0 + d) / s;
-                contrib = d;
-            }
-            for (int o = 32; o >= 1; o >>= 1) contrib += __shfl_xor(contrib, o);
-            if (lane == j) bcj = contrib;
-        }
-        if (Aj) acc_bc[vj] += bcj;
     }
+    double bcj = 0.0;
+    uint64_t am = __ballot(Aj != 0);
+    while (am) {
+        const int j = __ffsll((unsigned long long)am) - 1;
+        am &= am - 1;
+        const int32_t v = ch * 64 + j;
+        const uint64_t A = __shfl(Aj, j);
+        const int32_t b = __shfl(bj, j), e = __shfl(ej, j);
+        double acc = 0.0;
+        for (int32_t off = b; off < e; off += 64) {
+            const int32_t kk = off + lane;
+            uint64_t mk = 0;
+            int32_t wk = 0;
+            if (kk < e) {
+                wk = a.out_col[kk];
+                mk = Fn[wk] & A;
+            }
+            uint64_t hits = __ballot(mk != 0);
+            uint64_t unused = 0;
+            while (hits) gather_hits(hits, wk, mk, sig, lane, acc, unused);
+        }
+        double contrib = 0.0;
+        if ((A >> lane) & 1) {
+            const double s = sig[(size_t)v * 64 + lane];
+            const double d = s * acc;
+            sig[(size_t)v * 64 + lane] = (1.0 + d) / s;
+            contrib = d;
+        }
+        for (int o = 32; o >= 1; o >>= 1) contrib += __shfl_xor(contrib, o);
+        if (lane == j) bcj = contrib;
+    }
+    if (Aj) acc_bc[vj] += bcj;
+}
+
+// Backward step of one slot at level L over heavy-row segment sg: the lane
+// sums of the successors' coefficients over the segment go to the segment's
+// partial (nothing when the row's vertex has no lane on level L).
+__device__ void bwd_seg(const BcArgs& a, int slot, int32_t sg, int32_t L, int lane) {
+    const int32_t nv = a.nv;
+    const int32_t v = a.hout.seg_v[sg];
+    const uint64_t A = a.F[((size_t)slot * a.cap + L) * nv + v];
+    if (!A) return;
+    const uint64_t* Fn = a.F + ((size_t)slot * a.cap + L + 1) * nv;
+    const double* sig = a.sig + (size_t)slot * nv * 64;
+    const int32_t b = a.hout.seg_b[sg], e = a.hout.seg_e[sg];
+    double acc = 0.0;
+    for (int32_t off = b; off < e; off += 64) {
+        const int32_t kk = off + lane;
+        uint64_t mk = 0;
+        int32_t wk = 0;
+        if (kk < e) {
+            wk = a.out_col[kk];
+            mk = Fn[wk] & A;
+        }
+        uint64_t hits = __ballot(mk != 0);
+        uint64_t unused = 0;
+        while (hits) gather_hits(hits, wk, mk, sig, lane, acc, unused);
+    }
+    a.hpart[((size_t)slot * a.seg_stride + sg) * 64 + lane] = acc;
+}
+
+// Finishes heavy out-row h of one slot at level L after its segments: the
+// partials in segment order give delta, then as bwd_chunk.
+__device__ void bwd_combine(const BcArgs& a, int slot, int32_t h, int32_t L, int lane) {
+    const int32_t nv = a.nv;
+    const int32_t f0 = a.hout.h_first[h], nsg = a.hout.h_nseg[h];
+    const int32_t v = a.hout.seg_v[f0];
+    const uint64_t A = a.F[((size_t)slot * a.cap + L) * nv + v];
+    if (!A) return;
+    const double* part = a.hpart + ((size_t)slot * a.seg_stride + f0) * 64;
+    double tot = 0.0;
+    for (int32_t q = 0; q < nsg; q++) tot += part[(size_t)q * 64 + lane];
+    double* sig = a.sig + (size_t)slot * nv * 64;
+    double contrib = 0.0;
+    if ((A >> lane) & 1) {
+        const double sv = sig[(size_t)v * 64 + lane];
+        const double d = sv * tot;
+        sig[(size_t)v * 64 + lane] = (1.0 + d) / sv;
+        contrib = d;
+    }
+    for (int o = 32; o >= 1; o >>= 1) contrib += __shfl_xor(contrib, o);
+    if (lane == 0) a.bc_acc[(size_t)slot * nv + v] += contrib;
+}
+
+__global__ __launch_bounds__(kBlock) void bc_backward(BcArgs a, int32_t k) {
+    const int slot = blockIdx.y;
+    const int32_t L = a.ctrl[slot].maxlevel - k;
+    if (L < 1) return;
+    const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int32_t nseg = a.hout.nseg;
+    for (int32_t i = blockIdx.x * kWaves + wid; i < nseg + a.nchunks; i += gridDim.x * kWaves) {
+        if (i < nseg) bwd_seg(a, slot, i, L, lane);
+        else bwd_chunk(a, slot, i - nseg, L, lane);
+    }
+}
+
+// Backward level step, part 2: finishes the heavy rows of level maxlevel - k.
+__global__ __launch_bounds__(kBlock) void bc_backward_fin(BcArgs a, int32_t k) {
+    const int slot = blockIdx.y;
+    const int32_t L = a.ctrl[slot].maxlevel - k;
+    if (L < 1) return;
+    const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    for (int32_t h = blockIdx.x * kWaves + wid; h < a.hout.nh; h += gridDim.x * kWaves) bwd_combine(a, slot, h, L, lane);
+}
+
+constexpr int kMaxSlotsP = 1024;        // slots of one persistent launch
+constexpr int kMaxWordsP = kMaxSlotsP / 64;
+constexpr int kMaxGridP = 4096;
+
+struct PersistArgs {
+    BcPersistCtl* ctl;
+    uint64_t* found;
+    uint32_t* rel;        // [grid][32] per-block barrier flags
+    const int32_t* lane_src;
+    const double* endp;   // endpoints term or null
+    double* out;
+    double scale;
+    uint64_t timeout;     // wall-clock ticks a barrier may wait
+    uint64_t gather;      // wall-clock ticks without a new block that end the start-up
+    int32_t ns;
+    int32_t words;        // (ns + 63) / 64
+    int32_t do_scale;
+    int32_t pad;
+};
+
+// Barrier over the nblocks participating blocks of a launch (all resident):
+// release fence and an arrival count; the last block to arrive writes the
+// new generation into every block's own flag (one 128-byte line each, so
+// the waiting blocks poll different addresses and their polls do not pile
+// up on one memory channel beside the blocks still working); acquire
+// fence.  A wait longer than `timeout` ticks, or an abort seen on the way,
+// returns false in every thread of the block, which then leaves the kernel.
+__device__ bool grid_sync(BcPersistCtl* c, uint32_t* rel, uint32_t vb, uint32_t nblocks, uint32_t& gen,
+                          uint64_t timeout, int* s_ok) {
+    __shared__ int s_last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int ok = __hip_atomic_load(&c->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
+        int last = 0;
+        if (ok) {
+            const uint32_t prev = __hip_atomic_fetch_add(&c->bar_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            if (prev == nblocks - 1) {
+                __hip_atomic_store(&c->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                last = 1;
+            }
+        }
+        *s_ok = ok;
+        s_last = last;
+    }
+    __syncthreads();
+    if (s_last) {
+        for (uint32_t b = threadIdx.x; b < nblocks; b += blockDim.x)
+            __hip_atomic_store(&rel[(size_t)b * 32], gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else if (threadIdx.x == 0 && *s_ok) {
+        const uint64_t t0 = wall_clock64();
+        int nap = 0;
+        // relaxed polls: an acquiring load would invalidate the L2 each time
+        while (__hip_atomic_load(&rel[(size_t)vb * 32], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != gen + 1) {
+            if ((nap & 15) == 15 && __hip_atomic_load(&c->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+                *s_ok = 0;
+                break;
+            }
+            if (wall_clock64() - t0 > timeout) {
+                atomicExch(&c->abort, 1);
+                *s_ok = 0;
+                break;
+            }
+            if (nap < 8) __builtin_amdgcn_s_sleep(4);
+            else __builtin_amdgcn_s_sleep(32);
+            nap++;
+        }
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // one L2 invalidate
+    }
+    gen++;
+    __syncthreads();
+    return *s_ok != 0;
+}
+
+__device__ inline uint64_t wave_or64(uint64_t m) {
+    for (int o = 32; o >= 1; o >>= 1) m |= __shfl_xor(m, o);
+    return m;
+}
+
+// The whole computation of up to kMaxSlotsP lane groups in one launch:
+// clear, seed, forward levels until no slot finds a vertex, backward levels
+// deepest first, then the reduction over slots with the endpoints term and
+// the scale.  The chunk work is the same as bc_forward / bc_backward's,
+// and every slot's sums are formed in the same order, so the result is
+// bit-identical to the level-per-launch path.  Slot s finishes at the first
+// level L whose step finds no new vertex for it (maxlevel = L); every block
+// learns that from the per-block found masks after the barrier, so all
+// blocks run the same number of barriers.  With heavy rows every level step
+// has a second barrier, after the rows' partial sums are combined.
+__global__ __launch_bounds__(kBlock) void bc_persist(BcArgs a, PersistArgs p) {
+    __shared__ uint64_t s_found[kMaxWordsP];
+    __shared__ uint64_t s_live[kMaxWordsP];
+    __shared__ int32_t s_maxl[kMaxSlotsP];
+    __shared__ int s_ok;
+    __shared__ uint32_t s_vb, s_np;
+    BcPersistCtl* c = p.ctl;
+    const int tid = threadIdx.x, lane = tid & 63, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+    // Participants: a block may only wait for blocks that are resident.  The
+    // blocks take tickets as they start; the first waits until no ticket has
+    // been taken for kGatherTicks (or all are taken) and fixes the number of
+    // participants, rounded down so that ns divides their wavefront count.
+    // Blocks whose ticket is beyond it leave at once.  Which block does which
+    // item does not change any result.
+    if (tid == 0) {
+        const uint32_t t = atomicAdd(&c->arrive, 1u);
+        uint32_t np = 0;
+        if (t == 0) {
+            uint32_t last = 1;
+            uint64_t t_last = wall_clock64();
+            for (;;) {
+                const uint32_t now = __hip_atomic_load(&c->arrive, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                const uint64_t tn = wall_clock64();
+                if (now != last) {
+                    last = now;
+                    t_last = tn;
+                }
+                if (now >= gridDim.x || tn - t_last > p.gather) break;
+                __builtin_amdgcn_s_sleep(4);
+            }
+            np = last;
+            while (np > 0 && (np * kWaves) % (uint32_t)p.ns != 0) np--;
+            __hip_atomic_store(&c->closed, np + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            const uint64_t t0 = wall_clock64();
+            for (;;) {
+                const uint32_t cl = __hip_atomic_load(&c->closed, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (cl) {
+                    np = cl - 1;
+                    break;
+                }
+                if (wall_clock64() - t0 > p.timeout) {
+                    atomicExch(&c->abort, 1);
+                    break;
+                }
+                __builtin_amdgcn_s_sleep(16);
+            }
+        }
+        s_vb = t;
+        s_np = np;
+    }
+    __syncthreads();
+    const uint32_t vb = s_vb, G = s_np;   // this block's index among the G participants
+    if (vb >= G) {
+        if (G == 0 && vb == 0) atomicExch(&c->abort, 1);
+        return;
+    }
+    // Work items: the heavy-row segments, then the 64-vertex chunks, each
+    // once per slot, slot-minor.  Wavefront gw takes positions i = gw + r*nw
+    // (r = 0, 1, ...), and position i is item (i / ns, slot (i + r) % ns):
+    // as ns divides nw, every group of ns positions of one item is dealt in
+    // one round r, so this is a bijection, and a wavefront's slot changes
+    // from round to round, which spreads a slot with much work over all
+    // wavefronts.  The host keeps the item counts below 2^31.
+    const uint32_t gw = vb * kWaves + wid, nw = G * kWaves;
+    const uint32_t nitems_f = (uint32_t)p.ns * (uint32_t)(a.hin.nseg + a.nchunks);
+    const uint32_t nitems_b = (uint32_t)p.ns * (uint32_t)(a.hout.nseg + a.nchunks);
+    const uint32_t nrows_f = (uint32_t)p.ns * (uint32_t)a.hin.nh, nrows_b = (uint32_t)p.ns * (uint32_t)a.hout.nh;
+    const int64_t nv = a.nv;
+    const int ns = p.ns, W = p.words;
+    uint32_t gen = 0;
+    if (tid == 0) gen = __hip_atomic_load(&c->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (vb == 0 && tid == 0) {
+        c->overflow = 0;
+        c->levels = 0;
+        c->barriers = 0;
+    }
+    // clear seen, F_0 and the partial betweenness of every slot
+    for (int64_t i = (int64_t)vb * kBlock + tid; i < (int64_t)ns * nv; i += (int64_t)G * kBlock) {
+        const int64_t sl = i / nv;
+        a.seen[i] = 0;
+        a.F[sl * a.cap * nv + (i - sl * nv)] = 0;
+        a.bc_acc[i] = 0.0;
+    }
+    for (int w = tid; w < W; w += kBlock) s_live[w] = 0;
+    __syncthreads();
+    for (int sl = tid; sl < ns; sl += kBlock) {
+        s_maxl[sl] = 0;
+        if (a.active[sl]) atomicOr((unsigned long long*)&s_live[sl >> 6], 1ull << (sl & 63));
+    }
+    int nbar = 0;
+    if (!grid_sync(c, p.rel, vb, G, gen, p.timeout, &s_ok)) return;
+    nbar++;
+    // seed: lane l of slot s starts at lane_src[s][l]
+    for (int sl = (int)vb; sl < ns; sl += (int)G) {
+        if (tid < 64) {
+            const int32_t src = p.lane_src[sl * 64 + tid];
+            if (src >= 0) {
+                atomicOr((unsigned long long*)&a.seen[(size_t)sl * nv + src], 1ull << tid);
+                atomicOr((unsigned long long*)&a.F[(size_t)sl * a.cap * nv + src], 1ull << tid);
+                a.sig[((size_t)sl * nv + src) * 64 + tid] = 1.0;
+            }
+        }
+    }
+    if (!grid_sync(c, p.rel, vb, G, gen, p.timeout, &s_ok)) return;
+    nbar++;
+    if (vb == 0 && tid == 0) c->t_fwd = wall_clock64();
+    int32_t L = 0;
+    for (;;) {
+        bool live = false;
+        for (int w = 0; w < W; w++) live |= s_live[w] != 0;
+        if (!live) break;
+        if (L + 1 >= a.cap) {
+            if (vb == 0 && tid == 0) c->overflow = 1;
+            return;
+        }
+        for (int w = tid; w < W; w += kBlock) s_found[w] = 0;
+        __syncthreads();
+        for (uint32_t i = gw, r = 0; i < nitems_f; i += nw, r++) {
+            const int sl = (int)((i + r) % (uint32_t)ns);
+            const int32_t q = (int32_t)(i / (uint32_t)ns);
+            if (!((s_live[sl >> 6] >> (sl & 63)) & 1)) continue;
+            const bool got = q < a.hin.nseg ? fwd_seg(a, sl, q, L, lane) : fwd_chunk(a, sl, q - a.hin.nseg, L, lane);
+            if (got && lane == 0) atomicOr((unsigned long long*)&s_found[sl >> 6], 1ull << (sl & 63));
+        }
+        __syncthreads();
+        uint64_t* fo = p.found + ((size_t)(L & 1) * G + vb) * W;
+        for (int w = tid; w < W; w += kBlock) fo[w] = s_found[w];
+        if (!grid_sync(c, p.rel, vb, G, gen, p.timeout, &s_ok)) return;
+        nbar++;
+        // heavy rows: one wavefront per (row, live slot) adds the partials
+        for (uint32_t i = gw, r = 0; i < nrows_f; i += nw, r++) {
+            const int sl = (int)((i + r) % (uint32_t)ns);
+            if ((s_live[sl >> 6] >> (sl & 63)) & 1) fwd_combine(a, sl, (int32_t)(i / (uint32_t)ns), L, lane);
+        }
+        for (int w = tid; w < W; w += kBlock) s_found[w] = 0;
+        __syncthreads();
+        for (int w = 0; w < W; w++) {
+            uint64_t m = 0;
+            for (int b = tid; b < (int)G; b += kBlock)
+                m |= __hip_atomic_load(&p.found[((size_t)(L & 1) * G + b) * W + w], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+            m = wave_or64(m);
+            if (lane == 0 && m) atomicOr((unsigned long long*)&s_found[w], m);
+        }
+        __syncthreads();
+        for (int sl = tid; sl < ns; sl += kBlock)
+            if (((s_live[sl >> 6] >> (sl & 63)) & 1) && !((s_found[sl >> 6] >> (sl & 63)) & 1)) s_maxl[sl] = L;
+        __syncthreads();
+        for (int w = tid; w < W; w += kBlock) s_live[w] &= s_found[w];
+        if (nrows_f > 0) {
+            if (!grid_sync(c, p.rel, vb, G, gen, p.timeout, &s_ok)) return;
+            nbar++;
+        } else {
+            __syncthreads();
+        }
+        L++;
+    }
+    int32_t kmax = 0;
+    for (int sl = 0; sl < ns; sl++) kmax = max(kmax, s_maxl[sl]);
+    if (vb == 0) {
+        if (tid == 0) {
+            c->t_bwd = wall_clock64();
+            c->levels = L;
+        }
+        for (int sl = tid; sl < ns; sl += kBlock) {
+            a.ctrl[sl].maxlevel = s_maxl[sl];
+            a.ctrl[sl].done = 1;
+        }
+    }
+    for (int32_t k = 0; k < kmax; k++) {
+        for (uint32_t i = gw, r = 0; i < nitems_b; i += nw, r++) {
+            const int sl = (int)((i + r) % (uint32_t)ns);
+            const int32_t q = (int32_t)(i / (uint32_t)ns);
+            const int32_t Ls = s_maxl[sl] - k;
+            if (Ls < 1) continue;
+            if (q < a.hout.nseg) bwd_seg(a, sl, q, Ls, lane);
+            else bwd_chunk(a, sl, q - a.hout.nseg, Ls, lane);
+        }
+        if (!grid_sync(c, p.rel, vb, G, gen, p.timeout, &s_ok)) return;
+        nbar++;
+        if (nrows_b > 0) {
+            for (uint32_t i = gw, r = 0; i < nrows_b; i += nw, r++) {
+                const int sl = (int)((i + r) % (uint32_t)ns);
+                const int32_t Ls = s_maxl[sl] - k;
+                if (Ls >= 1) bwd_combine(a, sl, (int32_t)(i / (uint32_t)ns), Ls, lane);
+            }
+            if (!grid_sync(c, p.rel, vb, G, gen, p.timeout, &s_ok)) return;
+            nbar++;
+        }
+    }
+    if (vb == 0 && tid == 0) {
+        c->t_end = wall_clock64();
+        c->barriers = nbar;
+    }
+    for (int64_t v = (int64_t)vb * kBlock + tid; v < nv; v += (int64_t)G * kBlock) {
+        double sum = 0.0;
+        for (int k = 0; k < ns; k++) sum += a.bc_acc[(size_t)k * nv + v];
+        if (p.endp) sum += p.endp[v];
+        if (p.do_scale) sum *= p.scale;
+        p.out[v] = sum;
+    }
+}
+
+__global__ void bc_ctl_reset(BcPersistCtl* c) {
+    c->bar_count = 0;
+    c->abort = 0;
+    c->overflow = 0;
+}
+
+// Runs before every persistent launch (in the same captured graph).
+__global__ void bc_persist_prologue(BcPersistCtl* c) {
+    c->arrive = 0;
+    c->closed = 0;
 }
 
 // bc[v] = (sum over slots of bc_acc, in slot order, + endpoints term) * scale
@@ -327,6 +862,10 @@ void bc_free(BcWork* w) {
     dev_free(w->endp); dev_free(w->active); dev_free(w->lane_src); dev_free(w->ctrl);
     if (w->h_ctrl) (void)hipHostFree(w->h_ctrl);
     for (auto& e : w->ev) if (e) (void)hipEventDestroy(e);
+    if (w->pexec) (void)hipGraphExecDestroy(w->pexec);
+    dev_free(w->pctl); dev_free(w->found); dev_free(w->rel);
+    dev_free(w->hpart); dev_free(w->hgot);
+    if (w->h_pctl) (void)hipHostFree(w->h_pctl);
     delete w;
 }
 
@@ -337,9 +876,10 @@ static size_t free_device_bytes() {
 }
 
 // (Re)allocates the per-slot state for `nslots` slots and `cap` levels.
-static void bc_alloc(lgmi_graph* g, int32_t nslots, int32_t cap) {
+static void bc_alloc(lgmi_graph* g, int32_t nslots, int32_t cap, int32_t seg_stride) {
     BcWork* w = g->bc;
-    if (w && w->nslots >= nslots && w->cap_levels >= cap && w->nv == g->nv) return;
+    if (w && w->nslots >= nslots && w->cap_levels >= cap && w->nv == g->nv && w->seg_stride >= seg_stride)
+        return;
     bc_free(w);
     g->bc = nullptr;
     w = new BcWork();
@@ -359,6 +899,15 @@ static void bc_alloc(lgmi_graph* g, int32_t nslots, int32_t cap) {
         w->ctrl = dev_alloc_n<BcCtrl>(nslots);
         LGMI_HIP(hipHostMalloc((void**)&w->h_ctrl, sizeof(BcCtrl) * nslots, hipHostMallocDefault));
         for (auto& e : w->ev) LGMI_HIP(hipEventCreate(&e));
+        w->seg_stride = seg_stride;
+        w->hpart = dev_alloc_n<double>((size_t)std::max(seg_stride, 1) * nslots * 64);
+        w->hgot = dev_alloc_n<uint64_t>((size_t)std::max(seg_stride, 1) * nslots);
+        w->pctl = dev_alloc_n<BcPersistCtl>(1);
+        LGMI_HIP(hipHostMalloc((void**)&w->h_pctl, sizeof(BcPersistCtl), hipHostMallocDefault));
+        w->found = dev_alloc_n<uint64_t>((size_t)2 * kMaxGridP * kMaxWordsP);
+        w->rel = dev_alloc_n<uint32_t>((size_t)kMaxGridP * 32);
+        LGMI_HIP(hipMemsetAsync(w->pctl, 0, sizeof(BcPersistCtl), g->stream));
+        LGMI_HIP(hipMemsetAsync(w->rel, 0, sizeof(uint32_t) * kMaxGridP * 32, g->stream));
     } catch (...) {
         bc_free(w);
         throw;
@@ -380,17 +929,188 @@ static void bc_grow_levels(lgmi_graph* g, int32_t new_cap, int32_t keep) {
     w->cap_levels = new_cap;
 }
 
+// Host row pointers and heavy-row segments of g, built on first use.
+static BcGraph* bc_graph_get(lgmi_graph* g) {
+    if (g->bcg) return g->bcg;
+    const int32_t nv = g->nv;
+    BcGraph* b = new BcGraph();
+    try {
+        b->rp_out.resize((size_t)nv + 1);
+        LGMI_HIP(hipMemcpy(b->rp_out.data(), g->out.row_ptr, sizeof(int32_t) * ((size_t)nv + 1), hipMemcpyDeviceToHost));
+        if (g->directed) {
+            b->rp_in.resize((size_t)nv + 1);
+            LGMI_HIP(hipMemcpy(b->rp_in.data(), g->in.row_ptr, sizeof(int32_t) * ((size_t)nv + 1), hipMemcpyDeviceToHost));
+        }
+        // seg_v, seg_b, seg_e, seg_h, h_first, h_nseg of one CSR
+        auto build = [&](const std::vector<int32_t>& rp, std::vector<int32_t> (&arr)[6]) {
+            for (int32_t v = 0; v < nv; v++) {
+                const int32_t bb = rp[v], e = rp[v + 1];
+                if (e - bb <= kBcHeavyRow) continue;
+                const int32_t h = (int32_t)arr[4].size();
+                arr[4].push_back((int32_t)arr[0].size());
+                for (int32_t x = bb; x < e; x += kBcSegLen) {
+                    arr[0].push_back(v);
+                    arr[1].push_back(x);
+                    arr[2].push_back(std::min(e, x + kBcSegLen));
+                    arr[3].push_back(h);
+                }
+                arr[5].push_back((int32_t)arr[0].size() - arr[4].back());
+            }
+        };
+        std::vector<int32_t> ain[6], aout[6];
+        build(g->directed ? b->rp_in : b->rp_out, ain);
+        if (g->directed) build(b->rp_out, aout);
+        size_t total = 0;
+        for (auto& x : ain) total += x.size();
+        for (auto& x : aout) total += x.size();
+        b->dev = dev_alloc_n<int32_t>(std::max<size_t>(total, 1));
+        std::vector<int32_t> packed;
+        packed.reserve(total);
+        int32_t* base = b->dev;
+        for (int k = 0; k < 6; k++) {
+            b->seg_in[k] = base + packed.size();
+            packed.insert(packed.end(), ain[k].begin(), ain[k].end());
+        }
+        for (int k = 0; k < 6; k++) {
+            b->seg_out[k] = base + packed.size();
+            packed.insert(packed.end(), aout[k].begin(), aout[k].end());
+        }
+        b->nseg_in = (int32_t)ain[0].size();
+        b->nh_in = (int32_t)ain[4].size();
+        if (g->directed) {
+            b->nseg_out = (int32_t)aout[0].size();
+            b->nh_out = (int32_t)aout[4].size();
+        } else {
+            for (int k = 0; k < 6; k++) b->seg_out[k] = b->seg_in[k];
+            b->nseg_out = b->nseg_in;
+            b->nh_out = b->nh_in;
+        }
+        if (total > 0)
+            LGMI_HIP(hipMemcpy(b->dev, packed.data(), sizeof(int32_t) * total, hipMemcpyHostToDevice));
+    } catch (...) {
+        bc_graph_free(b);
+        throw;
+    }
+    g->bcg = b;
+    return b;
+}
+
+static BcSegs make_segs(int32_t* const (&arr)[6], int32_t nseg, int32_t nh) {
+    BcSegs r{};
+    r.seg_v = arr[0]; r.seg_b = arr[1]; r.seg_e = arr[2]; r.seg_h = arr[3]; r.h_first = arr[4]; r.h_nseg = arr[5];
+    r.nseg = nseg;
+    r.nh = nh;
+    return r;
+}
+
+// 0: one persistent launch when the slots fit (level-per-launch otherwise
+// and as the fallback), 1: level-per-launch only.
+static std::atomic<int> g_launch_mode{0};
+
+void bc_set_launch(int mode) { g_launch_mode = mode; }
+
+// One launch for all groups (ngroups <= slots): captured in a hipGraph when
+// the runtime captures cooperative launches, else launched cooperatively.
+// Returns false when the launch could not finish (a barrier timed out or a
+// slot ran past the level capacity); the caller then runs the level-per-launch
+// path, which overwrites everything this one wrote.
+static bool bc_persistent(lgmi_graph* g, BcArgs a, int32_t ns, const double* endp, int do_scale, double scale,
+                          int32_t* launch_kind, float* fwd_ms, float* bwd_ms, int32_t* max_levels) {
+    BcWork* w = g->bc;
+    hipStream_t st = g->stream;
+    int dev = 0;
+    LGMI_HIP(hipGetDevice(&dev));
+    int coop = 0, khz = 0, per_cu = 0;
+    LGMI_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    LGMI_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    if (!coop || khz <= 0) return false;
+    LGMI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(bc_persist), kBlock, 0));
+    const int cu = std::max(1, g->cu_count);
+    // the blocks that the occupancy calculation says fit; the kernel itself
+    // keeps only those that start together
+    const int grid = std::min(std::min(per_cu, 8) * cu, kMaxGridP);
+    if (per_cu < 1 || grid * kWaves < ns) return false;
+    PersistArgs p{};
+    p.ctl = w->pctl;
+    p.found = w->found;
+    p.rel = w->rel;
+    p.lane_src = w->lane_src;
+    p.endp = endp;
+    p.out = w->bc_out;
+    p.scale = scale;
+    p.timeout = (uint64_t)khz * 1000ull * 5ull;  // 5 s
+    p.gather = (uint64_t)khz / 50ull + 1;          // 20 us
+    p.ns = ns;
+    p.words = (ns + 63) / 64;
+    p.do_scale = do_scale;
+    void* args[] = {&a, &p};
+    const dim3 gd(grid), bd(kBlock);
+    std::vector<unsigned char> key(sizeof(a) + sizeof(p) + sizeof(int));
+    std::memcpy(key.data(), &a, sizeof(a));
+    std::memcpy(key.data() + sizeof(a), &p, sizeof(p));
+    std::memcpy(key.data() + sizeof(a) + sizeof(p), &grid, sizeof(int));
+    if (w->pexec && w->pkey != key) {
+        (void)hipGraphExecDestroy(w->pexec);
+        w->pexec = nullptr;
+    }
+    if (!w->pexec) {
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            bc_persist_prologue<<<1, 1, 0, st>>>(w->pctl);
+            hipError_t el = hipGetLastError();
+            if (el == hipSuccess)
+                el = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(bc_persist), gd, bd, args, 0, st);
+            e = hipStreamEndCapture(st, &graph);
+            if (el != hipSuccess) e = el;
+        }
+        if (e == hipSuccess && graph) e = hipGraphInstantiate(&w->pexec, graph, nullptr, nullptr, 0);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            if (w->pexec) (void)hipGraphExecDestroy(w->pexec);
+            w->pexec = nullptr;
+        } else {
+            w->pkey = key;
+        }
+    }
+    LGMI_HIP(hipEventRecord(w->ev[1], st));
+    if (w->pexec) {
+        LGMI_HIP(hipGraphLaunch(w->pexec, st));
+        *launch_kind = 2;
+    } else {
+        bc_persist_prologue<<<1, 1, 0, st>>>(w->pctl);
+        LGMI_HIP(hipGetLastError());
+        LGMI_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(bc_persist), gd, bd, args, 0, st));
+        *launch_kind = 1;
+    }
+    LGMI_HIP(hipEventRecord(w->ev[2], st));
+    LGMI_HIP(hipMemcpyAsync(w->h_pctl, w->pctl, sizeof(BcPersistCtl), hipMemcpyDeviceToHost, st));
+    LGMI_HIP(hipMemcpyAsync(w->h_ctrl, w->ctrl, sizeof(BcCtrl) * ns, hipMemcpyDeviceToHost, st));
+    LGMI_HIP(hipStreamSynchronize(st));
+    const BcPersistCtl c = *w->h_pctl;
+    if (c.abort || c.overflow) {
+        bc_ctl_reset<<<1, 1, 0, st>>>(w->pctl);
+        LGMI_HIP(hipGetLastError());
+        return false;
+    }
+    const double tick_ms = 1.0 / (double)khz;
+    *fwd_ms = (float)((double)(c.t_bwd - c.t_fwd) * tick_ms);
+    *bwd_ms = (float)((double)(c.t_end - c.t_bwd) * tick_ms);
+    int32_t lm = 0;
+    for (int s = 0; s < ns; s++) lm = std::max(lm, w->h_ctrl[s].maxlevel);
+    *max_levels = lm;
+    return true;
+}
+
 void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, int skip_degree0, int normalize,
             int endpoints, double* bc, lgmi_bc_stats* stats) {
     const int32_t nv = g->nv;
     hipStream_t st = g->stream;
     // degree(g, s) > 0 filter (betweenness.jl:58) on host copies of the row pointers
-    std::vector<int32_t> rp_out((size_t)nv + 1), rp_in;
-    LGMI_HIP(hipMemcpy(rp_out.data(), g->out.row_ptr, sizeof(int32_t) * ((size_t)nv + 1), hipMemcpyDeviceToHost));
-    if (g->directed) {
-        rp_in.resize((size_t)nv + 1);
-        LGMI_HIP(hipMemcpy(rp_in.data(), g->in.row_ptr, sizeof(int32_t) * ((size_t)nv + 1), hipMemcpyDeviceToHost));
-    }
+    BcGraph* bg = bc_graph_get(g);
+    const std::vector<int32_t>& rp_out = bg->rp_out;
+    const std::vector<int32_t>& rp_in = bg->rp_in;
     auto degree = [&](int32_t v) -> int64_t {
         int64_t d = rp_out[v + 1] - rp_out[v];
         if (g->directed) d += rp_in[v + 1] - rp_in[v];
@@ -405,12 +1125,13 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
 
     // slots: as many groups at once as fit a memory budget
     const int32_t cap0 = 32;
-    const size_t per_slot = (size_t)std::max(nv, 1) * (8 + 8 * (size_t)cap0 + 512 + 8);
+    const int32_t seg_stride = std::max(bg->nseg_in, bg->nseg_out);
+    const size_t per_slot = (size_t)std::max(nv, 1) * (8 + 8 * (size_t)cap0 + 512 + 8) + (size_t)seg_stride * 520;
     size_t budget = free_device_bytes() / 3;
     if (budget == 0) budget = (size_t)4 << 30;
     int32_t nslots = (int32_t)std::max<size_t>(1, std::min<size_t>((size_t)std::max(ngroups, 1), budget / per_slot));
-    nslots = std::min(nslots, 1024);
-    bc_alloc(g, nslots, cap0);
+    nslots = std::min(nslots, kMaxSlotsP);
+    bc_alloc(g, nslots, cap0, seg_stride);
     BcWork* w = g->bc;
     nslots = std::min(w->nslots, std::max(ngroups, 1));
 
@@ -421,77 +1142,18 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
     a.nchunks = (nv + 63) / 64;
     a.seen = w->seen; a.F = w->F; a.sig = w->sig; a.bc_acc = w->bc_acc;
     a.active = w->active; a.ctrl = w->ctrl; a.cap = w->cap_levels;
+    a.hin = make_segs(bg->seg_in, bg->nseg_in, bg->nh_in);
+    a.hout = make_segs(bg->seg_out, bg->nseg_out, bg->nh_out);
+    a.hpart = w->hpart; a.hgot = w->hgot;
+    a.seg_stride = w->seg_stride;
 
     const int cu = std::max(1, g->cu_count);
     const int target_blocks = cu * 8;
-    const int bx = std::max(1, std::min((a.nchunks + kWaves - 1) / kWaves, std::max(1, target_blocks / nslots)));
+    const int bx = std::max(1, std::min((seg_stride + a.nchunks + kWaves - 1) / kWaves, std::max(1, target_blocks / nslots)));
     const int ibx = std::max(1, std::min((nv + 255) / 256, std::max(1, target_blocks / nslots)));
+    const int hbx_in = std::max(1, std::min((a.hin.nh + kWaves - 1) / kWaves, std::max(1, target_blocks / nslots)));
+    const int hbx_out = std::max(1, std::min((a.hout.nh + kWaves - 1) / kWaves, std::max(1, target_blocks / nslots)));
 
-    LGMI_HIP(hipEventRecord(w->ev[0], st));
-    bc_clear_acc<<<std::max(1, std::min(target_blocks, (int)(((size_t)nv * nslots + 255) / 256))), 256, 0, st>>>(
-        w->bc_acc, (int64_t)nv * nslots);
-    float fwd_ms = 0.f, bwd_ms = 0.f;
-    int32_t max_levels = 0;
-    std::vector<int32_t> lane_src((size_t)nslots * 64);
-    std::vector<uint64_t> active((size_t)nslots);
-    for (int32_t g0 = 0; g0 < ngroups; g0 += nslots) {
-        const int32_t ns = std::min(nslots, ngroups - g0);
-        for (int32_t s = 0; s < nslots; s++) {
-            active[s] = 0;
-            for (int l = 0; l < 64; l++) {
-                const int64_t idx = (int64_t)(g0 + s) * 64 + l;
-                const bool on = s < ns && idx < nact;
-                lane_src[(size_t)s * 64 + l] = on ? act[(size_t)idx] : -1;
-                if (on) active[s] |= 1ull << l;
-            }
-        }
-        LGMI_HIP(hipMemcpyAsync(w->lane_src, lane_src.data(), sizeof(int32_t) * lane_src.size(), hipMemcpyHostToDevice, st));
-        LGMI_HIP(hipMemcpyAsync(w->active, active.data(), sizeof(uint64_t) * active.size(), hipMemcpyHostToDevice, st));
-        a.F = w->F; a.cap = w->cap_levels;
-        bc_init<<<dim3(ibx, ns), 256, 0, st>>>(a);
-        bc_seed<<<ns, 64, 0, st>>>(a, w->lane_src);
-        LGMI_HIP(hipGetLastError());
-        LGMI_HIP(hipEventRecord(w->ev[1], st));
-        // forward: level steps in batches until every slot is done
-        int32_t lcur = 0;
-        int batch = 8;
-        for (;;) {
-            if (lcur + batch + 2 >= w->cap_levels) {
-                int32_t nc = w->cap_levels;
-                while (lcur + batch + 2 >= nc) nc *= 2;
-                // a finished slot also needs its empty F_{maxlevel+1}
-                bc_grow_levels(g, nc, std::min(lcur + 2, w->cap_levels));
-                a.F = w->F; a.cap = w->cap_levels;
-            }
-            for (int i = 0; i < batch; i++) bc_forward<<<dim3(bx, ns), kBlock, 0, st>>>(a);
-            LGMI_HIP(hipGetLastError());
-            LGMI_HIP(hipMemcpyAsync(w->h_ctrl, w->ctrl, sizeof(BcCtrl) * ns, hipMemcpyDeviceToHost, st));
-            LGMI_HIP(hipStreamSynchronize(st));
-            bool all = true;
-            lcur = 0;
-            for (int s = 0; s < ns; s++) {
-                if (w->h_ctrl[s].overflow) throw Error(LGMI_E_HIP, "betweenness level capacity exceeded");
-                all = all && w->h_ctrl[s].done;
-                lcur = std::max(lcur, w->h_ctrl[s].done ? w->h_ctrl[s].maxlevel : w->h_ctrl[s].level);
-            }
-            if (all) break;
-            if (lcur > nv + 2) throw Error(LGMI_E_HIP, "betweenness forward pass did not terminate");
-            batch = std::min(batch * 2, 256);
-        }
-        int32_t lmax = 0;
-        for (int s = 0; s < ns; s++) lmax = std::max(lmax, w->h_ctrl[s].maxlevel);
-        max_levels = std::max(max_levels, lmax);
-        LGMI_HIP(hipEventRecord(w->ev[2], st));
-        for (int32_t k = 0; k < lmax; k++) bc_backward<<<dim3(bx, ns), kBlock, 0, st>>>(a, k);
-        LGMI_HIP(hipGetLastError());
-        LGMI_HIP(hipEventRecord(w->ev[3], st));
-        LGMI_HIP(hipEventSynchronize(w->ev[3]));
-        float t1 = 0.f, t2 = 0.f;
-        LGMI_HIP(hipEventElapsedTime(&t1, w->ev[1], w->ev[2]));
-        LGMI_HIP(hipEventElapsedTime(&t2, w->ev[2], w->ev[3]));
-        fwd_ms += t1;
-        bwd_ms += t2;
-    }
     // endpoints: every processed source s adds length(S)-1 = nv-1 to bc[s]
     // and 1 to every other vertex (betweenness.jl:122,130)
     const double* endp = nullptr;
@@ -515,12 +1177,100 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
         scale = 1.0 / 2.0;
     }
     if (do_scale && k_scale > 0) scale = scale * (double)nv / (double)k_scale;
-    if (nv > 0) {
-        bc_finish<<<std::max(1, std::min(target_blocks, (nv + 255) / 256)), 256, 0, st>>>(
-            w->bc_acc, nslots, nv, endp, do_scale, scale, w->bc_out);
-        LGMI_HIP(hipGetLastError());
-        LGMI_HIP(hipMemcpyAsync(bc, w->bc_out, sizeof(double) * (size_t)nv, hipMemcpyDeviceToHost, st));
+
+    LGMI_HIP(hipEventRecord(w->ev[0], st));
+    std::vector<int32_t> lane_src((size_t)nslots * 64);
+    std::vector<uint64_t> active((size_t)nslots);
+    auto fill_groups = [&](int32_t g0, int32_t ns) {
+        for (int32_t s = 0; s < nslots; s++) {
+            active[s] = 0;
+            for (int l = 0; l < 64; l++) {
+                const int64_t idx = (int64_t)(g0 + s) * 64 + l;
+                const bool on = s < ns && idx < nact;
+                lane_src[(size_t)s * 64 + l] = on ? act[(size_t)idx] : -1;
+                if (on) active[s] |= 1ull << l;
+            }
+        }
+        LGMI_HIP(hipMemcpyAsync(w->lane_src, lane_src.data(), sizeof(int32_t) * lane_src.size(), hipMemcpyHostToDevice, st));
+        LGMI_HIP(hipMemcpyAsync(w->active, active.data(), sizeof(uint64_t) * active.size(), hipMemcpyHostToDevice, st));
+    };
+
+    float fwd_ms = 0.f, bwd_ms = 0.f;
+    int32_t max_levels = 0;
+    int32_t launch_kind = 0;
+    const int64_t max_items = (int64_t)ngroups * ((int64_t)seg_stride + a.nchunks);
+    if (g_launch_mode == 0 && ngroups >= 1 && ngroups <= nslots && nv > 0 && max_items < ((int64_t)1 << 31)) {
+        fill_groups(0, ngroups);
+        if (!bc_persistent(g, a, ngroups, endp, do_scale, scale, &launch_kind, &fwd_ms, &bwd_ms, &max_levels)) {
+            launch_kind = 0;
+            fwd_ms = bwd_ms = 0.f;
+            max_levels = 0;
+        }
     }
+    if (launch_kind == 0) {
+        bc_clear_acc<<<std::max(1, std::min(target_blocks, (int)(((size_t)nv * nslots + 255) / 256))), 256, 0, st>>>(
+            w->bc_acc, (int64_t)nv * nslots);
+        for (int32_t g0 = 0; g0 < ngroups; g0 += nslots) {
+            const int32_t ns = std::min(nslots, ngroups - g0);
+            fill_groups(g0, ns);
+            a.F = w->F; a.cap = w->cap_levels;
+            bc_init<<<dim3(ibx, ns), 256, 0, st>>>(a);
+            bc_seed<<<ns, 64, 0, st>>>(a, w->lane_src);
+            LGMI_HIP(hipGetLastError());
+            LGMI_HIP(hipEventRecord(w->ev[1], st));
+            // forward: level steps in batches until every slot is done
+            int32_t lcur = 0;
+            int batch = 8;
+            for (;;) {
+                if (lcur + batch + 2 >= w->cap_levels) {
+                    int32_t nc = w->cap_levels;
+                    while (lcur + batch + 2 >= nc) nc *= 2;
+                    // a finished slot also needs its empty F_{maxlevel+1}
+                    bc_grow_levels(g, nc, std::min(lcur + 2, w->cap_levels));
+                    a.F = w->F; a.cap = w->cap_levels;
+                }
+                for (int i = 0; i < batch; i++) {
+                    bc_forward<<<dim3(bx, ns), kBlock, 0, st>>>(a);
+                    bc_forward_fin<<<dim3(hbx_in, ns), kBlock, 0, st>>>(a);
+                }
+                LGMI_HIP(hipGetLastError());
+                LGMI_HIP(hipMemcpyAsync(w->h_ctrl, w->ctrl, sizeof(BcCtrl) * ns, hipMemcpyDeviceToHost, st));
+                LGMI_HIP(hipStreamSynchronize(st));
+                bool all = true;
+                lcur = 0;
+                for (int s = 0; s < ns; s++) {
+                    if (w->h_ctrl[s].overflow) throw Error(LGMI_E_HIP, "betweenness level capacity exceeded");
+                    all = all && w->h_ctrl[s].done;
+                    lcur = std::max(lcur, w->h_ctrl[s].done ? w->h_ctrl[s].maxlevel : w->h_ctrl[s].level);
+                }
+                if (all) break;
+                if (lcur > nv + 2) throw Error(LGMI_E_HIP, "betweenness forward pass did not terminate");
+                batch = std::min(batch * 2, 256);
+            }
+            int32_t lmax = 0;
+            for (int s = 0; s < ns; s++) lmax = std::max(lmax, w->h_ctrl[s].maxlevel);
+            max_levels = std::max(max_levels, lmax);
+            LGMI_HIP(hipEventRecord(w->ev[2], st));
+            for (int32_t k = 0; k < lmax; k++) {
+                bc_backward<<<dim3(bx, ns), kBlock, 0, st>>>(a, k);
+                if (a.hout.nh > 0) bc_backward_fin<<<dim3(hbx_out, ns), kBlock, 0, st>>>(a, k);
+            }
+            LGMI_HIP(hipGetLastError());
+            LGMI_HIP(hipEventRecord(w->ev[3], st));
+            LGMI_HIP(hipEventSynchronize(w->ev[3]));
+            float t1 = 0.f, t2 = 0.f;
+            LGMI_HIP(hipEventElapsedTime(&t1, w->ev[1], w->ev[2]));
+            LGMI_HIP(hipEventElapsedTime(&t2, w->ev[2], w->ev[3]));
+            fwd_ms += t1;
+            bwd_ms += t2;
+        }
+        if (nv > 0) {
+            bc_finish<<<std::max(1, std::min(target_blocks, (nv + 255) / 256)), 256, 0, st>>>(
+                w->bc_acc, nslots, nv, endp, do_scale, scale, w->bc_out);
+            LGMI_HIP(hipGetLastError());
+        }
+    }
+    if (nv > 0) LGMI_HIP(hipMemcpyAsync(bc, w->bc_out, sizeof(double) * (size_t)nv, hipMemcpyDeviceToHost, st));
     LGMI_HIP(hipEventRecord(w->ev[3], st));
     LGMI_HIP(hipEventSynchronize(w->ev[3]));
     if (stats) {
@@ -528,6 +1278,7 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
         LGMI_HIP(hipEventElapsedTime(&tot, w->ev[0], w->ev[3]));
         stats->groups = ngroups;
         stats->max_levels = max_levels;
+        stats->launch = launch_kind;
         stats->device_ms = tot;
         stats->forward_ms = fwd_ms;
         stats->backward_ms = bwd_ms;

@@ -176,6 +176,7 @@ static void graph_free(lgmi_graph* g) {
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     bfs_free(g->bfs);
     bc_free(g->bc);
+    bc_graph_free(g->bcg);
     if (g->in.row_ptr != g->out.row_ptr) {
         dev_free(g->in.row_ptr); dev_free(g->in.col); dev_free(g->in.nz_bm);
         dev_free(g->in.item_v); dev_free(g->in.item_beg); dev_free(g->in.item_end);
@@ -373,6 +374,13 @@ int lgmi_betweenness(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int64_t 
     std::lock_guard<std::mutex> lk(g->mu);
     for (int32_t i = 0; i < nsrc; i++) check_src(g, srcs[i]);
     bc_run(g, srcs, nsrc, k_scale, skip_degree0, normalize, endpoints, bc, &g->bc_last);
+    LGMI_API_END
+}
+
+int lgmi_betweenness_set_launch(int mode) {
+    LGMI_API_BEGIN
+    if (mode != 0 && mode != 1) throw Error(LGMI_E_ARG, "launch mode must be 0 or 1");
+    bc_set_launch(mode);
     LGMI_API_END
 }
 

@@ -140,6 +140,7 @@ struct BfsWork {
 };
 
 struct BcWork;
+struct BcGraph;
 // Rows longer than kHeavyRow are split into work items of kHeavyRow
 // entries so that a hub vertex is expanded by many wavefronts at once.
 constexpr int32_t kHeavyRow = 256;
@@ -180,6 +181,7 @@ struct lgmi_graph {
     std::mutex mu;
     lgmi::BfsWork* bfs = nullptr;
     lgmi::BcWork* bc = nullptr;
+    lgmi::BcGraph* bcg = nullptr;   // betweenness: host row pointers, heavy-row segments
     int cu_count = 0;
     int bfs_forced = -1;           // -1 policy, 0/1 forced direction (label runs)
     lgmi_bc_stats bc_last{};       // statistics of the last betweenness call
@@ -197,6 +199,8 @@ void policy_upload(lgmi_graph* g);
 void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, int skip_degree0,
             int normalize, int endpoints, double* bc, lgmi_bc_stats* stats);
 void bc_free(BcWork* w);
+void bc_graph_free(BcGraph* b);
+void bc_set_launch(int mode);
 // global policy state (host copy), set by lgmi_bfs_set_policy
 struct PolicyHost {
     int mode = LGMI_POLICY_HEURISTIC;

@@ -110,6 +110,13 @@ def batch_tuning(wave_max_n=-1, task_work_target=0.0, lds_max_n=-1, lane_group=0
                                                          int(lane_group), int(slots)))
 
 
+def set_launch(mode: int) -> None:
+    """0: all lane groups in one persistent (graph-captured) launch when they
+    fit, level-per-launch otherwise (default); 1: level-per-launch only.
+    Both give bit-identical results."""
+    _lib.check(_lib.load().lgmi_betweenness_set_launch(int(mode)))
+
+
 def last_stats(g: _GraphBase) -> dict:
     st = _lib.BcStats()
     _lib.check(_lib.load().lgmi_betweenness_stats(g.handle(), _lib.ctypes.byref(st)))

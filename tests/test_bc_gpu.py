@@ -185,3 +185,100 @@ def test_rmat16_sources_vs_oracle(oracle):
     got = lgmi.betweenness_centrality(g, vs0 + 1)
     want = oracle.betweenness_threaded(rp, col, rp, False, vs0, nthreads=16, index_quirk=False)
     assert close(got, want)
+
+
+def _bits(x):
+    return np.ascontiguousarray(x, dtype=np.float64).view(np.uint64)
+
+
+def _directed_hubs():
+    """A directed graph whose out-row of vertex 0 and in-row of vertex 4999
+    are longer than the heavy-row threshold (rows cut into segments)."""
+    import lgmi
+    rng = np.random.default_rng(15)
+    n = 5000
+    src = np.concatenate([rng.integers(0, n, 20000), np.zeros(3000, np.int64), np.arange(1, 3001)])
+    dst = np.concatenate([rng.integers(0, n, 20000), np.arange(1, 3001), np.full(3000, n - 1)])
+    return lgmi.SimpleDiGraph.from_edges(n, src, dst, drop_self_loops=True)
+
+
+def test_heavy_rows_directed_vs_oracle(oracle):
+    """Heavy in-rows (forward) and out-rows (backward) of a directed graph,
+    both launch paths, against the oracle."""
+    import lgmi
+    from lgmi import centrality as C
+    g = _directed_hubs()
+    rp, col = g.csr()
+    irp, _ = g.in_csr()
+    assert np.diff(rp).max() > 1024 and np.diff(irp).max() > 1024
+    vs0 = np.concatenate([[0, 1, 2, 4999], np.random.default_rng(16).integers(0, 5000, 300)])
+    try:
+        for mode in (1, 0):
+            C.set_launch(mode)
+            say(f"directed hubs launch mode {mode}")
+            got = lgmi.betweenness_centrality(g, vs0 + 1, normalize=False, endpoints=True)
+            want = oracle.betweenness(rp, col, irp, True, vs0, normalize=False, endpoints=True)
+            assert close(got, want)
+    finally:
+        C.set_launch(0)
+
+
+def test_persistent_launch_matches_level_launches(oracle):
+    """One persistent launch (graph-captured or cooperative) against one
+    launch per level: bit-identical results, on undirected and directed
+    graphs, with and without endpoints, and a replay with other sources."""
+    import lgmi
+    from lgmi import centrality as C
+    cases = [
+        ("rmat14", lgmi.generators.rmat(14, 8, seed=11), 300),
+        ("rmat12-directed", lgmi.generators.rmat(12, 8, seed=12, directed=True), 130),
+        ("ba2000", lgmi.generators.barabasi_albert(2000, 3, seed=13), 2000),
+        ("directed-hubs", _directed_hubs(), 200),
+    ]
+    try:
+        for name, g, k in cases:
+            rng = np.random.default_rng(14)
+            for rep in range(2):
+                vs = rng.choice(g.nv(), size=min(k, g.nv()), replace=False) + 1
+                for endpoints in (False, True):
+                    say(f"persistent {name} rep={rep} endpoints={endpoints}")
+                    C.set_launch(1)
+                    ref = lgmi.betweenness_centrality(g, vs, normalize=False, endpoints=endpoints)
+                    assert C.last_stats(g)["launch"] == 0
+                    C.set_launch(0)
+                    got = lgmi.betweenness_centrality(g, vs, normalize=False, endpoints=endpoints)
+                    st = C.last_stats(g)
+                    assert st["launch"] in (1, 2), st
+                    assert (_bits(got) == _bits(ref)).all()
+        # a replay of the same call comes from the captured graph when the
+        # runtime captured it, and gives the same bits
+        g = cases[0][1]
+        vs = np.arange(1, 200)
+        a = lgmi.betweenness_centrality(g, vs)
+        b = lgmi.betweenness_centrality(g, vs)
+        assert (_bits(a) == _bits(b)).all()
+        rp, col = g.csr()
+        assert close(a, oracle.betweenness(rp, col, rp, False, vs - 1))
+    finally:
+        C.set_launch(0)
+
+
+def test_persistent_level_overflow_falls_back(oracle):
+    """A 60x60 grid needs more levels than the first allocation holds: the
+    persistent launch stops, the level-per-launch path (which grows the
+    level storage) gives the result, and the next call fits in one launch."""
+    import lgmi
+    from lgmi import centrality as C
+    C.set_launch(0)
+    g = lgmi.generators.grid([60, 60])
+    vs = np.arange(1, 3601, 37)
+    say("grid 60x60 first call")
+    a = lgmi.betweenness_centrality(g, vs)
+    assert C.last_stats(g)["launch"] == 0
+    say("grid 60x60 second call")
+    b = lgmi.betweenness_centrality(g, vs)
+    assert C.last_stats(g)["launch"] in (1, 2)
+    assert (_bits(a) == _bits(b)).all()
+    rp, col = g.csr()
+    assert close(a, oracle.betweenness(rp, col, rp, False, vs - 1))
+

@@ -94,6 +94,15 @@ def test_batch_tuning_argument_checks():
     assert lib.lgmi_betweenness_batch_tuning(-1, 0.0, -1, 0, 0) == _lib.LGMI_OK
 
 
+def test_bc_launch_mode_argument_checks():
+    from lgmi import _lib
+    lib = _lib.load()
+    assert lib.lgmi_betweenness_set_launch(2) == _lib.LGMI_E_ARG
+    assert lib.lgmi_betweenness_set_launch(-1) == _lib.LGMI_E_ARG
+    assert lib.lgmi_betweenness_set_launch(1) == _lib.LGMI_OK
+    assert lib.lgmi_betweenness_set_launch(0) == _lib.LGMI_OK
+
+
 def test_generators_shapes_and_determinism():
     import lgmi.generators as gen
     s1, d1 = gen.kronecker_edges(10, 16, seed=3)
