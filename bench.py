@@ -224,6 +224,21 @@ def bc_bench(args, lgmi, _lib, orc):
     teps = 1024 * m / wall
     log(f"BC 1024 sources ({launch}): {wall * 1e3:.1f} ms wall (mean of {reps}), device {stt['device_ms']:.1f} ms "
         f"(fwd {stt['forward_ms']:.1f}, bwd {stt['backward_ms']:.1f}), {teps / 1e9:.2f} GTEPS_BC")
+    # algorithmic bytes per source: 8 B sigma read (forward) + 8 B coefficient
+    # read (backward) per shortest-path DAG edge, 24 B per reached vertex
+    # (sigma write, sigma read + coefficient write); DAG edges counted exactly
+    # on a sample of the sources with the oracle's BFS distances
+    src_i = np.repeat(np.arange(nv, dtype=np.int64), np.diff(rp))
+    per_src = []
+    for s0 in srcs[:16]:
+        d = orc.gdistances(rp, col, [int(s0)]).astype(np.int64)
+        du, dv = d[src_i], d[col]
+        e_dag = int(np.count_nonzero((du >= 0) & (dv == du + 1)))
+        per_src.append(16 * e_dag + 24 * int(np.count_nonzero(d >= 0)))
+    bytes_alg = 1024 * float(np.mean(per_src))
+    bc_gbs = bytes_alg / (stt["device_ms"] * 1e-3) / 1e9
+    log(f"BC algorithmic bytes {bytes_alg / 1e9:.1f} GB (DAG-edge model, 16-source sample) -> {bc_gbs:.0f} GB/s "
+        f"= {bc_gbs / HBM_PEAK_GBS:.3f} of HBM peak")
     # CPU baseline: threaded oracle on a sample of the sources, scaled
     nthreads = min(16, os.cpu_count() or 1)
     ns = args.cpu_bc_sources
@@ -247,6 +262,11 @@ def bc_bench(args, lgmi, _lib, orc):
         "forward_ms": round(stt["forward_ms"], 2),
         "backward_ms": round(stt["backward_ms"], 2),
         "launch": launch,
+        "roofline": {"bound": "hbm", "kernel": "bc_persist" if stats[-1]["launch"] else "bc_forward/bc_backward",
+                     "achieved": round(bc_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(bc_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "bytes_model": "per source 16 B per shortest-path DAG edge + 24 B per reached vertex, "
+                                    "DAG edges from a 16-source sample"},
         "sample_max_rel_err": err,
         "cpu_baseline": {"value": round(cpu_teps / 1e9, 5), "unit": "GTEPS_BC", "cores": nthreads,
                          "kind": "port",
@@ -348,6 +368,7 @@ def main():
     ap.add_argument("--cpu-roots", type=int, default=8)
     ap.add_argument("--cpu-bc-sources", type=int, default=128)
     ap.add_argument("--no-bc", action="store_true")
+    ap.add_argument("--bc-only", action="store_true", help="run only the RMAT-20 betweenness workload")
     ap.add_argument("--config4", action="store_true",
                     help="run only the uneven small-graph batch (betweenness) workload")
     ap.add_argument("--config4-graphs", type=int, default=4096)
@@ -369,6 +390,9 @@ def main():
         return
     if args.config4:
         print(json.dumps(config4_bench(args, lgmi, orc)), flush=True)
+        return
+    if args.bc_only:
+        print(json.dumps(bc_bench(args, lgmi, _lib, orc)), flush=True)
         return
     result = bfs_bench(args, lgmi, _lib, orc)
     if not args.no_bc:

@@ -200,7 +200,19 @@ int lgmi_bfs_set_policy(int mode, float alpha, float beta, const float* mlp, int
  * summation order differs). */
 int lgmi_betweenness(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int64_t k_scale,
                      int skip_degree0, int normalize, int endpoints, double* bc);
-/* Statistics of the last betweenness run on g. */
+/* Replaces, for unweighted graphs:
+ *   stress_centrality(g, vs)          src/centrality/stress.jl:33-48 (with
+ *                                     _stress_accumulate_basic!, :53-77)
+ *   stress_centrality(g, k)           stress.jl:50-51 (the shim draws the
+ *                                     sample and passes it as srcs)
+ *   Parallel.stress_centrality(...; parallel = :distributed | :threads)
+ *                                     src/Parallel/centrality/stress.jl:1-51
+ * Sources of degree(g, s) == 0 are skipped (stress.jl:41); duplicates count
+ * twice.  stress[v] (nv entries) is the reference's Int64 sum, wrapping on
+ * overflow as Julia's Int does; exact. */
+int lgmi_stress(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int64_t* stress);
+
+/* Statistics of the last betweenness (or stress) run on g. */
 int lgmi_betweenness_stats(lgmi_graph_t g, lgmi_bc_stats* stats);
 
 /* How lgmi_betweenness launches its work (process-wide; no counterpart in

@@ -156,14 +156,23 @@ struct BcArgs {
     int32_t seg_stride;
 };
 
+// Two computations share these kernels: MODE 0 is betweenness (FP64 sigma
+// and delta), MODE 1 is stress centrality (src/centrality/stress.jl:33-77),
+// whose forward pass counts each vertex's predecessors per lane and whose
+// backward pass sums Int64 path counts (wrapping like Julia's Int).
+template <int MODE> struct BwdAcc { using T = double; };
+template <> struct BwdAcc<1> { using T = uint64_t; };
+
 // Up to eight of the hits in `hits` (lanes whose adjacency entry uk[t] has
-// a non-empty lane mask mk[t]): lane i adds to acc the FP64 rows' entry i of
-// the hits whose mask has bit i, one after another in ascending hit order
-// (the order of the row), with the eight loads in flight together.  Removes
-// the hits it took and ORs their masks into `got`.  Every lane calls it with
-// the same `hits`.
-__device__ inline void gather_hits(uint64_t& hits, int32_t uk, uint64_t mk, const double* sig, int lane,
-                                   double& acc, uint64_t& got) {
+// a non-empty lane mask mk[t]): lane i adds to acc entry i of the 64-entry
+// rows (of type T) of the hits whose mask has bit i -- or 1 per such hit
+// when LOAD is false -- one after another in ascending hit order (the order
+// of the row), with the eight loads in flight together.  Removes the hits it
+// took and ORs their masks into `got`.  Every lane calls it with the same
+// `hits`.
+template <typename T, bool LOAD>
+__device__ inline void gather_hits(uint64_t& hits, int32_t uk, uint64_t mk, const T* sig, int lane,
+                                   T& acc, uint64_t& got) {
     constexpr int kH = 8;
     int32_t uu[kH];
     uint64_t mm[kH];
@@ -180,9 +189,12 @@ __device__ inline void gather_hits(uint64_t& hits, int32_t uk, uint64_t mk, cons
                     (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)mk, t);
         }
     }
-    double x[kH];
+    T x[kH];
     #pragma unroll
-    for (int q = 0; q < kH; q++) x[q] = ((mm[q] >> lane) & 1) ? sig[(size_t)uu[q] * 64 + lane] : 0.0;
+    for (int q = 0; q < kH; q++) {
+        if constexpr (LOAD) x[q] = ((mm[q] >> lane) & 1) ? sig[(size_t)uu[q] * 64 + lane] : T(0);
+        else x[q] = ((mm[q] >> lane) & 1) ? T(1) : T(0);
+    }
     #pragma unroll
     for (int q = 0; q < kH; q++) {
         acc += x[q];
@@ -234,6 +246,7 @@ __global__ void bc_seed(BcArgs a, const int32_t* lane_src) {
 // wavefront): every vertex with an unseen lane ORs its in-neighbours'
 // frontier masks and adds their sigma per lane.  Returns, in every lane,
 // whether any vertex of the chunk got a new lane.
+template <int MODE>
 __device__ bool fwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int lane) {
     const int32_t nv = a.nv;
     const uint64_t act = a.active[slot];
@@ -271,7 +284,7 @@ __device__ bool fwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
                 mk = FL[uk] & unseen;
             }
             uint64_t hits = __ballot(mk != 0);
-            while (hits) gather_hits(hits, uk, mk, sig, lane, acc, got);
+            while (hits) gather_hits<double, MODE == 0>(hits, uk, mk, sig, lane, acc, got);
         }
         if ((got >> lane) & 1) sig[(size_t)v * 64 + lane] = acc;
         if (lane == j) newF = got;
@@ -288,6 +301,7 @@ __device__ bool fwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
 // the lanes they reach, go to the segment's partial.  fwd_combine finishes
 // the row once every segment is done.  Returns, in every lane, whether the
 // segment reached a new lane.
+template <int MODE>
 __device__ bool fwd_seg(const BcArgs& a, int slot, int32_t sg, int32_t L, int lane) {
     const int32_t nv = a.nv;
     const int32_t v = a.hin.seg_v[sg];
@@ -308,7 +322,7 @@ __device__ bool fwd_seg(const BcArgs& a, int slot, int32_t sg, int32_t L, int la
                 mk = FL[uk] & unseen;
             }
             uint64_t hits = __ballot(mk != 0);
-            while (hits) gather_hits(hits, uk, mk, sig, lane, acc, got);
+            while (hits) gather_hits<double, MODE == 0>(hits, uk, mk, sig, lane, acc, got);
         }
     }
     a.hpart[((size_t)slot * a.seg_stride + sg) * 64 + lane] = acc;
@@ -346,6 +360,7 @@ __device__ void fwd_combine(const BcArgs& a, int slot, int32_t h, int32_t L, int
 
 // Forward level step, part 1: every light vertex of every chunk and every
 // heavy-row segment; sets the slot's found flag when anything new is reached.
+template <int MODE>
 __global__ __launch_bounds__(kBlock) void bc_forward(BcArgs a) {
     const int slot = blockIdx.y;
     BcCtrl* c = &a.ctrl[slot];
@@ -366,7 +381,7 @@ __global__ __launch_bounds__(kBlock) void bc_forward(BcArgs a) {
     bool any = false;
     const int32_t nseg = a.hin.nseg;
     for (int32_t i = blockIdx.x * kWaves + wid; i < nseg + a.nchunks; i += gridDim.x * kWaves)
-        any |= i < nseg ? fwd_seg(a, slot, i, L, lane) : fwd_chunk(a, slot, i - nseg, L, lane);
+        any |= i < nseg ? fwd_seg<MODE>(a, slot, i, L, lane) : fwd_chunk<MODE>(a, slot, i - nseg, L, lane);
     if (any && lane == 0) s_found = 1;
     __syncthreads();
     if (tid == 0 && s_found) atomicOr(&c->found, 1);
@@ -416,12 +431,39 @@ __global__ __launch_bounds__(kBlock) void bc_forward_fin(BcArgs a) {
 // its successors' coefficients, the vertex's coefficient (1 + delta) / sigma
 // into its sigma slot, and the lane sum of delta into the slot's partial
 // betweenness.
+// One lane of a vertex on level L once its successor sum acc is complete:
+// betweenness turns sigma into the coefficient (1 + delta) / sigma and
+// returns delta = sigma * acc; stress (_stress_accumulate_basic!,
+// stress.jl:55-77) returns acc times the predecessor count the forward pass
+// left in the slot, and leaves acc + 1 there for the vertex's predecessors.
+template <int MODE>
+__device__ inline typename BwdAcc<MODE>::T bwd_finish_lane(double* slotp, typename BwdAcc<MODE>::T acc) {
+    if constexpr (MODE == 0) {
+        const double sv = *slotp;
+        const double d = sv * acc;
+        *slotp = (1.0 + d) / sv;
+        return d;
+    } else {
+        const uint64_t npred = (uint64_t)*slotp;
+        *reinterpret_cast<uint64_t*>(slotp) = acc + 1;
+        return acc * npred;
+    }
+}
+
+template <typename T>
+__device__ inline T lane_sum(T x) {
+    for (int o = 32; o >= 1; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+template <int MODE>
 __device__ void bwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int lane) {
+    using T = typename BwdAcc<MODE>::T;
     const int32_t nv = a.nv;
     const uint64_t* FL = a.F + ((size_t)slot * a.cap + L) * nv;
     const uint64_t* Fn = a.F + ((size_t)slot * a.cap + L + 1) * nv;
     double* sig = a.sig + (size_t)slot * nv * 64;
-    double* acc_bc = a.bc_acc + (size_t)slot * nv;
+    T* acc_bc = reinterpret_cast<T*>(a.bc_acc) + (size_t)slot * nv;
     const int32_t vj = ch * 64 + lane;
     uint64_t Aj = 0;
     int32_t bj = 0, ej = 0;
@@ -433,7 +475,7 @@ __device__ void bwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
             if (ej - bj > kBcHeavyRow) Aj = 0;   // a heavy row: its segments do it
         }
     }
-    double bcj = 0.0;
+    T bcj = 0;
     uint64_t am = __ballot(Aj != 0);
     while (am) {
         const int j = __ffsll((unsigned long long)am) - 1;
@@ -441,7 +483,7 @@ __device__ void bwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
         const int32_t v = ch * 64 + j;
         const uint64_t A = __shfl(Aj, j);
         const int32_t b = __shfl(bj, j), e = __shfl(ej, j);
-        double acc = 0.0;
+        T acc = 0;
         for (int32_t off = b; off < e; off += 64) {
             const int32_t kk = off + lane;
             uint64_t mk = 0;
@@ -452,16 +494,11 @@ __device__ void bwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
             }
             uint64_t hits = __ballot(mk != 0);
             uint64_t unused = 0;
-            while (hits) gather_hits(hits, wk, mk, sig, lane, acc, unused);
+            while (hits) gather_hits<T, true>(hits, wk, mk, reinterpret_cast<const T*>(sig), lane, acc, unused);
         }
-        double contrib = 0.0;
-        if ((A >> lane) & 1) {
-            const double s = sig[(size_t)v * 64 + lane];
-            const double d = s * acc;
-            sig[(size_t)v * 64 + lane] = (1.0 + d) / s;
-            contrib = d;
-        }
-        for (int o = 32; o >= 1; o >>= 1) contrib += __shfl_xor(contrib, o);
+        T contrib = 0;
+        if ((A >> lane) & 1) contrib = bwd_finish_lane<MODE>(&sig[(size_t)v * 64 + lane], acc);
+        contrib = lane_sum(contrib);
         if (lane == j) bcj = contrib;
     }
     if (Aj) acc_bc[vj] += bcj;
@@ -470,15 +507,17 @@ __device__ void bwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
 // Backward step of one slot at level L over heavy-row segment sg: the lane
 // sums of the successors' coefficients over the segment go to the segment's
 // partial (nothing when the row's vertex has no lane on level L).
+template <int MODE>
 __device__ void bwd_seg(const BcArgs& a, int slot, int32_t sg, int32_t L, int lane) {
+    using T = typename BwdAcc<MODE>::T;
     const int32_t nv = a.nv;
     const int32_t v = a.hout.seg_v[sg];
     const uint64_t A = a.F[((size_t)slot * a.cap + L) * nv + v];
     if (!A) return;
     const uint64_t* Fn = a.F + ((size_t)slot * a.cap + L + 1) * nv;
-    const double* sig = a.sig + (size_t)slot * nv * 64;
+    const T* sig = reinterpret_cast<const T*>(a.sig) + (size_t)slot * nv * 64;
     const int32_t b = a.hout.seg_b[sg], e = a.hout.seg_e[sg];
-    double acc = 0.0;
+    T acc = 0;
     for (int32_t off = b; off < e; off += 64) {
         const int32_t kk = off + lane;
         uint64_t mk = 0;
@@ -489,34 +528,32 @@ __device__ void bwd_seg(const BcArgs& a, int slot, int32_t sg, int32_t L, int la
         }
         uint64_t hits = __ballot(mk != 0);
         uint64_t unused = 0;
-        while (hits) gather_hits(hits, wk, mk, sig, lane, acc, unused);
+        while (hits) gather_hits<T, true>(hits, wk, mk, sig, lane, acc, unused);
     }
-    a.hpart[((size_t)slot * a.seg_stride + sg) * 64 + lane] = acc;
+    reinterpret_cast<T*>(a.hpart)[((size_t)slot * a.seg_stride + sg) * 64 + lane] = acc;
 }
 
 // Finishes heavy out-row h of one slot at level L after its segments: the
 // partials in segment order give delta, then as bwd_chunk.
+template <int MODE>
 __device__ void bwd_combine(const BcArgs& a, int slot, int32_t h, int32_t L, int lane) {
+    using T = typename BwdAcc<MODE>::T;
     const int32_t nv = a.nv;
     const int32_t f0 = a.hout.h_first[h], nsg = a.hout.h_nseg[h];
     const int32_t v = a.hout.seg_v[f0];
     const uint64_t A = a.F[((size_t)slot * a.cap + L) * nv + v];
     if (!A) return;
-    const double* part = a.hpart + ((size_t)slot * a.seg_stride + f0) * 64;
-    double tot = 0.0;
+    const T* part = reinterpret_cast<const T*>(a.hpart) + ((size_t)slot * a.seg_stride + f0) * 64;
+    T tot = 0;
     for (int32_t q = 0; q < nsg; q++) tot += part[(size_t)q * 64 + lane];
     double* sig = a.sig + (size_t)slot * nv * 64;
-    double contrib = 0.0;
-    if ((A >> lane) & 1) {
-        const double sv = sig[(size_t)v * 64 + lane];
-        const double d = sv * tot;
-        sig[(size_t)v * 64 + lane] = (1.0 + d) / sv;
-        contrib = d;
-    }
-    for (int o = 32; o >= 1; o >>= 1) contrib += __shfl_xor(contrib, o);
-    if (lane == 0) a.bc_acc[(size_t)slot * nv + v] += contrib;
+    T contrib = 0;
+    if ((A >> lane) & 1) contrib = bwd_finish_lane<MODE>(&sig[(size_t)v * 64 + lane], tot);
+    contrib = lane_sum(contrib);
+    if (lane == 0) reinterpret_cast<T*>(a.bc_acc)[(size_t)slot * nv + v] += contrib;
 }
 
+template <int MODE>
 __global__ __launch_bounds__(kBlock) void bc_backward(BcArgs a, int32_t k) {
     const int slot = blockIdx.y;
     const int32_t L = a.ctrl[slot].maxlevel - k;
@@ -524,18 +561,19 @@ __global__ __launch_bounds__(kBlock) void bc_backward(BcArgs a, int32_t k) {
     const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int32_t nseg = a.hout.nseg;
     for (int32_t i = blockIdx.x * kWaves + wid; i < nseg + a.nchunks; i += gridDim.x * kWaves) {
-        if (i < nseg) bwd_seg(a, slot, i, L, lane);
-        else bwd_chunk(a, slot, i - nseg, L, lane);
+        if (i < nseg) bwd_seg<MODE>(a, slot, i, L, lane);
+        else bwd_chunk<MODE>(a, slot, i - nseg, L, lane);
     }
 }
 
 // Backward level step, part 2: finishes the heavy rows of level maxlevel - k.
+template <int MODE>
 __global__ __launch_bounds__(kBlock) void bc_backward_fin(BcArgs a, int32_t k) {
     const int slot = blockIdx.y;
     const int32_t L = a.ctrl[slot].maxlevel - k;
     if (L < 1) return;
     const int lane = threadIdx.x & 63, wid = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    for (int32_t h = blockIdx.x * kWaves + wid; h < a.hout.nh; h += gridDim.x * kWaves) bwd_combine(a, slot, h, L, lane);
+    for (int32_t h = blockIdx.x * kWaves + wid; h < a.hout.nh; h += gridDim.x * kWaves) bwd_combine<MODE>(a, slot, h, L, lane);
 }
 
 constexpr int kMaxSlotsP = 1024;        // slots of one persistent launch
@@ -626,6 +664,7 @@ __device__ inline uint64_t wave_or64(uint64_t m) {
 // learns that from the per-block found masks after the barrier, so all
 // blocks run the same number of barriers.  With heavy rows every level step
 // has a second barrier, after the rows' partial sums are combined.
+template <int MODE>
 __global__ __launch_bounds__(kBlock) void bc_persist(BcArgs a, PersistArgs p) {
     __shared__ uint64_t s_found[kMaxWordsP];
     __shared__ uint64_t s_live[kMaxWordsP];
@@ -748,7 +787,7 @@ __global__ __launch_bounds__(kBlock) void bc_persist(BcArgs a, PersistArgs p) {
             const int sl = (int)((i + r) % (uint32_t)ns);
             const int32_t q = (int32_t)(i / (uint32_t)ns);
             if (!((s_live[sl >> 6] >> (sl & 63)) & 1)) continue;
-            const bool got = q < a.hin.nseg ? fwd_seg(a, sl, q, L, lane) : fwd_chunk(a, sl, q - a.hin.nseg, L, lane);
+            const bool got = q < a.hin.nseg ? fwd_seg<MODE>(a, sl, q, L, lane) : fwd_chunk<MODE>(a, sl, q - a.hin.nseg, L, lane);
             if (got && lane == 0) atomicOr((unsigned long long*)&s_found[sl >> 6], 1ull << (sl & 63));
         }
         __syncthreads();
@@ -802,8 +841,8 @@ __global__ __launch_bounds__(kBlock) void bc_persist(BcArgs a, PersistArgs p) {
             const int32_t q = (int32_t)(i / (uint32_t)ns);
             const int32_t Ls = s_maxl[sl] - k;
             if (Ls < 1) continue;
-            if (q < a.hout.nseg) bwd_seg(a, sl, q, Ls, lane);
-            else bwd_chunk(a, sl, q - a.hout.nseg, Ls, lane);
+            if (q < a.hout.nseg) bwd_seg<MODE>(a, sl, q, Ls, lane);
+            else bwd_chunk<MODE>(a, sl, q - a.hout.nseg, Ls, lane);
         }
         if (!grid_sync(c, p.rel, vb, G, gen, p.timeout, &s_ok)) return;
         nbar++;
@@ -811,7 +850,7 @@ __global__ __launch_bounds__(kBlock) void bc_persist(BcArgs a, PersistArgs p) {
             for (uint32_t i = gw, r = 0; i < nrows_b; i += nw, r++) {
                 const int sl = (int)((i + r) % (uint32_t)ns);
                 const int32_t Ls = s_maxl[sl] - k;
-                if (Ls >= 1) bwd_combine(a, sl, (int32_t)(i / (uint32_t)ns), Ls, lane);
+                if (Ls >= 1) bwd_combine<MODE>(a, sl, (int32_t)(i / (uint32_t)ns), Ls, lane);
             }
             if (!grid_sync(c, p.rel, vb, G, gen, p.timeout, &s_ok)) return;
             nbar++;
@@ -822,11 +861,17 @@ __global__ __launch_bounds__(kBlock) void bc_persist(BcArgs a, PersistArgs p) {
         c->barriers = nbar;
     }
     for (int64_t v = (int64_t)vb * kBlock + tid; v < nv; v += (int64_t)G * kBlock) {
-        double sum = 0.0;
-        for (int k = 0; k < ns; k++) sum += a.bc_acc[(size_t)k * nv + v];
-        if (p.endp) sum += p.endp[v];
-        if (p.do_scale) sum *= p.scale;
-        p.out[v] = sum;
+        if constexpr (MODE == 0) {
+            double sum = 0.0;
+            for (int k = 0; k < ns; k++) sum += a.bc_acc[(size_t)k * nv + v];
+            if (p.endp) sum += p.endp[v];
+            if (p.do_scale) sum *= p.scale;
+            p.out[v] = sum;
+        } else {
+            uint64_t sum = 0;
+            for (int k = 0; k < ns; k++) sum += reinterpret_cast<const uint64_t*>(a.bc_acc)[(size_t)k * nv + v];
+            reinterpret_cast<uint64_t*>(p.out)[v] = sum;
+        }
     }
 }
 
@@ -843,14 +888,21 @@ __global__ void bc_persist_prologue(BcPersistCtl* c) {
 }
 
 // bc[v] = (sum over slots of bc_acc, in slot order, + endpoints term) * scale
+template <int MODE>
 __global__ void bc_finish(const double* acc, int32_t nslots, int32_t nv, const double* endp, int do_scale,
                           double scale, double* out) {
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < nv; v += (int64_t)gridDim.x * blockDim.x) {
-        double s = 0.0;
-        for (int k = 0; k < nslots; k++) s += acc[(size_t)k * nv + v];
-        if (endp) s += endp[v];
-        if (do_scale) s *= scale;
-        out[v] = s;
+        if constexpr (MODE == 0) {
+            double s = 0.0;
+            for (int k = 0; k < nslots; k++) s += acc[(size_t)k * nv + v];
+            if (endp) s += endp[v];
+            if (do_scale) s *= scale;
+            out[v] = s;
+        } else {
+            uint64_t s = 0;
+            for (int k = 0; k < nslots; k++) s += reinterpret_cast<const uint64_t*>(acc)[(size_t)k * nv + v];
+            reinterpret_cast<uint64_t*>(out)[v] = s;
+        }
     }
 }
 
@@ -1015,7 +1067,8 @@ void bc_set_launch(int mode) { g_launch_mode = mode; }
 // slot ran past the level capacity); the caller then runs the level-per-launch
 // path, which overwrites everything this one wrote.
 static bool bc_persistent(lgmi_graph* g, BcArgs a, int32_t ns, const double* endp, int do_scale, double scale,
-                          int32_t* launch_kind, float* fwd_ms, float* bwd_ms, int32_t* max_levels) {
+                          int mode, int32_t* launch_kind, float* fwd_ms, float* bwd_ms, int32_t* max_levels) {
+    const void* kfn = mode ? reinterpret_cast<const void*>(bc_persist<1>) : reinterpret_cast<const void*>(bc_persist<0>);
     BcWork* w = g->bc;
     hipStream_t st = g->stream;
     int dev = 0;
@@ -1024,7 +1077,7 @@ static bool bc_persistent(lgmi_graph* g, BcArgs a, int32_t ns, const double* end
     LGMI_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
     LGMI_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
     if (!coop || khz <= 0) return false;
-    LGMI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(bc_persist), kBlock, 0));
+    LGMI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kBlock, 0));
     const int cu = std::max(1, g->cu_count);
     // the blocks that the occupancy calculation says fit; the kernel itself
     // keeps only those that start together
@@ -1045,10 +1098,11 @@ static bool bc_persistent(lgmi_graph* g, BcArgs a, int32_t ns, const double* end
     p.do_scale = do_scale;
     void* args[] = {&a, &p};
     const dim3 gd(grid), bd(kBlock);
-    std::vector<unsigned char> key(sizeof(a) + sizeof(p) + sizeof(int));
+    std::vector<unsigned char> key(sizeof(a) + sizeof(p) + 2 * sizeof(int));
     std::memcpy(key.data(), &a, sizeof(a));
     std::memcpy(key.data() + sizeof(a), &p, sizeof(p));
     std::memcpy(key.data() + sizeof(a) + sizeof(p), &grid, sizeof(int));
+    std::memcpy(key.data() + sizeof(a) + sizeof(p) + sizeof(int), &mode, sizeof(int));
     if (w->pexec && w->pkey != key) {
         (void)hipGraphExecDestroy(w->pexec);
         w->pexec = nullptr;
@@ -1060,7 +1114,7 @@ static bool bc_persistent(lgmi_graph* g, BcArgs a, int32_t ns, const double* end
             bc_persist_prologue<<<1, 1, 0, st>>>(w->pctl);
             hipError_t el = hipGetLastError();
             if (el == hipSuccess)
-                el = hipLaunchCooperativeKernel(reinterpret_cast<const void*>(bc_persist), gd, bd, args, 0, st);
+                el = hipLaunchCooperativeKernel(kfn, gd, bd, args, 0, st);
             e = hipStreamEndCapture(st, &graph);
             if (el != hipSuccess) e = el;
         }
@@ -1081,7 +1135,7 @@ static bool bc_persistent(lgmi_graph* g, BcArgs a, int32_t ns, const double* end
     } else {
         bc_persist_prologue<<<1, 1, 0, st>>>(w->pctl);
         LGMI_HIP(hipGetLastError());
-        LGMI_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(bc_persist), gd, bd, args, 0, st));
+        LGMI_HIP(hipLaunchCooperativeKernel(kfn, gd, bd, args, 0, st));
         *launch_kind = 1;
     }
     LGMI_HIP(hipEventRecord(w->ev[2], st));
@@ -1104,7 +1158,7 @@ static bool bc_persistent(lgmi_graph* g, BcArgs a, int32_t ns, const double* end
 }
 
 void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, int skip_degree0, int normalize,
-            int endpoints, double* bc, lgmi_bc_stats* stats) {
+            int endpoints, double* bc, lgmi_bc_stats* stats, int mode) {
     const int32_t nv = g->nv;
     hipStream_t st = g->stream;
     // degree(g, s) > 0 filter (betweenness.jl:58) on host copies of the row pointers
@@ -1157,7 +1211,7 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
     // endpoints: every processed source s adds length(S)-1 = nv-1 to bc[s]
     // and 1 to every other vertex (betweenness.jl:122,130)
     const double* endp = nullptr;
-    if (endpoints && nact > 0) {
+    if (mode == 0 && endpoints && nact > 0) {
         std::vector<double> e((size_t)nv, (double)nact);
         std::vector<int64_t> cnt((size_t)nv, 0);
         for (int32_t s : act) cnt[(size_t)s]++;
@@ -1177,6 +1231,7 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
         scale = 1.0 / 2.0;
     }
     if (do_scale && k_scale > 0) scale = scale * (double)nv / (double)k_scale;
+    if (mode != 0) do_scale = 0;   // stress: plain Int64 sums
 
     LGMI_HIP(hipEventRecord(w->ev[0], st));
     std::vector<int32_t> lane_src((size_t)nslots * 64);
@@ -1201,7 +1256,7 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
     const int64_t max_items = (int64_t)ngroups * ((int64_t)seg_stride + a.nchunks);
     if (g_launch_mode == 0 && ngroups >= 1 && ngroups <= nslots && nv > 0 && max_items < ((int64_t)1 << 31)) {
         fill_groups(0, ngroups);
-        if (!bc_persistent(g, a, ngroups, endp, do_scale, scale, &launch_kind, &fwd_ms, &bwd_ms, &max_levels)) {
+        if (!bc_persistent(g, a, ngroups, endp, do_scale, scale, mode, &launch_kind, &fwd_ms, &bwd_ms, &max_levels)) {
             launch_kind = 0;
             fwd_ms = bwd_ms = 0.f;
             max_levels = 0;
@@ -1230,7 +1285,8 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
                     a.F = w->F; a.cap = w->cap_levels;
                 }
                 for (int i = 0; i < batch; i++) {
-                    bc_forward<<<dim3(bx, ns), kBlock, 0, st>>>(a);
+                    if (mode) bc_forward<1><<<dim3(bx, ns), kBlock, 0, st>>>(a);
+                    else bc_forward<0><<<dim3(bx, ns), kBlock, 0, st>>>(a);
                     bc_forward_fin<<<dim3(hbx_in, ns), kBlock, 0, st>>>(a);
                 }
                 LGMI_HIP(hipGetLastError());
@@ -1252,8 +1308,12 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
             max_levels = std::max(max_levels, lmax);
             LGMI_HIP(hipEventRecord(w->ev[2], st));
             for (int32_t k = 0; k < lmax; k++) {
-                bc_backward<<<dim3(bx, ns), kBlock, 0, st>>>(a, k);
-                if (a.hout.nh > 0) bc_backward_fin<<<dim3(hbx_out, ns), kBlock, 0, st>>>(a, k);
+                if (mode) bc_backward<1><<<dim3(bx, ns), kBlock, 0, st>>>(a, k);
+                else bc_backward<0><<<dim3(bx, ns), kBlock, 0, st>>>(a, k);
+                if (a.hout.nh > 0) {
+                    if (mode) bc_backward_fin<1><<<dim3(hbx_out, ns), kBlock, 0, st>>>(a, k);
+                    else bc_backward_fin<0><<<dim3(hbx_out, ns), kBlock, 0, st>>>(a, k);
+                }
             }
             LGMI_HIP(hipGetLastError());
             LGMI_HIP(hipEventRecord(w->ev[3], st));
@@ -1265,8 +1325,9 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
             bwd_ms += t2;
         }
         if (nv > 0) {
-            bc_finish<<<std::max(1, std::min(target_blocks, (nv + 255) / 256)), 256, 0, st>>>(
-                w->bc_acc, nslots, nv, endp, do_scale, scale, w->bc_out);
+            const int fb = std::max(1, std::min(target_blocks, (nv + 255) / 256));
+            if (mode) bc_finish<1><<<fb, 256, 0, st>>>(w->bc_acc, nslots, nv, endp, do_scale, scale, w->bc_out);
+            else bc_finish<0><<<fb, 256, 0, st>>>(w->bc_acc, nslots, nv, endp, do_scale, scale, w->bc_out);
             LGMI_HIP(hipGetLastError());
         }
     }

@@ -373,7 +373,16 @@ int lgmi_betweenness(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int64_t 
     if (!g || !bc || nsrc < 0 || (nsrc > 0 && !srcs) || k_scale < 0) throw Error(LGMI_E_ARG, "bad arguments");
     std::lock_guard<std::mutex> lk(g->mu);
     for (int32_t i = 0; i < nsrc; i++) check_src(g, srcs[i]);
-    bc_run(g, srcs, nsrc, k_scale, skip_degree0, normalize, endpoints, bc, &g->bc_last);
+    bc_run(g, srcs, nsrc, k_scale, skip_degree0, normalize, endpoints, bc, &g->bc_last, 0);
+    LGMI_API_END
+}
+
+int lgmi_stress(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int64_t* stress) {
+    LGMI_API_BEGIN
+    if (!g || !stress || nsrc < 0 || (nsrc > 0 && !srcs)) throw Error(LGMI_E_ARG, "bad arguments");
+    std::lock_guard<std::mutex> lk(g->mu);
+    for (int32_t i = 0; i < nsrc; i++) check_src(g, srcs[i]);
+    bc_run(g, srcs, nsrc, 0, 1, 0, 0, reinterpret_cast<double*>(stress), &g->bc_last, 1);
     LGMI_API_END
 }
 

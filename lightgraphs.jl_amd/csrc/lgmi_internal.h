@@ -196,8 +196,10 @@ void bfs_fetch(lgmi_graph* g, int32_t* dist, int32_t* parent);
 void bfs_free(BfsWork* w);
 void policy_upload(lgmi_graph* g);
 // bc.hip
+// mode 0: betweenness into bc; mode 1: stress centrality, Int64 results
+// written into the same nv * 8 bytes
 void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, int skip_degree0,
-            int normalize, int endpoints, double* bc, lgmi_bc_stats* stats);
+            int normalize, int endpoints, double* bc, lgmi_bc_stats* stats, int mode);
 void bc_free(BcWork* w);
 void bc_graph_free(BcGraph* b);
 void bc_set_launch(int mode);

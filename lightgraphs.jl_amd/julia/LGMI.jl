@@ -1,6 +1,6 @@
 # LGMI.jl -- LightGraphs.jl drop-in for gdistances / bfs_parents /
-# betweenness_centrality (serial and Parallel) on an MI355X through
-# liblgmi.so (C ABI: include/lgmi.h).
+# betweenness_centrality / stress_centrality (serial and Parallel) on an
+# MI355X through liblgmi.so (C ABI: include/lgmi.h).
 #
 # Loading this module adds methods that are more specific than the
 # reference's (same names, same arguments, same results) for
@@ -16,7 +16,7 @@ module LGMI
 using LightGraphs
 using LightGraphs: AbstractGraph, SimpleGraph, SimpleDiGraph, DefaultDistance, nv, ne,
     is_directed, vertices, degree, sample
-import LightGraphs: gdistances, gdistances!, bfs_parents, bfs_tree, betweenness_centrality
+import LightGraphs: gdistances, gdistances!, bfs_parents, bfs_tree, betweenness_centrality, stress_centrality
 import LightGraphs.Parallel
 
 const liblgmi = joinpath(@__DIR__, "..", "lgmi", "liblgmi.so")
@@ -162,8 +162,33 @@ betweenness_centrality(g::SimpleG, k::Integer, distmx::DefaultDistance = weights
     betweenness_centrality(g, sample(vertices(g), k), distmx; normalize = normalize, endpoints = endpoints)
 
 # ---------------------------------------------------------------------
+# stress_centrality  (src/centrality/stress.jl:33-51)
+# ---------------------------------------------------------------------
+function device_stress(g::SimpleG, srcs::AbstractVector)
+    for s in srcs
+        1 <= s <= nv(g) || throw(BoundsError(vertices(g), s))
+    end
+    h = Handle(g)
+    out = Vector{Int64}(undef, nv(g))
+    s0 = Int32[s - 1 for s in srcs]
+    check(ccall((:lgmi_stress, liblgmi), Cint, (Ptr{Cvoid}, Ptr{Int32}, Int32, Ptr{Int64}),
+                h.ptr, s0, length(s0), out))
+    return out
+end
+
+function stress_centrality(g::SimpleG, vs::AbstractVector = vertices(g))
+    try
+        return device_stress(g, vs)
+    catch e
+        e isa Fallback || rethrow()
+        return invoke(stress_centrality, Tuple{AbstractGraph,Any}, g, vs)
+    end
+end
+stress_centrality(g::SimpleG, k::Integer) = stress_centrality(g, sample(vertices(g), k))
+
+# ---------------------------------------------------------------------
 # Parallel.*  (src/Parallel/traversals/{gdistances,bfs}.jl,
-#              src/Parallel/centrality/betweenness.jl)
+#              src/Parallel/centrality/{betweenness,stress}.jl)
 # ---------------------------------------------------------------------
 function Parallel.gdistances!(g::SimpleG{T}, sources::Vector{<:Integer}, vert_level::Vector{T};
                               queue_segment_size::Integer = 20) where T
@@ -198,5 +223,12 @@ Parallel.betweenness_centrality(g::SimpleG, k::Integer, distmx::DefaultDistance 
                                 normalize = true, endpoints = false, parallel = :distributed) =
     Parallel.betweenness_centrality(g, sample(vertices(g), k), distmx; normalize = normalize,
                                     endpoints = endpoints, parallel = parallel)
+
+# both modes sum the per-source Int64 vectors of the sources with degree > 0
+# (Parallel/centrality/stress.jl:9-51), i.e. the serial result
+Parallel.stress_centrality(g::SimpleG, vs::AbstractVector = vertices(g); parallel = :distributed) =
+    stress_centrality(g, vs)
+Parallel.stress_centrality(g::SimpleG, k::Integer; parallel = :distributed) =
+    stress_centrality(g, sample(vertices(g), k))
 
 end # module

@@ -1,7 +1,9 @@
-"""LightGraphs.Parallel mirrors: gdistances(!), bfs_tree(!), betweenness.
+"""LightGraphs.Parallel mirrors: gdistances(!), bfs_tree(!), betweenness,
+stress.
 
 src/Parallel/traversals/gdistances.jl:36-139, src/Parallel/traversals/
-bfs.jl:16-104 and src/Parallel/centrality/betweenness.jl:1-75.  The GPU runs
+bfs.jl:16-104, src/Parallel/centrality/betweenness.jl:1-75 and
+src/Parallel/centrality/stress.jl:1-51.  The GPU runs
 every variant; the thread/process knobs (``queue_segment_size``,
 ``parallel``) are accepted with the reference's meaning for results:
 
@@ -106,3 +108,11 @@ def betweenness_centrality(g: _GraphBase, vs=None, distmx=None, normalize=True, 
         raise IndexError("vertex out of range")
     pos = np.nonzero(deg[srcs - 1] > 0)[0].astype(np.int64) + 1 if srcs.size else np.zeros(0, np.int64)
     return _bc._run(g, pos, k, normalize, endpoints, skip_degree0=False)
+
+
+def stress_centrality(g: _GraphBase, vs=None, parallel="distributed", rng=None) -> np.ndarray:
+    """Parallel.stress_centrality(g[, vs | k]; parallel=:distributed |
+    :threads) (src/Parallel/centrality/stress.jl:1-51).  Both modes sum the
+    per-source Int64 vectors of the vertices in vs with degree > 0, which is
+    the serial result."""
+    return _bc._run_stress(g, _bc._vs(g, vs, rng=rng))

@@ -7,17 +7,18 @@ hand-written gfx950 HIP kernels in liblgmi.so through its C ABI
 * ``gdistances``, ``gdistances_`` (gdistances!), ``bfs_parents``,
   ``bfs_tree``, ``tree``            -- src/traversals/bfs.jl
 * ``betweenness_centrality``        -- src/centrality/betweenness.jl
+* ``stress_centrality``             -- src/centrality/stress.jl
 * ``Parallel.*``                    -- src/Parallel/{traversals,centrality}
 * ``SimpleGraph``, ``SimpleDiGraph`` -- src/SimpleGraphs (CSR-backed)
 """
 from . import Parallel, generators
 from ._lib import LgmiError, NoDeviceError
-from .centrality import betweenness_centrality, betweenness_centrality_batch, sample
+from .centrality import betweenness_centrality, betweenness_centrality_batch, sample, stress_centrality
 from .graphs import SimpleDiGraph, SimpleGraph
 from .traversals import bfs_parents, bfs_tree, gdistances, gdistances_, tree, typemax
 
 __all__ = [
     "Parallel", "generators", "LgmiError", "NoDeviceError", "betweenness_centrality", "sample",
-    "SimpleDiGraph", "SimpleGraph", "bfs_parents", "bfs_tree", "gdistances", "gdistances_", "tree",
-    "typemax",
+    "SimpleDiGraph", "SimpleGraph", "bfs_parents", "bfs_tree", "gdistances", "gdistances_", "stress_centrality",
+    "tree", "typemax",
 ]

@@ -1,4 +1,5 @@
-"""betweenness_centrality on the GPU (exact Brandes, FP64).
+"""betweenness_centrality (exact Brandes, FP64) and stress_centrality
+(Int64) on the GPU.
 
 Mirrors src/centrality/betweenness.jl:46-78: ``vs`` defaults to all
 vertices, an integer ``k`` samples k distinct vertices, sources with
@@ -64,6 +65,28 @@ def betweenness_centrality(g: _GraphBase, vs=None, distmx=None, normalize=True, 
     _check_distmx(distmx)
     srcs = _vs(g, vs, rng=rng)
     return _run(g, srcs, len(srcs), normalize, endpoints)
+
+
+def _run_stress(g: _GraphBase, srcs1: np.ndarray) -> np.ndarray:
+    n = g.nv()
+    if srcs1.size and (srcs1.min() < 1 or srcs1.max() > n):
+        raise IndexError(f"source vertex out of range 1:{n}")
+    out = np.zeros(max(n, 1), dtype=np.int64)
+    if n == 0:
+        return out[:0]
+    srcs0 = np.ascontiguousarray(srcs1 - 1, dtype=np.int32)
+    lib = _lib.load()
+    _lib.check(lib.lgmi_stress(g.handle(), _lib.i32p(srcs0 if len(srcs0) else np.zeros(1, np.int32)), len(srcs0),
+                               _lib.i64p(out)))
+    return out
+
+
+def stress_centrality(g: _GraphBase, vs=None, rng=None) -> np.ndarray:
+    """stress_centrality(g[, vs | k]) (src/centrality/stress.jl:33-51): the
+    Int64 stress of every vertex over the sources vs (all vertices by
+    default; an integer k samples k distinct vertices); sources of degree 0
+    are skipped.  Exact (wrapping on Int64 overflow like the reference)."""
+    return _run_stress(g, _vs(g, vs, rng=rng))
 
 
 def betweenness_centrality_batch(graphs, normalize=True, endpoints=False, return_ms=False):

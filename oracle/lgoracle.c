@@ -36,6 +36,11 @@
  *   orc_betweenness_threaded  equals orc_betweenness when vs = 1:n (the
  *                        reference test's only use); index-not-vertex quirk
  *                        pinned by a hand-derived case.
+ *   orc_stress           fixture test/testdata/graph-50-500-sc.txt (exact
+ *                        Int64); test/centrality/stress.jl:17-24 case
+ *                        [4, 2, 4, 10, 0]; doctests stress.jl:18-30; a
+ *                        direct restatement of the same recursion from the
+ *                        enumerated shortest-path DAG on random graphs.
  */
 #include <stdint.h>
 #include <stdlib.h>
@@ -530,6 +535,91 @@ int orc_betweenness_threaded(int32_t nv, const int32_t* row_ptr, const int32_t* 
     if (rc) return rc;
     orc_rescale(bc, nv, normalize, directed, nvs);
     return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* stress_centrality -- src/centrality/stress.jl:33-77                  */
+/* ------------------------------------------------------------------ */
+/* _stress_accumulate_basic! (:53-77): for w in reverse(closest_vertices),
+ * every predecessor v of w gets delta[v] += delta[w] + 1 (with delta[w]
+ * not yet multiplied), then delta[w] *= length(P[w]) and stress[w] +=
+ * delta[w] for w != s; P[s] is emptied first.  Int64 arithmetic wraps as
+ * Julia's Int does (done in uint64_t here). */
+static void stress_accumulate(int64_t* stress, dijk_ws* w, const int32_t* in_row_ptr, int32_t si) {
+    int32_t nv = w->nv;
+    uint64_t* delta = (uint64_t*)w->delta;      /* same 8-byte slots, integer use */
+    for (int32_t i = 0; i < nv; i++) delta[i] = 0;
+    for (int32_t k = w->nclosest - 1; k >= 0; k--) {
+        int32_t x = w->closest[k];
+        const int32_t* P = w->preds + in_row_ptr[x];
+        int32_t np = (x == si) ? 0 : w->pred_cnt[x];
+        for (int32_t j = 0; j < np; j++) delta[P[j]] += delta[x] + 1u;
+        delta[x] *= (uint64_t)np;
+        if (x != si) stress[x] = (int64_t)((uint64_t)stress[x] + delta[x]);
+    }
+}
+
+/* stress_centrality(g, vs) -- stress.jl:33-48 (degree-0 sources skipped,
+ * :41).  Parallel.stress_centrality (:distributed and :threads,
+ * src/Parallel/centrality/stress.jl:9-51) sums the same per-source
+ * vectors; integer addition makes the order irrelevant, so with nthreads
+ * > 1 the sources are shared among threads and the per-thread sums added. */
+typedef struct {
+    int32_t nv; const int32_t* row_ptr; const int32_t* col; const int32_t* in_row_ptr;
+    const int32_t* srcs; int64_t nsrc; int64_t* acc;
+    int64_t* next; pthread_mutex_t* mu; int rc;
+} sc_job;
+
+static void* sc_worker(void* arg) {
+    sc_job* j = (sc_job*)arg;
+    dijk_ws w;
+    if (dijk_ws_init(&w, j->nv, j->in_row_ptr[j->nv]) != 0) { dijk_ws_free(&w); j->rc = -1; return NULL; }
+    for (;;) {
+        int64_t i;
+        pthread_mutex_lock(j->mu);
+        i = (*j->next)++;
+        pthread_mutex_unlock(j->mu);
+        if (i >= j->nsrc) break;
+        int32_t s = j->srcs[i];
+        dijkstra_allpaths(&w, j->row_ptr, j->col, j->in_row_ptr, s);
+        stress_accumulate(j->acc, &w, j->in_row_ptr, s);
+    }
+    dijk_ws_free(&w);
+    j->rc = 0;
+    return NULL;
+}
+
+int orc_stress(int32_t nv, const int32_t* row_ptr, const int32_t* col, const int32_t* in_row_ptr,
+               int directed, const int32_t* vs, int32_t nvs, int nthreads, int64_t* stress) {
+    if (nthreads < 1) nthreads = 1;
+    for (int32_t i = 0; i < nvs; i++) if (vs[i] < 0 || vs[i] >= nv) return -2;
+    int32_t* act = (int32_t*)malloc(sizeof(int32_t) * ((size_t)nvs + 1));
+    int64_t* accs = (int64_t*)calloc((size_t)nthreads * (size_t)nv + 1, sizeof(int64_t));
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)nthreads);
+    sc_job* jobs = (sc_job*)malloc(sizeof(sc_job) * (size_t)nthreads);
+    if (!act || !accs || !th || !jobs) { free(act); free(accs); free(th); free(jobs); return -1; }
+    int64_t nact = 0;
+    for (int32_t i = 0; i < nvs; i++)
+        if (degree_of(vs[i], row_ptr, in_row_ptr, directed) > 0) act[nact++] = vs[i];
+    pthread_mutex_t mu;
+    pthread_mutex_init(&mu, NULL);
+    int64_t next = 0;
+    for (int t = 0; t < nthreads; t++) {
+        jobs[t].nv = nv; jobs[t].row_ptr = row_ptr; jobs[t].col = col; jobs[t].in_row_ptr = in_row_ptr;
+        jobs[t].srcs = act; jobs[t].nsrc = nact; jobs[t].acc = accs + (size_t)t * (size_t)nv;
+        jobs[t].next = &next; jobs[t].mu = &mu; jobs[t].rc = 0;
+        pthread_create(&th[t], NULL, sc_worker, &jobs[t]);
+    }
+    int rc = 0;
+    for (int t = 0; t < nthreads; t++) { pthread_join(th[t], NULL); if (jobs[t].rc) rc = jobs[t].rc; }
+    pthread_mutex_destroy(&mu);
+    for (int32_t v = 0; v < nv; v++) {
+        uint64_t x = 0;
+        for (int t = 0; t < nthreads; t++) x += (uint64_t)accs[(size_t)t * (size_t)nv + v];
+        stress[v] = (int64_t)x;
+    }
+    free(act); free(accs); free(th); free(jobs);
+    return rc;
 }
 
 /* Serial single-source BFS used as the CPU-baseline leg for

@@ -56,6 +56,8 @@ def _load():
                                              _i32p, ctypes.c_int32, ctypes.c_int, ctypes.c_int,
                                              ctypes.c_int, ctypes.c_int, _f64p]
     lib.orc_rescale.argtypes = [_f64p, ctypes.c_int32, ctypes.c_int, ctypes.c_int, ctypes.c_int64]
+    lib.orc_stress.argtypes = [ctypes.c_int32, _i32p, _i32p, _i32p, ctypes.c_int, _i32p, ctypes.c_int32,
+                               ctypes.c_int, _i64p]
     lib.orc_bfs_dist_parent.argtypes = [ctypes.c_int32, _i32p, _i32p, ctypes.c_int32, _i32p, _i32p]
     lib.orc_bfs_dist_parent.restype = ctypes.c_int64
     _lib = lib
@@ -170,6 +172,20 @@ def betweenness_threaded(row_ptr, col, in_row_ptr, directed, vs, normalize=True,
                                         int(directed), _p32(vs), len(vs), int(normalize),
                                         int(endpoints), int(nt), int(index_quirk),
                                         out.ctypes.data_as(_f64p)))
+    return out
+
+
+def stress(row_ptr, col, in_row_ptr, directed, vs, nthreads=1):
+    """stress_centrality(g, vs) (stress.jl:33-77); Int64, wrapping.  With
+    nthreads > 1 the Parallel form (Parallel/centrality/stress.jl), which
+    sums the same integers."""
+    lib = _load()
+    row_ptr, col, in_row_ptr = _c32(row_ptr), _c32(col), _c32(in_row_ptr)
+    vs = _c32(np.atleast_1d(vs) if len(np.atleast_1d(vs)) else np.zeros(0, np.int32))
+    nv = len(row_ptr) - 1
+    out = np.empty(nv, dtype=np.int64)
+    _check(lib.orc_stress(nv, _p32(row_ptr), _p32(col), _p32(in_row_ptr), int(directed), _p32(vs), len(vs),
+                          int(nthreads), out.ctypes.data_as(_i64p)))
     return out
 
 

@@ -3,9 +3,11 @@ container, where the reference tree and networkx are available; the GPU box
 only reads the committed .npz files).
 
 * ref_graph_50_500.npz -- the reference's own fixture: the directed graph of
-  test/testdata/graph-50-500.jgz (LG text format, gzip) and the FP64
+  test/testdata/graph-50-500.jgz (LG text format, gzip), the FP64
   betweenness of test/testdata/graph-50-500-bc.txt (normalize=true,
-  endpoints=false), as used by test/centrality/betweenness.jl:8-13.
+  endpoints=false), as used by test/centrality/betweenness.jl:8-13, and the
+  Int64 stress centrality of test/testdata/graph-50-500-sc.txt
+  (test/centrality/stress.jl:2-7).
 * nx_*.npz -- networkx 3.4.2 vectors for graphs drawn with this project's
   generators: BFS distances (single_source_shortest_path_length) and
   betweenness for the full vertex set and for explicit source lists.  For
@@ -97,7 +99,8 @@ def nx_bc_sources(G, n, directed, sources, normalize, endpoints):
 def write_ref_fixture():
     n, directed, src, dst = read_lg(os.path.join(REF, "test/testdata/graph-50-500.jgz"))
     bc = np.loadtxt(os.path.join(REF, "test/testdata/graph-50-500-bc.txt"), delimiter=",", dtype=np.float64)
-    np.savez(os.path.join(HERE, "ref_graph_50_500.npz"), n=n, directed=directed, src=src, dst=dst, bc=bc)
+    sc = np.loadtxt(os.path.join(REF, "test/testdata/graph-50-500-sc.txt"), delimiter=",", dtype=np.int64)
+    np.savez(os.path.join(HERE, "ref_graph_50_500.npz"), n=n, directed=directed, src=src, dst=dst, bc=bc, sc=sc)
     print("ref_graph_50_500.npz", n, len(src), directed)
 
 

@@ -293,3 +293,85 @@ def test_planted_mistakes_are_caught(oracle):
 def _transpose_col(rp, col):
     from lgmi.graphs import csr_transpose
     return csr_transpose(rp, col)[1]
+
+
+# ----------------------------------------------------------------------
+# stress centrality (src/centrality/stress.jl)
+# ----------------------------------------------------------------------
+
+def stress_by_levels(n, directed, rp, col, irp, icol, vs):
+    """The recursion of _stress_accumulate_basic! (stress.jl:53-77) written
+    from BFS levels instead of the Dijkstra state: P[w] = in-neighbours one
+    level closer, delta(w) = sum over x with w in P[x] of (delta(x) + 1),
+    stress[w] += delta(w) * |P[w]| for w != s.  Python ints, no wrapping."""
+    out = [0] * n
+    deg = np.diff(rp) + (np.diff(irp) if directed else 0)
+    for s in vs:
+        if deg[s] == 0:
+            continue
+        d = [-1] * n
+        d[s] = 0
+        order, q = [s], [s]
+        while q:
+            nq = []
+            for u in q:
+                for v in col[rp[u]:rp[u + 1]]:
+                    if d[v] < 0:
+                        d[v] = d[u] + 1
+                        nq.append(int(v))
+            order += nq
+            q = nq
+        preds = [[u for u in icol[irp[w]:irp[w + 1]] if d[u] >= 0 and d[u] == d[w] - 1] if w != s else []
+                 for w in range(n)]
+        delta = [0] * n
+        for w in reversed(order):
+            for v in preds[w]:
+                delta[v] += delta[w] + 1
+            delta[w] *= len(preds[w])
+            if w != s:
+                out[w] += delta[w]
+    return np.array(out, dtype=np.int64)
+
+
+def test_stress_reference_fixture(oracle):
+    """test/centrality/stress.jl:2-7 and test/parallel/centrality/stress.jl:2-10:
+    the directed graph-50-500 fixture, exact."""
+    z = load_golden("ref_graph_50_500.npz")
+    n = int(z["n"])
+    rp, col, irp, icol = csr_of(n, z["src"], z["dst"], True)
+    sc = oracle.stress(rp, col, irp, True, np.arange(n))
+    assert sc.tolist() == z["sc"].tolist()
+    assert oracle.stress(rp, col, irp, True, np.arange(n), nthreads=4).tolist() == z["sc"].tolist()
+    assert stress_by_levels(n, True, rp, col, irp, icol, range(n)).tolist() == z["sc"].tolist()
+
+
+def test_stress_small_reference_cases(oracle):
+    # test/centrality/stress.jl:17-24: cycle_graph(4) + vertex 5 + edge 4-5
+    n, d, (rp, col, irp, _) = mk((5, False, [(1, 2), (2, 3), (3, 4), (1, 4), (4, 5)]))
+    assert oracle.stress(rp, col, irp, False, np.arange(5)).tolist() == [4, 2, 4, 10, 0]
+    # doctests stress.jl:18-30: star_graph(3) and cycle_graph(4)
+    n, d, (rp, col, irp, _) = mk(star_spec(3))
+    assert oracle.stress(rp, col, irp, False, np.arange(3)).tolist() == [2, 0, 0]
+    n, d, (rp, col, irp, _) = mk((4, False, [(1, 2), (2, 3), (3, 4), (1, 4)]))
+    assert oracle.stress(rp, col, irp, False, np.arange(4)).tolist() == [2, 2, 2, 2]
+    # degree-0 sources are skipped; an empty source list gives zeros
+    n, d, (rp, col, irp, _) = mk((4, False, [(1, 2), (2, 3)]))
+    assert oracle.stress(rp, col, irp, False, [3]).tolist() == [0, 0, 0, 0]
+    assert oracle.stress(rp, col, irp, False, []).tolist() == [0, 0, 0, 0]
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+@pytest.mark.parametrize("directed", [False, True])
+def test_stress_matches_level_restatement(oracle, seed, directed):
+    """The C restatement (Dijkstra state, closest-vertices order) against the
+    level-based restatement above, with duplicate and degree-0 sources."""
+    rng = np.random.default_rng(seed)
+    n = 60
+    m = 150
+    src = rng.integers(0, n, m)
+    dst = rng.integers(0, n, m)
+    rp, col, irp, icol = csr_of(n, src, dst, directed, True)
+    vs = np.concatenate([rng.integers(0, n, 25), [0, 0]])
+    got = oracle.stress(rp, col, irp, directed, vs)
+    want = stress_by_levels(n, directed, rp, col, irp, icol, vs)
+    assert got.tolist() == want.tolist()
