@@ -212,6 +212,22 @@ int lgmi_betweenness(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int64_t 
  * overflow as Julia's Int does; exact. */
 int lgmi_stress(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int64_t* stress);
 
+/* Shortest-path distance statistics of every listed source (unit edge
+ * lengths), 64 sources per lane group, forward pass only: for source
+ * srcs[i], stats[3i] = the sum of the distances to the vertices it reaches,
+ * stats[3i+1] = the number of vertices it reaches (itself included),
+ * stats[3i+2] = the largest of those distances.  Exact.  The reference
+ * computes one dijkstra_shortest_paths per source for each of
+ *   closeness_centrality             src/centrality/closeness.jl:32-57
+ *   radiality_centrality             src/centrality/radiality.jl:34-47
+ *   eccentricity, diameter, radius, center, periphery
+ *                                    src/distance.jl:74-174
+ * and their Parallel forms (src/Parallel/centrality/{closeness,radiality}.jl,
+ * src/Parallel/distance.jl); the host forms those results from these three
+ * numbers (an unreachable vertex is typemax(Int) in the reference's
+ * distance vector, which the host restores). */
+int lgmi_distance_stats(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int64_t* stats);
+
 /* Statistics of the last betweenness (or stress) run on g. */
 int lgmi_betweenness_stats(lgmi_graph_t g, lgmi_bc_stats* stats);
 

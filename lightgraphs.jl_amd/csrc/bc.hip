@@ -86,6 +86,10 @@ struct BcWork {
     BcCtrl* ctrl = nullptr;      // [slot]
     BcCtrl* h_ctrl = nullptr;    // pinned mirror
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    bool has_sig = false;         // sig and bc_acc allocated (not needed by distances only)
+    uint64_t* st = nullptr;       // distances only: [slot][chunks + heavy rows][3][64]
+    uint64_t* st_out = nullptr;   // [slot][3][64]
+    size_t st_words = 0;
     double* hpart = nullptr;      // [slot][seg_stride][64]
     uint64_t* hgot = nullptr;     // [slot][seg_stride]
     int32_t seg_stride = 0;
@@ -154,12 +158,20 @@ struct BcArgs {
     double* hpart;           // [slot][seg_stride][64] per-segment lane sums
     uint64_t* hgot;          // [slot][seg_stride] per-segment new-lane masks
     int32_t seg_stride;
+    int32_t fmod;            // levels F keeps: cap (all), or 2 (distances only)
+    uint64_t* st;            // distances only: [slot][chunk or heavy row][3][64]
+                             // per-lane sums of distances, counts, last level
 };
 
-// Two computations share these kernels: MODE 0 is betweenness (FP64 sigma
+// Three computations share these kernels: MODE 0 is betweenness (FP64 sigma
 // and delta), MODE 1 is stress centrality (src/centrality/stress.jl:33-77),
 // whose forward pass counts each vertex's predecessors per lane and whose
-// backward pass sums Int64 path counts (wrapping like Julia's Int).
+// backward pass sums Int64 path counts (wrapping like Julia's Int), and
+// MODE 2 is the forward pass alone, which records per source lane the sum
+// of the distances, the number of vertices reached and the last level
+// (closeness, radiality and eccentricity, src/centrality/closeness.jl,
+// src/centrality/radiality.jl, src/distance.jl).  MODE 2 keeps two levels
+// of F (fmod = 2) and no sigma.
 template <int MODE> struct BwdAcc { using T = double; };
 template <> struct BwdAcc<1> { using T = uint64_t; };
 
@@ -229,7 +241,7 @@ __global__ void bc_seed(BcArgs a, const int32_t* lane_src) {
     if (s >= 0) {
         atomicOr((unsigned long long*)&F0[s], 1ull << lane);
         atomicOr((unsigned long long*)&seen[s], 1ull << lane);
-        sig[(size_t)s * 64 + lane] = 1.0;
+        if (a.sig) sig[(size_t)s * 64 + lane] = 1.0;
     }
     if (lane == 0) {
         BcCtrl* c = &a.ctrl[slot];
@@ -246,18 +258,39 @@ __global__ void bc_seed(BcArgs a, const int32_t* lane_src) {
 // wavefront): every vertex with an unseen lane ORs its in-neighbours'
 // frontier masks and adds their sigma per lane.  Returns, in every lane,
 // whether any vertex of the chunk got a new lane.
+__device__ inline uint64_t* flevel(const BcArgs& a, int slot, int32_t L) {
+    return a.F + ((size_t)slot * a.cap + (a.fmod == 2 ? (L & 1) : L)) * a.nv;
+}
+
+__device__ inline uint64_t wave_or_u64(uint64_t m) {
+    for (int o = 32; o >= 1; o >>= 1) m |= __shfl_xor(m, o);
+    return m;
+}
+
+// Distances only: adds level L + 1 for the lanes of `got` to one (slot,
+// chunk or heavy row) entry of the per-lane statistics.
+__device__ inline void st_add(const BcArgs& a, int slot, int32_t idx, int32_t L, uint64_t got, int lane) {
+    uint64_t* e = a.st + ((size_t)slot * (a.nchunks + a.hin.nh) + idx) * 192;
+    if ((got >> lane) & 1) {
+        e[lane] += (uint64_t)(L + 1);
+        e[64 + lane] += 1;
+        e[128 + lane] = (uint64_t)(L + 1);
+    }
+}
+
 template <int MODE>
 __device__ bool fwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int lane) {
     const int32_t nv = a.nv;
     const uint64_t act = a.active[slot];
-    const uint64_t* FL = a.F + ((size_t)slot * a.cap + L) * nv;
-    uint64_t* Fn = a.F + ((size_t)slot * a.cap + L + 1) * nv;
+    const uint64_t* FL = flevel(a, slot, L);
+    uint64_t* Fn = flevel(a, slot, L + 1);
     uint64_t* seen = a.seen + (size_t)slot * nv;
-    double* sig = a.sig + (size_t)slot * nv * 64;
+    double* sig = MODE == 2 ? nullptr : a.sig + (size_t)slot * nv * 64;
     const int32_t vj = ch * 64 + lane;
     uint64_t sj = 0;
     int32_t bj = 0, ej = 0;
     bool cand = false, heavy = false;
+    uint64_t st_d = 0, st_c = 0, st_e = 0;   // distances only
     if (vj < nv) {
         sj = seen[vj];
         bj = a.in_rp[vj];
@@ -283,17 +316,41 @@ __device__ bool fwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
                 uk = a.in_col[k];
                 mk = FL[uk] & unseen;
             }
-            uint64_t hits = __ballot(mk != 0);
-            while (hits) gather_hits<double, MODE == 0>(hits, uk, mk, sig, lane, acc, got);
+            if constexpr (MODE == 2) {
+                got |= wave_or_u64(mk);
+                if (got == unseen) break;    // every unseen lane found
+            } else {
+                uint64_t hits = __ballot(mk != 0);
+                while (hits) gather_hits<double, MODE == 0>(hits, uk, mk, sig, lane, acc, got);
+            }
         }
-        if ((got >> lane) & 1) sig[(size_t)v * 64 + lane] = acc;
+        if constexpr (MODE == 2) {
+            if ((got >> lane) & 1) {
+                st_d += (uint64_t)(L + 1);
+                st_c += 1;
+                st_e = (uint64_t)(L + 1);
+            }
+        } else {
+            if ((got >> lane) & 1) sig[(size_t)v * 64 + lane] = acc;
+        }
         if (lane == j) newF = got;
     }
     if (vj < nv && !heavy) {
         Fn[vj] = newF;
         if (newF) seen[vj] = sj | newF;
     }
-    return __ballot(newF != 0) != 0;
+    const bool any = __ballot(newF != 0) != 0;
+    if constexpr (MODE == 2) {
+        if (any) {
+            uint64_t* e = a.st + ((size_t)slot * (a.nchunks + a.hin.nh) + ch) * 192;
+            if (st_c) {
+                e[lane] += st_d;
+                e[64 + lane] += st_c;
+                e[128 + lane] = st_e;
+            }
+        }
+    }
+    return any;
 }
 
 // Forward step of one slot over heavy-row segment sg (one wavefront): the
@@ -306,8 +363,8 @@ __device__ bool fwd_seg(const BcArgs& a, int slot, int32_t sg, int32_t L, int la
     const int32_t nv = a.nv;
     const int32_t v = a.hin.seg_v[sg];
     const uint64_t act = a.active[slot];
-    const uint64_t* FL = a.F + ((size_t)slot * a.cap + L) * nv;
-    const double* sig = a.sig + (size_t)slot * nv * 64;
+    const uint64_t* FL = flevel(a, slot, L);
+    const double* sig = MODE == 2 ? nullptr : a.sig + (size_t)slot * nv * 64;
     const uint64_t unseen = ~a.seen[(size_t)slot * nv + v] & act;
     double acc = 0.0;
     uint64_t got = 0;
@@ -321,11 +378,16 @@ __device__ bool fwd_seg(const BcArgs& a, int slot, int32_t sg, int32_t L, int la
                 uk = a.in_col[k];
                 mk = FL[uk] & unseen;
             }
-            uint64_t hits = __ballot(mk != 0);
-            while (hits) gather_hits<double, MODE == 0>(hits, uk, mk, sig, lane, acc, got);
+            if constexpr (MODE == 2) {
+                got |= wave_or_u64(mk);
+                if (got == unseen) break;
+            } else {
+                uint64_t hits = __ballot(mk != 0);
+                while (hits) gather_hits<double, MODE == 0>(hits, uk, mk, sig, lane, acc, got);
+            }
         }
     }
-    a.hpart[((size_t)slot * a.seg_stride + sg) * 64 + lane] = acc;
+    if constexpr (MODE != 2) a.hpart[((size_t)slot * a.seg_stride + sg) * 64 + lane] = acc;
     if (lane == 0) a.hgot[(size_t)slot * a.seg_stride + sg] = got;
     return got != 0;
 }
@@ -333,25 +395,31 @@ __device__ bool fwd_seg(const BcArgs& a, int slot, int32_t sg, int32_t L, int la
 // Finishes heavy in-row h of one slot after its segments: the partials in
 // segment order (sigma is integer-valued, so the sum equals the row-order
 // one exactly) give the vertex's sigma; its F_{L+1} and seen follow.
+template <int MODE>
 __device__ void fwd_combine(const BcArgs& a, int slot, int32_t h, int32_t L, int lane) {
     const int32_t nv = a.nv;
     const int32_t f0 = a.hin.h_first[h], nsg = a.hin.h_nseg[h];
     const int32_t v = a.hin.seg_v[f0];
-    uint64_t* Fn = a.F + ((size_t)slot * a.cap + L + 1) * nv;
+    uint64_t* Fn = flevel(a, slot, L + 1);
     const uint64_t sv = a.seen[(size_t)slot * nv + v];
     if (!(~sv & a.active[slot])) {   // every lane seen: the segments did nothing
         if (lane == 0) Fn[v] = 0;
         return;
     }
-    const double* part = a.hpart + ((size_t)slot * a.seg_stride + f0) * 64;
     const uint64_t* pgot = a.hgot + (size_t)slot * a.seg_stride + f0;
-    double tot = 0.0;
     uint64_t gm = 0;
-    for (int32_t q = 0; q < nsg; q++) {
-        tot += part[(size_t)q * 64 + lane];
-        gm |= pgot[q];
+    if constexpr (MODE == 2) {
+        for (int32_t q = 0; q < nsg; q++) gm |= pgot[q];
+        if (gm) st_add(a, slot, a.nchunks + h, L, gm, lane);
+    } else {
+        const double* part = a.hpart + ((size_t)slot * a.seg_stride + f0) * 64;
+        double tot = 0.0;
+        for (int32_t q = 0; q < nsg; q++) {
+            tot += part[(size_t)q * 64 + lane];
+            gm |= pgot[q];
+        }
+        if ((gm >> lane) & 1) a.sig[((size_t)slot * nv + v) * 64 + lane] = tot;
     }
-    if ((gm >> lane) & 1) a.sig[((size_t)slot * nv + v) * 64 + lane] = tot;
     if (lane == 0) {
         Fn[v] = gm;
         if (gm) a.seen[(size_t)slot * nv + v] = sv | gm;
@@ -370,7 +438,7 @@ __global__ __launch_bounds__(kBlock) void bc_forward(BcArgs a) {
         s_done = c->done;
         s_L = c->level;
         s_found = 0;
-        if (!s_done && s_L + 1 >= a.cap) {
+        if (!s_done && a.fmod != 2 && s_L + 1 >= a.cap) {
             c->overflow = 1;
             s_done = 1;
         }
@@ -390,6 +458,7 @@ __global__ __launch_bounds__(kBlock) void bc_forward(BcArgs a) {
 // Forward level step, part 2: finishes the heavy rows; the last block of
 // the launch advances the slot's level, or marks the slot done when the
 // step found nothing (maxlevel = L).
+template <int MODE>
 __global__ __launch_bounds__(kBlock) void bc_forward_fin(BcArgs a) {
     const int slot = blockIdx.y;
     BcCtrl* c = &a.ctrl[slot];
@@ -399,12 +468,12 @@ __global__ __launch_bounds__(kBlock) void bc_forward_fin(BcArgs a) {
     if (tid == 0) {
         s_done = c->done;
         s_L = c->level;
-        if (!s_done && s_L + 1 >= a.cap) s_done = 1;
+        if (!s_done && a.fmod != 2 && s_L + 1 >= a.cap) s_done = 1;
     }
     __syncthreads();
     if (s_done) return;
     const int32_t L = s_L;
-    for (int32_t h = blockIdx.x * kWaves + wid; h < a.hin.nh; h += gridDim.x * kWaves) fwd_combine(a, slot, h, L, lane);
+    for (int32_t h = blockIdx.x * kWaves + wid; h < a.hin.nh; h += gridDim.x * kWaves) fwd_combine<MODE>(a, slot, h, L, lane);
     __syncthreads();
     if (tid == 0) {
         __threadfence();
@@ -576,6 +645,45 @@ __global__ __launch_bounds__(kBlock) void bc_backward_fin(BcArgs a, int32_t k) {
     for (int32_t h = blockIdx.x * kWaves + wid; h < a.hout.nh; h += gridDim.x * kWaves) bwd_combine<MODE>(a, slot, h, L, lane);
 }
 
+// Distances only: one block adds up the statistics entries of one slot
+// (sums of distances and counts, the largest last level) into
+// out[slot][3][64].  Every thread of the block calls it.
+__device__ void st_reduce(const BcArgs& a, int slot, uint64_t* out) {
+    __shared__ uint64_t s_part[kWaves][3][64];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int32_t n = a.nchunks + a.hin.nh;
+    const uint64_t* base = a.st + (size_t)slot * n * 192;
+    uint64_t d = 0, c = 0, e = 0;
+    for (int32_t i = wid; i < n; i += kWaves) {
+        d += base[(size_t)i * 192 + lane];
+        c += base[(size_t)i * 192 + 64 + lane];
+        e = max(e, base[(size_t)i * 192 + 128 + lane]);
+    }
+    s_part[wid][0][lane] = d;
+    s_part[wid][1][lane] = c;
+    s_part[wid][2][lane] = e;
+    __syncthreads();
+    if (wid == 0) {
+        for (int w = 1; w < kWaves; w++) {
+            d += s_part[w][0][lane];
+            c += s_part[w][1][lane];
+            e = max(e, s_part[w][2][lane]);
+        }
+        out[(size_t)slot * 192 + lane] = d;
+        out[(size_t)slot * 192 + 64 + lane] = c;
+        out[(size_t)slot * 192 + 128 + lane] = e;
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(kBlock) void bc_st_reduce(BcArgs a, uint64_t* out) { st_reduce(a, blockIdx.x, out); }
+
+__global__ void bc_st_clear(BcArgs a, int32_t ns) {
+    const int64_t n = (int64_t)ns * (a.nchunks + a.hin.nh) * 192;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        a.st[i] = 0;
+}
+
 constexpr int kMaxSlotsP = 1024;        // slots of one persistent launch
 constexpr int kMaxWordsP = kMaxSlotsP / 64;
 constexpr int kMaxGridP = 4096;
@@ -587,6 +695,7 @@ struct PersistArgs {
     const int32_t* lane_src;
     const double* endp;   // endpoints term or null
     double* out;
+    uint64_t* st_out;     // distances only: [slot][3][64]
     double scale;
     uint64_t timeout;     // wall-clock ticks a barrier may wait
     uint64_t gather;      // wall-clock ticks without a new block that end the start-up
@@ -747,7 +856,11 @@ __global__ __launch_bounds__(kBlock) void bc_persist(BcArgs a, PersistArgs p) {
         const int64_t sl = i / nv;
         a.seen[i] = 0;
         a.F[sl * a.cap * nv + (i - sl * nv)] = 0;
-        a.bc_acc[i] = 0.0;
+        if constexpr (MODE != 2) a.bc_acc[i] = 0.0;
+    }
+    if constexpr (MODE == 2) {
+        const int64_t nst = (int64_t)ns * (a.nchunks + a.hin.nh) * 192;
+        for (int64_t i = (int64_t)vb * kBlock + tid; i < nst; i += (int64_t)G * kBlock) a.st[i] = 0;
     }
     for (int w = tid; w < W; w += kBlock) s_live[w] = 0;
     __syncthreads();
@@ -765,7 +878,7 @@ __global__ __launch_bounds__(kBlock) void bc_persist(BcArgs a, PersistArgs p) {
             if (src >= 0) {
                 atomicOr((unsigned long long*)&a.seen[(size_t)sl * nv + src], 1ull << tid);
                 atomicOr((unsigned long long*)&a.F[(size_t)sl * a.cap * nv + src], 1ull << tid);
-                a.sig[((size_t)sl * nv + src) * 64 + tid] = 1.0;
+                if constexpr (MODE != 2) a.sig[((size_t)sl * nv + src) * 64 + tid] = 1.0;
             }
         }
     }
@@ -777,7 +890,7 @@ __global__ __launch_bounds__(kBlock) void bc_persist(BcArgs a, PersistArgs p) {
         bool live = false;
         for (int w = 0; w < W; w++) live |= s_live[w] != 0;
         if (!live) break;
-        if (L + 1 >= a.cap) {
+        if (a.fmod != 2 && L + 1 >= a.cap) {
             if (vb == 0 && tid == 0) c->overflow = 1;
             return;
         }
@@ -798,7 +911,7 @@ __global__ __launch_bounds__(kBlock) void bc_persist(BcArgs a, PersistArgs p) {
         // heavy rows: one wavefront per (row, live slot) adds the partials
         for (uint32_t i = gw, r = 0; i < nrows_f; i += nw, r++) {
             const int sl = (int)((i + r) % (uint32_t)ns);
-            if ((s_live[sl >> 6] >> (sl & 63)) & 1) fwd_combine(a, sl, (int32_t)(i / (uint32_t)ns), L, lane);
+            if ((s_live[sl >> 6] >> (sl & 63)) & 1) fwd_combine<MODE>(a, sl, (int32_t)(i / (uint32_t)ns), L, lane);
         }
         for (int w = tid; w < W; w += kBlock) s_found[w] = 0;
         __syncthreads();
@@ -834,6 +947,14 @@ __global__ __launch_bounds__(kBlock) void bc_persist(BcArgs a, PersistArgs p) {
             a.ctrl[sl].maxlevel = s_maxl[sl];
             a.ctrl[sl].done = 1;
         }
+    }
+    if constexpr (MODE == 2) {
+        if (vb == 0 && tid == 0) {
+            c->t_end = wall_clock64();
+            c->barriers = nbar;
+        }
+        for (int sl = (int)vb; sl < ns; sl += (int)G) st_reduce(a, sl, p.st_out);
+        return;
     }
     for (int32_t k = 0; k < kmax; k++) {
         for (uint32_t i = gw, r = 0; i < nitems_b; i += nw, r++) {
@@ -916,7 +1037,7 @@ void bc_free(BcWork* w) {
     for (auto& e : w->ev) if (e) (void)hipEventDestroy(e);
     if (w->pexec) (void)hipGraphExecDestroy(w->pexec);
     dev_free(w->pctl); dev_free(w->found); dev_free(w->rel);
-    dev_free(w->hpart); dev_free(w->hgot);
+    dev_free(w->hpart); dev_free(w->hgot); dev_free(w->st); dev_free(w->st_out);
     if (w->h_pctl) (void)hipHostFree(w->h_pctl);
     delete w;
 }
@@ -928,9 +1049,10 @@ static size_t free_device_bytes() {
 }
 
 // (Re)allocates the per-slot state for `nslots` slots and `cap` levels.
-static void bc_alloc(lgmi_graph* g, int32_t nslots, int32_t cap, int32_t seg_stride) {
+static void bc_alloc(lgmi_graph* g, int32_t nslots, int32_t cap, int32_t seg_stride, bool need_sig) {
     BcWork* w = g->bc;
-    if (w && w->nslots >= nslots && w->cap_levels >= cap && w->nv == g->nv && w->seg_stride >= seg_stride)
+    if (w && w->nslots >= nslots && w->cap_levels >= cap && w->nv == g->nv && w->seg_stride >= seg_stride &&
+        (w->has_sig || !need_sig))
         return;
     bc_free(w);
     g->bc = nullptr;
@@ -942,8 +1064,11 @@ static void bc_alloc(lgmi_graph* g, int32_t nslots, int32_t cap, int32_t seg_str
         const size_t nv = (size_t)std::max(g->nv, 1);
         w->seen = dev_alloc_n<uint64_t>(nv * nslots);
         w->F = dev_alloc_n<uint64_t>(nv * nslots * (size_t)cap);
-        w->sig = dev_alloc_n<double>(nv * nslots * 64);
-        w->bc_acc = dev_alloc_n<double>(nv * nslots);
+        if (need_sig) {
+            w->sig = dev_alloc_n<double>(nv * nslots * 64);
+            w->bc_acc = dev_alloc_n<double>(nv * nslots);
+            w->has_sig = true;
+        }
         w->bc_out = dev_alloc_n<double>(nv);
         w->endp = dev_alloc_n<double>(nv);
         w->active = dev_alloc_n<uint64_t>(nslots);
@@ -952,7 +1077,7 @@ static void bc_alloc(lgmi_graph* g, int32_t nslots, int32_t cap, int32_t seg_str
         LGMI_HIP(hipHostMalloc((void**)&w->h_ctrl, sizeof(BcCtrl) * nslots, hipHostMallocDefault));
         for (auto& e : w->ev) LGMI_HIP(hipEventCreate(&e));
         w->seg_stride = seg_stride;
-        w->hpart = dev_alloc_n<double>((size_t)std::max(seg_stride, 1) * nslots * 64);
+        if (need_sig) w->hpart = dev_alloc_n<double>((size_t)std::max(seg_stride, 1) * nslots * 64);
         w->hgot = dev_alloc_n<uint64_t>((size_t)std::max(seg_stride, 1) * nslots);
         w->pctl = dev_alloc_n<BcPersistCtl>(1);
         LGMI_HIP(hipHostMalloc((void**)&w->h_pctl, sizeof(BcPersistCtl), hipHostMallocDefault));
@@ -1068,7 +1193,9 @@ void bc_set_launch(int mode) { g_launch_mode = mode; }
 // path, which overwrites everything this one wrote.
 static bool bc_persistent(lgmi_graph* g, BcArgs a, int32_t ns, const double* endp, int do_scale, double scale,
                           int mode, int32_t* launch_kind, float* fwd_ms, float* bwd_ms, int32_t* max_levels) {
-    const void* kfn = mode ? reinterpret_cast<const void*>(bc_persist<1>) : reinterpret_cast<const void*>(bc_persist<0>);
+    const void* kfn = mode == 2 ? reinterpret_cast<const void*>(bc_persist<2>)
+                      : mode == 1 ? reinterpret_cast<const void*>(bc_persist<1>)
+                                  : reinterpret_cast<const void*>(bc_persist<0>);
     BcWork* w = g->bc;
     hipStream_t st = g->stream;
     int dev = 0;
@@ -1087,6 +1214,7 @@ static bool bc_persistent(lgmi_graph* g, BcArgs a, int32_t ns, const double* end
     p.ctl = w->pctl;
     p.found = w->found;
     p.rel = w->rel;
+    p.st_out = w->st_out;
     p.lane_src = w->lane_src;
     p.endp = endp;
     p.out = w->bc_out;
@@ -1158,7 +1286,7 @@ static bool bc_persistent(lgmi_graph* g, BcArgs a, int32_t ns, const double* end
 }
 
 void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, int skip_degree0, int normalize,
-            int endpoints, double* bc, lgmi_bc_stats* stats, int mode) {
+            int endpoints, double* bc, lgmi_bc_stats* stats, int mode, uint64_t* dist_stats) {
     const int32_t nv = g->nv;
     hipStream_t st = g->stream;
     // degree(g, s) > 0 filter (betweenness.jl:58) on host copies of the row pointers
@@ -1178,16 +1306,33 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
     const int32_t ngroups = (int32_t)((nact + 63) / 64);
 
     // slots: as many groups at once as fit a memory budget
-    const int32_t cap0 = 32;
+    const int32_t cap0 = mode == 2 ? 2 : 32;
     const int32_t seg_stride = std::max(bg->nseg_in, bg->nseg_out);
-    const size_t per_slot = (size_t)std::max(nv, 1) * (8 + 8 * (size_t)cap0 + 512 + 8) + (size_t)seg_stride * 520;
+    const int32_t nchunks = (nv + 63) / 64;
+    const size_t per_slot = mode == 2
+        ? (size_t)std::max(nv, 1) * (8 + 8 * (size_t)cap0 + 8) + (size_t)seg_stride * 8 +
+              ((size_t)nchunks + bg->nh_in) * 192 * 8
+        : (size_t)std::max(nv, 1) * (8 + 8 * (size_t)cap0 + 512 + 8) + (size_t)seg_stride * 520;
     size_t budget = free_device_bytes() / 3;
     if (budget == 0) budget = (size_t)4 << 30;
     int32_t nslots = (int32_t)std::max<size_t>(1, std::min<size_t>((size_t)std::max(ngroups, 1), budget / per_slot));
     nslots = std::min(nslots, kMaxSlotsP);
-    bc_alloc(g, nslots, cap0, seg_stride);
+    bc_alloc(g, nslots, cap0, seg_stride, mode != 2);
     BcWork* w = g->bc;
     nslots = std::min(w->nslots, std::max(ngroups, 1));
+    if (mode == 2) {
+        const size_t words = (size_t)nslots * ((size_t)nchunks + bg->nh_in) * 192;
+        if (w->st_words < words) {
+            dev_free(w->st);
+            dev_free(w->st_out);
+            w->st = nullptr;
+            w->st_out = nullptr;
+            w->st_words = 0;
+            w->st = dev_alloc_n<uint64_t>(words);
+            w->st_out = dev_alloc_n<uint64_t>((size_t)w->nslots * 192);
+            w->st_words = words;
+        }
+    }
 
     BcArgs a{};
     a.in_rp = g->in.row_ptr; a.in_col = g->in.col;
@@ -1200,6 +1345,9 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
     a.hout = make_segs(bg->seg_out, bg->nseg_out, bg->nh_out);
     a.hpart = w->hpart; a.hgot = w->hgot;
     a.seg_stride = w->seg_stride;
+    a.fmod = mode == 2 ? 2 : w->cap_levels;
+    a.st = w->st;
+    if (mode == 2) { a.sig = nullptr; a.bc_acc = nullptr; }
 
     const int cu = std::max(1, g->cu_count);
     const int target_blocks = cu * 8;
@@ -1250,6 +1398,20 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
         LGMI_HIP(hipMemcpyAsync(w->active, active.data(), sizeof(uint64_t) * active.size(), hipMemcpyHostToDevice, st));
     };
 
+    // distances only: slot s, lane l of groups g0.. is source (g0 + s) * 64 + l;
+    // dist_stats[i] = {sum of distances, vertices reached (the source
+    // included), largest distance}
+    std::vector<uint64_t> st_host(mode == 2 ? (size_t)nslots * 192 : 0);
+    auto scatter_stats = [&](int32_t g0, int32_t ns) {
+        for (int32_t s = 0; s < ns; s++)
+            for (int l = 0; l < 64; l++) {
+                const int64_t idx = (int64_t)(g0 + s) * 64 + l;
+                if (idx >= nact) continue;
+                dist_stats[idx * 3 + 0] = st_host[(size_t)s * 192 + l];
+                dist_stats[idx * 3 + 1] = st_host[(size_t)s * 192 + 64 + l] + 1;
+                dist_stats[idx * 3 + 2] = st_host[(size_t)s * 192 + 128 + l];
+            }
+    };
     float fwd_ms = 0.f, bwd_ms = 0.f;
     int32_t max_levels = 0;
     int32_t launch_kind = 0;
@@ -1260,16 +1422,24 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
             launch_kind = 0;
             fwd_ms = bwd_ms = 0.f;
             max_levels = 0;
+        } else if (mode == 2) {
+            LGMI_HIP(hipMemcpyAsync(st_host.data(), w->st_out, sizeof(uint64_t) * 192 * (size_t)ngroups,
+                                    hipMemcpyDeviceToHost, st));
+            LGMI_HIP(hipStreamSynchronize(st));
+            scatter_stats(0, ngroups);
         }
     }
     if (launch_kind == 0) {
-        bc_clear_acc<<<std::max(1, std::min(target_blocks, (int)(((size_t)nv * nslots + 255) / 256))), 256, 0, st>>>(
-            w->bc_acc, (int64_t)nv * nslots);
+        if (mode != 2)
+            bc_clear_acc<<<std::max(1, std::min(target_blocks, (int)(((size_t)nv * nslots + 255) / 256))), 256, 0, st>>>(
+                w->bc_acc, (int64_t)nv * nslots);
         for (int32_t g0 = 0; g0 < ngroups; g0 += nslots) {
             const int32_t ns = std::min(nslots, ngroups - g0);
             fill_groups(g0, ns);
             a.F = w->F; a.cap = w->cap_levels;
+            if (mode != 2) a.fmod = w->cap_levels;
             bc_init<<<dim3(ibx, ns), 256, 0, st>>>(a);
+            if (mode == 2) bc_st_clear<<<target_blocks, 256, 0, st>>>(a, ns);
             bc_seed<<<ns, 64, 0, st>>>(a, w->lane_src);
             LGMI_HIP(hipGetLastError());
             LGMI_HIP(hipEventRecord(w->ev[1], st));
@@ -1277,17 +1447,24 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
             int32_t lcur = 0;
             int batch = 8;
             for (;;) {
-                if (lcur + batch + 2 >= w->cap_levels) {
+                if (mode != 2 && lcur + batch + 2 >= w->cap_levels) {
                     int32_t nc = w->cap_levels;
                     while (lcur + batch + 2 >= nc) nc *= 2;
                     // a finished slot also needs its empty F_{maxlevel+1}
                     bc_grow_levels(g, nc, std::min(lcur + 2, w->cap_levels));
-                    a.F = w->F; a.cap = w->cap_levels;
+                    a.F = w->F; a.cap = w->cap_levels; a.fmod = w->cap_levels;
                 }
                 for (int i = 0; i < batch; i++) {
-                    if (mode) bc_forward<1><<<dim3(bx, ns), kBlock, 0, st>>>(a);
-                    else bc_forward<0><<<dim3(bx, ns), kBlock, 0, st>>>(a);
-                    bc_forward_fin<<<dim3(hbx_in, ns), kBlock, 0, st>>>(a);
+                    if (mode == 2) {
+                        bc_forward<2><<<dim3(bx, ns), kBlock, 0, st>>>(a);
+                        bc_forward_fin<2><<<dim3(hbx_in, ns), kBlock, 0, st>>>(a);
+                    } else if (mode == 1) {
+                        bc_forward<1><<<dim3(bx, ns), kBlock, 0, st>>>(a);
+                        bc_forward_fin<1><<<dim3(hbx_in, ns), kBlock, 0, st>>>(a);
+                    } else {
+                        bc_forward<0><<<dim3(bx, ns), kBlock, 0, st>>>(a);
+                        bc_forward_fin<0><<<dim3(hbx_in, ns), kBlock, 0, st>>>(a);
+                    }
                 }
                 LGMI_HIP(hipGetLastError());
                 LGMI_HIP(hipMemcpyAsync(w->h_ctrl, w->ctrl, sizeof(BcCtrl) * ns, hipMemcpyDeviceToHost, st));
@@ -1307,7 +1484,15 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
             for (int s = 0; s < ns; s++) lmax = std::max(lmax, w->h_ctrl[s].maxlevel);
             max_levels = std::max(max_levels, lmax);
             LGMI_HIP(hipEventRecord(w->ev[2], st));
-            for (int32_t k = 0; k < lmax; k++) {
+            if (mode == 2) {
+                bc_st_reduce<<<ns, kBlock, 0, st>>>(a, w->st_out);
+                LGMI_HIP(hipGetLastError());
+                LGMI_HIP(hipMemcpyAsync(st_host.data(), w->st_out, sizeof(uint64_t) * 192 * (size_t)ns,
+                                        hipMemcpyDeviceToHost, st));
+                LGMI_HIP(hipStreamSynchronize(st));
+                scatter_stats(g0, ns);
+            }
+            for (int32_t k = 0; mode != 2 && k < lmax; k++) {
                 if (mode) bc_backward<1><<<dim3(bx, ns), kBlock, 0, st>>>(a, k);
                 else bc_backward<0><<<dim3(bx, ns), kBlock, 0, st>>>(a, k);
                 if (a.hout.nh > 0) {
@@ -1324,14 +1509,14 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
             fwd_ms += t1;
             bwd_ms += t2;
         }
-        if (nv > 0) {
+        if (nv > 0 && mode != 2) {
             const int fb = std::max(1, std::min(target_blocks, (nv + 255) / 256));
             if (mode) bc_finish<1><<<fb, 256, 0, st>>>(w->bc_acc, nslots, nv, endp, do_scale, scale, w->bc_out);
             else bc_finish<0><<<fb, 256, 0, st>>>(w->bc_acc, nslots, nv, endp, do_scale, scale, w->bc_out);
             LGMI_HIP(hipGetLastError());
         }
     }
-    if (nv > 0) LGMI_HIP(hipMemcpyAsync(bc, w->bc_out, sizeof(double) * (size_t)nv, hipMemcpyDeviceToHost, st));
+    if (nv > 0 && mode != 2) LGMI_HIP(hipMemcpyAsync(bc, w->bc_out, sizeof(double) * (size_t)nv, hipMemcpyDeviceToHost, st));
     LGMI_HIP(hipEventRecord(w->ev[3], st));
     LGMI_HIP(hipEventSynchronize(w->ev[3]));
     if (stats) {

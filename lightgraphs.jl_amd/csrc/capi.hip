@@ -377,6 +377,16 @@ int lgmi_betweenness(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int64_t 
     LGMI_API_END
 }
 
+int lgmi_distance_stats(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int64_t* stats) {
+    LGMI_API_BEGIN
+    if (!g || !stats || nsrc < 0 || (nsrc > 0 && !srcs)) throw Error(LGMI_E_ARG, "bad arguments");
+    std::lock_guard<std::mutex> lk(g->mu);
+    for (int32_t i = 0; i < nsrc; i++) check_src(g, srcs[i]);
+    if (nsrc == 0) return LGMI_OK;
+    bc_run(g, srcs, nsrc, 0, 0, 0, 0, nullptr, &g->bc_last, 2, reinterpret_cast<uint64_t*>(stats));
+    LGMI_API_END
+}
+
 int lgmi_stress(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int64_t* stress) {
     LGMI_API_BEGIN
     if (!g || !stress || nsrc < 0 || (nsrc > 0 && !srcs)) throw Error(LGMI_E_ARG, "bad arguments");

@@ -197,9 +197,12 @@ void bfs_free(BfsWork* w);
 void policy_upload(lgmi_graph* g);
 // bc.hip
 // mode 0: betweenness into bc; mode 1: stress centrality, Int64 results
-// written into the same nv * 8 bytes
+// written into the same nv * 8 bytes; mode 2: per-source distance
+// statistics into dist_stats (nsrc x 3: sum of distances, vertices reached,
+// largest distance), bc unused
 void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, int skip_degree0,
-            int normalize, int endpoints, double* bc, lgmi_bc_stats* stats, int mode);
+            int normalize, int endpoints, double* bc, lgmi_bc_stats* stats, int mode,
+            uint64_t* dist_stats = nullptr);
 void bc_free(BcWork* w);
 void bc_graph_free(BcGraph* b);
 void bc_set_launch(int mode);

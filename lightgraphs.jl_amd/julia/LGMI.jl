@@ -1,6 +1,7 @@
 # LGMI.jl -- LightGraphs.jl drop-in for gdistances / bfs_parents /
-# betweenness_centrality / stress_centrality (serial and Parallel) on an
-# MI355X through liblgmi.so (C ABI: include/lgmi.h).
+# betweenness_centrality / stress_centrality / closeness_centrality /
+# radiality_centrality / eccentricity and the measures built on it (serial
+# and Parallel) on an MI355X through liblgmi.so (C ABI: include/lgmi.h).
 #
 # Loading this module adds methods that are more specific than the
 # reference's (same names, same arguments, same results) for
@@ -16,7 +17,8 @@ module LGMI
 using LightGraphs
 using LightGraphs: AbstractGraph, SimpleGraph, SimpleDiGraph, DefaultDistance, nv, ne,
     is_directed, vertices, degree, sample
-import LightGraphs: gdistances, gdistances!, bfs_parents, bfs_tree, betweenness_centrality, stress_centrality
+import LightGraphs: gdistances, gdistances!, bfs_parents, bfs_tree, betweenness_centrality, stress_centrality,
+    closeness_centrality, radiality_centrality, eccentricity, diameter, radius, center, periphery
 import LightGraphs.Parallel
 
 const liblgmi = joinpath(@__DIR__, "..", "lgmi", "liblgmi.so")
@@ -187,8 +189,69 @@ end
 stress_centrality(g::SimpleG, k::Integer) = stress_centrality(g, sample(vertices(g), k))
 
 # ---------------------------------------------------------------------
+# closeness / radiality / eccentricity  (src/centrality/closeness.jl:32-57,
+# src/centrality/radiality.jl:34-47, src/distance.jl:74-174).  With unit
+# lengths each needs, per source, only (sum of finite distances, vertices
+# reached, largest finite distance); an unreached vertex is typemax(Int) in
+# the reference's distance vector and enters its Int sums the same way here.
+# ---------------------------------------------------------------------
+function device_dstats(g::SimpleG, srcs::AbstractVector)
+    for s in srcs
+        1 <= s <= nv(g) || throw(BoundsError(vertices(g), s))
+    end
+    h = Handle(g)
+    st = Matrix{Int64}(undef, 3, length(srcs))
+    s0 = Int32[s - 1 for s in srcs]
+    isempty(s0) || check(ccall((:lgmi_distance_stats, liblgmi), Cint, (Ptr{Cvoid}, Ptr{Int32}, Int32, Ptr{Int64}),
+                               h.ptr, s0, length(s0), st))
+    return st
+end
+ecc_of(st, n) = [st[2, i] == n ? st[3, i] : typemax(Int) for i in axes(st, 2)]
+wrapped_sum(st, n) = [st[1, i] + (n - st[2, i]) * typemax(Int) for i in axes(st, 2)]   # Int wraps
+
+function eccentricity(g::SimpleG, v::Integer, distmx::DefaultDistance = weights(g))
+    e = ecc_of(device_dstats(g, [v]), nv(g))[1]
+    e == typemax(Int) && @warn("Infinite path length detected for vertex $v")
+    return e
+end
+eccentricity(g::SimpleG, vs::AbstractVector = vertices(g), distmx::DefaultDistance = weights(g)) =
+    ecc_of(device_dstats(g, vs), nv(g))
+diameter(g::SimpleG, distmx::DefaultDistance = weights(g)) = maximum(eccentricity(g, vertices(g), distmx))
+radius(g::SimpleG, distmx::DefaultDistance = weights(g)) = minimum(eccentricity(g, vertices(g), distmx))
+periphery(g::SimpleG, distmx::DefaultDistance = weights(g)) = periphery(eccentricity(g, vertices(g), distmx))
+center(g::SimpleG, distmx::DefaultDistance = weights(g)) = center(eccentricity(g, vertices(g), distmx))
+
+function closeness_centrality(g::SimpleG, distmx::DefaultDistance = weights(g); normalize = true)
+    n_v = nv(g)
+    closeness = zeros(n_v)
+    act = [u for u in vertices(g) if degree(g, u) != 0]
+    st = device_dstats(g, act)
+    for (i, u) in enumerate(act)
+        σ = st[1, i]
+        l = st[2, i] - 1
+        if σ > 0
+            closeness[u] = l / σ
+            if normalize
+                n = l / (n_v - 1)
+                closeness[u] *= n
+            end
+        end
+    end
+    return closeness
+end
+
+function radiality_centrality(g::SimpleG)::Vector{Float64}
+    n_v = nv(g)
+    st = device_dstats(g, vertices(g))
+    dmtr = max(0.0, maximum(ecc_of(st, n_v)))
+    meandists = wrapped_sum(st, n_v) ./ (n_v - 1)
+    return ((dmtr + 1) .- meandists) ./ dmtr
+end
+
+# ---------------------------------------------------------------------
 # Parallel.*  (src/Parallel/traversals/{gdistances,bfs}.jl,
-#              src/Parallel/centrality/{betweenness,stress}.jl)
+#              src/Parallel/centrality/{betweenness,stress,closeness,
+#              radiality}.jl, src/Parallel/distance.jl)
 # ---------------------------------------------------------------------
 function Parallel.gdistances!(g::SimpleG{T}, sources::Vector{<:Integer}, vert_level::Vector{T};
                               queue_segment_size::Integer = 20) where T
@@ -230,5 +293,15 @@ Parallel.stress_centrality(g::SimpleG, vs::AbstractVector = vertices(g); paralle
     stress_centrality(g, vs)
 Parallel.stress_centrality(g::SimpleG, k::Integer; parallel = :distributed) =
     stress_centrality(g, sample(vertices(g), k))
+
+Parallel.closeness_centrality(g::SimpleG, distmx::DefaultDistance = weights(g); normalize = true,
+                              parallel = :distributed) = closeness_centrality(g, distmx; normalize = normalize)
+Parallel.radiality_centrality(g::SimpleG; parallel = :distributed) = radiality_centrality(g)
+Parallel.eccentricity(g::SimpleG, vs::AbstractVector = vertices(g), distmx::DefaultDistance = weights(g)) =
+    eccentricity(g, vs, distmx)
+Parallel.diameter(g::SimpleG, distmx::DefaultDistance = weights(g)) = diameter(g, distmx)
+Parallel.radius(g::SimpleG, distmx::DefaultDistance = weights(g)) = radius(g, distmx)
+Parallel.periphery(g::SimpleG, distmx::DefaultDistance = weights(g)) = periphery(g, distmx)
+Parallel.center(g::SimpleG, distmx::DefaultDistance = weights(g)) = center(g, distmx)
 
 end # module

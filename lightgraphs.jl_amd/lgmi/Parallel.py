@@ -21,6 +21,7 @@ import numpy as np
 
 from . import traversals as _tr
 from . import centrality as _bc
+from . import distances as _dist
 from .graphs import _GraphBase
 
 
@@ -116,3 +117,39 @@ def stress_centrality(g: _GraphBase, vs=None, parallel="distributed", rng=None) 
     per-source Int64 vectors of the vertices in vs with degree > 0, which is
     the serial result."""
     return _bc._run_stress(g, _bc._vs(g, vs, rng=rng))
+
+
+# Parallel/centrality/{closeness,radiality}.jl and Parallel/distance.jl: the
+# same results as the serial functions for unit distances.
+def closeness_centrality(g: _GraphBase, distmx=None, normalize=True, parallel="distributed") -> np.ndarray:
+    """Parallel.closeness_centrality(g; normalize, parallel)
+    (src/Parallel/centrality/closeness.jl:1-56)."""
+    return _dist.closeness_centrality(g, distmx, normalize=normalize)
+
+
+def radiality_centrality(g: _GraphBase, parallel="distributed") -> np.ndarray:
+    """Parallel.radiality_centrality(g; parallel)
+    (src/Parallel/centrality/radiality.jl:1-38)."""
+    return _dist.radiality_centrality(g)
+
+
+def eccentricity(g: _GraphBase, vs=None, distmx=None) -> np.ndarray:
+    """Parallel.eccentricity(g[, vs]) (src/Parallel/distance.jl:3-18)."""
+    _dist._check_distmx(distmx)
+    return _dist.ecc_from_stats(_dist.distance_stats(g, vs), g.nv())
+
+
+def diameter(g: _GraphBase, distmx=None):
+    return _dist.diameter(eccentricity(g, None, distmx))
+
+
+def radius(g: _GraphBase, distmx=None):
+    return _dist.radius(eccentricity(g, None, distmx))
+
+
+def periphery(g: _GraphBase, distmx=None):
+    return _dist.periphery(eccentricity(g, None, distmx))
+
+
+def center(g: _GraphBase, distmx=None):
+    return _dist.center(eccentricity(g, None, distmx))

@@ -8,17 +8,24 @@ hand-written gfx950 HIP kernels in liblgmi.so through its C ABI
   ``bfs_tree``, ``tree``            -- src/traversals/bfs.jl
 * ``betweenness_centrality``        -- src/centrality/betweenness.jl
 * ``stress_centrality``             -- src/centrality/stress.jl
+* ``closeness_centrality``, ``radiality_centrality``, ``eccentricity``,
+  ``diameter``, ``radius``, ``center``, ``periphery``
+                                    -- src/centrality/{closeness,radiality}.jl,
+                                       src/distance.jl
 * ``Parallel.*``                    -- src/Parallel/{traversals,centrality}
 * ``SimpleGraph``, ``SimpleDiGraph`` -- src/SimpleGraphs (CSR-backed)
 """
 from . import Parallel, generators
 from ._lib import LgmiError, NoDeviceError
 from .centrality import betweenness_centrality, betweenness_centrality_batch, sample, stress_centrality
+from .distances import (center, closeness_centrality, diameter, eccentricity, periphery, radiality_centrality,
+                        radius)
 from .graphs import SimpleDiGraph, SimpleGraph
 from .traversals import bfs_parents, bfs_tree, gdistances, gdistances_, tree, typemax
 
 __all__ = [
     "Parallel", "generators", "LgmiError", "NoDeviceError", "betweenness_centrality", "sample",
     "SimpleDiGraph", "SimpleGraph", "bfs_parents", "bfs_tree", "gdistances", "gdistances_", "stress_centrality",
-    "tree", "typemax",
+    "tree", "typemax", "center", "closeness_centrality", "diameter", "eccentricity", "periphery",
+    "radiality_centrality", "radius",
 ]

@@ -189,6 +189,54 @@ def stress(row_ptr, col, in_row_ptr, directed, vs, nthreads=1):
     return out
 
 
+def _dists_int64(row_ptr, col, s):
+    """dijkstra_shortest_paths(g, s).dists with unit lengths: BFS distances,
+    typemax(Int) where unreached."""
+    d = gdistances(row_ptr, col, [int(s)]).astype(np.int64)
+    return np.where(d < 0, np.iinfo(np.int64).max, d)
+
+
+def eccentricity(row_ptr, col, vs):
+    """eccentricity(g, vs) (src/distance.jl:74-84): maximum(dists)."""
+    return np.array([int(_dists_int64(row_ptr, col, s).max()) for s in np.atleast_1d(vs)], dtype=np.int64)
+
+
+def closeness(row_ptr, col, in_row_ptr, directed, normalize=True):
+    """closeness_centrality(g; normalize) (src/centrality/closeness.jl:32-57),
+    one BFS per vertex."""
+    nv = len(row_ptr) - 1
+    deg = np.diff(np.asarray(row_ptr)) + (np.diff(np.asarray(in_row_ptr)) if directed else 0)
+    out = np.zeros(nv, dtype=np.float64)
+    tmax = np.iinfo(np.int64).max
+    for u in range(nv):
+        if deg[u] == 0:
+            continue
+        d = _dists_int64(row_ptr, col, u)
+        delta = d[d != tmax]
+        sigma = int(delta.sum())
+        l = len(delta) - 1
+        if sigma > 0:
+            out[u] = l / sigma
+            if normalize:
+                out[u] *= l / (nv - 1)
+    return out
+
+
+def radiality(row_ptr, col):
+    """radiality_centrality(g) (src/centrality/radiality.jl:34-47), one BFS
+    per vertex; sum(d.dists) wraps in Int64 as Julia's does."""
+    nv = len(row_ptr) - 1
+    meandists = np.zeros(nv, dtype=np.float64)
+    dmtr = 0.0
+    for v in range(nv):
+        d = _dists_int64(row_ptr, col, v)
+        dmtr = max(dmtr, float(d.max()))
+        total = int(d.astype(np.uint64).sum(dtype=np.uint64).view(np.int64)) if nv else 0
+        meandists[v] = total / (nv - 1)
+    meandists = (dmtr + 1) - meandists
+    return meandists / dmtr
+
+
 def rescale(bc, n, normalize, directed, k):
     lib = _load()
     bc = np.ascontiguousarray(bc, dtype=np.float64).copy()

@@ -7,7 +7,8 @@ only reads the committed .npz files).
   betweenness of test/testdata/graph-50-500-bc.txt (normalize=true,
   endpoints=false), as used by test/centrality/betweenness.jl:8-13, and the
   Int64 stress centrality of test/testdata/graph-50-500-sc.txt
-  (test/centrality/stress.jl:2-7).
+  (test/centrality/stress.jl:2-7) and the FP64 radiality of
+  test/testdata/graph-50-500-rc.txt (test/centrality/radiality.jl:2-8).
 * nx_*.npz -- networkx 3.4.2 vectors for graphs drawn with this project's
   generators: BFS distances (single_source_shortest_path_length) and
   betweenness for the full vertex set and for explicit source lists.  For
@@ -100,7 +101,9 @@ def write_ref_fixture():
     n, directed, src, dst = read_lg(os.path.join(REF, "test/testdata/graph-50-500.jgz"))
     bc = np.loadtxt(os.path.join(REF, "test/testdata/graph-50-500-bc.txt"), delimiter=",", dtype=np.float64)
     sc = np.loadtxt(os.path.join(REF, "test/testdata/graph-50-500-sc.txt"), delimiter=",", dtype=np.int64)
-    np.savez(os.path.join(HERE, "ref_graph_50_500.npz"), n=n, directed=directed, src=src, dst=dst, bc=bc, sc=sc)
+    rc = np.loadtxt(os.path.join(REF, "test/testdata/graph-50-500-rc.txt"), delimiter=",", dtype=np.float64)
+    np.savez(os.path.join(HERE, "ref_graph_50_500.npz"), n=n, directed=directed, src=src, dst=dst, bc=bc, sc=sc,
+             rc=rc)
     print("ref_graph_50_500.npz", n, len(src), directed)
 
 

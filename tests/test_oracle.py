@@ -375,3 +375,104 @@ def test_stress_matches_level_restatement(oracle, seed, directed):
     got = oracle.stress(rp, col, irp, directed, vs)
     want = stress_by_levels(n, directed, rp, col, irp, icol, vs)
     assert got.tolist() == want.tolist()
+
+
+# ----------------------------------------------------------------------
+# closeness / radiality / eccentricity (src/centrality/{closeness,
+# radiality}.jl, src/distance.jl) and the host formulas of lgmi.distances
+# ----------------------------------------------------------------------
+
+def stats_from_oracle(oracle, rp, col, vs):
+    """(sum of finite distances, reached, largest finite distance) per source."""
+    rows = []
+    for s in vs:
+        d = oracle.gdistances(rp, col, [int(s)]).astype(np.int64)
+        r = d[d >= 0]
+        rows.append((int(r.sum()), len(r), int(r.max())))
+    return np.array(rows, dtype=np.int64).reshape(-1, 3)
+
+
+def test_radiality_reference_fixture(oracle):
+    """test/centrality/radiality.jl:2-8 (exact ==) on the directed
+    graph-50-500 fixture; the host formula gives the same bits."""
+    from lgmi import distances as D
+    z = load_golden("ref_graph_50_500.npz")
+    n = int(z["n"])
+    rp, col, irp, icol = csr_of(n, z["src"], z["dst"], True)
+    r = oracle.radiality(rp, col)
+    assert r.tolist() == z["rc"].tolist()
+    st = stats_from_oracle(oracle, rp, col, range(n))
+    assert D.radiality_from_stats(st, n).tolist() == z["rc"].tolist()
+
+
+def test_radiality_small_reference_cases(oracle):
+    # radiality.jl test :10-16 (cycle_graph(4) + pendant) and doctests :18-31
+    n, d, (rp, col, irp, _) = mk((5, False, [(1, 2), (2, 3), (3, 4), (1, 4), (4, 5)]))
+    assert np.allclose(oracle.radiality(rp, col), [5 / 6, 3 / 4, 5 / 6, 11 / 12, 2 / 3], rtol=1e-15)
+    n, d, (rp, col, irp, _) = mk(star_spec(4))
+    assert oracle.radiality(rp, col).tolist() == [1.0, 0.6666666666666666, 0.6666666666666666, 0.6666666666666666]
+    n, d, (rp, col, irp, _) = mk(path_spec(3))
+    assert oracle.radiality(rp, col).tolist() == [0.75, 1.0, 0.75]
+
+
+def test_closeness_reference_cases(oracle):
+    from lgmi import distances as D
+    # test/centrality/closeness.jl:2-11: digraph 1->2, 2->3, 1->3, 3->4
+    n, d, (rp, col, irp, _) = mk((4, True, [(1, 2), (2, 3), (1, 3), (3, 4)]))
+    assert oracle.closeness(rp, col, irp, True, normalize=False).tolist() == [0.75, 0.6666666666666666, 1.0, 0.0]
+    assert oracle.closeness(rp, col, irp, True).tolist() == [0.75, 0.4444444444444444, 0.3333333333333333, 0.0]
+    st = stats_from_oracle(oracle, rp, col, range(4))
+    assert D.closeness_from_stats(st, 4, np.arange(4), True).tolist() == [0.75, 0.4444444444444444,
+                                                                          0.3333333333333333, 0.0]
+    # :24-30: SimpleGraph(5) with edge 1-2
+    n, d, (rp, col, irp, _) = mk((5, False, [(1, 2)]))
+    assert oracle.closeness(rp, col, irp, False).tolist() == [0.25, 0.25, 0.0, 0.0, 0.0]
+    # doctests closeness.jl:14-28
+    n, d, (rp, col, irp, _) = mk(star_spec(5))
+    assert oracle.closeness(rp, col, irp, False).tolist() == [1.0] + [0.5714285714285714] * 4
+    n, d, (rp, col, irp, _) = mk(path_spec(4))
+    assert oracle.closeness(rp, col, irp, False).tolist() == [0.5, 0.75, 0.75, 0.5]
+
+
+def test_eccentricity_reference_cases(oracle):
+    from lgmi import distances as D
+    # doctests src/distance.jl:47-55, 92-99, 113-122, 140-147, 159-168
+    n, d, (rp, col, irp, _) = mk(path_spec(3))
+    assert oracle.eccentricity(rp, col, [0]).tolist() == [2]
+    assert oracle.eccentricity(rp, col, [0, 1]).tolist() == [2, 1]
+    n, d, (rp, col, irp, _) = mk(star_spec(5))
+    e = oracle.eccentricity(rp, col, range(5))
+    assert (D.diameter(e), D.radius(e)) == (2, 1)
+    assert D.periphery(e).tolist() == [2, 3, 4, 5] and D.center(e).tolist() == [1]
+    n, d, (rp, col, irp, _) = mk(path_spec(5))
+    e = oracle.eccentricity(rp, col, range(5))
+    assert (D.diameter(e), D.radius(e)) == (4, 2)
+    assert D.periphery(e).tolist() == [1, 5] and D.center(e).tolist() == [3]
+    # test/distance.jl:59-64: two isolated vertices -> typemax(Int)
+    n, d, (rp, col, irp, _) = mk((2, False, []))
+    assert oracle.eccentricity(rp, col, [0, 1]).tolist() == [D.TYPEMAX_INT] * 2
+    st = stats_from_oracle(oracle, rp, col, [0, 1])
+    assert D.ecc_from_stats(st, 2).tolist() == [D.TYPEMAX_INT] * 2
+
+
+@pytest.mark.parametrize("seed", [4, 5])
+@pytest.mark.parametrize("directed", [False, True])
+def test_distance_host_formulas_vs_oracle(oracle, seed, directed):
+    """The host formulas of lgmi.distances (from the three per-source
+    integers) against the oracle's direct restatements (from the whole
+    distance vectors) on disconnected random graphs: bit-identical."""
+    from lgmi import distances as D
+    rng = np.random.default_rng(seed)
+    n = 80
+    src = rng.integers(0, n, 120)
+    dst = rng.integers(0, n, 120)
+    rp, col, irp, icol = csr_of(n, src, dst, directed, True)
+    st = stats_from_oracle(oracle, rp, col, range(n))
+    rad = oracle.radiality(rp, col)
+    assert np.array_equal(D.radiality_from_stats(st, n), rad, equal_nan=True)
+    deg = np.diff(rp) + (np.diff(irp) if directed else 0)
+    active = np.nonzero(deg > 0)[0]
+    for normalize in (True, False):
+        want = oracle.closeness(rp, col, irp, directed, normalize=normalize)
+        assert D.closeness_from_stats(st[active], n, active, normalize).tolist() == want.tolist()
+    assert D.ecc_from_stats(st, n).tolist() == oracle.eccentricity(rp, col, range(n)).tolist()
