@@ -185,11 +185,11 @@ def bfs_bench(args, lgmi, _lib, orc):
     return result
 
 
-def read_traffic():
-    """HBM bytes per launch of bfs_level from the committed rocprofv3 --pmc
+def read_traffic(name="bfs_level_rmat22_pmc.json"):
+    """HBM bytes per launch of a kernel from its committed rocprofv3 --pmc
     summary (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel factor), or
     None when no summary is present."""
-    p = os.path.join(REPO, "profiles", "bfs_level_rmat22_pmc.json")
+    p = os.path.join(REPO, "profiles", name)
     if not os.path.exists(p):
         return None
     with open(p) as f:
@@ -264,7 +264,8 @@ def bc_bench(args, lgmi, _lib, orc):
         "launch": launch,
         "roofline": {"bound": "hbm", "kernel": "bc_persist" if stats[-1]["launch"] else "bc_forward/bc_backward",
                      "achieved": round(bc_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(bc_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(bc_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": read_traffic("bc_persist_rmat20_pmc.json") if stats[-1]["launch"] else None,
                      "bytes_model": "per source 16 B per shortest-path DAG edge + 24 B per reached vertex, "
                                     "DAG edges from a 16-source sample"},
         "sample_max_rel_err": err,
