@@ -212,6 +212,28 @@ int lgmi_betweenness(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int64_t 
  * overflow as Julia's Int does; exact. */
 int lgmi_stress(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int64_t* stress);
 
+/* connected_components!(label, g) / connected_components(g)
+ * (src/connectivity.jl:13-95; is_connected :113-114): label[v] = the
+ * smallest vertex id (0-based) of v's component -- weak components for a
+ * directed graph, as the reference's all_neighbors gives -- which is the
+ * label the reference's outer loop assigns.  The shim forms the component
+ * lists from it (components, :52-64). */
+int lgmi_connected_components(lgmi_graph_t g, int32_t* label);
+
+/* has_path(g, u, v; exclude_vertices) (src/traversals/bfs.jl:150-172):
+ * *result = 1 when v is reachable from u over out-neighbours without
+ * entering an excluded vertex, or u == v; 0 when not, or when u or v is
+ * excluded. */
+int lgmi_has_path(lgmi_graph_t g, int32_t u, int32_t v, const int32_t* exclude, int32_t nexclude,
+                  int* result);
+
+/* period(g) for a directed graph (src/connectivity.jl:495-516): LGMI_E_ARG
+ * ("Graph must be strongly connected") unless vertex 0 reaches every vertex
+ * and every vertex reaches it; else the gcd over the edges outside the BFS
+ * tree from vertex 0 of level[src] - level[dst] + 1 (1 when there is a
+ * self-loop). */
+int lgmi_period(lgmi_graph_t g, int64_t* period);
+
 /* Shortest-path distance statistics of every listed source (unit edge
  * lengths), 64 sources per lane group, forward pass only: for source
  * srcs[i], stats[3i] = the sum of the distances to the vertices it reaches,

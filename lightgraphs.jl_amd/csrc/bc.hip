@@ -57,13 +57,16 @@ struct BcCtrl {
 // times out (the blocks of a launch were not all resident), overflow when a
 // slot needs more levels than F holds; either makes the host redo the
 // computation level by level.
+// Each word that blocks update concurrently has a 128-byte line of its own:
+// measured on gfx950, stores into a line that other work-groups update with
+// atomics can lose those updates.
 struct BcPersistCtl {
-    uint32_t arrive;      // blocks that have started (reset before each launch)
-    uint32_t closed;      // participants + 1 once fixed, 0 before
-    uint32_t bar_count;
-    uint32_t bar_gen;
-    int32_t abort;
-    int32_t overflow;
+    alignas(128) uint32_t arrive;   // blocks that have started (reset before each launch)
+    alignas(128) uint32_t closed;   // participants + 1 once fixed, 0 before
+    alignas(128) uint32_t bar_count;
+    alignas(128) uint32_t bar_gen;
+    alignas(128) int32_t abort;
+    alignas(128) int32_t overflow;  // the rest: written by block 0 only, read by the host
     int32_t levels;
     int32_t barriers;
     uint64_t t_fwd;
@@ -722,7 +725,7 @@ __device__ bool grid_sync(BcPersistCtl* c, uint32_t* rel, uint32_t vb, uint32_t 
         if (ok) {
             const uint32_t prev = __hip_atomic_fetch_add(&c->bar_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
             if (prev == nblocks - 1) {
-                __hip_atomic_store(&c->bar_count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                atomicExch(&c->bar_count, 0u);
                 last = 1;
             }
         }

@@ -1111,6 +1111,15 @@ void bfs_free(BfsWork* w) {
     delete w;
 }
 
+// Marks the wall vertices visited before the first level, so that no level
+// claims them (has_path's exclude_vertices, src/traversals/bfs.jl:150-157).
+__global__ void bfs_walls(StepArgs a, const int32_t* walls, int32_t n) {
+    for (int32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int32_t x = walls[i];
+        if (x >= 0 && x < a.nv) atomicOr(&a.vis[x >> 5], 1u << (x & 31));
+    }
+}
+
 void bfs_prepare(lgmi_graph* g) {
     if (g->bfs) return;
     BfsWork* w = new BfsWork();
@@ -1183,7 +1192,7 @@ void policy_upload(lgmi_graph* g) {
 }
 
 void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_src, int dir_in,
-             int want_parents, int forced_dir, lgmi_bfs_stats* stats) {
+             int want_parents, int forced_dir, lgmi_bfs_stats* stats, const int32_t* h_walls, int32_t nwalls) {
     bfs_prepare(g);
     BfsWork* w = g->bfs;
     if (nsrc > g->nv) throw Error(LGMI_E_ARG, "source list longer than nv (de-duplicate first)");
@@ -1209,6 +1218,11 @@ void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_
     bfs_init<<<init_grid, 256, 0, s>>>(a, single_src, nnz_scan, forced_dir, mode, alpha, beta);
     if (single_src < 0)
         bfs_seed<<<1, 256, 0, s>>>(a, w->srcbuf, nsrc, nnz_scan, forced_dir, mode, alpha, beta);
+    if (nwalls > 0) {
+        if (single_src < 0) throw Error(LGMI_E_ARG, "walls need a single source");
+        LGMI_HIP(hipMemcpyAsync(w->srcbuf, h_walls, sizeof(int32_t) * (size_t)nwalls, hipMemcpyHostToDevice, s));
+        bfs_walls<<<std::max(1, std::min(w->grid, (nwalls + 255) / 256)), 256, 0, s>>>(a, w->srcbuf, nwalls);
+    }
     LGMI_HIP(hipGetLastError());
     // Enqueue level launches in batches (even-sized, so that a captured
     // graph always starts on an even level); a launch after the last level

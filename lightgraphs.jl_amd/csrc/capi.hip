@@ -377,6 +377,56 @@ int lgmi_betweenness(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int64_t 
     LGMI_API_END
 }
 
+int lgmi_connected_components(lgmi_graph_t g, int32_t* label) {
+    LGMI_API_BEGIN
+    if (!g || !label) throw Error(LGMI_E_ARG, "bad arguments");
+    std::lock_guard<std::mutex> lk(g->mu);
+    connected_components(g, label);
+    LGMI_API_END
+}
+
+int lgmi_has_path(lgmi_graph_t g, int32_t u, int32_t v, const int32_t* exclude, int32_t nexclude, int* result) {
+    LGMI_API_BEGIN
+    if (!g || !result || nexclude < 0 || (nexclude > 0 && !exclude)) throw Error(LGMI_E_ARG, "bad arguments");
+    std::lock_guard<std::mutex> lk(g->mu);
+    check_src(g, u);
+    check_src(g, v);
+    std::vector<int32_t> ex(exclude, exclude + nexclude);
+    for (int32_t x : ex) check_src(g, x);
+    std::sort(ex.begin(), ex.end());
+    ex.erase(std::unique(ex.begin(), ex.end()), ex.end());
+    // bfs.jl:153-157: an excluded end is no path; u == v is a path
+    if (std::binary_search(ex.begin(), ex.end(), u) || std::binary_search(ex.begin(), ex.end(), v)) {
+        *result = 0;
+        return LGMI_OK;
+    }
+    if (u == v) {
+        *result = 1;
+        return LGMI_OK;
+    }
+    bfs_run(g, nullptr, 1, u, 0, 0, -1, nullptr, ex.data(), (int32_t)ex.size());
+    int32_t dv = -1;
+    LGMI_HIP(hipMemcpy(&dv, g->bfs->dist + v, sizeof(int32_t), hipMemcpyDeviceToHost));
+    *result = dv >= 0 ? 1 : 0;
+    LGMI_API_END
+}
+
+int lgmi_period(lgmi_graph_t g, int64_t* period) {
+    LGMI_API_BEGIN
+    if (!g || !period) throw Error(LGMI_E_ARG, "bad arguments");
+    std::lock_guard<std::mutex> lk(g->mu);
+    if (!g->directed) throw Error(LGMI_E_ARG, "period is defined for directed graphs");
+    if (g->nv == 0) throw Error(LGMI_E_ARG, "Graph must be strongly connected");
+    // is_strongly_connected: vertex 1 reaches every vertex and every vertex reaches it
+    lgmi_bfs_stats st{};
+    bfs_run(g, nullptr, 1, 0, 1, 0, -1, &st);
+    if (st.reached != g->nv) throw Error(LGMI_E_ARG, "Graph must be strongly connected");
+    bfs_run(g, nullptr, 1, 0, 0, 0, -1, &st);
+    if (st.reached != g->nv) throw Error(LGMI_E_ARG, "Graph must be strongly connected");
+    *period = period_from_levels(g, g->bfs->dist);
+    LGMI_API_END
+}
+
 int lgmi_distance_stats(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int64_t* stats) {
     LGMI_API_BEGIN
     if (!g || !stats || nsrc < 0 || (nsrc > 0 && !srcs)) throw Error(LGMI_E_ARG, "bad arguments");

@@ -191,7 +191,8 @@ namespace lgmi {
 // bfs.hip
 void bfs_prepare(lgmi_graph* g);
 void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_src, int dir_in,
-             int want_parents, int forced_dir, lgmi_bfs_stats* stats);
+             int want_parents, int forced_dir, lgmi_bfs_stats* stats, const int32_t* h_walls = nullptr,
+             int32_t nwalls = 0);
 void bfs_fetch(lgmi_graph* g, int32_t* dist, int32_t* parent);
 void bfs_free(BfsWork* w);
 void policy_upload(lgmi_graph* g);
@@ -206,6 +207,9 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
 void bc_free(BcWork* w);
 void bc_graph_free(BcGraph* b);
 void bc_set_launch(int mode);
+// conn.hip
+void connected_components(lgmi_graph* g, int32_t* label);
+int64_t period_from_levels(lgmi_graph* g, const int32_t* d_dist);
 // global policy state (host copy), set by lgmi_bfs_set_policy
 struct PolicyHost {
     int mode = LGMI_POLICY_HEURISTIC;

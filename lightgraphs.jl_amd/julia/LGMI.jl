@@ -18,7 +18,8 @@ using LightGraphs
 using LightGraphs: AbstractGraph, SimpleGraph, SimpleDiGraph, DefaultDistance, nv, ne,
     is_directed, vertices, degree, sample
 import LightGraphs: gdistances, gdistances!, bfs_parents, bfs_tree, betweenness_centrality, stress_centrality,
-    closeness_centrality, radiality_centrality, eccentricity, diameter, radius, center, periphery
+    closeness_centrality, radiality_centrality, eccentricity, diameter, radius, center, periphery,
+    connected_components, connected_components!, is_connected, has_path, period
 import LightGraphs.Parallel
 
 const liblgmi = joinpath(@__DIR__, "..", "lgmi", "liblgmi.so")
@@ -246,6 +247,46 @@ function radiality_centrality(g::SimpleG)::Vector{Float64}
     dmtr = max(0.0, maximum(ecc_of(st, n_v)))
     meandists = wrapped_sum(st, n_v) ./ (n_v - 1)
     return ((dmtr + 1) .- meandists) ./ dmtr
+end
+
+# ---------------------------------------------------------------------
+# connectivity  (src/connectivity.jl:13-134, 495-516; has_path
+# src/traversals/bfs.jl:150-172)
+# ---------------------------------------------------------------------
+function connected_components!(label::AbstractVector, g::SimpleG{T}) where T
+    lab = Vector{Int32}(undef, nv(g))
+    nv(g) > 0 && check(ccall((:lgmi_connected_components, liblgmi), Cint, (Ptr{Cvoid}, Ptr{Int32}),
+                             Handle(g).ptr, lab))
+    for v in eachindex(lab)
+        label[v] = lab[v] + 1          # the smallest vertex id of v's component
+    end
+    return label
+end
+
+function connected_components(g::SimpleG{T}) where T
+    label = zeros(T, nv(g))
+    connected_components!(label, g)
+    c, d = LightGraphs.components(label)
+    return c
+end
+
+is_connected(g::SimpleG) = (is_directed(g) ? 2 : 1) * ne(g) + 1 >= nv(g) && length(connected_components(g)) == 1
+
+function has_path(g::SimpleG{T}, u::Integer, v::Integer; exclude_vertices::AbstractVector = Vector{T}()) where T
+    for x in (u, v)
+        1 <= x <= nv(g) || throw(BoundsError(vertices(g), x))
+    end
+    ex = Int32[x - 1 for x in exclude_vertices]
+    res = Ref{Cint}(0)
+    check(ccall((:lgmi_has_path, liblgmi), Cint, (Ptr{Cvoid}, Int32, Int32, Ptr{Int32}, Int32, Ptr{Cint}),
+                Handle(g).ptr, u - 1, v - 1, ex, length(ex), res))
+    return res[] != 0
+end
+
+function period(g::SimpleDiGraph)
+    p = Ref{Int64}(0)
+    check(ccall((:lgmi_period, liblgmi), Cint, (Ptr{Cvoid}, Ptr{Int64}), Handle(g).ptr, p))   # ArgumentError if not strongly connected
+    return p[]
 end
 
 # ---------------------------------------------------------------------

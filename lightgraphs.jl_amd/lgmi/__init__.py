@@ -12,12 +12,16 @@ hand-written gfx950 HIP kernels in liblgmi.so through its C ABI
   ``diameter``, ``radius``, ``center``, ``periphery``
                                     -- src/centrality/{closeness,radiality}.jl,
                                        src/distance.jl
+* ``connected_components``, ``is_connected``, ``has_path``, ``period``
+                                    -- src/connectivity.jl, src/traversals/bfs.jl
 * ``Parallel.*``                    -- src/Parallel/{traversals,centrality}
 * ``SimpleGraph``, ``SimpleDiGraph`` -- src/SimpleGraphs (CSR-backed)
 """
 from . import Parallel, generators
 from ._lib import LgmiError, NoDeviceError
 from .centrality import betweenness_centrality, betweenness_centrality_batch, sample, stress_centrality
+from .connectivity import (components, components_dict, connected_components, connected_components_,
+                           has_path, is_connected, is_weakly_connected, period)
 from .distances import (center, closeness_centrality, diameter, eccentricity, periphery, radiality_centrality,
                         radius)
 from .graphs import SimpleDiGraph, SimpleGraph
@@ -27,5 +31,6 @@ __all__ = [
     "Parallel", "generators", "LgmiError", "NoDeviceError", "betweenness_centrality", "sample",
     "SimpleDiGraph", "SimpleGraph", "bfs_parents", "bfs_tree", "gdistances", "gdistances_", "stress_centrality",
     "tree", "typemax", "center", "closeness_centrality", "diameter", "eccentricity", "periphery",
-    "radiality_centrality", "radius",
+    "radiality_centrality", "radius", "components", "components_dict", "connected_components",
+    "connected_components_", "has_path", "is_connected", "is_weakly_connected", "period",
 ]

@@ -237,6 +237,83 @@ def radiality(row_ptr, col):
     return meandists / dmtr
 
 
+def connected_component_labels(row_ptr, col, in_row_ptr, in_col, directed):
+    """connected_components!(label, g) (src/connectivity.jl:13-34), 0-based
+    labels: the outer loop over vertices in order, a FIFO over
+    all_neighbors (out- and in-neighbours for a directed graph)."""
+    nv = len(row_ptr) - 1
+    label = [-1] * nv
+    for u in range(nv):
+        if label[u] >= 0:
+            continue
+        label[u] = u
+        q = [u]
+        head = 0
+        while head < len(q):
+            src = q[head]
+            head += 1
+            nbrs = list(col[row_ptr[src]:row_ptr[src + 1]])
+            if directed:
+                nbrs += list(in_col[in_row_ptr[src]:in_row_ptr[src + 1]])
+            for x in nbrs:
+                if label[x] < 0:
+                    q.append(int(x))
+                    label[x] = u
+    return np.array(label, dtype=np.int64)
+
+
+def has_path(row_ptr, col, u, v, exclude=()):
+    """has_path(g, u, v; exclude_vertices) (src/traversals/bfs.jl:150-172),
+    0-based."""
+    nv = len(row_ptr) - 1
+    seen = [False] * nv
+    for x in exclude:
+        seen[x] = True
+    if seen[u] or seen[v]:
+        return False
+    if u == v:
+        return True
+    q = [u]
+    seen[u] = True
+    head = 0
+    while head < len(q):
+        src = q[head]
+        head += 1
+        for x in col[row_ptr[src]:row_ptr[src + 1]]:
+            if x == v:
+                return True
+            if not seen[x]:
+                q.append(int(x))
+                seen[x] = True
+    return False
+
+
+def period(row_ptr, col, in_row_ptr, in_col):
+    """period(g) of a directed graph (src/connectivity.jl:497-516): raises
+    ValueError unless strongly connected; 1 with a self-loop; else the gcd
+    over the edges outside bfs_tree(g, 1) of levels[s] - levels[d] + 1."""
+    import math
+    nv = len(row_ptr) - 1
+    fwd = gdistances(row_ptr, col, [0])
+    bwd = gdistances(in_row_ptr, in_col, [0])
+    if nv == 0 or (fwd < 0).any() or (bwd < 0).any():
+        raise ValueError("Graph must be strongly connected")
+    src = np.repeat(np.arange(nv), np.diff(np.asarray(row_ptr)))
+    if (src == np.asarray(col)).any():
+        return 1
+    par = bfs_parents(row_ptr, col, 0)
+    tree = {(int(par[v]), v) for v in range(nv) if v != 0}
+    # levels of the tree from vertex 1 are the BFS distances
+    divisor = 0
+    for s, d in zip(src.tolist(), np.asarray(col).tolist()):
+        if (s, d) in tree:
+            continue
+        divisor = math.gcd(divisor, int(fwd[s]) - int(fwd[d]) + 1)
+        if divisor == 1:
+            return 1
+    return divisor
+
+
 def rescale(bc, n, normalize, directed, k):
     lib = _load()
     bc = np.ascontiguousarray(bc, dtype=np.float64).copy()

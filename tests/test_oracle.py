@@ -476,3 +476,87 @@ def test_distance_host_formulas_vs_oracle(oracle, seed, directed):
         want = oracle.closeness(rp, col, irp, directed, normalize=normalize)
         assert D.closeness_from_stats(st[active], n, active, normalize).tolist() == want.tolist()
     assert D.ecc_from_stats(st, n).tolist() == oracle.eccentricity(rp, col, range(n)).tolist()
+
+
+# ----------------------------------------------------------------------
+# connectivity: connected_components, has_path, period
+# ----------------------------------------------------------------------
+
+def test_connected_components_reference_case(oracle):
+    """test/connectivity.jl:2-22 (path_graph(4) + 10 vertices + edges)."""
+    from lgmi import connectivity as CC
+    edges = [(1, 2), (2, 3), (3, 4), (5, 6), (6, 7), (8, 9), (10, 9)]
+    n, d, (rp, col, irp, icol) = mk((14, False, edges))
+    lab = oracle.connected_component_labels(rp, col, irp, icol, False) + 1
+    assert lab[:10].tolist() == [1, 1, 1, 1, 5, 5, 5, 8, 8, 8]
+    cd = CC.components_dict(lab)
+    assert cd[1] == [1, 2, 3, 4] and cd[5] == [5, 6, 7] and cd[8] == [8, 9, 10]
+    c, _ = CC.components(lab)
+    assert len(c) >= 3 and sorted(c[2]) == [8, 9, 10]
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_connected_components_vs_scipy(oracle, directed):
+    """Labels against scipy's (weak) components mapped to each component's
+    smallest vertex id."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import connected_components as sp_cc
+    rng = np.random.default_rng(41)
+    n = 500
+    src = rng.integers(0, n, 420)
+    dst = rng.integers(0, n, 420)
+    rp, col, irp, icol = csr_of(n, src, dst, directed, True)
+    lab = oracle.connected_component_labels(rp, col, irp, icol, directed)
+    m = csr_matrix((np.ones(len(col)), np.asarray(col), np.asarray(rp)), shape=(n, n))
+    _, sl = sp_cc(m, directed=directed, connection="weak")
+    mins = {}
+    for v, l in enumerate(sl.tolist()):
+        mins.setdefault(l, v)
+    assert lab.tolist() == [mins[l] for l in sl.tolist()]
+
+
+def test_has_path_reference_cases(oracle):
+    """test/traversals/bfs.jl:48-72."""
+    n, d, (rp, col, irp, _) = mk((6, False, [(1, 2), (2, 3), (2, 4), (4, 5), (3, 5)]))
+    hp = lambda u, v, ex=(): oracle.has_path(rp, col, u - 1, v - 1, [x - 1 for x in ex])  # noqa: E731
+    assert hp(1, 5) and hp(1, 2) and hp(1, 5, [3]) and hp(1, 5, [4]) and not hp(1, 5, [3, 4])
+    assert hp(5, 1) and hp(5, 1, [3]) and hp(5, 1, [4]) and not hp(5, 1, [3, 4])
+    assert not hp(1, 6) and not hp(6, 1) and hp(1, 1)
+    assert not hp(1, 2, [2]) and not hp(1, 2, [1])
+
+
+def test_period_reference_cases(oracle):
+    """test/connectivity.jl:206-222: cycle_digraph(10) has period 10, with the
+    shortcut 1 -> 4 period 2; a graph that is not strongly connected raises."""
+    ring = [(i, i % 10 + 1) for i in range(1, 11)]
+    n, d, (rp, col, irp, icol) = mk((10, True, ring))
+    assert oracle.period(rp, col, irp, icol) == 10
+    n, d, (rp, col, irp, icol) = mk((10, True, ring + [(1, 4)]))
+    assert oracle.period(rp, col, irp, icol) == 2
+    n, d, (rp, col, irp, icol) = mk((3, True, [(1, 2), (2, 3)]))
+    with pytest.raises(ValueError):
+        oracle.period(rp, col, irp, icol)
+
+
+@pytest.mark.parametrize("seed", [42, 43, 44])
+def test_period_root_invariance(oracle, seed):
+    """The period of a strongly connected digraph is the gcd over all edges
+    of level[s] - level[d] + 1 for levels from any root (the tree edges give
+    0): the restatement from vertex 1 equals the all-edge gcd from the last
+    vertex, on cycles of several lengths joined by chords."""
+    import math
+    rng = np.random.default_rng(seed)
+    k = int(rng.integers(2, 6))
+    n = k * int(rng.integers(3, 12))
+    ring = [(i, (i + 1) % n) for i in range(n)]
+    # chords that keep the period a multiple of k
+    chords = [(int(a), int((a + k * rng.integers(1, n // k) + 1) % n)) for a in rng.integers(0, n, 5)]
+    e = np.array(ring + chords)
+    rp, col, irp, icol = csr_of(n, e[:, 0], e[:, 1], True, True)
+    p1 = oracle.period(rp, col, irp, icol)
+    d = oracle.gdistances(rp, col, [n - 1]).astype(np.int64)
+    src = np.repeat(np.arange(n), np.diff(rp))
+    g = 0
+    for s_, d_ in zip(src.tolist(), np.asarray(col).tolist()):
+        g = math.gcd(g, int(d[s_] - d[d_] + 1))
+    assert p1 == g and p1 % k == 0
