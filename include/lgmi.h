@@ -227,6 +227,16 @@ int lgmi_connected_components(lgmi_graph_t g, int32_t* label);
 int lgmi_has_path(lgmi_graph_t g, int32_t u, int32_t v, const int32_t* exclude, int32_t nexclude,
                   int* result);
 
+/* neighborhood_dists(g, v, d; dir) / neighborhood(g, v, d; dir) with unit
+ * lengths (src/connectivity.jl:634-702): the vertices within distance d of
+ * v (d < 0: none) in the reference's FIFO queue order -- by level, and
+ * within a level by the queue position of the first vertex of the previous
+ * level that reaches them, then by adjacency order -- over out-neighbours,
+ * or in-neighbours when dir_in.  verts (and dists when not NULL) need room
+ * for nv entries; *count gets the number written. */
+int lgmi_neighborhood(lgmi_graph_t g, int32_t v, int64_t d, int dir_in, int32_t* verts, int32_t* dists,
+                      int32_t* count);
+
 /* period(g) for a directed graph (src/connectivity.jl:495-516): LGMI_E_ARG
  * ("Graph must be strongly connected") unless vertex 0 reaches every vertex
  * and every vertex reaches it; else the gcd over the edges outside the BFS

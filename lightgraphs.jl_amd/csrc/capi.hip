@@ -411,6 +411,16 @@ int lgmi_has_path(lgmi_graph_t g, int32_t u, int32_t v, const int32_t* exclude, 
     LGMI_API_END
 }
 
+int lgmi_neighborhood(lgmi_graph_t g, int32_t v, int64_t d, int dir_in, int32_t* verts, int32_t* dists,
+                      int32_t* count) {
+    LGMI_API_BEGIN
+    if (!g || !verts || !count) throw Error(LGMI_E_ARG, "bad arguments");
+    std::lock_guard<std::mutex> lk(g->mu);
+    check_src(g, v);
+    *count = neighborhood(g, v, d, dir_in, verts, dists);
+    LGMI_API_END
+}
+
 int lgmi_period(lgmi_graph_t g, int64_t* period) {
     LGMI_API_BEGIN
     if (!g || !period) throw Error(LGMI_E_ARG, "bad arguments");

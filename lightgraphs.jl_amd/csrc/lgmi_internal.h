@@ -210,6 +210,7 @@ void bc_set_launch(int mode);
 // conn.hip
 void connected_components(lgmi_graph* g, int32_t* label);
 int64_t period_from_levels(lgmi_graph* g, const int32_t* d_dist);
+int32_t neighborhood(lgmi_graph* g, int32_t v, int64_t d, int dir_in, int32_t* verts, int32_t* dists);
 // global policy state (host copy), set by lgmi_bfs_set_policy
 struct PolicyHost {
     int mode = LGMI_POLICY_HEURISTIC;

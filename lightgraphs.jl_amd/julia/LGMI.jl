@@ -19,7 +19,7 @@ using LightGraphs: AbstractGraph, SimpleGraph, SimpleDiGraph, DefaultDistance, n
     is_directed, vertices, degree, sample
 import LightGraphs: gdistances, gdistances!, bfs_parents, bfs_tree, betweenness_centrality, stress_centrality,
     closeness_centrality, radiality_centrality, eccentricity, diameter, radius, center, periphery,
-    connected_components, connected_components!, is_connected, has_path, period
+    connected_components, connected_components!, is_connected, has_path, period, neighborhood, neighborhood_dists
 import LightGraphs.Parallel
 
 const liblgmi = joinpath(@__DIR__, "..", "lgmi", "liblgmi.so")
@@ -282,6 +282,21 @@ function has_path(g::SimpleG{T}, u::Integer, v::Integer; exclude_vertices::Abstr
                 Handle(g).ptr, u - 1, v - 1, ex, length(ex), res))
     return res[] != 0
 end
+
+function neighborhood_dists(g::SimpleG{T}, v::Integer, d, distmx::DefaultDistance = weights(g); dir = :out) where T
+    1 <= v <= nv(g) || throw(BoundsError(vertices(g), v))
+    d < 0 && return Tuple{T,Int}[]
+    dd = isinf(d) ? nv(g) : min(floor(Int64, d), nv(g))
+    verts = Vector{Int32}(undef, nv(g))
+    dists = Vector{Int32}(undef, nv(g))
+    cnt = Ref{Int32}(0)
+    check(ccall((:lgmi_neighborhood, liblgmi), Cint,
+                (Ptr{Cvoid}, Int32, Int64, Cint, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}),
+                Handle(g).ptr, v - 1, dd, dir == :out ? 0 : 1, verts, dists, cnt))
+    return [(T(verts[i] + 1), Int(dists[i])) for i in 1:cnt[]]     # FIFO queue order, as _neighborhood
+end
+neighborhood(g::SimpleG, v::Integer, d, distmx::DefaultDistance = weights(g); dir = :out) =
+    first.(neighborhood_dists(g, v, d, distmx; dir = dir))
 
 function period(g::SimpleDiGraph)
     p = Ref{Int64}(0)

@@ -21,7 +21,7 @@ from . import Parallel, generators
 from ._lib import LgmiError, NoDeviceError
 from .centrality import betweenness_centrality, betweenness_centrality_batch, sample, stress_centrality
 from .connectivity import (components, components_dict, connected_components, connected_components_,
-                           has_path, is_connected, is_weakly_connected, period)
+                           has_path, is_connected, is_weakly_connected, neighborhood, neighborhood_dists, period)
 from .distances import (center, closeness_centrality, diameter, eccentricity, periphery, radiality_centrality,
                         radius)
 from .graphs import SimpleDiGraph, SimpleGraph
@@ -32,5 +32,6 @@ __all__ = [
     "SimpleDiGraph", "SimpleGraph", "bfs_parents", "bfs_tree", "gdistances", "gdistances_", "stress_centrality",
     "tree", "typemax", "center", "closeness_centrality", "diameter", "eccentricity", "periphery",
     "radiality_centrality", "radius", "components", "components_dict", "connected_components",
-    "connected_components_", "has_path", "is_connected", "is_weakly_connected", "period",
+    "connected_components_", "has_path", "is_connected", "is_weakly_connected", "period", "neighborhood",
+    "neighborhood_dists",
 ]

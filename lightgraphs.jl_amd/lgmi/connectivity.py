@@ -101,3 +101,30 @@ def period(g: _GraphBase) -> int:
     out = np.zeros(1, dtype=np.int64)
     _lib.check(_lib.load().lgmi_period(g.handle(), _lib.i64p(out)))
     return int(out[0])
+
+
+def neighborhood_dists(g: _GraphBase, v: int, d, distmx=None, dir: str = "out") -> list:
+    """neighborhood_dists(g, v, d; dir) (connectivity.jl:681-702) with unit
+    lengths: (vertex, distance) for every vertex within distance d of v, in
+    the reference's FIFO queue order; dir=:in follows in-neighbours."""
+    import math
+    if distmx is not None:
+        raise NotImplementedError("weighted distmx is not on the accelerated path")
+    n = g.nv()
+    if not 1 <= int(v) <= n:
+        raise IndexError(f"vertex {v} out of range 1:{n}")
+    if d < 0:
+        return []
+    dd = n if math.isinf(d) else int(math.floor(d))
+    verts = np.empty(n, dtype=np.int32)
+    dists = np.empty(n, dtype=np.int32)
+    cnt = np.zeros(1, dtype=np.int32)
+    _lib.check(_lib.load().lgmi_neighborhood(g.handle(), int(v) - 1, int(min(dd, n)), int(dir in ("in", ":in")),
+                                             _lib.i32p(verts), _lib.i32p(dists), _lib.i32p(cnt)))
+    k = int(cnt[0])
+    return list(zip((verts[:k].astype(np.int64) + 1).tolist(), dists[:k].astype(np.int64).tolist()))
+
+
+def neighborhood(g: _GraphBase, v: int, d, distmx=None, dir: str = "out") -> list:
+    """neighborhood(g, v, d; dir) (connectivity.jl:634-635)."""
+    return [x for x, _ in neighborhood_dists(g, v, d, distmx, dir)]

@@ -288,6 +288,31 @@ def has_path(row_ptr, col, u, v, exclude=()):
     return False
 
 
+def neighborhood_dists(row_ptr, col, v, d):
+    """_neighborhood (src/connectivity.jl:685-702) with unit lengths over the
+    given CSR (out- or in-neighbours), 0-based: the FIFO queue of
+    (vertex, distance)."""
+    nv = len(row_ptr) - 1
+    q = []
+    if d < 0:
+        return q
+    q.append((v, 0))
+    seen = [False] * nv
+    seen[v] = True
+    i = 0
+    while i < len(q):
+        src, cur = q[i]
+        i += 1
+        if cur >= d:
+            continue
+        for dst in col[row_ptr[src]:row_ptr[src + 1]]:
+            if not seen[dst]:
+                seen[dst] = True
+                if cur + 1 <= d:
+                    q.append((int(dst), cur + 1))
+    return q
+
+
 def period(row_ptr, col, in_row_ptr, in_col):
     """period(g) of a directed graph (src/connectivity.jl:497-516): raises
     ValueError unless strongly connected; 1 with a self-loop; else the gcd

@@ -125,3 +125,50 @@ def test_period_vs_oracle(oracle):
         irp, icol = g.in_csr()
         say(f"period k={k} n={n}")
         assert lgmi.period(g) == oracle.period(rp, col, irp, icol)
+
+
+def test_neighborhood_reference_cases():
+    import lgmi
+    e = np.array([(1, 2), (2, 3), (3, 1), (3, 4), (4, 5), (5, 4)])
+    g = lgmi.SimpleDiGraph.from_edges(5, e[:, 0], e[:, 1], one_based=True)
+    assert lgmi.neighborhood_dists(g, 1, 3) == [(1, 0), (2, 1), (3, 2), (4, 3)]
+    assert lgmi.neighborhood(g, 1, 2) == [1, 2, 3]
+    assert lgmi.neighborhood_dists(g, 4, 3) == [(4, 0), (5, 1)]
+    assert lgmi.neighborhood_dists(g, 4, 3, dir="in") == [(4, 0), (3, 1), (5, 1), (2, 2), (1, 3)]
+    star = lgmi.SimpleGraph.from_edges(10, [1] * 9, list(range(2, 11)), one_based=True)
+    assert lgmi.neighborhood_dists(star, 1, 0) == [(1, 0)]
+    assert [len(lgmi.neighborhood(star, v, dd)) for v, dd in [(1, 1), (2, 1), (1, 2), (2, 2), (2, -1)]] == [
+        10, 2, 10, 10, 0]
+    dstar = lgmi.SimpleDiGraph.from_edges(10, [1] * 9, list(range(2, 11)), one_based=True)
+    assert [len(lgmi.neighborhood(dstar, v, dd)) for v, dd in [(1, 1), (2, 1), (1, 2), (2, 2)]] == [10, 1, 10, 1]
+    assert [len(lgmi.neighborhood(dstar, v, dd, dir="in")) for v, dd in [(1, 1), (2, 1), (1, 2), (2, 2)]] == [
+        1, 2, 1, 2]
+    assert lgmi.neighborhood_dists(dstar, 1, 2.5) == lgmi.neighborhood_dists(dstar, 1, 2)
+
+
+def test_neighborhood_vs_oracle(oracle):
+    """Queue order against the oracle: random directed graph (both
+    directions) and RMAT-16 with hubs, several depths."""
+    import lgmi
+    rng = np.random.default_rng(62)
+    n = 2000
+    src = rng.integers(0, n, 6000)
+    dst = rng.integers(0, n, 6000)
+    g = lgmi.SimpleDiGraph.from_edges(n, src, dst)
+    rp, col = g.csr()
+    irp, icol = g.in_csr()
+    say("neighborhood random digraph")
+    for v in (0, 7, 1999):
+        for dd in (0, 1, 3, 50):
+            want = [(x + 1, y) for x, y in oracle.neighborhood_dists(rp, col, v, dd)]
+            assert lgmi.neighborhood_dists(g, v + 1, dd) == want
+            want = [(x + 1, y) for x, y in oracle.neighborhood_dists(irp, icol, v, dd)]
+            assert lgmi.neighborhood_dists(g, v + 1, dd, dir="in") == want
+    g = lgmi.generators.rmat(16, 16, seed=63)
+    rp, col = g.csr()
+    hub = int(np.argmax(np.diff(rp)))
+    say("neighborhood rmat16")
+    for v in (hub, 3):
+        for dd in (1, 2, 6):
+            want = [(x + 1, y) for x, y in oracle.neighborhood_dists(rp, col, v, dd)]
+            assert lgmi.neighborhood_dists(g, v + 1, dd) == want

@@ -560,3 +560,46 @@ def test_period_root_invariance(oracle, seed):
     for s_, d_ in zip(src.tolist(), np.asarray(col).tolist()):
         g = math.gcd(g, int(d[s_] - d[d_] + 1))
     assert p1 == g and p1 % k == 0
+
+
+def test_neighborhood_reference_cases(oracle):
+    """Doctests src/connectivity.jl:607-632, 646-676 and test/connectivity.jl:225-253."""
+    e = [(1, 2), (2, 3), (3, 1), (3, 4), (4, 5), (5, 4)]
+    n, d, (rp, col, irp, icol) = mk((5, True, e))
+    nd = lambda v, dd, r=rp, c=col: [(x + 1, y) for x, y in oracle.neighborhood_dists(r, c, v - 1, dd)]  # noqa: E731
+    assert nd(1, 3) == [(1, 0), (2, 1), (3, 2), (4, 3)]
+    assert [x for x, _ in nd(1, 2)] == [1, 2, 3]
+    assert nd(4, 3) == [(4, 0), (5, 1)]
+    assert nd(4, 3, irp, icol) == [(4, 0), (3, 1), (5, 1), (2, 2), (1, 3)]
+    n, d, (rp, col, irp, icol) = mk(star_spec(10))
+    assert nd(1, 0, rp, col) == [(1, 0)]
+    assert [len(nd(v, dd, rp, col)) for v, dd in [(1, 1), (2, 1), (1, 2), (2, 2), (2, -1)]] == [10, 2, 10, 10, 0]
+    n, d, (rp, col, irp, icol) = mk((10, True, [(1, i) for i in range(2, 11)]))
+    assert [len(nd(v, dd, rp, col)) for v, dd in [(1, 1), (2, 1), (1, 2), (2, 2)]] == [10, 1, 10, 1]
+    assert [len(nd(v, dd, irp, icol)) for v, dd in [(1, 1), (2, 1), (1, 2), (2, 2)]] == [1, 2, 1, 2]
+
+
+def test_neighborhood_order_vs_networkx(oracle):
+    """The queue order restated from networkx's BFS edge order (its
+    bfs_edges visits sources FIFO and neighbours in insertion order, which is
+    sorted here), cut at depth d."""
+    import networkx as nx
+    rng = np.random.default_rng(61)
+    n = 300
+    src = rng.integers(0, n, 700)
+    dst = rng.integers(0, n, 700)
+    rp, col, irp, icol = csr_of(n, src, dst, True, True)
+    G = nx.DiGraph()
+    G.add_nodes_from(range(n))
+    for u in range(n):
+        for v in col[rp[u]:rp[u + 1]]:
+            G.add_edge(u, int(v))
+    for v in (0, 5, 17):
+        for dd in (0, 1, 2, 4, 100):
+            want = [(v, 0)]
+            depth = {v: 0}
+            for a, b in nx.bfs_edges(G, v):
+                depth[b] = depth[a] + 1
+                if depth[b] <= dd:
+                    want.append((b, depth[b]))
+            assert oracle.neighborhood_dists(rp, col, v, dd) == want
