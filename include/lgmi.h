@@ -103,6 +103,24 @@ int lgmi_graph_create(int32_t nv, int64_t nnz, const int32_t* row_ptr, const int
                       int directed, lgmi_graph_t* out);
 void lgmi_graph_destroy(lgmi_graph_t g);
 
+/* SimpleGraph(edge_list) / SimpleDiGraph(edge_list) built on the device
+ *   src/SimpleGraphs/simplegraph.jl:163-244 (cleanupedges!, constructor)
+ *   src/SimpleGraphs/simpledigraph.jl:174-253
+ * from m edges (src[i], dst[i]), 0-based; an edge with a negative id is
+ * skipped (the reference skips ids below 1).  nv = max(nv_min, largest id
+ * used + 1) (the reference: nv_min = 0).  Undirected: each edge enters both
+ * rows, a self-loop once; rows sorted, duplicates removed; *ne = the
+ * reference's ne (an edge once, a self-loop once).  Directed: out- and
+ * in-CSR, *ne = the number of distinct edges.  The CSR is built and kept on
+ * the device; *out is a handle like lgmi_graph_create's. */
+int lgmi_graph_from_edges(const int32_t* src, const int32_t* dst, int64_t m, int directed, int32_t nv_min,
+                          lgmi_graph_t* out, int64_t* ne);
+/* nv and the adjacency-entry count of a handle; the CSR copied to the host
+ * (in_row_ptr / in_col may be NULL; for an undirected graph they repeat the
+ * out-CSR). */
+int lgmi_graph_size(lgmi_graph_t g, int32_t* nv, int64_t* nnz);
+int lgmi_graph_csr(lgmi_graph_t g, int32_t* row_ptr, int32_t* col, int32_t* in_row_ptr, int32_t* in_col);
+
 /* gdistances(g, source) / gdistances!(g, source, vert_level)
  *   src/traversals/bfs.jl:95-141
  * Parallel.gdistances(g, sources) / Parallel.gdistances!(...)

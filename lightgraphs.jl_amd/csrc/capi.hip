@@ -108,10 +108,17 @@ static void check_csr(int32_t nv, int64_t nnz, const int32_t* rp, const int32_t*
 static void upload_csr(lgmi_graph* g, DevCSR& d, const int32_t* rp, const int32_t* col, int64_t nnz) {
     d.row_ptr = dev_alloc_n<int32_t>((size_t)g->nv + 1);
     d.col = dev_alloc_n<int32_t>((size_t)std::max<int64_t>(nnz, 1));
-    const int32_t nwords = 2 * ((g->nv + 63) / 64);
-    d.nz_bm = dev_alloc_n<uint32_t>((size_t)std::max(nwords, 1));
     LGMI_HIP(hipMemcpyAsync(d.row_ptr, rp, sizeof(int32_t) * ((size_t)g->nv + 1), hipMemcpyHostToDevice, g->stream));
     if (nnz > 0) LGMI_HIP(hipMemcpyAsync(d.col, col, sizeof(int32_t) * (size_t)nnz, hipMemcpyHostToDevice, g->stream));
+    finish_csr(g, d, rp);
+}
+
+// The derived structures of a CSR whose row_ptr and col are on the device
+// (rp: a host copy of row_ptr): non-empty-row bitmap, heavy-row items and
+// the range table of the BFS.
+void finish_csr(lgmi_graph* g, DevCSR& d, const int32_t* rp) {
+    const int32_t nwords = 2 * ((g->nv + 63) / 64);
+    d.nz_bm = dev_alloc_n<uint32_t>((size_t)std::max(nwords, 1));
     if (nwords > 0) {
         nz_bitmap_kernel<<<(nwords + 255) / 256, 256, 0, g->stream>>>(d.row_ptr, g->nv, nwords, d.nz_bm);
         LGMI_HIP(hipGetLastError());
@@ -189,6 +196,8 @@ static void graph_free(lgmi_graph* g) {
     delete g;
 }
 
+void graph_release(lgmi_graph* g) { graph_free(g); }
+
 }  // namespace lgmi
 
 using namespace lgmi;
@@ -263,6 +272,37 @@ int lgmi_graph_create(int32_t nv, int64_t nnz, const int32_t* row_ptr, const int
 }
 
 void lgmi_graph_destroy(lgmi_graph_t g) { graph_free(g); }
+
+int lgmi_graph_from_edges(const int32_t* src, const int32_t* dst, int64_t m, int directed, int32_t nv_min,
+                          lgmi_graph_t* out, int64_t* ne) {
+    LGMI_API_BEGIN
+    if (!out || !ne || m < 0 || (m > 0 && (!src || !dst)) || nv_min < 0) throw Error(LGMI_E_ARG, "bad arguments");
+    *out = nullptr;
+    ensure_device();
+    *out = graph_from_edges(src, dst, m, directed, nv_min, ne);
+    LGMI_API_END
+}
+
+int lgmi_graph_size(lgmi_graph_t g, int32_t* nv, int64_t* nnz) {
+    LGMI_API_BEGIN
+    if (!g || !nv || !nnz) throw Error(LGMI_E_ARG, "bad arguments");
+    *nv = g->nv;
+    *nnz = g->nnz;
+    LGMI_API_END
+}
+
+int lgmi_graph_csr(lgmi_graph_t g, int32_t* row_ptr, int32_t* col, int32_t* in_row_ptr, int32_t* in_col) {
+    LGMI_API_BEGIN
+    if (!g || !row_ptr || !col) throw Error(LGMI_E_ARG, "bad arguments");
+    std::lock_guard<std::mutex> lk(g->mu);
+    LGMI_HIP(hipMemcpy(row_ptr, g->out.row_ptr, sizeof(int32_t) * ((size_t)g->nv + 1), hipMemcpyDeviceToHost));
+    if (g->nnz > 0) LGMI_HIP(hipMemcpy(col, g->out.col, sizeof(int32_t) * (size_t)g->nnz, hipMemcpyDeviceToHost));
+    if (in_row_ptr && in_col) {
+        LGMI_HIP(hipMemcpy(in_row_ptr, g->in.row_ptr, sizeof(int32_t) * ((size_t)g->nv + 1), hipMemcpyDeviceToHost));
+        if (g->nnz > 0) LGMI_HIP(hipMemcpy(in_col, g->in.col, sizeof(int32_t) * (size_t)g->nnz, hipMemcpyDeviceToHost));
+    }
+    LGMI_API_END
+}
 
 static void check_src(lgmi_graph_t g, int32_t s) {
     if (s < 0 || s >= g->nv) throw Error(LGMI_E_BOUNDS, "source vertex " + std::to_string(s) + " out of range");

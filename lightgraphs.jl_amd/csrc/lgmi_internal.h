@@ -207,6 +207,12 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
 void bc_free(BcWork* w);
 void bc_graph_free(BcGraph* b);
 void bc_set_launch(int mode);
+// capi.hip
+void finish_csr(lgmi_graph* g, DevCSR& d, const int32_t* rp);
+void graph_release(lgmi_graph* g);
+// ingest.hip
+lgmi_graph* graph_from_edges(const int32_t* h_src, const int32_t* h_dst, int64_t m, int directed, int32_t nv_min,
+                             int64_t* ne_out);
 // conn.hip
 void connected_components(lgmi_graph* g, int32_t* label);
 int64_t period_from_levels(lgmi_graph* g, const int32_t* d_dist);

@@ -250,6 +250,33 @@ function radiality_centrality(g::SimpleG)::Vector{Float64}
 end
 
 # ---------------------------------------------------------------------
+# device-side ingest: SimpleGraph(edge_list) / SimpleDiGraph(edge_list)
+# (src/SimpleGraphs/simplegraph.jl:208-244, simpledigraph.jl:212-253).
+# The CSR is sorted and de-duplicated on the GPU; the handle is kept for the
+# calls above and the fadjlist is filled from the CSR it returns.
+# ---------------------------------------------------------------------
+function device_graph(edge_list::Vector{<:LightGraphs.AbstractEdge{T}}, directed::Bool) where T
+    s0 = Int32[src(e) - 1 for e in edge_list]
+    d0 = Int32[dst(e) - 1 for e in edge_list]
+    ptr = Ref{Ptr{Cvoid}}(C_NULL)
+    ne_ = Ref{Int64}(0)
+    check(ccall((:lgmi_graph_from_edges, liblgmi), Cint,
+                (Ptr{Int32}, Ptr{Int32}, Int64, Cint, Int32, Ptr{Ptr{Cvoid}}, Ptr{Int64}),
+                s0, d0, length(s0), directed, 0, ptr, ne_))
+    n = Ref{Int32}(0); nnz = Ref{Int64}(0)
+    check(ccall((:lgmi_graph_size, liblgmi), Cint, (Ptr{Cvoid}, Ptr{Int32}, Ptr{Int64}), ptr[], n, nnz))
+    rp = Vector{Int32}(undef, n[] + 1); col = Vector{Int32}(undef, nnz[])
+    irp = Vector{Int32}(undef, n[] + 1); icol = Vector{Int32}(undef, nnz[])
+    check(ccall((:lgmi_graph_csr, liblgmi), Cint, (Ptr{Cvoid}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}),
+                ptr[], rp, col, irp, icol))
+    ccall((:lgmi_graph_destroy, liblgmi), Cvoid, (Ptr{Cvoid},), ptr[])
+    rows(p, c) = [T[c[k] + 1 for k in (p[v] + 1):p[v + 1]] for v in 1:n[]]
+    return directed ? SimpleDiGraph{T}(ne_[], rows(rp, col), rows(irp, icol)) : SimpleGraph{T}(ne_[], rows(rp, col))
+end
+device_simplegraph(edge_list) = device_graph(edge_list, false)
+device_simpledigraph(edge_list) = device_graph(edge_list, true)
+
+# ---------------------------------------------------------------------
 # connectivity  (src/connectivity.jl:13-134, 495-516; has_path
 # src/traversals/bfs.jl:150-172)
 # ---------------------------------------------------------------------

@@ -37,6 +37,7 @@ EXPORTED = [
     "lgmi_betweenness_batch_tuning",
     "lgmi_betweenness_set_launch", "lgmi_stress", "lgmi_distance_stats",
     "lgmi_connected_components", "lgmi_has_path", "lgmi_period", "lgmi_neighborhood",
+    "lgmi_graph_from_edges", "lgmi_graph_size", "lgmi_graph_csr",
     "lgmi_last_error", "lgmi_device_count", "lgmi_synchronize",
     "lgmi_gen_kronecker", "lgmi_gen_erdos_renyi", "lgmi_gen_barabasi_albert",
     "lgmi_gen_watts_strogatz", "lgmi_gen_grid2d", "lgmi_csr_build", "lgmi_csr_transpose",
@@ -126,6 +127,9 @@ def load():
         "lgmi_connected_components": ([_vp, _i32p], c.c_int),
         "lgmi_has_path": ([_vp, c.c_int32, c.c_int32, _i32p, c.c_int32, c.POINTER(c.c_int)], c.c_int),
         "lgmi_period": ([_vp, _i64p], c.c_int),
+        "lgmi_graph_from_edges": ([_i32p, _i32p, c.c_int64, c.c_int, c.c_int32, c.POINTER(_vp), _i64p], c.c_int),
+        "lgmi_graph_size": ([_vp, _i32p, _i64p], c.c_int),
+        "lgmi_graph_csr": ([_vp, _i32p, _i32p, _i32p, _i32p], c.c_int),
         "lgmi_neighborhood": ([_vp, c.c_int32, c.c_int64, c.c_int, _i32p, _i32p, _i32p], c.c_int),
         "lgmi_last_error": ([], c.c_char_p),
         "lgmi_device_count": ([], c.c_int),

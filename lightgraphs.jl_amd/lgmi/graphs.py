@@ -75,6 +75,39 @@ class _GraphBase:
         return g
 
     @classmethod
+    def from_edge_list(cls, src, dst, T=np.int64, one_based: bool = True, nv_min: int = 0):
+        """SimpleGraph(edge_list) / SimpleDiGraph(edge_list)
+        (src/SimpleGraphs/simplegraph.jl:208-244, simpledigraph.jl:212-253)
+        built on the device (lgmi_graph_from_edges): nv is the largest id
+        used (at least nv_min), edges with an id below 1 are skipped, rows
+        are sorted and de-duplicated.  Returns the graph, whose device handle
+        is already built, and the reference's ne."""
+        src = np.asarray(src, dtype=np.int64)
+        dst = np.asarray(dst, dtype=np.int64)
+        if one_based:
+            src, dst = src - 1, dst - 1
+        lib = _lib.load()
+        h = _lib.ctypes.c_void_p()
+        ne = np.zeros(1, dtype=np.int64)
+        s32 = _as_i32(np.where(src < 0, -1, src))
+        d32 = _as_i32(np.where(dst < 0, -1, dst))
+        _lib.check(lib.lgmi_graph_from_edges(_lib.i32p(_nonempty(s32)), _lib.i32p(_nonempty(d32)), len(s32),
+                                             int(cls.directed), int(nv_min), _lib.ctypes.byref(h), _lib.i64p(ne)))
+        nv = np.zeros(1, dtype=np.int32)
+        nnz = np.zeros(1, dtype=np.int64)
+        _lib.check(lib.lgmi_graph_size(h, _lib.i32p(nv), _lib.i64p(nnz)))
+        n, m = int(nv[0]), int(nnz[0])
+        rp = np.zeros(n + 1, dtype=np.int32)
+        col = np.zeros(max(m, 1), dtype=np.int32)
+        irp = np.zeros(n + 1, dtype=np.int32) if cls.directed else None
+        icol = np.zeros(max(m, 1), dtype=np.int32) if cls.directed else None
+        _lib.check(lib.lgmi_graph_csr(h, _lib.i32p(rp), _lib.i32p(col), _lib.i32p(irp) if cls.directed else None,
+                                      _lib.i32p(icol) if cls.directed else None))
+        g = cls.from_csr(rp, col[:m], T=T, in_row_ptr=irp, in_col=icol[:m] if cls.directed else None)
+        g._handle = h
+        return g, int(ne[0])
+
+    @classmethod
     def from_edges(cls, n: int, src, dst, T=np.int64, one_based: bool = False, drop_self_loops=False):
         src = np.asarray(src, dtype=np.int64)
         dst = np.asarray(dst, dtype=np.int64)

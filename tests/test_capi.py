@@ -103,6 +103,19 @@ def test_bc_launch_mode_argument_checks():
     assert lib.lgmi_betweenness_set_launch(0) == _lib.LGMI_OK
 
 
+def test_graph_from_edges_argument_checks():
+    from lgmi import _lib
+    lib = _lib.load()
+    h = _lib.ctypes.c_void_p()
+    ne = np.zeros(1, dtype=np.int64)
+    a = np.zeros(2, dtype=np.int32)
+    assert lib.lgmi_graph_from_edges(_lib.i32p(a), _lib.i32p(a), -1, 0, 0, _lib.ctypes.byref(h), _lib.i64p(ne)) == \
+        _lib.LGMI_E_ARG
+    assert lib.lgmi_graph_from_edges(None, None, 2, 0, 0, _lib.ctypes.byref(h), _lib.i64p(ne)) == _lib.LGMI_E_ARG
+    assert lib.lgmi_graph_from_edges(_lib.i32p(a), _lib.i32p(a), 2, 0, -3, _lib.ctypes.byref(h), _lib.i64p(ne)) == \
+        _lib.LGMI_E_ARG
+
+
 def test_generators_shapes_and_determinism():
     import lgmi.generators as gen
     s1, d1 = gen.kronecker_edges(10, 16, seed=3)
