@@ -359,6 +359,117 @@ def profile_pass(lgmi, _lib, nbfs):
     log(f"profile pass done: copy {nbytes} B x3, {nbfs} BFS")
 
 
+def extras_bench(args, lgmi, _lib, orc):
+    """The widened rows at scale, each beside a CPU run of the same work:
+    stress centrality and the distance statistics (closeness / radiality /
+    eccentricity) over the 1024 RMAT-20 sources of the betweenness workload,
+    connected components and device ingest of RMAT-22, and a depth-2
+    neighbourhood of an RMAT-22 hub."""
+    import ctypes
+    from lgmi import distances as D
+    from lgmi.graphs import csr_from_edges
+    out = {}
+    nthreads = min(16, os.cpu_count() or 1)
+    g = lgmi.generators.rmat(20, 16, seed=1)
+    rp, col = g.csr()
+    nv, nnz = g.nv(), len(col)
+    rng = np.random.default_rng(4242)
+    srcs = rng.choice(np.nonzero(np.diff(rp) > 0)[0], size=1024, replace=False).astype(np.int64)
+    g.handle()
+    from lgmi.centrality import last_stats
+    # stress: 1024 sources, one persistent launch
+    lgmi.stress_centrality(g, srcs + 1)
+    t1 = time.perf_counter()
+    st = lgmi.stress_centrality(g, srcs + 1)
+    wall = time.perf_counter() - t1
+    st_stats = last_stats(g)
+    ns = 16
+    c0 = time.perf_counter()
+    want = orc.stress(rp, col, rp, False, srcs[:ns], nthreads=nthreads)
+    t_cpu = time.perf_counter() - c0
+    got = lgmi.stress_centrality(g, srcs[:ns] + 1)
+    out["stress_rmat20_1024"] = {"ms": round(wall * 1e3, 2), "device_ms": round(st_stats["device_ms"], 2),
+                                 "gteps": round(1024 * nnz / 2 / wall / 1e9, 3),
+                                 "cpu_ms_per_source": round(t_cpu / ns * 1e3, 2), "cpu_threads": nthreads,
+                                 "speedup_vs_cpu": round(t_cpu / ns * 1024 / wall, 1),
+                                 "sample_exact": bool((got == want).all())}
+    log(f"extras: stress 1024 sources {wall * 1e3:.1f} ms, cpu {t_cpu / ns * 1e3:.1f} ms/source "
+        f"({nthreads} threads), exact={out['stress_rmat20_1024']['sample_exact']}")
+    # distance statistics: 1024 sources
+    D.distance_stats(g, srcs + 1)
+    t1 = time.perf_counter()
+    ds = D.distance_stats(g, srcs + 1)
+    wall = time.perf_counter() - t1
+    c0 = time.perf_counter()
+    ok = True
+    for i in range(8):
+        d = orc.gdistances(rp, col, [int(srcs[i])]).astype(np.int64)
+        r = d[d >= 0]
+        ok = ok and [int(r.sum()), len(r), int(r.max())] == ds[i].tolist()
+    t_cpu = (time.perf_counter() - c0) / 8
+    out["distance_stats_rmat20_1024"] = {"ms": round(wall * 1e3, 2), "gteps": round(1024 * nnz / 2 / wall / 1e9, 3),
+                                         "cpu_ms_per_source": round(t_cpu * 1e3, 2), "cpu_threads": 1,
+                                         "speedup_vs_cpu": round(t_cpu * 1024 / wall, 1), "sample_exact": ok}
+    log(f"extras: distance stats 1024 sources {wall * 1e3:.1f} ms, cpu BFS {t_cpu * 1e3:.1f} ms/source, exact={ok}")
+    # RMAT-22: ingest, components, neighbourhood
+    g22 = lgmi.generators.rmat(22, 16, seed=1)
+    rp22, col22 = g22.csr()
+    src22 = np.repeat(np.arange(g22.nv()), np.diff(rp22))
+    keep = src22 <= col22
+    es, ed = src22[keep], col22[keep]
+    perm = np.random.default_rng(5).permutation(len(es))
+    es, ed = np.ascontiguousarray(es[perm], np.int32), np.ascontiguousarray(ed[perm], np.int32)
+    lib = _lib.load()
+    for rep in range(2):
+        h = ctypes.c_void_p()
+        ne = np.zeros(1, dtype=np.int64)
+        t1 = time.perf_counter()
+        _lib.check(lib.lgmi_graph_from_edges(_lib.i32p(es), _lib.i32p(ed), len(es), 0, g22.nv(), ctypes.byref(h),
+                                             _lib.i64p(ne)))
+        wall = time.perf_counter() - t1
+        lib.lgmi_graph_destroy(h)
+    c0 = time.perf_counter()
+    rp_h, col_h = csr_from_edges(g22.nv(), es, ed, symmetrize=True)
+    t_cpu = time.perf_counter() - c0
+    out["ingest_rmat22"] = {"edges": int(len(es)), "ms": round(wall * 1e3, 1), "ne": int(ne[0]),
+                            "cpu_ms": round(t_cpu * 1e3, 1), "cpu": "lgmi_csr_build (host C++, 1 thread)",
+                            "same_csr": bool((rp_h == rp22).all() and (col_h == col22).all())}
+    log(f"extras: ingest RMAT-22 ({len(es)} edges) {wall * 1e3:.1f} ms incl. host copies; host builder "
+        f"{t_cpu * 1e3:.1f} ms")
+    lab = np.zeros(g22.nv(), dtype=np.int64)
+    g22.handle()
+    lgmi.connected_components_(lab, g22)
+    t1 = time.perf_counter()
+    lgmi.connected_components_(lab, g22)
+    wall = time.perf_counter() - t1
+    try:
+        from scipy.sparse import csr_matrix
+        from scipy.sparse.csgraph import connected_components as sp_cc
+        m = csr_matrix((np.ones(len(col22), np.int8), col22, rp22), shape=(g22.nv(), g22.nv()))
+        c0 = time.perf_counter()
+        ncomp, _ = sp_cc(m, directed=False)
+        t_cpu = time.perf_counter() - c0
+    except ImportError:
+        ncomp, t_cpu = -1, float("nan")
+    out["components_rmat22"] = {"ms": round(wall * 1e3, 2), "components": int(len(np.unique(lab))),
+                                "cpu_ms": round(t_cpu * 1e3, 1), "cpu": "scipy.sparse.csgraph (1 thread)",
+                                "same_count": int(len(np.unique(lab))) == int(ncomp)}
+    log(f"extras: components RMAT-22 {wall * 1e3:.1f} ms ({len(np.unique(lab))} components), scipy "
+        f"{t_cpu * 1e3:.1f} ms")
+    hub = int(np.argmax(np.diff(rp22)))
+    verts = np.empty(g22.nv(), dtype=np.int32)
+    dists = np.empty(g22.nv(), dtype=np.int32)
+    cnt = np.zeros(1, dtype=np.int32)
+    for rep in range(2):
+        t1 = time.perf_counter()
+        _lib.check(lib.lgmi_neighborhood(g22.handle(), hub, 2, 0, _lib.i32p(verts), _lib.i32p(dists), _lib.i32p(cnt)))
+        wall = time.perf_counter() - t1
+    out["neighborhood_rmat22_hub_d2"] = {"ms": round(wall * 1e3, 2), "vertices": int(cnt[0]),
+                                         "note": "C-ABI call incl. the host copies of the queue and distances"}
+    log(f"extras: neighborhood(hub, 2) on RMAT-22: {int(cnt[0])} vertices in {wall * 1e3:.1f} ms")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--profile-pass", type=int, default=0,
@@ -370,6 +481,8 @@ def main():
     ap.add_argument("--cpu-bc-sources", type=int, default=128)
     ap.add_argument("--no-bc", action="store_true")
     ap.add_argument("--bc-only", action="store_true", help="run only the RMAT-20 betweenness workload")
+    ap.add_argument("--extras", action="store_true",
+                    help="run only the widened rows (stress, distance measures, ingest, components, neighborhood)")
     ap.add_argument("--config4", action="store_true",
                     help="run only the uneven small-graph batch (betweenness) workload")
     ap.add_argument("--config4-graphs", type=int, default=4096)
@@ -391,6 +504,9 @@ def main():
         return
     if args.config4:
         print(json.dumps(config4_bench(args, lgmi, orc)), flush=True)
+        return
+    if args.extras:
+        print(json.dumps(extras_bench(args, lgmi, _lib, orc)), flush=True)
         return
     if args.bc_only:
         print(json.dumps(bc_bench(args, lgmi, _lib, orc)), flush=True)
