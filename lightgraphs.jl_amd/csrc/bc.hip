@@ -162,6 +162,7 @@ struct BcArgs {
     uint64_t* hgot;          // [slot][seg_stride] per-segment new-lane masks
     int32_t seg_stride;
     int32_t fmod;            // levels F keeps: cap (all), or 2 (distances only)
+    int32_t fstride;         // slots interleaved in F: F[level][v][slot]
     uint64_t* st;            // distances only: [slot][chunk or heavy row][3][64]
                              // per-lane sums of distances, counts, last level
 };
@@ -226,10 +227,9 @@ __global__ void bc_clear_acc(double* acc, int64_t n) {
 __global__ void bc_init(BcArgs a) {
     const int slot = blockIdx.y;
     uint64_t* seen = a.seen + (size_t)slot * a.nv;
-    uint64_t* F0 = a.F + (size_t)slot * a.cap * a.nv;
     for (int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; v < a.nv; v += (int64_t)gridDim.x * blockDim.x) {
         seen[v] = 0;
-        F0[v] = 0;
+        a.F[(size_t)v * a.fstride + slot] = 0;   // level 0 of F[level][v][slot]
     }
 }
 
@@ -239,10 +239,9 @@ __global__ void bc_seed(BcArgs a, const int32_t* lane_src) {
     const int lane = threadIdx.x;
     const int32_t s = lane_src[slot * 64 + lane];
     uint64_t* seen = a.seen + (size_t)slot * a.nv;
-    uint64_t* F0 = a.F + (size_t)slot * a.cap * a.nv;
     double* sig = a.sig + (size_t)slot * a.nv * 64;
     if (s >= 0) {
-        atomicOr((unsigned long long*)&F0[s], 1ull << lane);
+        atomicOr((unsigned long long*)&a.F[(size_t)s * a.fstride + slot], 1ull << lane);
         atomicOr((unsigned long long*)&seen[s], 1ull << lane);
         if (a.sig) sig[(size_t)s * 64 + lane] = 1.0;
     }
@@ -261,8 +260,11 @@ __global__ void bc_seed(BcArgs a, const int32_t* lane_src) {
 // wavefront): every vertex with an unseen lane ORs its in-neighbours'
 // frontier masks and adds their sigma per lane.  Returns, in every lane,
 // whether any vertex of the chunk got a new lane.
-__device__ inline uint64_t* flevel(const BcArgs& a, int slot, int32_t L) {
-    return a.F + ((size_t)slot * a.cap + (a.fmod == 2 ? (L & 1) : L)) * a.nv;
+// F[level][v][slot]: the masks of one vertex for all slots share a line,
+// which the items of one chunk for the slots of a level read together.
+__device__ inline size_t fidx(const BcArgs& a, int slot, int32_t L, int32_t v) {
+    const int32_t lev = a.fmod == 2 ? (L & 1) : L;
+    return ((size_t)lev * a.nv + v) * a.fstride + slot;
 }
 
 __device__ inline uint64_t wave_or_u64(uint64_t m) {
@@ -285,8 +287,6 @@ template <int MODE>
 __device__ bool fwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int lane) {
     const int32_t nv = a.nv;
     const uint64_t act = a.active[slot];
-    const uint64_t* FL = flevel(a, slot, L);
-    uint64_t* Fn = flevel(a, slot, L + 1);
     uint64_t* seen = a.seen + (size_t)slot * nv;
     double* sig = MODE == 2 ? nullptr : a.sig + (size_t)slot * nv * 64;
     const int32_t vj = ch * 64 + lane;
@@ -317,7 +317,7 @@ __device__ bool fwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
             int32_t uk = 0;
             if (k < e) {
                 uk = a.in_col[k];
-                mk = FL[uk] & unseen;
+                mk = a.F[fidx(a, slot, L, uk)] & unseen;
             }
             if constexpr (MODE == 2) {
                 got |= wave_or_u64(mk);
@@ -339,7 +339,7 @@ __device__ bool fwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
         if (lane == j) newF = got;
     }
     if (vj < nv && !heavy) {
-        Fn[vj] = newF;
+        a.F[fidx(a, slot, L + 1, vj)] = newF;
         if (newF) seen[vj] = sj | newF;
     }
     const bool any = __ballot(newF != 0) != 0;
@@ -366,7 +366,6 @@ __device__ bool fwd_seg(const BcArgs& a, int slot, int32_t sg, int32_t L, int la
     const int32_t nv = a.nv;
     const int32_t v = a.hin.seg_v[sg];
     const uint64_t act = a.active[slot];
-    const uint64_t* FL = flevel(a, slot, L);
     const double* sig = MODE == 2 ? nullptr : a.sig + (size_t)slot * nv * 64;
     const uint64_t unseen = ~a.seen[(size_t)slot * nv + v] & act;
     double acc = 0.0;
@@ -379,7 +378,7 @@ __device__ bool fwd_seg(const BcArgs& a, int slot, int32_t sg, int32_t L, int la
             int32_t uk = 0;
             if (k < e) {
                 uk = a.in_col[k];
-                mk = FL[uk] & unseen;
+                mk = a.F[fidx(a, slot, L, uk)] & unseen;
             }
             if constexpr (MODE == 2) {
                 got |= wave_or_u64(mk);
@@ -403,10 +402,10 @@ __device__ void fwd_combine(const BcArgs& a, int slot, int32_t h, int32_t L, int
     const int32_t nv = a.nv;
     const int32_t f0 = a.hin.h_first[h], nsg = a.hin.h_nseg[h];
     const int32_t v = a.hin.seg_v[f0];
-    uint64_t* Fn = flevel(a, slot, L + 1);
+    uint64_t* Fn = a.F + fidx(a, slot, L + 1, v);
     const uint64_t sv = a.seen[(size_t)slot * nv + v];
     if (!(~sv & a.active[slot])) {   // every lane seen: the segments did nothing
-        if (lane == 0) Fn[v] = 0;
+        if (lane == 0) *Fn = 0;
         return;
     }
     const uint64_t* pgot = a.hgot + (size_t)slot * a.seg_stride + f0;
@@ -424,7 +423,7 @@ __device__ void fwd_combine(const BcArgs& a, int slot, int32_t h, int32_t L, int
         if ((gm >> lane) & 1) a.sig[((size_t)slot * nv + v) * 64 + lane] = tot;
     }
     if (lane == 0) {
-        Fn[v] = gm;
+        *Fn = gm;
         if (gm) a.seen[(size_t)slot * nv + v] = sv | gm;
     }
 }
@@ -532,15 +531,13 @@ template <int MODE>
 __device__ void bwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int lane) {
     using T = typename BwdAcc<MODE>::T;
     const int32_t nv = a.nv;
-    const uint64_t* FL = a.F + ((size_t)slot * a.cap + L) * nv;
-    const uint64_t* Fn = a.F + ((size_t)slot * a.cap + L + 1) * nv;
     double* sig = a.sig + (size_t)slot * nv * 64;
     T* acc_bc = reinterpret_cast<T*>(a.bc_acc) + (size_t)slot * nv;
     const int32_t vj = ch * 64 + lane;
     uint64_t Aj = 0;
     int32_t bj = 0, ej = 0;
     if (vj < nv) {
-        Aj = FL[vj];
+        Aj = a.F[fidx(a, slot, L, vj)];
         if (Aj) {
             bj = a.out_rp[vj];
             ej = a.out_rp[vj + 1];
@@ -562,7 +559,7 @@ __device__ void bwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
             int32_t wk = 0;
             if (kk < e) {
                 wk = a.out_col[kk];
-                mk = Fn[wk] & A;
+                mk = a.F[fidx(a, slot, L + 1, wk)] & A;
             }
             uint64_t hits = __ballot(mk != 0);
             uint64_t unused = 0;
@@ -584,9 +581,8 @@ __device__ void bwd_seg(const BcArgs& a, int slot, int32_t sg, int32_t L, int la
     using T = typename BwdAcc<MODE>::T;
     const int32_t nv = a.nv;
     const int32_t v = a.hout.seg_v[sg];
-    const uint64_t A = a.F[((size_t)slot * a.cap + L) * nv + v];
+    const uint64_t A = a.F[fidx(a, slot, L, v)];
     if (!A) return;
-    const uint64_t* Fn = a.F + ((size_t)slot * a.cap + L + 1) * nv;
     const T* sig = reinterpret_cast<const T*>(a.sig) + (size_t)slot * nv * 64;
     const int32_t b = a.hout.seg_b[sg], e = a.hout.seg_e[sg];
     T acc = 0;
@@ -596,7 +592,7 @@ __device__ void bwd_seg(const BcArgs& a, int slot, int32_t sg, int32_t L, int la
         int32_t wk = 0;
         if (kk < e) {
             wk = a.out_col[kk];
-            mk = Fn[wk] & A;
+            mk = a.F[fidx(a, slot, L + 1, wk)] & A;
         }
         uint64_t hits = __ballot(mk != 0);
         uint64_t unused = 0;
@@ -613,7 +609,7 @@ __device__ void bwd_combine(const BcArgs& a, int slot, int32_t h, int32_t L, int
     const int32_t nv = a.nv;
     const int32_t f0 = a.hout.h_first[h], nsg = a.hout.h_nseg[h];
     const int32_t v = a.hout.seg_v[f0];
-    const uint64_t A = a.F[((size_t)slot * a.cap + L) * nv + v];
+    const uint64_t A = a.F[fidx(a, slot, L, v)];
     if (!A) return;
     const T* part = reinterpret_cast<const T*>(a.hpart) + ((size_t)slot * a.seg_stride + f0) * 64;
     T tot = 0;
@@ -858,7 +854,7 @@ __global__ __launch_bounds__(kBlock) void bc_persist(BcArgs a, PersistArgs p) {
     for (int64_t i = (int64_t)vb * kBlock + tid; i < (int64_t)ns * nv; i += (int64_t)G * kBlock) {
         const int64_t sl = i / nv;
         a.seen[i] = 0;
-        a.F[sl * a.cap * nv + (i - sl * nv)] = 0;
+        a.F[fidx(a, (int)sl, 0, (int32_t)(i - sl * nv))] = 0;
         if constexpr (MODE != 2) a.bc_acc[i] = 0.0;
     }
     if constexpr (MODE == 2) {
@@ -880,7 +876,7 @@ __global__ __launch_bounds__(kBlock) void bc_persist(BcArgs a, PersistArgs p) {
             const int32_t src = p.lane_src[sl * 64 + tid];
             if (src >= 0) {
                 atomicOr((unsigned long long*)&a.seen[(size_t)sl * nv + src], 1ull << tid);
-                atomicOr((unsigned long long*)&a.F[(size_t)sl * a.cap * nv + src], 1ull << tid);
+                atomicOr((unsigned long long*)&a.F[fidx(a, sl, 0, src)], 1ull << tid);
                 if constexpr (MODE != 2) a.sig[((size_t)sl * nv + src) * 64 + tid] = 1.0;
             }
         }
@@ -1100,9 +1096,9 @@ static void bc_grow_levels(lgmi_graph* g, int32_t new_cap, int32_t keep) {
     BcWork* w = g->bc;
     const size_t nv = (size_t)std::max(g->nv, 1);
     uint64_t* F2 = dev_alloc_n<uint64_t>(nv * w->nslots * (size_t)new_cap);
-    for (int s = 0; s < w->nslots; s++)
-        LGMI_HIP(hipMemcpyAsync(F2 + (size_t)s * new_cap * nv, w->F + (size_t)s * w->cap_levels * nv,
-                                sizeof(uint64_t) * nv * (size_t)keep, hipMemcpyDeviceToDevice, g->stream));
+    // F[level][v][slot]: the first `keep` levels are a prefix
+    LGMI_HIP(hipMemcpyAsync(F2, w->F, sizeof(uint64_t) * nv * (size_t)w->nslots * (size_t)keep, hipMemcpyDeviceToDevice,
+                            g->stream));
     LGMI_HIP(hipStreamSynchronize(g->stream));
     dev_free(w->F);
     w->F = F2;
@@ -1349,6 +1345,7 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
     a.hpart = w->hpart; a.hgot = w->hgot;
     a.seg_stride = w->seg_stride;
     a.fmod = mode == 2 ? 2 : w->cap_levels;
+    a.fstride = w->nslots;
     a.st = w->st;
     if (mode == 2) { a.sig = nullptr; a.bc_acc = nullptr; }
 
