@@ -39,8 +39,9 @@
 //    most one per fresh edge plus one per new vertex.
 //  * bottom-up: every candidate scans its sorted reverse row for the first
 //    neighbour in F_k, which is its minimum-id parent.  Stage one tests the
-//    first entry of every candidate (it is the parent for most vertices on
-//    the large levels); the rest wait in a per-wavefront LDS queue and scan
+//    first entry of every candidate, read from a contiguous first-entry
+//    array (it is the parent for most vertices on the large levels); the
+//    rest wait in a per-wavefront LDS queue and scan
 //    four entries per round trip for up to four round trips, after which a
 //    whole wavefront scans each row still open, 64 entries per round trip.
 //    The tile's F_{k+1} words are gathered in LDS and written whole.
@@ -84,6 +85,7 @@ struct StepArgs {
     const int32_t* bu_rp;     // reverse adjacency (bottom-up scans it)
     const int32_t* bu_col;
     const uint32_t* bu_nz;    // bit v: row v of the reverse adjacency is non-empty
+    const int32_t* bu_first;  // [nv]: first entry of reverse row v (-1: empty)
     int32_t nv;
     int32_t nwords;           // 2 * ceil(nv / 64)
     int32_t* dist;
@@ -534,13 +536,14 @@ __device__ void bu_step(const StepArgs& a, int32_t L, const uint32_t* FL, uint32
         for (int32_t base = 0; base < total; base += 64) {
             int lo, bit;
             const int32_t v = batch_vertex(cw, excl, total, base, v0, lane, lo, bit);
-            int32_t beg = 0, end = 0;
-            if (v >= 0) { beg = a.bu_rp[v]; end = a.bu_rp[v + 1]; }
-            // stage one: the first entry
+            // stage one: the first entry, from the contiguous first-entry
+            // array (a tile's candidates read one run of it) rather than
+            // from a scattered line of the adjacency
+            int32_t beg = 0, end = 0, u0 = -1;
+            if (v >= 0) { beg = a.bu_rp[v]; end = a.bu_rp[v + 1]; u0 = a.bu_first[v]; }
             int32_t par = -1;
             bool open = false;
             if (v >= 0 && end > beg) {
-                const int32_t u0 = a.bu_col[beg];
                 k.scan += 1ull;
                 if ((FL[u0 >> 5] >> (u0 & 31)) & 1u) par = u0;
                 else open = end - beg > 1;
@@ -1162,7 +1165,7 @@ static StepArgs make_args(lgmi_graph* g, int dir_in, bool parents) {
     a.td_heavy_id = td.heavy_id;
     a.td_rtab = td.rtab;
     a.range_size = (td.range_size > 0 && td.range_grid == w->grid) ? td.range_size : 0;
-    a.bu_rp = bu.row_ptr; a.bu_col = bu.col; a.bu_nz = bu.nz_bm;
+    a.bu_rp = bu.row_ptr; a.bu_col = bu.col; a.bu_nz = bu.nz_bm; a.bu_first = bu.first;
     a.nv = g->nv;
     a.nwords = w->nwords;
     a.dist = w->dist;

@@ -23,6 +23,15 @@ __global__ void nz_bitmap_kernel(const int32_t* rp, int32_t nv, int32_t nwords, 
     nz[w] = bits;
 }
 
+// first[v] = the first (smallest) neighbour of v, -1 for an empty row: the
+// entry a bottom-up step tests first, read from one contiguous array.
+__global__ void first_entry_kernel(const int32_t* rp, const int32_t* col, int32_t nv, int32_t* first) {
+    const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nv) return;
+    const int32_t b = rp[v];
+    first[v] = rp[v + 1] > b ? col[b] : -1;
+}
+
 // rtab[h][r] = first entry of heavy row h with neighbour id >= r * rs (a
 // binary search over the sorted row); one thread per (row, boundary).
 __global__ void range_table_kernel(const int32_t* rp, const int32_t* col, const int32_t* heavy_v, int32_t n_heavy,
@@ -123,6 +132,11 @@ void finish_csr(lgmi_graph* g, DevCSR& d, const int32_t* rp) {
         nz_bitmap_kernel<<<(nwords + 255) / 256, 256, 0, g->stream>>>(d.row_ptr, g->nv, nwords, d.nz_bm);
         LGMI_HIP(hipGetLastError());
     }
+    d.first = dev_alloc_n<int32_t>((size_t)std::max(g->nv, 1));
+    if (g->nv > 0) {
+        first_entry_kernel<<<(unsigned)((g->nv + 255) / 256), 256, 0, g->stream>>>(d.row_ptr, d.col, g->nv, d.first);
+        LGMI_HIP(hipGetLastError());
+    }
     // rows longer than kHeavyRow -> items of at most kHeavyRow entries
     std::vector<int32_t> iv, ib, ie;
     for (int32_t v = 0; v < g->nv; v++) {
@@ -185,11 +199,11 @@ static void graph_free(lgmi_graph* g) {
     bc_free(g->bc);
     bc_graph_free(g->bcg);
     if (g->in.row_ptr != g->out.row_ptr) {
-        dev_free(g->in.row_ptr); dev_free(g->in.col); dev_free(g->in.nz_bm);
+        dev_free(g->in.row_ptr); dev_free(g->in.col); dev_free(g->in.nz_bm); dev_free(g->in.first);
         dev_free(g->in.item_v); dev_free(g->in.item_beg); dev_free(g->in.item_end);
         dev_free(g->in.heavy_id); dev_free(g->in.rtab);
     }
-    dev_free(g->out.row_ptr); dev_free(g->out.col); dev_free(g->out.nz_bm);
+    dev_free(g->out.row_ptr); dev_free(g->out.col); dev_free(g->out.nz_bm); dev_free(g->out.first);
     dev_free(g->out.item_v); dev_free(g->out.item_beg); dev_free(g->out.item_end);
     dev_free(g->out.heavy_id); dev_free(g->out.rtab);
     if (g->stream) (void)hipStreamDestroy(g->stream);

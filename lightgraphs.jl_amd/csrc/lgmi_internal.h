@@ -149,6 +149,7 @@ struct DevCSR {
     int32_t* row_ptr = nullptr;
     int32_t* col = nullptr;
     uint32_t* nz_bm = nullptr;   // bit v set when the row of v is non-empty
+    int32_t* first = nullptr;    // [nv]: col[row_ptr[v]], or -1 for an empty row
     int32_t* item_v = nullptr;   // heavy-row work items: vertex
     int32_t* item_beg = nullptr; //   first entry
     int32_t* item_end = nullptr; //   and one past the last (at most kHeavyRow entries)
