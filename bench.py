@@ -359,6 +359,44 @@ def profile_pass(lgmi, _lib, nbfs):
     log(f"profile pass done: copy {nbytes} B x3, {nbfs} BFS")
 
 
+def lg_load_bench(lgmi, g):
+    """loadgraph of an LG text file holding symmetrised RMAT-20 (15.7M edge
+    lines): header and edge-line parsing on the host (lgmi_lg_parse_edges),
+    then the device ingest, beside the same parse followed by the host CSR
+    builder."""
+    import tempfile
+
+    from lgmi import persistence as P
+    from lgmi.graphs import csr_from_edges
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "rmat20.lg")
+        with open(path, "w") as f:
+            P.savegraph(f, g, "rmat20")
+        size = os.path.getsize(path)
+        t0 = time.perf_counter()
+        g2 = P.loadgraph(path, "rmat20")
+        t_load = time.perf_counter() - t0
+        # the same file: parse only, then the host builder
+        t0 = time.perf_counter()
+        r = P._Reader(P._auto_decompress(P._read_all(path)))
+        h = P._parse_header(r.readline().strip())
+        s, dd = P._parse_edges(r, h)
+        t_parse = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        rp_h, col_h = csr_from_edges(h.nv, s - 1, dd - 1, symmetrize=True)
+        t_host = time.perf_counter() - t0
+    rp, col = g.csr()
+    same = bool(np.array_equal(g2.csr()[0], rp) and np.array_equal(g2.csr()[1], col) and
+                np.array_equal(rp_h, rp) and np.array_equal(col_h, col))
+    res = {"lines": int(h.ne), "mb": round(size / 1e6, 1), "load_ms": round(t_load * 1e3, 1),
+           "parse_ms": round(t_parse * 1e3, 1), "device_ingest_ms": round((t_load - t_parse) * 1e3, 1),
+           "host_builder_ms": round(t_host * 1e3, 1), "same_csr": same,
+           "cpu": "the same parse, then lgmi_csr_build (host C++, 1 thread)"}
+    log(f"extras: loadgraph RMAT-20 LG ({size / 1e6:.0f} MB, {h.ne} lines) {t_load * 1e3:.0f} ms "
+        f"(parse {t_parse * 1e3:.0f} ms); parse + host builder {(t_parse + t_host) * 1e3:.0f} ms; same={same}")
+    return res
+
+
 def extras_bench(args, lgmi, _lib, orc):
     """The widened rows at scale, each beside a CPU run of the same work:
     stress centrality and the distance statistics (closeness / radiality /
@@ -436,6 +474,7 @@ def extras_bench(args, lgmi, _lib, orc):
                             "same_csr": bool((rp_h == rp22).all() and (col_h == col22).all())}
     log(f"extras: ingest RMAT-22 ({len(es)} edges) {wall * 1e3:.1f} ms incl. host copies; host builder "
         f"{t_cpu * 1e3:.1f} ms")
+    out["loadgraph_rmat20"] = lg_load_bench(lgmi, g)
     lab = np.zeros(g22.nv(), dtype=np.int64)
     g22.handle()
     lgmi.connected_components_(lab, g22)

@@ -56,6 +56,21 @@ int lgmi_csr_build(int32_t nv, int64_t ne, const int32_t* src, const int32_t* ds
 int lgmi_csr_transpose(int32_t nv, const int32_t* row_ptr, const int32_t* col, int32_t* t_row_ptr,
                        int32_t* t_col);
 
+/* The edge lines of one graph in the LG text format, read as
+ * _lg_read_one_graph reads them (src/persistence/lg.jl:35-52).  At most
+ * nlines lines of text[0..len) are read: an empty line (after its line break
+ * is removed) is skipped but counts; any other line gives one pair, its first
+ * two comma-separated fields parsed as Int64 (blanks around commas and
+ * numbers allowed, optional sign, 0x/0o/0b prefixes; further fields are
+ * ignored).  The pairs, as written (1-based, not range-checked), go to
+ * src/dst, which hold nlines entries; *npairs = their count, *consumed = the
+ * bytes of text the lines took (where the next header starts).  Returns 0,
+ * or -1 for a line with a single field, -2 for a field that is not an
+ * integer, -3 for one beyond Int64 (*bad_line = the 1-based line, counted
+ * from the first edge line), -4 for bad arguments. */
+int lgmi_lg_parse_edges(const char* text, int64_t len, int64_t nlines, int64_t* src, int64_t* dst,
+                        int64_t* npairs, int64_t* consumed, int64_t* bad_line);
+
 #ifdef __cplusplus
 }
 #endif

@@ -255,14 +255,12 @@ end
 # The CSR is sorted and de-duplicated on the GPU; the handle is kept for the
 # calls above and the fadjlist is filled from the CSR it returns.
 # ---------------------------------------------------------------------
-function device_graph(edge_list::Vector{<:LightGraphs.AbstractEdge{T}}, directed::Bool) where T
-    s0 = Int32[src(e) - 1 for e in edge_list]
-    d0 = Int32[dst(e) - 1 for e in edge_list]
+function device_graph_ids(::Type{T}, s0::Vector{Int32}, d0::Vector{Int32}, directed::Bool, nv_min::Integer) where T
     ptr = Ref{Ptr{Cvoid}}(C_NULL)
     ne_ = Ref{Int64}(0)
     check(ccall((:lgmi_graph_from_edges, liblgmi), Cint,
                 (Ptr{Int32}, Ptr{Int32}, Int64, Cint, Int32, Ptr{Ptr{Cvoid}}, Ptr{Int64}),
-                s0, d0, length(s0), directed, 0, ptr, ne_))
+                s0, d0, length(s0), directed, nv_min, ptr, ne_))
     n = Ref{Int32}(0); nnz = Ref{Int64}(0)
     check(ccall((:lgmi_graph_size, liblgmi), Cint, (Ptr{Cvoid}, Ptr{Int32}, Ptr{Int64}), ptr[], n, nnz))
     rp = Vector{Int32}(undef, n[] + 1); col = Vector{Int32}(undef, nnz[])
@@ -273,8 +271,83 @@ function device_graph(edge_list::Vector{<:LightGraphs.AbstractEdge{T}}, directed
     rows(p, c) = [T[c[k] + 1 for k in (p[v] + 1):p[v + 1]] for v in 1:n[]]
     return directed ? SimpleDiGraph{T}(ne_[], rows(rp, col), rows(irp, icol)) : SimpleGraph{T}(ne_[], rows(rp, col))
 end
+
+function device_graph(edge_list::Vector{<:LightGraphs.AbstractEdge{T}}, directed::Bool) where T
+    s0 = Int32[src(e) - 1 for e in edge_list]
+    d0 = Int32[dst(e) - 1 for e in edge_list]
+    return device_graph_ids(T, s0, d0, directed, 0)
+end
 device_simplegraph(edge_list) = device_graph(edge_list, false)
 device_simpledigraph(edge_list) = device_graph(edge_list, true)
+
+# ---------------------------------------------------------------------
+# LG files (src/persistence/lg.jl:35-112): headers parsed by the
+# reference's _parse_header, edge lines by lgmi_lg_parse_edges (host C++),
+# each graph built on the device from its in-range pairs (add_edge! drops
+# the others, ignores duplicates and keeps self-loops).
+# ---------------------------------------------------------------------
+function lg_edges_device(text::Vector{UInt8}, pos::Int, h)
+    T = h.dtype
+    cap = max(h.ne, 1)
+    s = Vector{Int64}(undef, cap); d = Vector{Int64}(undef, cap)
+    np_ = Ref{Int64}(0); used = Ref{Int64}(0); bad = Ref{Int64}(0)
+    avail = length(text) - pos + 1
+    rc = avail <= 0 ? Cint(0) : GC.@preserve text ccall((:lgmi_lg_parse_edges, liblgmi), Cint,
+        (Ptr{UInt8}, Int64, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Int64}),
+        pointer(text, pos), avail, h.ne, s, d, np_, used, bad)
+    rc == -3 && throw(OverflowError("LG edge line $(bad[]) of graph $(h.name)"))
+    rc == 0 || throw(ArgumentError("LG edge line $(bad[]) of graph $(h.name): expected src,dst"))
+    s0 = Int32[]; d0 = Int32[]
+    for i in 1:np_[]
+        (typemin(T) <= s[i] <= typemax(T) && typemin(T) <= d[i] <= typemax(T)) ||
+            throw(OverflowError("LG edge line of graph $(h.name) does not fit $T"))
+        if 1 <= s[i] <= h.nv && 1 <= d[i] <= h.nv
+            push!(s0, Int32(s[i] - 1)); push!(d0, Int32(d[i] - 1))
+        end
+    end
+    return device_graph_ids(T, s0, d0, h.is_directed, h.nv), pos + used[]
+end
+
+function lg_next_line(text::Vector{UInt8}, pos::Int)
+    e = something(findnext(==(UInt8('\n')), text, pos), length(text) + 1)
+    return strip(String(text[pos:e-1])), e + 1
+end
+
+"""
+    loadgraphs_device(file) / loadgraph_device(file, gname="graph")
+
+loadgraphs / loadgraph (src/persistence/common.jl:3-39, lg.jl:87-112) for
+LG files, with each graph's adjacency sorted and de-duplicated on the GPU.
+"""
+function loadgraphs_device(fn::AbstractString)
+    text = open(fn, "r") do io read(LightGraphs.auto_decompress(io)) end
+    graphs = Dict{String,AbstractGraph}()
+    pos = 1
+    while pos <= length(text)
+        line, pos = lg_next_line(text, pos)
+        (startswith(line, "#") || line == "") && continue
+        h = LightGraphs._parse_header(line)
+        g, pos = lg_edges_device(text, pos, h)
+        graphs[h.name] = g
+    end
+    return graphs
+end
+
+function loadgraph_device(fn::AbstractString, gname::AbstractString="graph")
+    text = open(fn, "r") do io read(LightGraphs.auto_decompress(io)) end
+    pos = 1
+    while pos <= length(text)
+        line, pos = lg_next_line(text, pos)
+        (startswith(line, "#") || line == "") && continue
+        h = LightGraphs._parse_header(line)
+        h.name == gname && return first(lg_edges_device(text, pos, h))
+        for _ in 1:h.ne                        # _lg_skip_one_graph
+            pos > length(text) && break
+            _, pos = lg_next_line(text, pos)
+        end
+    end
+    throw(ArgumentError("graph $gname not found"))
+end
 
 # ---------------------------------------------------------------------
 # connectivity  (src/connectivity.jl:13-134, 495-516; has_path

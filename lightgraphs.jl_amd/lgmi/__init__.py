@@ -16,6 +16,8 @@ hand-written gfx950 HIP kernels in liblgmi.so through its C ABI
                                     -- src/connectivity.jl, src/traversals/bfs.jl
 * ``Parallel.*``                    -- src/Parallel/{traversals,centrality}
 * ``SimpleGraph``, ``SimpleDiGraph`` -- src/SimpleGraphs (CSR-backed)
+* ``loadgraph``, ``loadgraphs``, ``savegraph`` -- src/persistence (LG format;
+                                       graphs built on the device)
 """
 from . import Parallel, generators
 from ._lib import LgmiError, NoDeviceError
@@ -25,6 +27,7 @@ from .connectivity import (components, components_dict, connected_components, co
 from .distances import (center, closeness_centrality, diameter, eccentricity, periphery, radiality_centrality,
                         radius)
 from .graphs import SimpleDiGraph, SimpleGraph
+from .persistence import loadgraph, loadgraphs, savegraph
 from .traversals import bfs_parents, bfs_tree, gdistances, gdistances_, tree, typemax
 
 __all__ = [
@@ -33,5 +36,5 @@ __all__ = [
     "tree", "typemax", "center", "closeness_centrality", "diameter", "eccentricity", "periphery",
     "radiality_centrality", "radius", "components", "components_dict", "connected_components",
     "connected_components_", "has_path", "is_connected", "is_weakly_connected", "period", "neighborhood",
-    "neighborhood_dists",
+    "neighborhood_dists", "loadgraph", "loadgraphs", "savegraph",
 ]

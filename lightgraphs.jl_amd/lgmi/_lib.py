@@ -41,6 +41,7 @@ EXPORTED = [
     "lgmi_last_error", "lgmi_device_count", "lgmi_synchronize",
     "lgmi_gen_kronecker", "lgmi_gen_erdos_renyi", "lgmi_gen_barabasi_albert",
     "lgmi_gen_watts_strogatz", "lgmi_gen_grid2d", "lgmi_csr_build", "lgmi_csr_transpose",
+    "lgmi_lg_parse_edges",
 ]
 
 
@@ -143,6 +144,7 @@ def load():
         "lgmi_csr_build": ([c.c_int32, c.c_int64, _i32p, _i32p, c.c_int, c.c_int, _i32p, _i32p, c.c_int64,
                             _i64p], c.c_int),
         "lgmi_csr_transpose": ([c.c_int32, _i32p, _i32p, _i32p, _i32p], c.c_int),
+        "lgmi_lg_parse_edges": ([_vp, c.c_int64, c.c_int64, _i64p, _i64p, _i64p, _i64p, _i64p], c.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)  # AttributeError if a declared symbol is missing
