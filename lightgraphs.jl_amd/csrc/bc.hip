@@ -218,6 +218,27 @@ __device__ inline void gather_hits(uint64_t& hits, int32_t uk, uint64_t mk, cons
     }
 }
 
+// For source i: its out-neighbour of largest out-degree (the smallest id
+// among ties), -1 for an empty row.  One wavefront per source.
+__global__ void bc_source_hubs(const int32_t* rp, const int32_t* col, const int32_t* srcs, int32_t n, int32_t* hub) {
+    const int lane = threadIdx.x & 63;
+    const int32_t i = (int32_t)blockIdx.x * (int32_t)(blockDim.x / 64) + (int32_t)(threadIdx.x >> 6);
+    if (i >= n) return;   // uniform over the wavefront
+    const int32_t s = srcs[i];
+    const int32_t b = rp[s], e = rp[s + 1];
+    long long best = -1;   // (degree << 32) | (0x7fffffff - id)
+    for (int32_t k = b + lane; k < e; k += 64) {
+        const int32_t u = col[k];
+        const long long key = ((long long)(rp[u + 1] - rp[u]) << 32) | (long long)(0x7fffffff - u);
+        best = key > best ? key : best;
+    }
+    for (int o = 32; o >= 1; o >>= 1) {
+        const long long x = __shfl_xor(best, o);
+        best = x > best ? x : best;
+    }
+    if (lane == 0) hub[i] = best < 0 ? -1 : 0x7fffffff - (int32_t)(best & 0xffffffffll);
+}
+
 __global__ void bc_clear_acc(double* acc, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         acc[i] = 0.0;
@@ -1284,6 +1305,48 @@ static bool bc_persistent(lgmi_graph* g, BcArgs a, int32_t ns, const double* end
     return true;
 }
 
+// Lane assignment of the sources.  A hit of the gathers reads the 64-lane
+// row of one vertex for the lanes whose searches place it on the level
+// before; the fewer 128-byte quarters of the row those lanes span, the
+// fewer lines it costs.  Sources that share their highest-degree neighbour
+// have nearly the same distances to everything else (offset by one at most),
+// so the sources are ordered by that neighbour (highest degree first), then
+// by their own degree, then by position: sources with alike searches end up
+// in the same lane group and the same quarter.  Betweenness and stress sum
+// over the sources, so only the floating-point summation order changes; the
+// order is a function of the graph and the source list alone.
+static void order_sources(lgmi_graph* g, std::vector<int32_t>& act) {
+    const int32_t n = (int32_t)act.size();
+    int32_t* d = dev_alloc_n<int32_t>((size_t)n * 2);
+    std::vector<int32_t> hub((size_t)n);
+    try {
+        LGMI_HIP(hipMemcpyAsync(d, act.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, g->stream));
+        bc_source_hubs<<<(unsigned)((n + 3) / 4), 256, 0, g->stream>>>(g->out.row_ptr, g->out.col, d, n, d + n);
+        LGMI_HIP(hipGetLastError());
+        LGMI_HIP(hipMemcpyAsync(hub.data(), d + n, sizeof(int32_t) * (size_t)n, hipMemcpyDeviceToHost, g->stream));
+        LGMI_HIP(hipStreamSynchronize(g->stream));
+    } catch (...) {
+        dev_free(d);
+        throw;
+    }
+    dev_free(d);
+    const std::vector<int32_t>& rp = g->bcg->rp_out;
+    auto odeg = [&](int32_t v) { return v < 0 ? -1 : rp[(size_t)v + 1] - rp[(size_t)v]; };
+    std::vector<int32_t> idx((size_t)n);
+    for (int32_t i = 0; i < n; i++) idx[(size_t)i] = i;
+    std::stable_sort(idx.begin(), idx.end(), [&](int32_t x, int32_t y) {
+        const int32_t hx = hub[(size_t)x], hy = hub[(size_t)y];
+        const int32_t dx = odeg(hx), dy = odeg(hy);
+        if (dx != dy) return dx > dy;
+        if (hx != hy) return hx < hy;
+        const int32_t sx = odeg(act[(size_t)x]), sy = odeg(act[(size_t)y]);
+        return sx > sy;
+    });
+    std::vector<int32_t> out((size_t)n);
+    for (int32_t i = 0; i < n; i++) out[(size_t)i] = act[(size_t)idx[(size_t)i]];
+    act.swap(out);
+}
+
 void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, int skip_degree0, int normalize,
             int endpoints, double* bc, lgmi_bc_stats* stats, int mode, uint64_t* dist_stats) {
     const int32_t nv = g->nv;
@@ -1303,6 +1366,7 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
         if (!skip_degree0 || degree(srcs[i]) > 0) act.push_back(srcs[i]);
     const int64_t nact = (int64_t)act.size();
     const int32_t ngroups = (int32_t)((nact + 63) / 64);
+    if (mode != 2 && nact > 16) order_sources(g, act);
 
     // slots: as many groups at once as fit a memory budget
     const int32_t cap0 = mode == 2 ? 2 : 32;
