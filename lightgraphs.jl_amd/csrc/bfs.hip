@@ -450,7 +450,10 @@ __device__ void bu_drain(const StepArgs& a, const Pending* q, int n, int32_t v0,
                          Counters& k, uint32_t* s_hist, int32_t* qout, uint32_t* s_fn, int lane) {
     constexpr int kLaneRounds = 4;
     int32_t v = -1, b = 0, e = 0, b0 = 0;
-    if (lane < n) { const Pending p = q[lane]; v = p.v; b = p.b; e = p.e; b0 = p.b0; }
+    // field by field: a whole-struct copy can leave a private copy that the
+    // compiler places in LDS at a per-thread address, which needs the
+    // dispatch packet (block dimensions) in every wave of every launch
+    if (lane < n) { v = q[lane].v; b = q[lane].b; e = q[lane].e; b0 = q[lane].b0; }
     int32_t par = -1;
     int32_t x = b;
     if (v >= 0) {
@@ -489,6 +492,91 @@ __device__ void bu_drain(const StepArgs& a, const Pending* q, int n, int32_t v0,
     int bin = 0;
     if (par >= 0) bin = bu_found<PARENTS>(a, v, par, v - v0, b0, e, L, k, s_fn);
     push_new(par >= 0, v, bin, s_hist, qout, lane);
+}
+
+// The candidates of one tile (byte cw of lane l: vertices v0 + 8l ..
+// v0 + 8l + 7; excl its exclusive popcount prefix, total > 0 their number):
+// stage one, the found vertices recorded, and the open ones drained.
+template <bool PARENTS>
+__device__ inline Counters bu_tile(const StepArgs& a, int32_t L, const uint32_t* FL, uint32_t* s_hist,
+                                         int32_t* qout, uint32_t* s_fn, Pending* s_q, uint32_t cw, int32_t excl,
+                                         int32_t total, int32_t word0, int lane) {
+    Counters k;
+    const int32_t v0 = word0 * 32;
+    int qn = 0;   // queued candidates (uniform over the wavefront)
+    // Stage one for all batches of the tile (kTileVerts / 64 at most) at
+    // once: every batch's row bounds and first entry are loaded, then
+    // every first entry's frontier word, so the tile pays two memory
+    // round trips rather than two per batch.  The first entry comes from
+    // the contiguous first-entry array (a tile's candidates read one run
+    // of it), not from a scattered line of the adjacency.
+    constexpr int kTB = kTileVerts / 64;
+    const int nb = (total + 63) >> 6;   // uniform
+    int32_t qv[kTB], qb[kTB], qe[kTB], qp[kTB];
+    #pragma unroll
+    for (int q = 0; q < kTB; q++) {
+        qv[q] = -1;
+        if (q < nb) {
+            int lo, bit;
+            qv[q] = batch_vertex(cw, excl, total, q * 64, v0, lane, lo, bit);
+        }
+    }
+    #pragma unroll
+    for (int q = 0; q < kTB; q++) {
+        qb[q] = 0;
+        qe[q] = 0;
+        qp[q] = -1;
+        if (qv[q] >= 0) { qb[q] = a.bu_rp[qv[q]]; qe[q] = a.bu_rp[qv[q] + 1]; qp[q] = a.bu_first[qv[q]]; }
+    }
+    uint32_t qw[kTB];
+    #pragma unroll
+    for (int q = 0; q < kTB; q++) qw[q] = qp[q] >= 0 ? FL[qp[q] >> 5] : 0u;
+    #pragma unroll
+    for (int q = 0; q < kTB; q++) qp[q] = (qp[q] >= 0 && ((qw[q] >> (qp[q] & 31)) & 1u)) ? qp[q] : -1;
+    // record the batches in order; the registers of batch 0 are used and
+    // the others move down one place per batch (constant indices only)
+    for (int b = 0; b < nb; b++) {
+        const int32_t v = qv[0], beg = qb[0], end = qe[0], par = qp[0];
+        #pragma unroll
+        for (int q = 0; q + 1 < kTB; q++) {
+            qv[q] = qv[q + 1];
+            qb[q] = qb[q + 1];
+            qe[q] = qe[q + 1];
+            qp[q] = qp[q + 1];
+        }
+        bool open = false;
+        if (v >= 0 && end > beg) {
+            k.scan += 1ull;
+            open = par < 0 && end - beg > 1;
+        }
+        int bin = 0;
+        if (par >= 0) bin = bu_found<PARENTS>(a, v, par, v - v0, beg, end, L, k, s_fn);
+        push_new(par >= 0, v, bin, s_hist, qout, lane);
+        const uint64_t om = __ballot(open);
+        if (open) {
+            const int pos = qn + (int)__popcll(om & ((1ull << lane) - 1ull));
+            s_q[pos] = Pending{v, beg + 1, end, beg};
+        }
+        qn += (int)__popcll(om);
+        if (qn >= 64) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            bu_drain<PARENTS>(a, s_q, 64, v0, L, FL, k, s_hist, qout, s_fn, lane);
+            qn -= 64;
+            int32_t kv = -1, kb = 0, ke = 0, kb0 = 0;   // field by field, as in bu_drain
+            if (lane < qn) { kv = s_q[64 + lane].v; kb = s_q[64 + lane].b; ke = s_q[64 + lane].e; kb0 = s_q[64 + lane].b0; }
+            __builtin_amdgcn_wave_barrier();
+            if (lane < qn) s_q[lane] = Pending{kv, kb, ke, kb0};
+        }
+    }
+    if (qn > 0) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        bu_drain<PARENTS>(a, s_q, qn, v0, L, FL, k, s_hist, qout, s_fn, lane);
+    }
+    return k;
 }
 
 // Bottom-up step for one wavefront.  Candidates are the unvisited vertices
@@ -531,49 +619,14 @@ __device__ void bu_step(const StepArgs& a, int32_t L, const uint32_t* FL, uint32
         int32_t total;
         const int32_t excl = wave_excl_scan(__popc(cw), lane, total);
         k.cand += (lane == 0) ? (unsigned long long)total : 0ull;
-        const int32_t v0 = word0 * 32;
-        int qn = 0;   // queued candidates (uniform over the wavefront)
-        for (int32_t base = 0; base < total; base += 64) {
-            int lo, bit;
-            const int32_t v = batch_vertex(cw, excl, total, base, v0, lane, lo, bit);
-            // stage one: the first entry, from the contiguous first-entry
-            // array (a tile's candidates read one run of it) rather than
-            // from a scattered line of the adjacency
-            int32_t beg = 0, end = 0, u0 = -1;
-            if (v >= 0) { beg = a.bu_rp[v]; end = a.bu_rp[v + 1]; u0 = a.bu_first[v]; }
-            int32_t par = -1;
-            bool open = false;
-            if (v >= 0 && end > beg) {
-                k.scan += 1ull;
-                if ((FL[u0 >> 5] >> (u0 & 31)) & 1u) par = u0;
-                else open = end - beg > 1;
-            }
-            int bin = 0;
-            if (par >= 0) bin = bu_found<PARENTS>(a, v, par, v - v0, beg, end, L, k, s_fn);
-            push_new(par >= 0, v, bin, s_hist, qout, lane);
-            const uint64_t om = __ballot(open);
-            if (open) {
-                const int pos = qn + (int)__popcll(om & ((1ull << lane) - 1ull));
-                s_q[pos] = Pending{v, beg + 1, end, beg};
-            }
-            qn += (int)__popcll(om);
-            if (qn >= 64) {
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-                bu_drain<PARENTS>(a, s_q, 64, v0, L, FL, k, s_hist, qout, s_fn, lane);
-                qn -= 64;
-                Pending keep{-1, 0, 0, 0};
-                if (lane < qn) keep = s_q[64 + lane];
-                __builtin_amdgcn_wave_barrier();
-                if (lane < qn) s_q[lane] = keep;
-            }
-        }
-        if (qn > 0) {
-            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-            __builtin_amdgcn_wave_barrier();
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            bu_drain<PARENTS>(a, s_q, qn, v0, L, FL, k, s_hist, qout, s_fn, lane);
+        if (total > 0) {
+            const Counters kt = bu_tile<PARENTS>(a, L, FL, s_hist, qout, s_fn, s_q, cw, excl, total, word0, lane);
+            k.n += kt.n;
+            k.out += kt.out;
+            k.in += kt.in;
+            k.scan += kt.scan;
+            k.cand += kt.cand;
+            k.light += kt.light;
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -804,7 +857,9 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
     __shared__ uint32_t s_lds[kLdsWords];
     Pending* s_q = reinterpret_cast<Pending*>(s_lds);
 
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    // the wavefront index is uniform over a wavefront; readfirstlane lets the
+    // compiler keep it, and the tile loops that depend on it, scalar
+    const int tid = threadIdx.x, lane = tid & 63, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint64_t t_now = wall_clock64();
     const int lpar = karg >= 0 ? (karg & 1) : (-1 - karg);   // parity of p
     const bool first = karg == 0;
