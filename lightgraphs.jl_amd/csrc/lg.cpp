@@ -13,7 +13,11 @@
 // element type (add_edge! drops out-of-range endpoints).
 #include "../../include/lgmi_gen.h"
 
+#include <algorithm>
 #include <cstdint>
+#include <cstring>
+#include <thread>
+#include <vector>
 
 namespace {
 
@@ -55,21 +59,16 @@ int parse_i64(const char* t, int64_t b, int64_t e, int64_t* out) {
     return 0;
 }
 
-}  // namespace
-
-extern "C" int lgmi_lg_parse_edges(const char* text, int64_t len, int64_t nlines, int64_t* src, int64_t* dst,
-                                   int64_t* npairs, int64_t* consumed, int64_t* bad_line) {
-    if (!npairs || !consumed || !bad_line || len < 0 || nlines < 0 || (len > 0 && !text) ||
-        (nlines > 0 && (!src || !dst)))
-        return -4;
-    *npairs = 0;
-    *consumed = 0;
-    *bad_line = 0;
-    int64_t pos = 0, np = 0;
-    for (int64_t line = 1; line <= nlines; line++) {
+// Parses lines [first_line, first_line + nl) starting at text[pos] (the
+// lines are known to end before `len`); pairs go to src/dst from index 0.
+// Returns 0 or an error code with *bad = the 1-based line number.
+int parse_lines(const char* text, int64_t len, int64_t pos, int64_t first_line, int64_t nl, int64_t* src,
+                int64_t* dst, int64_t* np_out, int64_t* bad) {
+    int64_t np = 0;
+    for (int64_t line = first_line; line < first_line + nl; line++) {
         if (pos >= len) break;   // readline at the end of the stream gives ""
-        int64_t e = pos;
-        while (e < len && text[e] != '\n') e++;
+        const char* nlp = static_cast<const char*>(std::memchr(text + pos, '\n', (size_t)(len - pos)));
+        const int64_t e = nlp ? (int64_t)(nlp - text) : len;
         const int64_t next = e < len ? e + 1 : e;
         int64_t le = e;
         if (e < len && le > pos && text[le - 1] == '\r') le--;   // chomp removes "\r\n" whole
@@ -78,29 +77,135 @@ extern "C" int lgmi_lg_parse_edges(const char* text, int64_t len, int64_t nlines
             int64_t c1 = pos;
             while (c1 < le && text[c1] != ',') c1++;
             if (c1 >= le) {
-                *bad_line = line;
-                *consumed = pos;
-                *npairs = np;
+                *bad = line + 1;
+                *np_out = np;
                 return -1;
             }
             int64_t c2 = c1 + 1;
             while (c2 < le && text[c2] != ',') c2++;
-            int64_t s = 0, d = 0;
-            int rc = parse_i64(text, pos, c1, &s);
-            if (rc == 0) rc = parse_i64(text, c1 + 1, c2, &d);
+            int64_t sv = 0, dv = 0;
+            int rc = parse_i64(text, pos, c1, &sv);
+            if (rc == 0) rc = parse_i64(text, c1 + 1, c2, &dv);
             if (rc != 0) {
-                *bad_line = line;
-                *consumed = pos;
-                *npairs = np;
+                *bad = line + 1;
+                *np_out = np;
                 return rc;
             }
-            src[np] = s;
-            dst[np] = d;
+            src[np] = sv;
+            dst[np] = dv;
             np++;
         }
         pos = next;
     }
+    *np_out = np;
+    return 0;
+}
+
+}  // namespace
+
+// Small inputs are parsed by the calling thread.  Larger ones are cut by
+// bytes into up to 16 ranges that start at line starts; each thread counts
+// the line breaks of its range, a prefix sum gives each range's first line,
+// and each range parses the lines it holds among the first nlines into its
+// own part of src/dst.  The parts are then moved together in line order; an
+// error reports the first bad line.
+extern "C" int lgmi_lg_parse_edges(const char* text, int64_t len, int64_t nlines, int64_t* src, int64_t* dst,
+                                   int64_t* npairs, int64_t* consumed, int64_t* bad_line) {
+    if (!npairs || !consumed || !bad_line || len < 0 || nlines < 0 || (len > 0 && !text) ||
+        (nlines > 0 && (!src || !dst)))
+        return -4;
+    *npairs = 0;
+    *consumed = 0;
+    *bad_line = 0;
+    const int64_t kMinBytes = (int64_t)1 << 20;   // bytes per thread at least
+    const unsigned hw = std::max(1u, std::min(std::thread::hardware_concurrency(), 16u));
+    const int nt = (int)std::min<int64_t>((int64_t)hw, std::max<int64_t>(1, len / kMinBytes));
+    if (nt == 1 || nlines < 4096) {
+        int64_t np = 0, bad = 0;
+        const int rc = parse_lines(text, len, 0, 0, nlines, src, dst, &np, &bad);
+        *npairs = np;
+        *bad_line = bad;
+        if (rc != 0) return rc;
+        // where the nlines lines end
+        int64_t pos = 0;
+        for (int64_t line = 0; line < nlines && pos < len; line++) {
+            const char* nlp = static_cast<const char*>(std::memchr(text + pos, '\n', (size_t)(len - pos)));
+            pos = nlp ? (int64_t)(nlp - text) + 1 : len;
+        }
+        *consumed = pos;
+        return 0;
+    }
+    // range starts: the first line start at or after r * len / nt
+    std::vector<int64_t> rs((size_t)nt + 1, len);
+    rs[0] = 0;
+    for (int r = 1; r < nt; r++) {
+        int64_t p = std::max(rs[(size_t)r - 1], (int64_t)((double)len * r / nt));
+        if (p > 0 && p < len && text[p - 1] != '\n') {
+            const char* nlp = static_cast<const char*>(std::memchr(text + p, '\n', (size_t)(len - p)));
+            p = nlp ? (int64_t)(nlp - text) + 1 : len;
+        }
+        rs[(size_t)r] = p;
+    }
+    // line breaks per range; a range's last line may lack one only at the end
+    std::vector<int64_t> cnt((size_t)nt, 0);
+    {
+        std::vector<std::thread> th;
+        for (int r = 0; r < nt; r++)
+            th.emplace_back([&, r] {
+                cnt[(size_t)r] = (int64_t)std::count(text + rs[(size_t)r], text + rs[(size_t)r + 1], '\n');
+            });
+        for (auto& t : th) t.join();
+    }
+    std::vector<int64_t> first((size_t)nt + 1, 0);   // first line of each range
+    for (int r = 0; r < nt; r++) {
+        const bool tail = rs[(size_t)r + 1] == len && len > rs[(size_t)r] && text[len - 1] != '\n';
+        first[(size_t)r + 1] = first[(size_t)r] + cnt[(size_t)r] + (tail ? 1 : 0);
+    }
+    std::vector<int64_t> got((size_t)nt, 0), bad((size_t)nt, 0);
+    std::vector<int> rcs((size_t)nt, 0);
+    {
+        std::vector<std::thread> th;
+        for (int r = 0; r < nt; r++) {
+            const int64_t f = first[(size_t)r];
+            if (f >= nlines || rs[(size_t)r] >= rs[(size_t)r + 1]) continue;
+            const int64_t nl = std::min(first[(size_t)r + 1], nlines) - f;
+            th.emplace_back([&, r, f, nl] {
+                rcs[(size_t)r] = parse_lines(text, rs[(size_t)r + 1], rs[(size_t)r], f, nl, src + f, dst + f,
+                                             &got[(size_t)r], &bad[(size_t)r]);
+            });
+        }
+        for (auto& t : th) t.join();
+    }
+    // pairs of range r were written from index first[r]; move them down
+    int64_t np = 0;
+    for (int r = 0; r < nt; r++) {
+        const int64_t off = first[(size_t)r];
+        if (off >= nlines) break;
+        if (np != off && got[(size_t)r] > 0) {
+            std::memmove(src + np, src + off, sizeof(int64_t) * (size_t)got[(size_t)r]);
+            std::memmove(dst + np, dst + off, sizeof(int64_t) * (size_t)got[(size_t)r]);
+        }
+        np += got[(size_t)r];
+        if (rcs[(size_t)r] != 0) {
+            *npairs = np;
+            *consumed = rs[(size_t)r];
+            *bad_line = bad[(size_t)r];
+            return rcs[(size_t)r];
+        }
+    }
     *npairs = np;
+    // where the nlines lines end: inside the range that holds line nlines - 1
+    int64_t pos = len;
+    for (int r = 0; r < nt; r++) {
+        if (first[(size_t)r + 1] >= nlines) {
+            pos = rs[(size_t)r];
+            for (int64_t line = first[(size_t)r]; line < nlines && pos < len; line++) {
+                const char* nlp = static_cast<const char*>(std::memchr(text + pos, '\n', (size_t)(len - pos)));
+                pos = nlp ? (int64_t)(nlp - text) + 1 : len;
+            }
+            break;
+        }
+    }
     *consumed = pos;
     return 0;
 }

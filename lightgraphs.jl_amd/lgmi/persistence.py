@@ -163,9 +163,14 @@ def _parse_edges(r: _Reader, h: LGHeader):
     r.pos += int(consumed[0])
     k = int(npairs[0])
     s, d = s[:k], d[:k]
+    if k == 0:
+        return s, d
     info = np.iinfo(h.dtype)
-    if k and (min(s.min(), d.min()) < info.min or max(s.max(), d.max()) > info.max):
+    lo, hi = min(s.min(), d.min()), max(s.max(), d.max())
+    if lo < info.min or hi > info.max:
         raise OverflowError(f"graph {h.name!r}: an endpoint does not fit {_JULIA_NAME[h.dtype]}")
+    if lo >= 1 and hi <= h.nv:
+        return s, d
     keep = (s >= 1) & (s <= h.nv) & (d >= 1) & (d <= h.nv)
     return s[keep], d[keep]
 

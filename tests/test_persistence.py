@@ -169,3 +169,65 @@ def test_skip_and_comments(tmp_path):
     p.write_bytes(gzip.compress(text))
     from lgmi.persistence import _auto_decompress, _read_all
     assert _auto_decompress(_read_all(str(p))) == text
+
+
+def _py_parse(text: bytes, nlines: int):
+    """The line rules of _lg_read_one_graph (lg.jl:39-50) in plain Python:
+    (pairs, bytes consumed, first bad line or 0)."""
+    pos, pairs = 0, []
+    for line in range(1, nlines + 1):
+        if pos >= len(text):
+            break
+        e = text.find(b"\n", pos)
+        e = len(text) if e < 0 else e
+        nxt = e + 1 if e < len(text) else e
+        body = text[pos:e]
+        if e < len(text) and body.endswith(b"\r"):
+            body = body[:-1]
+        if body:
+            f = body.split(b",")
+            if len(f) < 2:
+                return pairs, pos, line
+            try:
+                pairs.append((int(f[0]), int(f[1])))
+            except ValueError:
+                return pairs, pos, line
+        pos = nxt
+    return pairs, pos, 0
+
+
+@pytest.mark.parametrize("trail_newline", [True, False])
+def test_parse_edges_large_threaded(trail_newline):
+    """Several MB of lines, so that the parser splits them over threads:
+    blank lines, "\\r\\n" endings, blanks around numbers, third fields, a
+    second graph after the first one's lines, and a last line without a
+    line break; against the plain-Python reading of the same rules."""
+    rng = np.random.default_rng(7)
+    n = 300_000
+    parts = []
+    for i in range(n):
+        r = rng.integers(0, 20)
+        a, b = rng.integers(-5, 10**9, size=2)
+        if r == 0:
+            parts.append(b"")
+        elif r == 1:
+            parts.append(b" %d , %d\r" % (a, b))
+        elif r == 2:
+            parts.append(b"%d,%d,%d" % (a, b, i))
+        else:
+            parts.append(b"%d,%d" % (a, b))
+    first = b"\n".join(parts) + b"\n"
+    second = b"3,1,u,next\n1,2" + (b"\n" if trail_newline else b"")
+    text = first + second
+    for nl in (n, n - 12_345, n + 7):
+        rc, pairs, consumed, bad = parse(text, nl)
+        want, wpos, wbad = _py_parse(text, nl)
+        assert rc == 0 and bad == 0 and wbad == 0
+        assert pairs == want and consumed == wpos
+    # a bad line late in the text is the one reported
+    bad_text = first[: len(first) // 2] + b"oops\n" + first[len(first) // 2:]
+    bad_line = bad_text[: len(first) // 2].count(b"\n") + 1
+    rc, pairs, _, bad = parse(bad_text, n + 1)
+    assert rc in (-1, -2) and bad == bad_line
+    want, _, wbad = _py_parse(bad_text, n + 1)
+    assert wbad == bad_line and pairs == want
