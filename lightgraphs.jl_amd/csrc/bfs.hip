@@ -71,6 +71,7 @@ constexpr int kGraphLevels = 16;    // level launches per captured graph (even: 
 constexpr int kPartRows = kBlock / kPart;
 constexpr int32_t kRangeFront = 2048;    // range mode: frontier vertices at most
 constexpr int64_t kRangeLight = 32768;   // range mode: light-row frontier edges at most
+constexpr int kEmit = 8;                 // range mode: 1024-vertex slices emitted per round
 
 struct StepArgs {
     const int32_t* td_rp;     // traversal adjacency (top-down expands it)
@@ -660,6 +661,34 @@ __device__ void range_step(const StepArgs& a, int32_t L, const uint32_t* FL, uin
     int32_t* s_sb = s_su + kRangeFront;           // [kRangeFront] segment first entry
     int32_t* s_sx = s_sb + kRangeFront;           // [kRangeFront + 1] segment offsets (prefix)
     int32_t* s_ws = s_sx + kRangeFront + 1;       // [kWaves] wavefront totals
+    // segments: thread t takes frontier entries 2t and 2t + 1.  Their
+    // vertices, row bounds and heavy-row ids are loaded first (the three
+    // loads of a vertex together), so that their round trips overlap the
+    // sweep of the range's bitmap words below.
+    const int32_t* qin = a.qbuf + (size_t)(lpar ^ 1) * a.grid * kQCap;
+    int32_t fu[2] = {-1, -1};
+    #pragma unroll
+    for (int q = 0; q < 2; q++) {
+        const int32_t j = 2 * tid + q;
+        if (j < nf) {
+            int lo = 0, hi = a.grid;                  // largest b with s_qoff[b] <= j
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_qoff[mid] <= j) lo = mid;
+                else hi = mid;
+            }
+            fu[q] = qin[(size_t)lo * kQCap + (j - s_qoff[lo])];
+        }
+    }
+    int32_t fb[2] = {0, 0}, fe[2] = {0, 0}, fh[2] = {-1, -1};
+    #pragma unroll
+    for (int q = 0; q < 2; q++) {
+        if (fu[q] >= 0) {
+            fb[q] = a.td_rp[fu[q]];
+            fe[q] = a.td_rp[fu[q] + 1];
+            fh[q] = a.td_heavy_id[fu[q]];
+        }
+    }
     for (int32_t i = tid; i < nwr; i += kBlock) {
         const int32_t w = w0 + i;
         uint32_t sw = 0xffffffffu;
@@ -672,27 +701,18 @@ __device__ void range_step(const StepArgs& a, int32_t L, const uint32_t* FL, uin
         s_seen[i] = sw;
     }
     for (int32_t i = tid; i < RS; i += kBlock) s_par[i] = 0xffffffffu;
-    // segments: thread t takes frontier entries 2t and 2t + 1
-    const int32_t* qin = a.qbuf + (size_t)(lpar ^ 1) * a.grid * kQCap;
     int32_t len[2] = {0, 0};
     #pragma unroll
     for (int q = 0; q < 2; q++) {
-        const int32_t j = 2 * tid + q;
-        if (j < nf) {
-            int lo = 0, hi = a.grid;                  // largest b with s_qoff[b] <= j
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (s_qoff[mid] <= j) lo = mid;
-                else hi = mid;
-            }
-            const int32_t u = qin[(size_t)lo * kQCap + (j - s_qoff[lo])];
-            int32_t sb = a.td_rp[u], se = a.td_rp[u + 1];
+        if (fu[q] >= 0) {
+            int32_t sb = fb[q], se = fe[q];
             if (se - sb > kHeavyRow) {
-                const int32_t* t = a.td_rtab + (size_t)a.td_heavy_id[u] * (a.grid + 1) + blockIdx.x;
+                const int32_t* t = a.td_rtab + (size_t)fh[q] * (a.grid + 1) + blockIdx.x;
                 sb = t[0];
                 se = t[1];
             }
-            s_su[j] = u;
+            const int32_t j = 2 * tid + q;
+            s_su[j] = fu[q];
             s_sb[j] = sb;
             len[q] = se - sb;
         }
@@ -750,14 +770,14 @@ __device__ void range_step(const StepArgs& a, int32_t L, const uint32_t* FL, uin
         }
     }
     lds_barrier();
-    // emit: four 1024-vertex slices per round, so that the degree loads of
+    // emit: eight 1024-vertex slices per round, so that the degree loads of
     // their new vertices are in flight together
-    for (int32_t c0 = 0; c0 < RS; c0 += 4 * kBlock) {
-        int32_t v[4], d0[4], d1[4];
-        uint32_t par[4];
-        bool fresh[4];
+    for (int32_t c0 = 0; c0 < RS; c0 += kEmit * kBlock) {
+        int32_t v[kEmit], d0[kEmit], d1[kEmit];
+        uint32_t par[kEmit];
+        bool fresh[kEmit];
         #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < kEmit; q++) {
             const int32_t loc = c0 + q * kBlock + tid;
             par[q] = loc < RS ? s_par[loc] : 0xffffffffu;
             v[q] = R0 + loc;
@@ -766,7 +786,7 @@ __device__ void range_step(const StepArgs& a, int32_t L, const uint32_t* FL, uin
             if (fresh[q]) { d0[q] = a.td_rp[v[q]]; d1[q] = a.td_rp[v[q] + 1]; }
         }
         #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < kEmit; q++) {
             const int32_t base = c0 + q * kBlock + wid * 64;
             const uint64_t m = __ballot(fresh[q]);
             if (lane < 2 && base < RS) {
