@@ -708,6 +708,8 @@ constexpr int kMaxSlotsP = 1024;        // slots of one persistent launch
 constexpr int kMaxWordsP = kMaxSlotsP / 64;
 constexpr int kMaxGridP = 4096;
 
+constexpr int kPush0 = 16;   // level-0 push: wavefronts per source row
+
 struct PersistArgs {
     BcPersistCtl* ctl;
     uint64_t* found;
@@ -871,12 +873,16 @@ __global__ __launch_bounds__(kBlock) void bc_persist(BcArgs a, PersistArgs p) {
         c->levels = 0;
         c->barriers = 0;
     }
-    // clear seen, F_0 and the partial betweenness of every slot
+    // clear seen, F_0 (and F_1, which the level-0 push below fills with
+    // atomics) and the partial betweenness of every slot
     for (int64_t i = (int64_t)vb * kBlock + tid; i < (int64_t)ns * nv; i += (int64_t)G * kBlock) {
         const int64_t sl = i / nv;
         a.seen[i] = 0;
         a.F[fidx(a, (int)sl, 0, (int32_t)(i - sl * nv))] = 0;
-        if constexpr (MODE != 2) a.bc_acc[i] = 0.0;
+        if constexpr (MODE != 2) {
+            a.F[fidx(a, (int)sl, 1, (int32_t)(i - sl * nv))] = 0;
+            a.bc_acc[i] = 0.0;
+        }
     }
     if constexpr (MODE == 2) {
         const int64_t nst = (int64_t)ns * (a.nchunks + a.hin.nh) * 192;
@@ -916,12 +922,41 @@ __global__ __launch_bounds__(kBlock) void bc_persist(BcArgs a, PersistArgs p) {
         }
         for (int w = tid; w < W; w += kBlock) s_found[w] = 0;
         __syncthreads();
-        for (uint32_t i = gw, r = 0; i < nitems_f; i += nw, r++) {
-            const int sl = (int)((i + r) % (uint32_t)ns);
-            const int32_t q = (int32_t)(i / (uint32_t)ns);
-            if (!((s_live[sl >> 6] >> (sl & 63)) & 1)) continue;
-            const bool got = q < a.hin.nseg ? fwd_seg<MODE>(a, sl, q, L, lane) : fwd_chunk<MODE>(a, sl, q - a.hin.nseg, L, lane);
-            if (got && lane == 0) atomicOr((unsigned long long*)&s_found[sl >> 6], 1ull << (sl & 63));
+        const bool push = MODE != 2 && L == 0;
+        if (push) {
+            // Level 0 -> 1 by a push from the sources rather than a pull over
+            // every vertex: lane i's level 1 is out(s_i) minus s_i itself, with
+            // sigma 1 (a simple graph has one edge s_i -> v) -- for stress, one
+            // predecessor -- which is what the pull computes.  Item (slot,
+            // lane, part) takes every kPush0-th 64-entry round of the source's
+            // row, so that a hub source is spread over kPush0 wavefronts.
+            for (uint32_t i = gw; i < (uint32_t)ns * 64u * (uint32_t)kPush0; i += nw) {
+                const int sl = (int)(i / (64u * (uint32_t)kPush0));
+                const int ln = (int)((i / (uint32_t)kPush0) & 63u);
+                const int part = (int)(i % (uint32_t)kPush0);
+                if (!((s_live[sl >> 6] >> (sl & 63)) & 1)) continue;
+                const int32_t src = p.lane_src[sl * 64 + ln];
+                if (src < 0) continue;
+                const int32_t b = a.out_rp[src], e = a.out_rp[src + 1];
+                bool got = false;
+                for (int32_t k = b + part * 64 + lane; k < e; k += kPush0 * 64) {
+                    const int32_t v = a.out_col[k];
+                    if (v == src) continue;
+                    atomicOr((unsigned long long*)&a.F[fidx(a, sl, 1, v)], 1ull << ln);
+                    atomicOr((unsigned long long*)&a.seen[(size_t)sl * nv + v], 1ull << ln);
+                    a.sig[((size_t)sl * nv + v) * 64 + ln] = 1.0;
+                    got = true;
+                }
+                if (__ballot(got) && lane == 0) atomicOr((unsigned long long*)&s_found[sl >> 6], 1ull << (sl & 63));
+            }
+        } else {
+            for (uint32_t i = gw, r = 0; i < nitems_f; i += nw, r++) {
+                const int sl = (int)((i + r) % (uint32_t)ns);
+                const int32_t q = (int32_t)(i / (uint32_t)ns);
+                if (!((s_live[sl >> 6] >> (sl & 63)) & 1)) continue;
+                const bool got = q < a.hin.nseg ? fwd_seg<MODE>(a, sl, q, L, lane) : fwd_chunk<MODE>(a, sl, q - a.hin.nseg, L, lane);
+                if (got && lane == 0) atomicOr((unsigned long long*)&s_found[sl >> 6], 1ull << (sl & 63));
+            }
         }
         __syncthreads();
         uint64_t* fo = p.found + ((size_t)(L & 1) * G + vb) * W;
@@ -929,7 +964,8 @@ __global__ __launch_bounds__(kBlock) void bc_persist(BcArgs a, PersistArgs p) {
         if (!grid_sync(c, p.rel, vb, G, gen, p.timeout, &s_ok)) return;
         nbar++;
         // heavy rows: one wavefront per (row, live slot) adds the partials
-        for (uint32_t i = gw, r = 0; i < nrows_f; i += nw, r++) {
+        // (none after the level-0 push, which wrote every row directly)
+        for (uint32_t i = gw, r = 0; !push && i < nrows_f; i += nw, r++) {
             const int sl = (int)((i + r) % (uint32_t)ns);
             if ((s_live[sl >> 6] >> (sl & 63)) & 1) fwd_combine<MODE>(a, sl, (int32_t)(i / (uint32_t)ns), L, lane);
         }
