@@ -168,7 +168,7 @@ __device__ void brandes_source(const int32_t* __restrict__ rp, const int32_t* __
     constexpr int NW = TEAM / 64;
     constexpr int kRows = 64 / KG;
     constexpr int kHeavy = 2 * KG * SL; // longer rows use a whole wavefront
-    const int lane = t & 63, w = t >> 6;
+    const int lane = t & 63, w = __builtin_amdgcn_readfirstlane(t >> 6);
     const int gl = lane & (KG - 1), grp = lane / KG;
     const int32_t step = NW * kRows;
     if (t == 0) {
@@ -309,7 +309,7 @@ __global__ __launch_bounds__(kBB) void bc_batch_kernel(BatchArgs a) {
     extern __shared__ double s_sig[];
     __shared__ int s_task;
     __shared__ int s_qc[kBW];
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     int32_t* queue_b = a.queue + (size_t)blockIdx.x * a.scr;
     double* coef_b = a.coef + (size_t)blockIdx.x * a.scr;
     int32_t* loff_b = a.loff + (size_t)blockIdx.x * (a.scr + 2 * kBW);

@@ -234,7 +234,7 @@ constexpr int kScanTile = kScanBlock * kScanPer;
 
 __global__ __launch_bounds__(kScanBlock) void scan_tiles(const int32_t* in, int32_t* out, int32_t n, int32_t* sums) {
     __shared__ int32_t s_wave[kScanBlock / 64];
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int64_t base = (int64_t)blockIdx.x * kScanTile + (int64_t)tid * kScanPer;
     int32_t x[kScanPer];
     int32_t t = 0;

@@ -72,7 +72,7 @@ __device__ inline void mlp_forward(const float* p, const float* xrow, int lane, 
 __global__ __launch_bounds__(kBlock) void mlp_grad_kernel(MlpArgs a) {
     __shared__ float s_red[kWaves][kFeat + 3];     // per wave, per lane staged below
     __shared__ float s_g[kWaves][kTrain + 1];
-    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     const float* W2 = a.p + kHidden * kFeat + kHidden;
     float gW1[kFeat];
     #pragma unroll
