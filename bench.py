@@ -88,6 +88,26 @@ def bfs_bench(args, lgmi, _lib, orc):
     gteps = edges / elapsed / 1e9
     log(f"timed {K} BFS: {elapsed * 1e3:.2f} ms total, {gteps:.2f} GTEPS")
 
+    # the same timed runs with the learned direction switch (outputs are the
+    # same by construction; the GPU tests check it)
+    gteps_learned = None
+    from lgmi import policy
+    params, _ = policy.load_weights()
+    if params is not None:
+        try:
+            policy.set_learned(params)
+            for i in range(args.warmup):
+                _lib.check(lib.lgmi_bfs_run(h, int(roots[i % len(roots)]), 0, 1, None))
+            _lib.check(lib.lgmi_synchronize())
+            t_l = time.perf_counter()
+            for i in range(K):
+                _lib.check(lib.lgmi_bfs_run(h, int(roots[i % len(roots)]), 0, 1, None))
+            _lib.check(lib.lgmi_synchronize())
+            gteps_learned = edges / (time.perf_counter() - t_l) / 1e9
+            log(f"timed {K} BFS with the learned switch: {gteps_learned:.2f} GTEPS")
+        finally:
+            policy.set_heuristic()
+
     # per-root device times (HIP events) for the harmonic mean and median
     per_root = []
     for i, r in enumerate(roots):
@@ -159,6 +179,7 @@ def bfs_bench(args, lgmi, _lib, orc):
             "nv": nv, "nnz": nnz, "roots": len(roots), "policy": "alpha=15/beta=18 heuristic",
             "gteps_harmonic_mean_device": round(float(len(per_root) / np.sum(1.0 / per_root)), 3),
             "gteps_median_device": round(float(np.median(per_root)), 3),
+            "gteps_learned_switch": round(gteps_learned, 4) if gteps_learned is not None else None,
         },
         "roofline": {
             "kernel": "bfs_level",
