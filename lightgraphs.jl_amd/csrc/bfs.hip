@@ -196,7 +196,7 @@ __device__ inline int32_t batch_vertex(uint32_t cw, int32_t excl, int32_t total,
 
 // Features of the level state (LGMI_POLICY_FEATURES), in FP32 (the
 // hardware log2).
-__device__ void compute_features(const BfsCtrl* c, float* f) {
+__device__ __attribute__((always_inline)) inline void compute_features(const BfsCtrl* c, float* f) {
     const float nf = (float)c->n_front, mf = (float)c->m_front, mu = (float)c->m_unvisited;
     const float nu = (float)c->n_unvisited;
     const float lmf = flog2p(mf), lmu = flog2p(mu), lnf = flog2p(nf);
@@ -333,7 +333,7 @@ __device__ inline void expand_row(const StepArgs& a, int32_t u, int32_t b, int32
 
 // Top-down step L -> L+1 of one wavefront (gw of nw).
 template <bool PARENTS>
-__device__ void td_step(const StepArgs& a, const BfsCtrl& c, int32_t L, const uint32_t* FL, uint32_t* Fn,
+__device__ __attribute__((always_inline)) inline void td_step(const StepArgs& a, const BfsCtrl& c, int32_t L, const uint32_t* FL, uint32_t* Fn,
                         uint32_t* Fc, Counters& k, uint32_t* s_hist, int32_t* qout, int gw, int nw, int lane) {
     const int32_t ntiles = (a.nwords + kTileWords - 1) / kTileWords;
     const int sh8 = (lane & 3) * kLaneBits;
@@ -447,7 +447,7 @@ __device__ inline int bu_found(const StepArgs& a, int32_t v, int32_t par, int32_
 // kLaneRounds rounds; rows still open after that are scanned by the whole
 // wavefront, 64 entries per round trip.
 template <bool PARENTS>
-__device__ void bu_drain(const StepArgs& a, const Pending* q, int n, int32_t v0, int32_t L, const uint32_t* FL,
+__device__ __attribute__((always_inline)) inline void bu_drain(const StepArgs& a, const Pending* q, int n, int32_t v0, int32_t L, const uint32_t* FL,
                          Counters& k, uint32_t* s_hist, int32_t* qout, uint32_t* s_fn, int lane) {
     constexpr int kLaneRounds = 4;
     int32_t v = -1, b = 0, e = 0, b0 = 0;
@@ -499,7 +499,7 @@ __device__ void bu_drain(const StepArgs& a, const Pending* q, int n, int32_t v0,
 // v0 + 8l + 7; excl its exclusive popcount prefix, total > 0 their number):
 // stage one, the found vertices recorded, and the open ones drained.
 template <bool PARENTS>
-__device__ inline Counters bu_tile(const StepArgs& a, int32_t L, const uint32_t* FL, uint32_t* s_hist,
+__device__ __attribute__((always_inline)) inline Counters bu_tile(const StepArgs& a, int32_t L, const uint32_t* FL, uint32_t* s_hist,
                                          int32_t* qout, uint32_t* s_fn, Pending* s_q, uint32_t cw, int32_t excl,
                                          int32_t total, int32_t word0, int lane) {
     Counters k;
@@ -586,7 +586,7 @@ __device__ inline Counters bu_tile(const StepArgs& a, int32_t L, const uint32_t*
 // every candidate's first entry; candidates left open go to a per-wavefront
 // LDS queue that bu_drain empties 64 at a time.
 template <bool PARENTS>
-__device__ void bu_step(const StepArgs& a, int32_t L, const uint32_t* FL, uint32_t* Fn, uint32_t* Fc,
+__device__ __attribute__((always_inline)) inline void bu_step(const StepArgs& a, int32_t L, const uint32_t* FL, uint32_t* Fn, uint32_t* Fc,
                         Counters& k, uint32_t* s_hist, int32_t* qout, uint32_t* s_fn, Pending* s_q, int gw, int nw,
                         int lane) {
     const int32_t ntiles = (a.nwords + kTileWords - 1) / kTileWords;
@@ -648,7 +648,7 @@ __device__ void bu_step(const StepArgs& a, int32_t L, const uint32_t* FL, uint32
 // is formed with LDS atomics, and distances, parents and the range's
 // F_{k+1} words are written with plain stores.  No device-scope atomics.
 template <bool PARENTS>
-__device__ void range_step(const StepArgs& a, int32_t L, const uint32_t* FL, uint32_t* Fn, uint32_t* Fc,
+__device__ __attribute__((always_inline)) inline void range_step(const StepArgs& a, int32_t L, const uint32_t* FL, uint32_t* Fn, uint32_t* Fc,
                            Counters& k, uint32_t* s_hist, int32_t* qout, uint32_t* s_lds, const int32_t* s_qoff,
                            int32_t nf, int lpar, int tid, int lane, int wid) {
     const int32_t RS = a.range_size;              // a multiple of 64
@@ -860,6 +860,9 @@ __device__ inline int mlp_direction(const float* xs, const PolicyRegs& pr) {
 // (captured graphs): p is unknown to the host but has parity -1 - karg, and
 // p is read from the state the launch before stored.  forced >= 0 forces
 // the direction.
+// Every device function the level kernel calls is inlined (always_inline
+// on the large ones): a call that stays a call makes the kernel pass its
+// arguments through per-lane scratch memory.
 template <bool PARENTS>
 __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, int32_t forced) {
     __shared__ BfsCtrl s_c;
