@@ -113,6 +113,13 @@ class _Reader:
             line = line[:-1]
         return line.decode()
 
+    def lines_left(self) -> int:
+        """Lines from the current position to the end (the last one may
+        lack its line break)."""
+        if self._nl is None:
+            self._nl = np.flatnonzero(self._view[:len(self.data)] == 10)
+        return int(len(self._nl) - np.searchsorted(self._nl, self.pos)) + 1
+
     def skip_lines(self, n: int):
         """_lg_skip_one_graph (lg.jl:54-58): n readlines, not parsed."""
         if n <= 0:
@@ -144,13 +151,15 @@ def _parse_edges(r: _Reader, h: LGHeader):
     """The header's ne edge lines (lg.jl:39-50) as 1-based (src, dst)
     arrays, the pairs add_edge! would drop (an endpoint outside 1:nv)
     removed; advances the reader past the lines."""
-    cap = max(h.ne, 1)
+    # a header may promise more lines than the text holds; those read as ""
+    nl = max(0, min(h.ne, r.lines_left()))
+    cap = max(nl, 1)
     s = np.empty(cap, dtype=np.int64)
     d = np.empty(cap, dtype=np.int64)
     npairs = np.zeros(1, dtype=np.int64)
     consumed = np.zeros(1, dtype=np.int64)
     bad = np.zeros(1, dtype=np.int64)
-    rc = _lib.load().lgmi_lg_parse_edges(r.address() + r.pos, len(r.data) - r.pos, h.ne, _lib.i64p(s),
+    rc = _lib.load().lgmi_lg_parse_edges(r.address() + r.pos, len(r.data) - r.pos, nl, _lib.i64p(s),
                                          _lib.i64p(d), _lib.i64p(npairs), _lib.i64p(consumed), _lib.i64p(bad))
     if rc == -1:
         raise IndexError(f"graph {h.name!r}, edge line {int(bad[0])}: expected 'src,dst'")

@@ -231,3 +231,15 @@ def test_parse_edges_large_threaded(trail_newline):
     assert rc in (-1, -2) and bad == bad_line
     want, _, wbad = _py_parse(bad_text, n + 1)
     assert wbad == bad_line and pairs == want
+
+
+def test_header_promising_more_lines_than_the_text():
+    """A header's ne beyond the end of the text: the missing lines read as
+    empty (readline at the end of a stream), and no ne-sized buffer is
+    made."""
+    from lgmi import persistence as P
+    text = b"5,1000000000000,u,g\n1,2\n3,4"
+    r = P._Reader(text)
+    h = P._parse_header(r.readline().strip())
+    s, d = P._parse_edges(r, h)
+    assert s.tolist() == [1, 3] and d.tolist() == [2, 4] and r.eof()
