@@ -352,7 +352,7 @@ if __name__ == "__main__":
     import argparse
     import sys
     ap = argparse.ArgumentParser(description="train and evaluate the learned BFS direction policy")
-    ap.add_argument("--out", default="gpurun_out/policy")
+    ap.add_argument("--out", default="policy_out")
     ap.add_argument("--train-roots", type=int, default=2000)
     ap.add_argument("--eval-roots", type=int, default=200)
     ap.add_argument("--grid-train-roots", type=int, default=40)

@@ -1,6 +1,6 @@
-"""Writes the golden fixtures under tests/golden/ (run in the build
-container, where the reference tree and networkx are available; the GPU box
-only reads the committed .npz files).
+"""Writes the golden fixtures under tests/golden/ (run where the reference
+tree and networkx are available; the GPU test runs only read the committed
+.npz files).
 
 * ref_graph_50_500.npz -- the reference's own fixture: the directed graph of
   test/testdata/graph-50-500.jgz (LG text format, gzip), the FP64
