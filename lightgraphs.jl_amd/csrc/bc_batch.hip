@@ -8,9 +8,13 @@
 // (src/shortestpaths/dijkstra.jl:59-141).
 //
 // Work decomposition: a task is (graph, contiguous source range); graphs
-// whose n * (nnz + n) exceeds the per-task target are split.  Tasks are
-// ordered by decreasing work and claimed by persistent 1024-thread
-// work-groups from an atomic counter.  A task runs in one of two modes:
+// whose n * (nnz + n) exceeds the per-task target are split (by default the
+// largest graph into about 128 tasks).  The graphs are dealt to eight task
+// queues, one per XCD, largest first to the least-loaded queue; persistent
+// 1024-thread work-groups claim tasks from their XCD's queue (decreasing
+// work) with an atomic counter, then from the other queues, so that the
+// work-groups sharing an L2 work on the same few graphs at a time.  A task
+// runs in one of two modes:
 //  * block mode: the whole work-group runs one single-source search at a
 //    time, with the per-vertex search state in LDS (160 KB holds 20 000
 //    vertices), or in global scratch for larger graphs;
@@ -56,6 +60,11 @@ namespace {
 
 constexpr int kBB = 1024;           // threads per work-group
 constexpr int kBW = kBB / 64;       // wavefronts per work-group
+// Task queues, one per XCD: work-group b (placed on XCD b mod 8 by the
+// dispatcher's round-robin) takes from queue b mod kXq first, so that the
+// work-groups sharing an L2 work on the same few graphs and share their
+// adjacency there; a work-group whose queue is empty takes from the others.
+constexpr int kXq = 8;
 
 struct BatchArgs {
     const int32_t* nv;        // [ngraphs]
@@ -72,7 +81,8 @@ struct BatchArgs {
     int32_t lds_max_n;        // the state fits in LDS for n <= this (block mode)
     int32_t max_n;
     double* part;             // task partial betweenness
-    int32_t* task_ctr;
+    int32_t* task_ctr;        // [kXq][32]: one counter per queue, 128 bytes apart
+    const int32_t* qoff;      // [kXq + 1] queue q holds tasks qoff[q] .. qoff[q + 1] - 1
     // per-block scratch (global); wave mode slices it per wavefront
     int32_t* queue;           // [blocks][scr]
     double* coef;             // [blocks][scr] new backward states, by queue position
@@ -313,12 +323,24 @@ __global__ __launch_bounds__(kBB) void bc_batch_kernel(BatchArgs a) {
     int32_t* queue_b = a.queue + (size_t)blockIdx.x * a.scr;
     double* coef_b = a.coef + (size_t)blockIdx.x * a.scr;
     int32_t* loff_b = a.loff + (size_t)blockIdx.x * (a.scr + 2 * kBW);
+    uint32_t empty = 0;   // queues this work-group has found empty (thread 0)
     for (;;) {
-        if (tid == 0) s_task = atomicAdd(a.task_ctr, 1);
+        if (tid == 0) {
+            int task = -1;
+            for (int k = 0; k < kXq && task < 0; k++) {
+                const int q = (int)((blockIdx.x + (unsigned)k) % kXq);
+                if ((empty >> q) & 1u) continue;
+                const int len = a.qoff[q + 1] - a.qoff[q];
+                const int t = len > 0 ? atomicAdd(&a.task_ctr[q * 32], 1) : len;
+                if (t < len) task = a.qoff[q] + t;
+                else empty |= 1u << q;
+            }
+            s_task = task;
+        }
         __syncthreads();
         const int task = s_task;
         __syncthreads();
-        if (task >= a.ntasks) break;
+        if (task < 0) break;
         const int32_t gi = a.task_graph[task];
         const int32_t n = a.nv[gi];
         const int32_t s0 = a.task_s0[task], s1 = a.task_s1[task];
@@ -498,15 +520,28 @@ extern "C" int lgmi_betweenness_batch(int32_t ngraphs, const int32_t* nv, const 
 
         // ---- tasks: split graphs by work, largest first ----
         std::vector<double> work((size_t)ngraphs);
-        double total_work = 0.0;
         for (int32_t g = 0; g < ngraphs; g++) {
             const int32_t* rp = row_ptr_cat + row_off[g];
             work[(size_t)g] = (double)nv[g] * (double)(rp[nv[g]] + nv[g]);
-            total_work += work[(size_t)g];
         }
         const int nblocks = std::max(1, cu);
-        const double target = tuning().task_work_target > 0 ? tuning().task_work_target
-                                                             : std::max(1.0, total_work / ((double)nblocks * 6.0));
+        // Task size: by default the largest graph is cut into about 128 tasks,
+        // so that the 32 work-groups of its queue's XCD all work on it, with
+        // the tasks' partial arrays held to 8 GB (each task keeps one FP64
+        // value per vertex of its graph; the target doubles until they fit).
+        double target = tuning().task_work_target;
+        if (!(target > 0)) {
+            double wmax = 1.0;
+            for (int32_t g = 0; g < ngraphs; g++) wmax = std::max(wmax, work[(size_t)g]);
+            target = std::max(1.0, wmax / 128.0);
+            auto part_bytes = [&](double t) {
+                double b = 0.0;
+                for (int32_t g = 0; g < ngraphs; g++)
+                    b += std::max(1.0, std::min((double)nv[g], std::ceil(work[(size_t)g] / t))) * (double)nv[g] * 8.0;
+                return b;
+            };
+            while (part_bytes(target) > 8e9) target *= 2.0;
+        }
         struct Task { int32_t g, s0, s1; double w; };
         std::vector<Task> tasks;
         for (int32_t g = 0; g < ngraphs; g++) {
@@ -520,9 +555,33 @@ extern "C" int lgmi_betweenness_batch(int32_t ngraphs, const int32_t* nv, const 
                 if (s1 > s0) tasks.push_back({g, s0, s1, work[(size_t)g] * (double)(s1 - s0) / (double)n});
             }
         }
+        // graphs to queues: largest work first, each to the least-loaded
+        // queue; a queue's tasks in decreasing work
+        std::vector<int32_t> gq((size_t)ngraphs, 0);
+        {
+            std::vector<int32_t> gord((size_t)ngraphs);
+            std::iota(gord.begin(), gord.end(), 0);
+            std::stable_sort(gord.begin(), gord.end(),
+                             [&](int32_t x, int32_t y) { return work[(size_t)x] > work[(size_t)y]; });
+            double load[kXq] = {};
+            for (int32_t g : gord) {
+                int q = 0;
+                for (int k = 1; k < kXq; k++)
+                    if (load[k] < load[q]) q = k;
+                gq[(size_t)g] = q;
+                load[q] += work[(size_t)g];
+            }
+        }
         std::vector<int32_t> order(tasks.size());
         std::iota(order.begin(), order.end(), 0);
-        std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) { return tasks[x].w > tasks[y].w; });
+        std::stable_sort(order.begin(), order.end(), [&](int32_t x, int32_t y) {
+            const int32_t qx = gq[(size_t)tasks[x].g], qy = gq[(size_t)tasks[y].g];
+            if (qx != qy) return qx < qy;
+            return tasks[x].w > tasks[y].w;
+        });
+        std::vector<int32_t> qoff(kXq + 1, 0);
+        for (const Task& t : tasks) qoff[(size_t)gq[(size_t)t.g] + 1]++;
+        for (int q = 0; q < kXq; q++) qoff[(size_t)q + 1] += qoff[(size_t)q];
         const int32_t ntasks = (int32_t)tasks.size();
         std::vector<int32_t> tg(ntasks), ts0(ntasks), ts1(ntasks);
         std::vector<int64_t> tpart(ntasks);
@@ -593,8 +652,9 @@ extern "C" int lgmi_betweenness_batch(int32_t ngraphs, const int32_t* nv, const 
             a.lds_max_n = lds_max_n;
             a.max_n = max_n;
             a.part = (double*)dalloc(sizeof(double) * (size_t)std::max<int64_t>(part_total, 1));
-            a.task_ctr = (int32_t*)dalloc(sizeof(int32_t));
-            LGMI_HIP(hipMemsetAsync(a.task_ctr, 0, sizeof(int32_t), st));
+            a.task_ctr = (int32_t*)dalloc(sizeof(int32_t) * kXq * 32);
+            LGMI_HIP(hipMemsetAsync(a.task_ctr, 0, sizeof(int32_t) * kXq * 32, st));
+            a.qoff = (const int32_t*)up(qoff.data(), sizeof(int32_t) * (kXq + 1));
             a.scr = std::max<int64_t>({(int64_t)max_n, (int64_t)kBW * wave_max_n, 1});
             a.queue = (int32_t*)dalloc(sizeof(int32_t) * (size_t)nblocks * a.scr);
             a.coef = (double*)dalloc(sizeof(double) * (size_t)nblocks * a.scr);
