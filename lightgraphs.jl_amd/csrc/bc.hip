@@ -1348,10 +1348,13 @@ static bool bc_persistent(lgmi_graph* g, BcArgs a, int32_t ns, const double* end
 // have nearly the same distances to everything else (offset by one at most),
 // so the sources are ordered by that neighbour (highest degree first), then
 // by their own degree, then by position: sources with alike searches end up
-// in the same lane group and the same quarter.  Betweenness and stress sum
-// over the sources, so only the floating-point summation order changes; the
-// order is a function of the graph and the source list alone.
-static void order_sources(lgmi_graph* g, std::vector<int32_t>& act) {
+// in the same lane group and the same quarter (and, for the distance
+// statistics, a vertex's unseen lanes tend to be found by the same
+// neighbour, which ends its scan early).  Betweenness and stress sum over
+// the sources, so only the floating-point summation order changes; the
+// distance statistics go back to their sources' positions (pos).  The order
+// is a function of the graph and the source list alone.
+static void order_sources(lgmi_graph* g, std::vector<int32_t>& act, std::vector<int32_t>& pos) {
     const int32_t n = (int32_t)act.size();
     int32_t* d = dev_alloc_n<int32_t>((size_t)n * 2);
     std::vector<int32_t> hub((size_t)n);
@@ -1379,8 +1382,13 @@ static void order_sources(lgmi_graph* g, std::vector<int32_t>& act) {
         return sx > sy;
     });
     std::vector<int32_t> out((size_t)n);
-    for (int32_t i = 0; i < n; i++) out[(size_t)i] = act[(size_t)idx[(size_t)i]];
+    std::vector<int32_t> pos2((size_t)n);
+    for (int32_t i = 0; i < n; i++) {
+        out[(size_t)i] = act[(size_t)idx[(size_t)i]];
+        pos2[(size_t)i] = pos[(size_t)idx[(size_t)i]];
+    }
     act.swap(out);
+    pos.swap(pos2);
 }
 
 void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, int skip_degree0, int normalize,
@@ -1402,7 +1410,11 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
         if (!skip_degree0 || degree(srcs[i]) > 0) act.push_back(srcs[i]);
     const int64_t nact = (int64_t)act.size();
     const int32_t ngroups = (int32_t)((nact + 63) / 64);
-    if (mode != 2 && nact > 16) order_sources(g, act);
+    // pos[i]: the position among the active sources that act[i] had before
+    // the ordering (the distance statistics are returned in that order)
+    std::vector<int32_t> pos((size_t)nact);
+    for (int64_t i = 0; i < nact; i++) pos[(size_t)i] = (int32_t)i;
+    if (nact > 16) order_sources(g, act, pos);
 
     // slots: as many groups at once as fit a memory budget
     const int32_t cap0 = mode == 2 ? 2 : 32;
@@ -1507,9 +1519,10 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
             for (int l = 0; l < 64; l++) {
                 const int64_t idx = (int64_t)(g0 + s) * 64 + l;
                 if (idx >= nact) continue;
-                dist_stats[idx * 3 + 0] = st_host[(size_t)s * 192 + l];
-                dist_stats[idx * 3 + 1] = st_host[(size_t)s * 192 + 64 + l] + 1;
-                dist_stats[idx * 3 + 2] = st_host[(size_t)s * 192 + 128 + l];
+                const int64_t o = pos[(size_t)idx];
+                dist_stats[o * 3 + 0] = st_host[(size_t)s * 192 + l];
+                dist_stats[o * 3 + 1] = st_host[(size_t)s * 192 + 64 + l] + 1;
+                dist_stats[o * 3 + 2] = st_host[(size_t)s * 192 + 128 + l];
             }
     };
     float fwd_ms = 0.f, bwd_ms = 0.f;
