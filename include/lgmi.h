@@ -307,7 +307,9 @@ int lgmi_betweenness_batch(int32_t ngraphs, const int32_t* nv, const int64_t* ro
  * larger ones one source per work-group (-1: the largest n whose 16 sigma
  * slices fit in LDS, at most 1024); task_work_target is the n * (nnz + n)
  * work above which a graph's sources are split over several tasks
- * (0: total work / (6 * number of CUs)); graphs with more than lds_max_n
+ * (0: the largest graph's work / 128, doubled until the tasks' partial
+ * arrays, one FP64 value per vertex each, fit in 8 GB); graphs with more
+ * than lds_max_n
  * vertices keep the per-vertex search state in global memory instead of LDS
  * (-1: as many as LDS holds; values above that are clamped); lane_group
  * (1, 2, 4 or 8; 0: 2) is the number of lanes that expand one adjacency
