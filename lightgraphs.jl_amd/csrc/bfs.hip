@@ -66,7 +66,7 @@ constexpr int kLaneBits = 8;                    // frontier bits per lane in a t
 constexpr int kTileVerts = 64 * kLaneBits;      // vertices per wavefront tile
 constexpr int kTileWords = kTileVerts / 32;     // bitmap words per tile
 constexpr int32_t kWaveRow = 32;    // top-down: rows this long or longer are expanded by a wavefront
-constexpr int kUnroll = 4;          // top-down: adjacency entries per lane per round
+constexpr int kUnroll = 2;          // top-down: adjacency entries per lane per round
 constexpr int kGraphLevels = 16;    // level launches per captured graph (even: parity is static)
 constexpr int kPartRows = kBlock / kPart;
 constexpr int32_t kRangeFront = 2048;    // range mode: frontier vertices at most
