@@ -117,15 +117,6 @@ __device__ inline int hist_bin(int32_t deg) {
 
 __device__ inline float flog2p(float x) { return __log2f(1.0f + (x > 0.f ? x : 0.f)); }
 
-// Position of the r-th (0-based) set bit of the byte x (r < popcount(x)).
-__device__ inline int select_bit8(uint32_t x, int r) {
-    int pos = 0, c;
-    c = __popc(x & 0xfu); if (r >= c) { r -= c; x >>= 4; pos += 4; }
-    c = __popc(x & 0x3u); if (r >= c) { r -= c; x >>= 2; pos += 2; }
-    c = (int)(x & 1u);    if (r >= c) { pos += 1; }
-    return pos;
-}
-
 // Exclusive prefix sum over the 64 lanes; total in `total`.
 __device__ inline int32_t wave_excl_scan(int32_t x, int lane, int32_t& total) {
     int32_t incl = x;
@@ -177,21 +168,27 @@ __device__ inline int owner_lane(int32_t excl, int32_t g) {
     return lo;
 }
 
-// Lane `lane` of a batch starting at set-bit index `base` of a tile (lane l
-// holds the byte cw of vertices v0 + 8l .. v0 + 8l + 7, with exclusive
-// popcount prefix excl) gets the vertex of set bit base + lane, or -1 past
-// the end.  Must be called by all 64 lanes.  lo/bit return the owning lane
-// and the bit inside its byte.
-__device__ inline int32_t batch_vertex(uint32_t cw, int32_t excl, int32_t total, int32_t base, int32_t v0,
-                                       int lane, int& lo, int& bit) {
-    const int32_t g = base + lane;
-    lo = owner_lane(excl, g);
-    const uint32_t byte = __shfl(cw, lo);
-    const int32_t r = g - __shfl(excl, lo);
-    bit = 0;
-    if (g >= total) return -1;
-    bit = select_bit8(byte, r);
-    return v0 + lo * kLaneBits + bit;
+// The set bits of a tile as a list: lane l holds the byte cw of vertices
+// v0 + 8l .. v0 + 8l + 7, with exclusive popcount prefix excl, and writes
+// those vertices in ascending order to entries excl .. of the wavefront's
+// LDS list (kTileVerts entries); the lanes then read it 64 entries (one
+// batch) at a time.  One LDS write per bit position and one read per batch
+// take the place of a per-batch rank search over the lanes (a chain of
+// dependent cross-lane reads).  Must be called by all 64 lanes.
+__device__ __attribute__((always_inline)) inline void tile_list(int32_t* s_list, uint32_t cw, int32_t excl,
+                                                                int32_t v0, int lane) {
+    int32_t p = excl;
+    const int32_t vb = v0 + lane * kLaneBits;
+    #pragma unroll
+    for (int j = 0; j < kLaneBits; j++) {
+        if ((cw >> j) & 1u) {
+            s_list[p] = vb + j;
+            p++;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 // Features of the level state (LGMI_POLICY_FEATURES), in FP32 (the
@@ -334,7 +331,8 @@ __device__ inline void expand_row(const StepArgs& a, int32_t u, int32_t b, int32
 // Top-down step L -> L+1 of one wavefront (gw of nw).
 template <bool PARENTS>
 __device__ __attribute__((always_inline)) inline void td_step(const StepArgs& a, const BfsCtrl& c, int32_t L, const uint32_t* FL, uint32_t* Fn,
-                        uint32_t* Fc, Counters& k, uint32_t* s_hist, int32_t* qout, int gw, int nw, int lane) {
+                        uint32_t* Fc, Counters& k, uint32_t* s_hist, int32_t* qout, int32_t* s_list, int gw, int nw,
+                        int lane) {
     const int32_t ntiles = (a.nwords + kTileWords - 1) / kTileWords;
     const int sh8 = (lane & 3) * kLaneBits;
     // Heavy rows: 256-entry items; lane j of wavefront gw tests item
@@ -388,10 +386,9 @@ __device__ __attribute__((always_inline)) inline void td_step(const StepArgs& a,
         }
         int32_t total;
         const int32_t excl = wave_excl_scan(__popc(cw), lane, total);
-        const int32_t v0 = word0 * 32;
+        if (total > 0) tile_list(s_list, cw, excl, word0 * 32, lane);
         for (int32_t base = 0; base < total; base += 64) {
-            int lo, bit;
-            const int32_t u = batch_vertex(cw, excl, total, base, v0, lane, lo, bit);
+            const int32_t u = base + lane < total ? s_list[base + lane] : -1;
             int32_t beg = 0, deg = 0;
             if (u >= 0) {
                 beg = a.td_rp[u];
@@ -500,8 +497,8 @@ __device__ __attribute__((always_inline)) inline void bu_drain(const StepArgs& a
 // stage one, the found vertices recorded, and the open ones drained.
 template <bool PARENTS>
 __device__ __attribute__((always_inline)) inline Counters bu_tile(const StepArgs& a, int32_t L, const uint32_t* FL, uint32_t* s_hist,
-                                         int32_t* qout, uint32_t* s_fn, Pending* s_q, uint32_t cw, int32_t excl,
-                                         int32_t total, int32_t word0, int lane) {
+                                         int32_t* qout, uint32_t* s_fn, Pending* s_q, int32_t* s_list, uint32_t cw,
+                                         int32_t excl, int32_t total, int32_t word0, int lane) {
     Counters k;
     const int32_t v0 = word0 * 32;
     int qn = 0;   // queued candidates (uniform over the wavefront)
@@ -514,14 +511,9 @@ __device__ __attribute__((always_inline)) inline Counters bu_tile(const StepArgs
     constexpr int kTB = kTileVerts / 64;
     const int nb = (total + 63) >> 6;   // uniform
     int32_t qv[kTB], qb[kTB], qe[kTB], qp[kTB];
+    tile_list(s_list, cw, excl, v0, lane);
     #pragma unroll
-    for (int q = 0; q < kTB; q++) {
-        qv[q] = -1;
-        if (q < nb) {
-            int lo, bit;
-            qv[q] = batch_vertex(cw, excl, total, q * 64, v0, lane, lo, bit);
-        }
-    }
+    for (int q = 0; q < kTB; q++) qv[q] = (q < nb && q * 64 + lane < total) ? s_list[q * 64 + lane] : -1;
     #pragma unroll
     for (int q = 0; q < kTB; q++) {
         qb[q] = 0;
@@ -587,8 +579,8 @@ __device__ __attribute__((always_inline)) inline Counters bu_tile(const StepArgs
 // LDS queue that bu_drain empties 64 at a time.
 template <bool PARENTS>
 __device__ __attribute__((always_inline)) inline void bu_step(const StepArgs& a, int32_t L, const uint32_t* FL, uint32_t* Fn, uint32_t* Fc,
-                        Counters& k, uint32_t* s_hist, int32_t* qout, uint32_t* s_fn, Pending* s_q, int gw, int nw,
-                        int lane) {
+                        Counters& k, uint32_t* s_hist, int32_t* qout, uint32_t* s_fn, Pending* s_q, int32_t* s_list,
+                        int gw, int nw, int lane) {
     const int32_t ntiles = (a.nwords + kTileWords - 1) / kTileWords;
     const int sh8 = (lane & 3) * kLaneBits;
     // the next tile's words are loaded before the current tile is scanned
@@ -621,7 +613,7 @@ __device__ __attribute__((always_inline)) inline void bu_step(const StepArgs& a,
         const int32_t excl = wave_excl_scan(__popc(cw), lane, total);
         k.cand += (lane == 0) ? (unsigned long long)total : 0ull;
         if (total > 0) {
-            const Counters kt = bu_tile<PARENTS>(a, L, FL, s_hist, qout, s_fn, s_q, cw, excl, total, word0, lane);
+            const Counters kt = bu_tile<PARENTS>(a, L, FL, s_hist, qout, s_fn, s_q, s_list, cw, excl, total, word0, lane);
             k.n += kt.n;
             k.out += kt.out;
             k.in += kt.in;
@@ -879,6 +871,9 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
     constexpr int kLdsWords = kRangeWords > (kWaves * 128 * 4) ? kRangeWords : (kWaves * 128 * 4);
     __shared__ uint32_t s_lds[kLdsWords];
     Pending* s_q = reinterpret_cast<Pending*>(s_lds);
+    // per-wavefront tile lists of the sweep steps, after the queues
+    static_assert(kLdsWords >= kWaves * 128 * 4 + kWaves * kTileVerts, "tile lists exceed the LDS area");
+    int32_t* s_lists = reinterpret_cast<int32_t*>(s_lds + kWaves * 128 * 4);
 
     // the wavefront index is uniform over a wavefront; readfirstlane lets the
     // compiler keep it, and the tile loops that depend on it, scalar
@@ -1047,9 +1042,10 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
     const int gw = blockIdx.x * kWaves + wid, nw = gridDim.x * kWaves;
     int32_t* qout = a.qbuf + ((size_t)lpar * a.grid + blockIdx.x) * kQCap;
     Counters k;
-    if (dir == 1) bu_step<PARENTS>(a, L, FL, Fn, Fc, k, s_hist, qout, s_fn[wid], s_q + wid * 128, gw, nw, lane);
+    if (dir == 1) bu_step<PARENTS>(a, L, FL, Fn, Fc, k, s_hist, qout, s_fn[wid], s_q + wid * 128,
+                                  s_lists + wid * kTileVerts, gw, nw, lane);
     else if (range) range_step<PARENTS>(a, L, FL, Fn, Fc, k, s_hist, qout, s_lds, s_qoff, s_qoff[a.grid], lpar, tid, lane, wid);
-    else td_step<PARENTS>(a, s_c, L, FL, Fn, Fc, k, s_hist, qout, gw, nw, lane);
+    else td_step<PARENTS>(a, s_c, L, FL, Fn, Fc, k, s_hist, qout, s_lists + wid * kTileVerts, gw, nw, lane);
 
     // block reduction -> this block's counters (plain stores)
     // A work-group's counters of one step fit in 32 bits (at most nnz < 2^31
