@@ -214,6 +214,7 @@ __device__ __attribute__((always_inline)) inline void compute_features(const Bfs
 
 struct Counters {
     unsigned long long n = 0, out = 0, in = 0, scan = 0, cand = 0, light = 0;
+    uint32_t hist[kHistBins] = {};   // new vertices per degree bin (wavefront-uniform)
 };
 
 // A new vertex v of F_{k+1}: its degrees feed the next level's state.
@@ -235,23 +236,44 @@ __device__ inline int account_new(const StepArgs& a, int32_t v, int32_t din_know
     return hist_bin(dout);
 }
 
-// Records the lanes' new vertices (flag set): their histogram bins, with one
-// LDS add per non-empty bin, and the work-group's queue, the first kQCap of
-// them (s_hist[kHistBins] counts the places taken), with one LDS atomic per
-// wavefront.  Every lane of the wavefront calls it.
-__device__ inline void push_new(bool flag, int32_t v, int bin, uint32_t* s_hist, int32_t* qout, int lane) {
-    const uint64_t m = __ballot(flag);
-    if (!m) return;
+// Records the lanes' new vertices of N rounds (flag[q] set): their degree
+// bins go to the wavefront's histogram counters (three ballots give every
+// lane's bin; no LDS traffic), and their ids to the work-group's queue, the
+// first kQCap of them, with one LDS atomic on the queue count *s_qn per call.
+// Every lane of the wavefront calls it.
+template <int N>
+__device__ __attribute__((always_inline)) inline void push_new_n(const bool* flag, const int32_t* v, const int* bin,
+                                                                 Counters& k, uint32_t* s_qn, int32_t* qout, int lane) {
+    uint64_t m[N];
+    uint32_t tot = 0;
     #pragma unroll
-    for (int b = 0; b < 7; b++) {
-        const uint64_t mb = __ballot(flag && bin == b);
-        if (lane == 0 && mb) atomicAdd(&s_hist[b], (uint32_t)__popcll(mb));
+    for (int q = 0; q < N; q++) {
+        m[q] = __ballot(flag[q]);
+        tot += (uint32_t)__popcll(m[q]);
+    }
+    if (!tot) return;
+    #pragma unroll
+    for (int q = 0; q < N; q++) {
+        const uint64_t b0 = __ballot(bin[q] & 1), b1 = __ballot(bin[q] & 2), b2 = __ballot(bin[q] & 4);
+        #pragma unroll
+        for (int b = 0; b < 7; b++)
+            k.hist[b] += (uint32_t)__popcll(m[q] & ((b & 1) ? b0 : ~b0) & ((b & 2) ? b1 : ~b1) & ((b & 4) ? b2 : ~b2));
     }
     uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&s_hist[kHistBins], (uint32_t)__popcll(m));
+    if (lane == 0) base = atomicAdd(s_qn, tot);
     base = __shfl(base, 0);
-    const uint32_t pos = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
-    if (flag && pos < (uint32_t)kQCap) qout[pos] = v;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    #pragma unroll
+    for (int q = 0; q < N; q++) {
+        const uint32_t pos = base + (uint32_t)__popcll(m[q] & lt);
+        if (flag[q] && pos < (uint32_t)kQCap) qout[pos] = v[q];
+        base += (uint32_t)__popcll(m[q]);
+    }
+}
+
+__device__ __attribute__((always_inline)) inline void push_new(bool flag, int32_t v, int bin, Counters& k,
+                                                               uint32_t* s_qn, int32_t* qout, int lane) {
+    push_new_n<1>(&flag, &v, &bin, k, s_qn, qout, lane);
 }
 
 // Work-group barrier that orders LDS accesses only: the global stores a
@@ -268,7 +290,7 @@ __device__ inline void lds_barrier() {
 // without parents the claim is an atomicOr on F_{k+1}.
 template <bool PARENTS, int N>
 __device__ inline void visit_n(const StepArgs& a, const int32_t* u, const int32_t* v, int32_t L,
-                               const uint32_t* FL, uint32_t* Fn, Counters& k, uint32_t* s_hist, int32_t* qout,
+                               const uint32_t* FL, uint32_t* Fn, Counters& k, uint32_t* s_qn, int32_t* qout,
                                int lane) {
     uint32_t seen[N], pre[N];
     #pragma unroll
@@ -297,24 +319,26 @@ __device__ inline void visit_n(const StepArgs& a, const int32_t* u, const int32_
             else old[q] = atomicOr(&Fn[v[q] >> 5], 1u << (v[q] & 31));
         }
     }
+    bool won[N];
+    int bin[N];
     #pragma unroll
     for (int q = 0; q < N; q++) {
-        const bool won = claim[q] && (PARENTS ? old[q] == 0xffffffffu : !((old[q] >> (v[q] & 31)) & 1u));
-        int bin = 0;
-        if (won) {
+        won[q] = claim[q] && (PARENTS ? old[q] == 0xffffffffu : !((old[q] >> (v[q] & 31)) & 1u));
+        bin[q] = 0;
+        if (won[q]) {
             if (PARENTS) atomicOr(&Fn[v[q] >> 5], 1u << (v[q] & 31));
             a.dist[v[q]] = L + 1;
-            bin = account_new(a, v[q], -1, k);
+            bin[q] = account_new(a, v[q], -1, k);
         }
-        push_new(won, v[q], bin, s_hist, qout, lane);
     }
+    push_new_n<N>(won, v, bin, k, s_qn, qout, lane);
 }
 
 // Top-down expansion of row entries [b, e) of frontier vertex u by a whole
 // wavefront, kUnroll entries per lane per round.
 template <bool PARENTS>
 __device__ inline void expand_row(const StepArgs& a, int32_t u, int32_t b, int32_t e, int32_t L,
-                                  const uint32_t* FL, uint32_t* Fn, Counters& k, uint32_t* s_hist, int32_t* qout,
+                                  const uint32_t* FL, uint32_t* Fn, Counters& k, uint32_t* s_qn, int32_t* qout,
                                   int lane) {
     for (int32_t x0 = b; x0 < e; x0 += 64 * kUnroll) {
         int32_t uu[kUnroll], vv[kUnroll];
@@ -324,14 +348,14 @@ __device__ inline void expand_row(const StepArgs& a, int32_t u, int32_t b, int32
             uu[q] = u;
             vv[q] = x < e ? a.td_col[x] : -1;
         }
-        visit_n<PARENTS, kUnroll>(a, uu, vv, L, FL, Fn, k, s_hist, qout, lane);
+        visit_n<PARENTS, kUnroll>(a, uu, vv, L, FL, Fn, k, s_qn, qout, lane);
     }
 }
 
 // Top-down step L -> L+1 of one wavefront (gw of nw).
 template <bool PARENTS>
 __device__ __attribute__((always_inline)) inline void td_step(const StepArgs& a, const BfsCtrl& c, int32_t L, const uint32_t* FL, uint32_t* Fn,
-                        uint32_t* Fc, Counters& k, uint32_t* s_hist, int32_t* qout, int32_t* s_list, int gw, int nw,
+                        uint32_t* Fc, Counters& k, uint32_t* s_qn, int32_t* qout, int32_t* s_list, int gw, int nw,
                         int lane) {
     const int32_t ntiles = (a.nwords + kTileWords - 1) / kTileWords;
     const int sh8 = (lane & 3) * kLaneBits;
@@ -355,7 +379,7 @@ __device__ __attribute__((always_inline)) inline void td_step(const StepArgs& a,
                 m &= m - 1;
                 const int32_t u = __shfl(iu, j), b = __shfl(ib, j), e = __shfl(ie, j);
                 if (lane == 0) k.scan += (unsigned long long)(e - b);
-                expand_row<PARENTS>(a, u, b, e, L, FL, Fn, k, s_hist, qout, lane);
+                expand_row<PARENTS>(a, u, b, e, L, FL, Fn, k, s_qn, qout, lane);
             }
         }
     }
@@ -402,7 +426,7 @@ __device__ __attribute__((always_inline)) inline void td_step(const StepArgs& a,
                 const int j = __ffsll((unsigned long long)m) - 1;
                 m &= m - 1;
                 const int32_t wu = __shfl(u, j), wb = __shfl(beg, j), wd = __shfl(deg, j);
-                expand_row<PARENTS>(a, wu, wb, wb + wd, L, FL, Fn, k, s_hist, qout, lane);
+                expand_row<PARENTS>(a, wu, wb, wb + wd, L, FL, Fn, k, s_qn, qout, lane);
             }
             if (deg >= kWaveRow) deg = 0;
             // short rows: scan-based gather over the batch, kUnroll entries per lane
@@ -418,7 +442,7 @@ __device__ __attribute__((always_inline)) inline void td_step(const StepArgs& a,
                     uu[q] = ou;
                     vv[q] = e < rtot ? a.td_col[ob + (e - oe)] : -1;
                 }
-                visit_n<PARENTS, kUnroll>(a, uu, vv, L, FL, Fn, k, s_hist, qout, lane);
+                visit_n<PARENTS, kUnroll>(a, uu, vv, L, FL, Fn, k, s_qn, qout, lane);
             }
         }
     }
@@ -445,7 +469,7 @@ __device__ inline int bu_found(const StepArgs& a, int32_t v, int32_t par, int32_
 // wavefront, 64 entries per round trip.
 template <bool PARENTS>
 __device__ __attribute__((always_inline)) inline void bu_drain(const StepArgs& a, const Pending* q, int n, int32_t v0, int32_t L, const uint32_t* FL,
-                         Counters& k, uint32_t* s_hist, int32_t* qout, uint32_t* s_fn, int lane) {
+                         Counters& k, uint32_t* s_qn, int32_t* qout, uint32_t* s_fn, int lane) {
     constexpr int kLaneRounds = 4;
     int32_t v = -1, b = 0, e = 0, b0 = 0;
     // field by field: a whole-struct copy can leave a private copy that the
@@ -489,14 +513,14 @@ __device__ __attribute__((always_inline)) inline void bu_drain(const StepArgs& a
     }
     int bin = 0;
     if (par >= 0) bin = bu_found<PARENTS>(a, v, par, v - v0, b0, e, L, k, s_fn);
-    push_new(par >= 0, v, bin, s_hist, qout, lane);
+    push_new(par >= 0, v, bin, k, s_qn, qout, lane);
 }
 
 // The candidates of one tile (byte cw of lane l: vertices v0 + 8l ..
 // v0 + 8l + 7; excl its exclusive popcount prefix, total > 0 their number):
 // stage one, the found vertices recorded, and the open ones drained.
 template <bool PARENTS>
-__device__ __attribute__((always_inline)) inline Counters bu_tile(const StepArgs& a, int32_t L, const uint32_t* FL, uint32_t* s_hist,
+__device__ __attribute__((always_inline)) inline Counters bu_tile(const StepArgs& a, int32_t L, const uint32_t* FL, uint32_t* s_qn,
                                          int32_t* qout, uint32_t* s_fn, Pending* s_q, int32_t* s_list, uint32_t cw,
                                          int32_t excl, int32_t total, int32_t word0, int lane) {
     Counters k;
@@ -544,7 +568,7 @@ __device__ __attribute__((always_inline)) inline Counters bu_tile(const StepArgs
         }
         int bin = 0;
         if (par >= 0) bin = bu_found<PARENTS>(a, v, par, v - v0, beg, end, L, k, s_fn);
-        push_new(par >= 0, v, bin, s_hist, qout, lane);
+        push_new(par >= 0, v, bin, k, s_qn, qout, lane);
         const uint64_t om = __ballot(open);
         if (open) {
             const int pos = qn + (int)__popcll(om & ((1ull << lane) - 1ull));
@@ -555,7 +579,7 @@ __device__ __attribute__((always_inline)) inline Counters bu_tile(const StepArgs
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
             __builtin_amdgcn_wave_barrier();
             __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-            bu_drain<PARENTS>(a, s_q, 64, v0, L, FL, k, s_hist, qout, s_fn, lane);
+            bu_drain<PARENTS>(a, s_q, 64, v0, L, FL, k, s_qn, qout, s_fn, lane);
             qn -= 64;
             int32_t kv = -1, kb = 0, ke = 0, kb0 = 0;   // field by field, as in bu_drain
             if (lane < qn) { kv = s_q[64 + lane].v; kb = s_q[64 + lane].b; ke = s_q[64 + lane].e; kb0 = s_q[64 + lane].b0; }
@@ -567,7 +591,7 @@ __device__ __attribute__((always_inline)) inline Counters bu_tile(const StepArgs
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        bu_drain<PARENTS>(a, s_q, qn, v0, L, FL, k, s_hist, qout, s_fn, lane);
+        bu_drain<PARENTS>(a, s_q, qn, v0, L, FL, k, s_qn, qout, s_fn, lane);
     }
     return k;
 }
@@ -579,7 +603,7 @@ __device__ __attribute__((always_inline)) inline Counters bu_tile(const StepArgs
 // LDS queue that bu_drain empties 64 at a time.
 template <bool PARENTS>
 __device__ __attribute__((always_inline)) inline void bu_step(const StepArgs& a, int32_t L, const uint32_t* FL, uint32_t* Fn, uint32_t* Fc,
-                        Counters& k, uint32_t* s_hist, int32_t* qout, uint32_t* s_fn, Pending* s_q, int32_t* s_list,
+                        Counters& k, uint32_t* s_qn, int32_t* qout, uint32_t* s_fn, Pending* s_q, int32_t* s_list,
                         int gw, int nw, int lane) {
     const int32_t ntiles = (a.nwords + kTileWords - 1) / kTileWords;
     const int sh8 = (lane & 3) * kLaneBits;
@@ -613,13 +637,15 @@ __device__ __attribute__((always_inline)) inline void bu_step(const StepArgs& a,
         const int32_t excl = wave_excl_scan(__popc(cw), lane, total);
         k.cand += (lane == 0) ? (unsigned long long)total : 0ull;
         if (total > 0) {
-            const Counters kt = bu_tile<PARENTS>(a, L, FL, s_hist, qout, s_fn, s_q, s_list, cw, excl, total, word0, lane);
+            const Counters kt = bu_tile<PARENTS>(a, L, FL, s_qn, qout, s_fn, s_q, s_list, cw, excl, total, word0, lane);
             k.n += kt.n;
             k.out += kt.out;
             k.in += kt.in;
             k.scan += kt.scan;
             k.cand += kt.cand;
             k.light += kt.light;
+            #pragma unroll
+            for (int b = 0; b < kHistBins; b++) k.hist[b] += kt.hist[b];
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
@@ -641,7 +667,7 @@ __device__ __attribute__((always_inline)) inline void bu_step(const StepArgs& a,
 // F_{k+1} words are written with plain stores.  No device-scope atomics.
 template <bool PARENTS>
 __device__ __attribute__((always_inline)) inline void range_step(const StepArgs& a, int32_t L, const uint32_t* FL, uint32_t* Fn, uint32_t* Fc,
-                           Counters& k, uint32_t* s_hist, int32_t* qout, uint32_t* s_lds, const int32_t* s_qoff,
+                           Counters& k, uint32_t* s_qn, int32_t* qout, uint32_t* s_lds, const int32_t* s_qoff,
                            int32_t nf, int lpar, int tid, int lane, int wid) {
     const int32_t RS = a.range_size;              // a multiple of 64
     const int32_t R0 = (int32_t)blockIdx.x * RS;
@@ -777,6 +803,7 @@ __device__ __attribute__((always_inline)) inline void range_step(const StepArgs&
             d0[q] = d1[q] = 0;
             if (fresh[q]) { d0[q] = a.td_rp[v[q]]; d1[q] = a.td_rp[v[q] + 1]; }
         }
+        int bin[kEmit];
         #pragma unroll
         for (int q = 0; q < kEmit; q++) {
             const int32_t base = c0 + q * kBlock + wid * 64;
@@ -785,7 +812,7 @@ __device__ __attribute__((always_inline)) inline void range_step(const StepArgs&
                 const int32_t w = w0 + (base >> 5) + lane;
                 if (w < a.nwords) Fn[w] = (uint32_t)(m >> (32 * lane));
             }
-            int bin = 0;
+            bin[q] = 0;
             if (fresh[q]) {
                 a.dist[v[q]] = L + 1;
                 if (PARENTS) a.parent[v[q]] = par[q];
@@ -795,10 +822,10 @@ __device__ __attribute__((always_inline)) inline void range_step(const StepArgs&
                 k.out += (unsigned long long)dout;
                 k.in += (unsigned long long)din;
                 if (dout <= kHeavyRow) k.light += (unsigned long long)dout;
-                bin = hist_bin(dout);
+                bin[q] = hist_bin(dout);
             }
-            push_new(fresh[q], v[q], bin, s_hist, qout, lane);
         }
+        push_new_n<kEmit>(fresh, v, bin, k, s_qn, qout, lane);
     }
 }
 
@@ -862,7 +889,7 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
     __shared__ unsigned long long s_sum[kPart];
     __shared__ float s_feat[kFeat];
     __shared__ int32_t s_p, s_dir, s_exit, s_dl, s_range;
-    __shared__ uint32_t s_hist[kHistBins + 1];   // + the queue counter
+    __shared__ uint32_t s_qn[1];                 // the work-group queue's count
     __shared__ uint32_t s_fn[kWaves][kTileWords];
     __shared__ int32_t s_qoff[kMaxGrid + 1];      // per-work-group new-vertex counts, then queue offsets
     // bottom-up: the wavefronts' pending queues; range mode: parents and
@@ -917,7 +944,7 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
         s_c = sh->state[lpar ^ 1];   // the first launch reads S_0 from state[1]
         s_dl = sh->done_level;
     }
-    if (tid <= kHistBins) s_hist[tid] = 0;
+    if (tid == 0) s_qn[0] = 0;
     // lanes l, l+16, l+32, l+48 of a wavefront hold the same slot
     acc += __shfl_xor(acc, 16);
     acc += __shfl_xor(acc, 32);
@@ -1005,14 +1032,25 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
                            c.front_hist[5] + c.front_hist[6] > 0;
         if (lane == 0) s_range = range ? 1 : 0;
         if (range) {
-            // queue offsets: exclusive scan of the per-work-group counts
+            // queue offsets: exclusive scan of the per-work-group counts,
+            // four consecutive counts per lane (one wavefront scan per 256)
             int32_t carry = 0;
-            for (int32_t b0 = 0; b0 < a.grid; b0 += 64) {
-                const int32_t b = b0 + lane;
-                const int32_t cnt = b < a.grid ? s_qoff[b] : 0;
+            for (int32_t b0 = 0; b0 < a.grid; b0 += 256) {
+                int32_t cnt[4], sum = 0;
+                #pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int32_t b = b0 + lane * 4 + j;
+                    cnt[j] = b < a.grid ? s_qoff[b] : 0;
+                    sum += cnt[j];
+                }
                 int32_t tot;
-                const int32_t ex = wave_excl_scan(cnt, lane, tot);
-                if (b < a.grid) s_qoff[b] = carry + ex;
+                int32_t ex = carry + wave_excl_scan(sum, lane, tot);
+                #pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const int32_t b = b0 + lane * 4 + j;
+                    if (b < a.grid) s_qoff[b] = ex;
+                    ex += cnt[j];
+                }
                 carry += tot;
             }
             if (lane == 0) s_qoff[a.grid] = carry;
@@ -1042,10 +1080,10 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
     const int gw = blockIdx.x * kWaves + wid, nw = gridDim.x * kWaves;
     int32_t* qout = a.qbuf + ((size_t)lpar * a.grid + blockIdx.x) * kQCap;
     Counters k;
-    if (dir == 1) bu_step<PARENTS>(a, L, FL, Fn, Fc, k, s_hist, qout, s_fn[wid], s_q + wid * 128,
+    if (dir == 1) bu_step<PARENTS>(a, L, FL, Fn, Fc, k, s_qn, qout, s_fn[wid], s_q + wid * 128,
                                   s_lists + wid * kTileVerts, gw, nw, lane);
-    else if (range) range_step<PARENTS>(a, L, FL, Fn, Fc, k, s_hist, qout, s_lds, s_qoff, s_qoff[a.grid], lpar, tid, lane, wid);
-    else td_step<PARENTS>(a, s_c, L, FL, Fn, Fc, k, s_hist, qout, s_lists + wid * kTileVerts, gw, nw, lane);
+    else if (range) range_step<PARENTS>(a, L, FL, Fn, Fc, k, s_qn, qout, s_lds, s_qoff, s_qoff[a.grid], lpar, tid, lane, wid);
+    else td_step<PARENTS>(a, s_c, L, FL, Fn, Fc, k, s_qn, qout, s_lists + wid * kTileVerts, gw, nw, lane);
 
     // block reduction -> this block's counters (plain stores)
     // A work-group's counters of one step fit in 32 bits (at most nnz < 2^31
@@ -1054,8 +1092,11 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
                      (uint32_t)k.light};
     #pragma unroll
     for (int i = 0; i < 6; i++) r[i] = wave_sum(r[i]);
-    if (lane == 0)
+    static_assert(6 + kHistBins <= kPart, "block reduction slots");
+    if (lane == 0) {
         for (int i = 0; i < 6; i++) s_red[wid][i] = r[i];
+        for (int b = 0; b < kHistBins; b++) s_red[wid][6 + b] = k.hist[b];
+    }
     lds_barrier();
     unsigned long long* out = a.part + ((size_t)lpar * a.grid + blockIdx.x) * kPart;
     if (tid < 6) {
@@ -1063,8 +1104,10 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
         for (int q = 0; q < kWaves; q++) s += s_red[q][tid];
         out[tid < 5 ? tid : P_LIGHT] = s;
         if (tid == P_NEXT) out[P_QOVF] = s > (unsigned long long)kQCap ? 1ull : 0ull;
-    } else if (tid >= 16 && tid < 16 + kHistBins) {
-        out[P_HIST + tid - 16] = s_hist[tid - 16];
+    } else if (tid >= 6 && tid < 6 + kHistBins) {   // the degree histogram
+        unsigned long long s = 0;
+        for (int q = 0; q < kWaves; q++) s += s_red[q][tid];
+        out[P_HIST + tid - 6] = s;
     }
 }
 
