@@ -190,6 +190,34 @@ def test_direction_never_changes_output(oracle):
         assert (outs[0][1] == oracle.bfs_parents(rp, col, root)).all()
 
 
+def test_full_tiles_every_direction(oracle):
+    """Full 512-vertex tiles: a star whose 2999 leaves also form a ring and a
+    complete graph on 600 vertices, so that whole tiles are frontier
+    (top-down) or candidates (bottom-up); forced directions and the heuristic
+    against the oracle from the hub and from a leaf."""
+    from lgmi import _lib
+    from lgmi.graphs import SimpleGraph
+    import lgmi.generators as gen
+    lib = _lib.load()
+    n = 3000
+    a = np.arange(1, n)
+    src = np.concatenate([np.zeros(n - 1, np.int64), a])
+    dst = np.concatenate([a, np.where(a + 1 < n, a + 1, 1)])
+    for tag, g in (("star-ring", SimpleGraph.from_edges(n, src, dst)), ("k600", gen.complete_graph(600))):
+        rp, col = g.csr()
+        h = g.handle()
+        for root in (0, 7):
+            for forced in (-1, 0, 1):
+                say(f"{tag} root {root} forced {forced}")
+                _lib.check(lib.lgmi_bfs_force_direction(h, forced))
+                d = np.empty(g.nv(), np.int32)
+                p = np.empty(g.nv(), np.int32)
+                _lib.check(lib.lgmi_bfs(h, root, 0, _lib.i32p(d), _lib.i32p(p)))
+                assert (d == oracle.gdistances(rp, col, [root])).all()
+                assert (p == oracle.bfs_parents(rp, col, root)).all()
+        _lib.check(lib.lgmi_bfs_force_direction(h, -1))
+
+
 def test_repeat_bit_identical():
     import lgmi.generators as gen
     g = gen.rmat(16, 16, seed=31)
