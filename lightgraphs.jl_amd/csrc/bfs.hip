@@ -1003,7 +1003,8 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
                         sh->state[p & 1] = c;
                         sh->done_level = p;
                     }
-                } else {
+                } else if (may_learn || (blockIdx.x == 0 && k < a.max_log)) {
+                    // the features feed the network and work-group 0's level log only
                     compute_features(&c, s_feat);
                 }
                 s_c = c;
