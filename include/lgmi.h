@@ -150,6 +150,27 @@ int lgmi_bfs_run(lgmi_graph_t g, int32_t src, int dir_in, int want_parents, lgmi
 /* Copies the last run's results to the host; either pointer may be NULL. */
 int lgmi_bfs_fetch(lgmi_graph_t g, int32_t* dist, int32_t* parent);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code that must not call hipMalloc: tight serving loops, harnesses
+ * that forbid it): the single-source BFS of lgmi_bfs_run / lgmi_bfs with the
+ * results written to caller-owned DEVICE buffers -- dist (nv Int32, -1
+ * unreached) and parent (nv Int32, parent[src] = src, -1 unreached; NULL:
+ * distances only) -- and every scratch buffer of the search (frontier and
+ * visited bitmaps, per-work-group counters and queues, the level state, the
+ * policy weights) in one fixed device array of LGMI_NOALLOC_SCRATCH_BYTES
+ * that each GPU holds from the library's first use until the process ends.
+ * The call allocates nothing (no hipMalloc, hipFree, hipHostMalloc, event or
+ * graph creation).  nv must be at most LGMI_NOALLOC_MAX_NV (LGMI_E_TOO_LARGE
+ * otherwise).  All calls on one GPU share its array: the library serialises
+ * the calls it sees on a device, and calls that overlap in time on one GPU
+ * by other means race.  Work the caller queued on dist / parent must be
+ * complete before the call (the library runs on its own non-blocking
+ * stream).  Results are identical to lgmi_bfs's and complete when the call
+ * returns. */
+#define LGMI_NOALLOC_MAX_NV         (1 << 24)
+#define LGMI_NOALLOC_SCRATCH_BYTES  (17u << 20)
+int lgmi_bfs_run_noalloc(lgmi_graph_t g, int32_t src, int dir_in, int32_t* dist, int32_t* parent);
+
 /* Per-step records of the last BFS run (policy features and labels,
  * algorithmic-byte counters).  Writes up to max_steps records and returns
  * the number written (or a negative code). */

@@ -1272,8 +1272,9 @@ void bfs_prepare(lgmi_graph* g) {
     g->bfs = w;
 }
 
-static StepArgs make_args(lgmi_graph* g, int dir_in, bool parents) {
-    BfsWork* w = g->bfs;
+// The adjacency half of the step arguments (traversal and reverse CSR with
+// their derived structures) for a launch grid of `grid` work-groups.
+static StepArgs graph_args(lgmi_graph* g, int dir_in, int32_t grid) {
     StepArgs a{};
     const DevCSR& td = dir_in ? g->in : g->out;
     const DevCSR& bu = dir_in ? g->out : g->in;
@@ -1282,10 +1283,17 @@ static StepArgs make_args(lgmi_graph* g, int dir_in, bool parents) {
     a.td_items = td.n_items;
     a.td_heavy_id = td.heavy_id;
     a.td_rtab = td.rtab;
-    a.range_size = (td.range_size > 0 && td.range_grid == w->grid) ? td.range_size : 0;
+    a.range_size = (td.range_size > 0 && td.range_grid == grid) ? td.range_size : 0;
     a.bu_rp = bu.row_ptr; a.bu_col = bu.col; a.bu_nz = bu.nz_bm; a.bu_first = bu.first;
     a.nv = g->nv;
-    a.nwords = w->nwords;
+    a.nwords = 2 * ((g->nv + 63) / 64);
+    a.grid = grid;
+    return a;
+}
+
+static StepArgs make_args(lgmi_graph* g, int dir_in, bool parents) {
+    BfsWork* w = g->bfs;
+    StepArgs a = graph_args(g, dir_in, w->grid);
     a.dist = w->dist;
     a.parent = parents ? reinterpret_cast<uint32_t*>(w->parent) : nullptr;
     a.bm0 = w->bm[0]; a.bm1 = w->bm[1]; a.bm2 = w->bm[2];
@@ -1448,7 +1456,126 @@ void bfs_fetch(lgmi_graph* g, int32_t* dist, int32_t* parent) {
     LGMI_HIP(hipStreamSynchronize(g->stream));
 }
 
+// ----------------------------------------------------------------------
+// BFS without device allocation: the search state lives in one fixed
+// device array (`scratch`), the results go to caller-owned device buffers.
+// ----------------------------------------------------------------------
+namespace {
+
+constexpr size_t na_align(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr int kNaMaxGrid = 256;                                   // work-groups at most
+constexpr size_t kNaWords = 2 * (((size_t)LGMI_NOALLOC_MAX_NV + 63) / 64);   // bitmap words at the largest nv
+constexpr size_t kNaPartOff = 0;                                  // [2][grid][kPart] u64
+constexpr size_t kNaBmOff = na_align(kNaPartOff + sizeof(unsigned long long) * 2 * kNaMaxGrid * kPart);
+constexpr size_t kNaQOff = na_align(kNaBmOff + sizeof(uint32_t) * 4 * kNaWords);   // bm0, bm1, bm2, vis
+constexpr size_t kNaShOff = na_align(kNaQOff + sizeof(int32_t) * 2 * kNaMaxGrid * kQCap);
+constexpr size_t kNaPolOff = na_align(kNaShOff + sizeof(BfsShared));
+constexpr size_t kNaBytes = na_align(kNaPolOff + sizeof(PolicyWeights));
+static_assert(kNaBytes <= (size_t)LGMI_NOALLOC_SCRATCH_BYTES, "no-alloc scratch layout exceeds the stated size");
+
+// Each GPU holds its own copy until the process ends; host code reaches it
+// only through the address hipGetSymbolAddress returns for that device.
+static __device__ __align__(256) unsigned char scratch[LGMI_NOALLOC_SCRATCH_BYTES];
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mu[kNaMaxDevices];         // one no-alloc search per device at a time
+unsigned char* g_na_ptr[kNaMaxDevices] = {};
+
+// The device address of `scratch` on device dev, looked up once per device
+// with that device current (the caller holds g_na_mu[dev]).
+unsigned char* na_scratch(int dev) {
+    if (!g_na_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            throw Error(LGMI_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        }
+        g_na_ptr[dev] = static_cast<unsigned char*>(p);
+    }
+    return g_na_ptr[dev];
+}
+
+}  // namespace
+
+void bfs_run_noalloc(lgmi_graph* g, int32_t src, int dir_in, int32_t* dist, int32_t* parent) {
+    if (g->nv > LGMI_NOALLOC_MAX_NV)
+        throw Error(LGMI_E_TOO_LARGE, "lgmi_bfs_run_noalloc: nv = " + std::to_string(g->nv) +
+                                          " exceeds LGMI_NOALLOC_MAX_NV = " + std::to_string(LGMI_NOALLOC_MAX_NV));
+    int dev = 0;
+    LGMI_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNaMaxDevices) throw Error(LGMI_E_ARG, "device index beyond the no-alloc table");
+    std::lock_guard<std::mutex> lk(g_na_mu[dev]);
+    unsigned char* base = na_scratch(dev);
+    const int32_t grid = std::min(std::max(1, g->cu_count), kNaMaxGrid);
+    int mode;
+    float alpha, beta;
+    bool have_mlp;
+    PolicyWeights pw;
+    {
+        std::lock_guard<std::mutex> pl(policy_mutex());
+        PolicyHost& p = policy_host();
+        mode = p.mode; alpha = p.alpha; beta = p.beta; have_mlp = p.have_mlp;
+        if (have_mlp) pw = p.w;
+    }
+    if (mode == LGMI_POLICY_LEARNED && !have_mlp) mode = LGMI_POLICY_HEURISTIC;
+    StepArgs a = graph_args(g, dir_in, grid);
+    a.dist = dist;
+    a.parent = reinterpret_cast<uint32_t*>(parent);
+    a.part = reinterpret_cast<unsigned long long*>(base + kNaPartOff);
+    uint32_t* bm = reinterpret_cast<uint32_t*>(base + kNaBmOff);
+    a.bm0 = bm;
+    a.bm1 = bm + kNaWords;
+    a.bm2 = bm + 2 * kNaWords;
+    a.vis = bm + 3 * kNaWords;
+    a.qbuf = reinterpret_cast<int32_t*>(base + kNaQOff);
+    a.sh = reinterpret_cast<BfsShared*>(base + kNaShOff);
+    a.log = nullptr;
+    a.max_log = 0;
+    PolicyWeights* dpol = reinterpret_cast<PolicyWeights*>(base + kNaPolOff);
+    a.policy = mode == LGMI_POLICY_LEARNED ? dpol : nullptr;
+    hipStream_t s = g->stream;
+    if (a.policy) LGMI_HIP(hipMemcpyAsync(dpol, &pw, sizeof(PolicyWeights), hipMemcpyHostToDevice, s));
+    const int forced = g->bfs ? g->bfs_forced : -1;
+    const bool parents = parent != nullptr;
+    // bfs_init writes every word a search reads before writing it: the
+    // bitmaps, dist / parent and the level state (the first level launch
+    // reads no per-block counters)
+    const int init_grid = std::max(1, std::min<int>(grid * 8, (g->nv + 255) / 256));
+    bfs_init<<<init_grid, 256, 0, s>>>(a, src, g->nnz, forced, mode, alpha, beta);
+    LGMI_HIP(hipGetLastError());
+    int32_t launched = 0, batch = 8;
+    for (;;) {
+        for (int i = 0; i < batch; i++) launch_level(a, parents, launched + i, forced, s);
+        launched += batch;
+        LGMI_HIP(hipGetLastError());
+        int32_t done = 0;
+        LGMI_HIP(hipMemcpyAsync(&done, &a.sh->done_level, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        LGMI_HIP(hipStreamSynchronize(s));
+        if (done < launched) break;
+        if (launched > g->nv + 8) throw Error(LGMI_E_HIP, "BFS did not terminate");
+        batch = std::min(batch * 2, 1024);
+    }
+}
+
 }  // namespace lgmi
+
+extern "C" int lgmi_bfs_run_noalloc(lgmi_graph_t g, int32_t src, int dir_in, int32_t* dist, int32_t* parent) {
+    using namespace lgmi;
+    try {
+        if (!g || !dist) throw Error(LGMI_E_ARG, "bad arguments");
+        std::lock_guard<std::mutex> lk(g->mu);
+        if (src < 0 || src >= g->nv) throw Error(LGMI_E_BOUNDS, "source vertex " + std::to_string(src) + " out of range");
+        bfs_run_noalloc(g, src, dir_in ? 1 : 0, dist, parent);
+        return LGMI_OK;
+    } catch (const Error& e) {
+        set_last_error(e.what());
+        return e.code;
+    } catch (const std::exception& e) {
+        set_last_error(e.what());
+        return LGMI_E_HIP;
+    }
+}
 
 // ----------------------------------------------------------------------
 // step records export

@@ -27,11 +27,13 @@ LGMI_POLICY_TOPDOWN = 2
 LGMI_POLICY_FEATURES = 16
 LGMI_POLICY_HIDDEN = 64
 LGMI_POLICY_NWEIGHTS = 64 * 16 + 64 + 2 * 64 + 2 + 16 + 16
+LGMI_NOALLOC_MAX_NV = 1 << 24
+LGMI_NOALLOC_SCRATCH_BYTES = 17 << 20
 
 # Every symbol include/lgmi.h and include/lgmi_gen.h declare.
 EXPORTED = [
     "lgmi_graph_create", "lgmi_graph_destroy", "lgmi_gdistances", "lgmi_bfs_parents", "lgmi_bfs",
-    "lgmi_bfs_run", "lgmi_bfs_fetch", "lgmi_bfs_steps", "lgmi_bfs_time_steps", "lgmi_bfs_force_direction",
+    "lgmi_bfs_run", "lgmi_bfs_fetch", "lgmi_bfs_run_noalloc", "lgmi_bfs_steps", "lgmi_bfs_time_steps", "lgmi_bfs_force_direction",
     "lgmi_bfs_probe", "lgmi_mlp_loss_grad", "lgmi_mlp_train", "lgmi_mlp_forward",
     "lgmi_bfs_set_policy", "lgmi_betweenness", "lgmi_betweenness_stats", "lgmi_betweenness_batch",
     "lgmi_betweenness_batch_tuning",
@@ -109,6 +111,7 @@ def load():
         "lgmi_bfs": ([_vp, c.c_int32, c.c_int, _i32p, _i32p], c.c_int),
         "lgmi_bfs_run": ([_vp, c.c_int32, c.c_int, c.c_int, c.POINTER(BfsStats)], c.c_int),
         "lgmi_bfs_fetch": ([_vp, _i32p, _i32p], c.c_int),
+        "lgmi_bfs_run_noalloc": ([_vp, c.c_int32, c.c_int, _vp, _vp], c.c_int),
         "lgmi_bfs_steps": ([_vp, c.c_int32, c.POINTER(BfsStep)], c.c_int),
         "lgmi_bfs_time_steps": ([_vp, c.c_int], c.c_int),
         "lgmi_bfs_force_direction": ([_vp, c.c_int32], c.c_int),
