@@ -8,7 +8,12 @@ of degree >= 1 (cycled).  value = sum over timed BFS runs of m_comp divided
 by the wall time of the timed region, where m_comp = (sum of degrees of the
 reached vertices) / 2.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-bc]
+Further legs of the default run, each its own key of the JSON line: the
+uneven 4096-graph betweenness batch ("batch"), the learned direction switch
+against the alpha/beta rule on its three graphs ("learned_switch") and the
+per-call drop-in cost on RMAT-22 ("dropin").
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-bc] [--quick]
 """
 from __future__ import annotations
 
@@ -30,6 +35,28 @@ METRIC = "BFS GTEPS (RMAT-22) + exact-BC TEPS; learned vs α/β switch, same GPU
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_threads():
+    """Threads of the threaded CPU legs: the CPUs this process may run on
+    (its affinity mask), capped by OMP_NUM_THREADS where the host sets it
+    (a GPU host that shares its cores between GPUs states each GPU's share
+    there)."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(env))) if env.isdigit() and int(env) > 0 else n
+
+
+def cpu_median(fn, runs=3):
+    """One untimed warm-up run of fn, then the median wall time of `runs`
+    timed runs (seconds) and the list of those times."""
+    fn()
+    ts = []
+    for _ in range(runs):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), ts
 
 
 def step_bytes(st, nv):
@@ -146,17 +173,19 @@ def bfs_bench(args, lgmi, _lib, orc):
         f"{tot_bytes / max(launches, 1) / 1e6:.2f} MB/launch, {achieved:.0f} GB/s; "
         f"bottom-up steps {n_bu}: {bu_achieved:.0f} GB/s; top-down steps {n_td}")
 
-    # CPU baseline: serial oracle BFS (distances + parents) on a sample of roots
+    # CPU baseline: serial oracle BFS (distances + parents) on a sample of
+    # roots; one warm-up pass over the sample, then the median of three
     sample_roots = roots[: args.cpu_roots]
-    t_cpu = 0.0
-    e_cpu = 0.0
-    for i, r in enumerate(sample_roots):
-        c0 = time.perf_counter()
-        orc.bfs_dist_parent(rp, col, int(r))
-        t_cpu += time.perf_counter() - c0
-        e_cpu += mcomp[i]
+    e_cpu = float(mcomp[: len(sample_roots)].sum())
+
+    def cpu_pass():
+        for r in sample_roots:
+            orc.bfs_dist_parent(rp, col, int(r))
+
+    t_cpu, t_runs = cpu_median(cpu_pass)
     cpu_gteps = e_cpu / t_cpu / 1e9
-    log(f"cpu baseline: {len(sample_roots)} roots in {t_cpu:.1f}s -> {cpu_gteps:.4f} GTEPS")
+    log(f"cpu baseline: {len(sample_roots)} roots, median of 3 passes {t_cpu:.2f}s "
+        f"({', '.join(f'{t:.2f}' for t in t_runs)}) -> {cpu_gteps:.4f} GTEPS")
 
     traffic = read_traffic()
     result = {
@@ -199,8 +228,8 @@ def bfs_bench(args, lgmi, _lib, orc):
             "unit": "GTEPS",
             "cores": 1,
             "kind": "port",
-            "sample": f"{len(sample_roots)} of the 64 roots, serial oracle BFS with distances and parents "
-                      f"({t_cpu:.1f} s)",
+            "sample": f"{len(sample_roots)} of the 64 roots, serial oracle BFS with distances and parents; "
+                      f"1 warm-up pass, median of 3 passes ({t_cpu:.2f} s per pass)",
         },
     }
     return result
@@ -239,7 +268,7 @@ def bc_bench(args, lgmi, _lib, orc):
         stats.append(last_stats(g))
     wall = float(np.mean(walls))
     stt = {k: float(np.mean([s[k] for s in stats])) for k in ("device_ms", "forward_ms", "backward_ms")}
-    launch = {0: "one launch per level", 1: "one persistent cooperative launch",
+    launch = {0: "one launch per level", 1: "one persistent launch",
               2: "one persistent launch replayed from a captured hipGraph"}[int(stats[-1]["launch"])]
     m = nnz / 2
     teps = 1024 * m / wall
@@ -260,19 +289,26 @@ def bc_bench(args, lgmi, _lib, orc):
     bc_gbs = bytes_alg / (stt["device_ms"] * 1e-3) / 1e9
     log(f"BC algorithmic bytes {bytes_alg / 1e9:.1f} GB (DAG-edge model, 16-source sample) -> {bc_gbs:.0f} GB/s "
         f"= {bc_gbs / HBM_PEAK_GBS:.3f} of HBM peak")
-    # CPU baseline: threaded oracle on a sample of the sources, scaled
-    nthreads = min(16, os.cpu_count() or 1)
+    # CPU baseline: threaded oracle on a sample of the sources, scaled; one
+    # warm-up pass, then the median of three
+    nthreads = cpu_threads()
     ns = args.cpu_bc_sources
-    c0 = time.perf_counter()
-    want = orc.betweenness_threaded(rp, col, rp, False, srcs[:ns], nthreads=nthreads, index_quirk=False)
-    t_cpu = time.perf_counter() - c0
+    box = {}
+
+    def cpu_pass():
+        box["want"] = orc.betweenness_threaded(rp, col, rp, False, srcs[:ns], nthreads=nthreads,
+                                               index_quirk=False)
+
+    t_cpu, t_runs = cpu_median(cpu_pass)
+    want = box["want"]
     cpu_teps = ns * m / t_cpu
     # parity spot check on the sample: rescale to the 1024-source k
     got = lgmi.betweenness_centrality(g, srcs[:ns] + 1)
     floor = 1e-15 * max(1.0, float(np.abs(want).max()))
     err = float(np.max(np.abs(got - want) / np.maximum(np.abs(want), floor)))
-    log(f"BC cpu baseline: {ns} sources, {nthreads} threads, {t_cpu:.1f}s -> {cpu_teps / 1e9:.4f} GTEPS_BC; "
-        f"sample max rel err {err:.2e}")
+    log(f"BC cpu baseline: {ns} sources, {nthreads} threads, median of 3 passes {t_cpu:.2f}s "
+        f"({', '.join(f'{t:.2f}' for t in t_runs)}) -> {cpu_teps / 1e9:.4f} GTEPS_BC; sample max rel err {err:.2e}")
+    g._release()
     return {
         "workload": "exact Brandes betweenness, symmetrised RMAT scale 20 edge factor 16, "
                     "1024 explicit seeded sources in 64-lane groups, FP64",
@@ -292,64 +328,195 @@ def bc_bench(args, lgmi, _lib, orc):
         "sample_max_rel_err": err,
         "cpu_baseline": {"value": round(cpu_teps / 1e9, 5), "unit": "GTEPS_BC", "cores": nthreads,
                          "kind": "port",
-                         "sample": f"{ns} of the 1024 sources, threaded oracle Brandes ({t_cpu:.1f} s)"},
+                         "sample": f"{ns} of the 1024 sources, threaded oracle Brandes; 1 warm-up pass, "
+                                   f"median of 3 passes ({t_cpu:.2f} s per pass)"},
         "speedup_vs_cpu": round(teps / cpu_teps, 1),
     }
 
 
-def config4_bench(args, lgmi, orc):
+LDS_BYTES_PER_CLK_CU = 128      # LDS bytes per clock per CU (CDNA3's rate, a floor for CDNA4)
+ENGINE_CLOCK_GHZ = 2.4          # MI355X peak engine clock
+CU_COUNT = 256
+
+
+def config4_bench(args, lgmi, orc, tune=False):
     """4096 graphs, n log-uniform in [200, 20000], half barabasi_albert(n, 4),
     half watts_strogatz(n, 8, 0.1), seeded per graph: all-source exact
     betweenness (normalize=true, endpoints=false) for every graph in one
-    persistent launch after autotuning; the CPU oracle runs a seeded
+    persistent launch, replayed from a captured hipGraph, with the
+    autotuner's degree-bin and lane-group thresholds (batch_tuning.json, or
+    a fresh autotune with --config4); the CPU oracle runs a seeded
     subsample and its time is scaled by sum n*m."""
     from lgmi import batch
     t0 = time.time()
     graphs = lgmi.generators.uneven_batch(args.config4_graphs, 200, 20_000, seed=1)
     nm = [g.nv() * (len(g.csr()[1]) // 2) for g in graphs]
     nm_tot = float(sum(nm))
-    log(f"batch built: {len(graphs)} graphs, sum n {sum(g.nv() for g in graphs)}, sum n*m {nm_tot:.3e} "
-        f"({time.time() - t0:.1f}s)")
-    tune = batch.autotune(graphs, log=log)
-    if args.save_tuning:
-        batch.save_tuning(tune, args.save_tuning)
+    sum_n = sum(g.nv() for g in graphs)
+    sum_n2 = float(sum(float(g.nv()) ** 2 for g in graphs))
+    sum_n_nnz = float(sum(float(g.nv()) * len(g.csr()[1]) for g in graphs))
+    log(f"batch built: {len(graphs)} graphs, sum n {sum_n}, sum n*m {nm_tot:.3e} ({time.time() - t0:.1f}s)")
+    if tune:
+        tuned = batch.autotune(graphs, log=log)
+        if args.save_tuning:
+            batch.save_tuning(tuned, args.save_tuning)
+        best = tuned["best"]
+        cands = tuned["candidates"]
+    else:
+        batch.load_tuning()
+        with open(batch.TUNING_PATH) as f:
+            best = json.load(f)["best"]
+        cands = None
     runs = []
     for _ in range(2):
         t1 = time.perf_counter()
         res, ms = lgmi.betweenness_centrality_batch(graphs, return_ms=True)
         runs.append((ms, time.perf_counter() - t1))
-    ms = min(r[0] for r in runs)
-    wall = min(r[1] for r in runs)
-    log(f"batch: device {ms:.1f} ms, wall {wall * 1e3:.1f} ms, {nm_tot / (ms * 1e-3) / 1e9:.2f} G n*m/s")
+    ms = runs[-1][0]
+    wall = runs[-1][1]
+    log(f"batch: device {ms:.1f} ms, wall {wall * 1e3:.1f} ms (first run {runs[0][0]:.1f} ms), "
+        f"{nm_tot / (ms * 1e-3) / 1e9:.2f} G n*m/s")
+    # LDS traffic model of the Brandes passes (per source): forward, 8 B
+    # state read per adjacency entry plus a 16 B atomic add for about half
+    # of them (the entries towards the next level); backward, 8 B state read
+    # per adjacency entry; per vertex 32 B (level list, sigma, coefficient,
+    # partial).  Adjacency entries come from L2 (a graph's CSR stays
+    # cached while its sources run): 4 B per entry and pass.
+    lds_bytes = 24.0 * sum_n_nnz + 32.0 * sum_n2
+    l2_bytes = 8.0 * sum_n_nnz
+    lds_peak = LDS_BYTES_PER_CLK_CU * ENGINE_CLOCK_GHZ * CU_COUNT          # GB/s
+    lds_gbs = lds_bytes / (ms * 1e-3) / 1e9
     rng = np.random.default_rng(7)
     idx = rng.choice(len(graphs), size=min(args.cpu_batch_graphs, len(graphs)), replace=False)
-    nthreads = min(16, os.cpu_count() or 1)
-    t_cpu = nm_s = 0.0
+    nthreads = cpu_threads()
+    nm_s = float(sum(nm[i] for i in idx))
+    wants = {}
+
+    def cpu_pass():
+        for i in idx:
+            rp, col = graphs[i].csr()
+            wants[i] = orc.betweenness_threaded(rp, col, rp, False, np.arange(graphs[i].nv()), nthreads=nthreads,
+                                                index_quirk=False)
+
+    t_cpu, t_runs = cpu_median(cpu_pass)
     maxerr = 0.0
     for i in idx:
-        g = graphs[i]
-        rp, col = g.csr()
-        c0 = time.perf_counter()
-        want = orc.betweenness_threaded(rp, col, rp, False, np.arange(g.nv()), nthreads=nthreads, index_quirk=False)
-        t_cpu += time.perf_counter() - c0
-        nm_s += nm[i]
+        want = wants[i]
         floor = 1e-15 * max(1.0, float(np.abs(want).max()))
         maxerr = max(maxerr, float((np.abs(res[i] - want) / np.maximum(np.abs(want), floor)).max()))
     cpu_s = t_cpu * nm_tot / nm_s
-    log(f"cpu oracle: {len(idx)} graphs in {t_cpu:.1f}s -> {cpu_s:.0f}s extrapolated; max rel err {maxerr:.2e}")
-    return {
+    log(f"cpu oracle: {len(idx)} graphs, {nthreads} threads, median of 3 passes {t_cpu:.2f}s "
+        f"({', '.join(f'{t:.2f}' for t in t_runs)}) -> {cpu_s:.0f}s extrapolated; max rel err {maxerr:.2e}")
+    out = {
         "workload": "4096 graphs, n log-uniform in [200, 20000], half barabasi_albert(n, 4), half "
-                    "watts_strogatz(n, 8, 0.1); all-source exact betweenness, FP64, one persistent launch",
+                    "watts_strogatz(n, 8, 0.1); all-source exact betweenness, FP64, one persistent launch "
+                    "replayed from a captured hipGraph",
         "value": round(nm_tot / (ms * 1e-3) / 1e9, 4), "unit": "G (sum n*m)/s",
         "device_ms": round(ms, 2), "wall_ms": round(wall * 1e3, 2), "sum_nm": nm_tot,
-        "tuning": tune["best"], "autotune_candidates": tune["candidates"],
+        "tuning": best,
+        "roofline": {"bound": "lds", "kernel": "bc_batch", "achieved": round(lds_gbs, 1), "peak": round(lds_peak, 1),
+                     "unit": "GB/s", "frac": round(lds_gbs / lds_peak, 4), "traffic": None,
+                     "l2_adjacency_gbs": round(l2_bytes / (ms * 1e-3) / 1e9, 1),
+                     "bytes_model": "LDS: per source 24 B per adjacency entry + 32 B per vertex; peak "
+                                    f"{LDS_BYTES_PER_CLK_CU} B/clk/CU x {CU_COUNT} CUs x {ENGINE_CLOCK_GHZ} GHz"},
         "sample_max_rel_err": maxerr,
         "cpu_baseline": {"value": round(nm_tot / cpu_s / 1e9, 5), "unit": "G (sum n*m)/s", "cores": nthreads,
                          "kind": "port",
-                         "sample": f"{len(idx)} seeded graphs of the batch, threaded oracle Brandes "
-                                   f"({t_cpu:.1f} s), time scaled by sum n*m"},
+                         "sample": f"{len(idx)} seeded graphs of the batch, threaded oracle Brandes; 1 warm-up "
+                                   f"pass, median of 3 passes ({t_cpu:.2f} s per pass), time scaled by sum n*m"},
         "speedup_vs_cpu": round(cpu_s / (ms * 1e-3), 1),
     }
+    if cands is not None:
+        out["autotune_candidates"] = cands
+    return out
+
+
+def config5_bench(args, lgmi, _lib):
+    """The learned direction switch against the alpha=15 / beta=18 rule on
+    the three graphs it is specified on (directed RMAT-23 with edge factor
+    16, barabasi_albert(8_000_000, 8), the 4096 x 4096 grid), with the
+    committed network: per graph `config5_roots` of the held-out roots the
+    training run kept aside, each BFS run under both policies, outputs
+    compared bit for bit, device times summed."""
+    import zlib
+    from lgmi import policy
+    params, meta = policy.load_weights()
+    if params is None:
+        return None
+    gen = lgmi.generators
+    specs = [("rmat23_directed", lambda: gen.rmat(23, 16, seed=23, directed=True)),
+             ("ba_8M_8", lambda: gen.barabasi_albert(8_000_000, 8, seed=8)),
+             ("grid_4096x4096", lambda: gen.grid([4096, 4096]))]
+    n_train = {"rmat23_directed": 2000, "ba_8M_8": 2000, "grid_4096x4096": 40}
+    out = {"graphs": {}}
+    tot_h = tot_l = 0.0
+    same_all = True
+    for name, make in specs:
+        t0 = time.time()
+        g = make()
+        rp = g.csr()[0]
+        rng = np.random.default_rng(zlib.crc32(name.encode()))   # the training run's root draw
+        cand = np.nonzero(np.diff(rp) > 0)[0]
+        ntr = n_train[name]
+        pick = rng.choice(cand, size=min(len(cand), ntr + 200), replace=False)
+        roots = pick[ntr:ntr + args.config5_roots]
+        g.handle()
+        policy.evaluate(g, roots[:2], params)                    # warm-up
+        th, tl, same = policy.evaluate(g, roots, params)
+        g._release()
+        out["graphs"][name] = {"nv": g.nv(), "roots": int(len(roots)), "sum_ms_heuristic": round(float(th.sum()), 3),
+                               "sum_ms_learned": round(float(tl.sum()), 3),
+                               "ratio": round(float(th.sum() / tl.sum()), 4), "outputs_identical": bool(same)}
+        tot_h += float(th.sum())
+        tot_l += float(tl.sum())
+        same_all = same_all and same
+        log(f"config 5 {name}: heuristic {th.sum():.1f} ms, learned {tl.sum():.1f} ms, ratio "
+            f"{th.sum() / tl.sum():.3f}, identical={same} ({time.time() - t0:.1f}s)")
+    out["ratio_all"] = round(tot_h / tot_l, 4)
+    out["outputs_identical"] = same_all
+    out["workload"] = ("learned MLP vs alpha=15/beta=18 direction switch; held-out roots of the training draw; "
+                       "device time of full BFS runs (distances + parents)")
+    return out
+
+
+def dropin_bench(lgmi, _lib):
+    """The per-call drop-in path on RMAT-22: a caller that holds only a
+    host CSR (what the Julia shim builds from fadjlist) creates a handle
+    (upload and derived structures), runs one BFS with host outputs
+    (lgmi_bfs: device search + the dist / parent copy back) and destroys
+    the handle; beside it the same BFS on a resident handle."""
+    import ctypes
+    lib = _lib.load()
+    g = lgmi.generators.rmat(22, 16, seed=1)
+    rp, col = g.csr()
+    nv = g.nv()
+    root = int(np.argmax(np.diff(rp)))
+    d = np.empty(nv, np.int32)
+    p = np.empty(nv, np.int32)
+    rows = []
+    for rep in range(3):
+        h = ctypes.c_void_p()
+        t0 = time.perf_counter()
+        _lib.check(lib.lgmi_graph_create(nv, len(col), _lib.i32p(rp), _lib.i32p(col), 0, None, None, 0,
+                                         ctypes.byref(h)))
+        _lib.check(lib.lgmi_synchronize())
+        t1 = time.perf_counter()
+        _lib.check(lib.lgmi_bfs(h, root, 0, _lib.i32p(d), _lib.i32p(p)))
+        t2 = time.perf_counter()
+        _lib.check(lib.lgmi_bfs(h, root, 0, _lib.i32p(d), _lib.i32p(p)))
+        t3 = time.perf_counter()
+        lib.lgmi_graph_destroy(h)
+        t4 = time.perf_counter()
+        rows.append((t1 - t0, t2 - t1, t3 - t2, t4 - t3))
+    med = np.median(np.array(rows), axis=0) * 1e3
+    out = {"graph": "RMAT-22 (host CSR, 128.3M entries)", "create_ms": round(float(med[0]), 2),
+           "first_bfs_host_outputs_ms": round(float(med[1]), 2), "resident_bfs_host_outputs_ms": round(float(med[2]), 2),
+           "destroy_ms": round(float(med[3]), 2),
+           "per_call_ms": round(float(med[0] + med[1] + med[3]), 2),
+           "note": "median of 3; per_call = create + first BFS with host outputs + destroy"}
+    log(f"drop-in per call on RMAT-22: create {med[0]:.1f} ms, first BFS with host outputs {med[1]:.1f} ms, "
+        f"resident BFS with host outputs {med[2]:.1f} ms, destroy {med[3]:.1f} ms")
+    return out
 
 
 def profile_pass(lgmi, _lib, nbfs):
@@ -537,16 +704,19 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=8)
-    ap.add_argument("--cpu-roots", type=int, default=8)
-    ap.add_argument("--cpu-bc-sources", type=int, default=128)
+    ap.add_argument("--cpu-roots", type=int, default=4)
+    ap.add_argument("--cpu-bc-sources", type=int, default=32)
     ap.add_argument("--no-bc", action="store_true")
+    ap.add_argument("--quick", action="store_true",
+                    help="only the RMAT-22 BFS and RMAT-20 betweenness legs (no batch, learned-switch or drop-in legs)")
     ap.add_argument("--bc-only", action="store_true", help="run only the RMAT-20 betweenness workload")
     ap.add_argument("--extras", action="store_true",
                     help="run only the widened rows (stress, distance measures, ingest, components, neighborhood)")
     ap.add_argument("--config4", action="store_true",
-                    help="run only the uneven small-graph batch (betweenness) workload")
+                    help="run only the uneven small-graph batch (betweenness) workload, autotuning first")
     ap.add_argument("--config4-graphs", type=int, default=4096)
-    ap.add_argument("--cpu-batch-graphs", type=int, default=24)
+    ap.add_argument("--cpu-batch-graphs", type=int, default=8)
+    ap.add_argument("--config5-roots", type=int, default=20)
     ap.add_argument("--save-tuning", default="", help="write the batch autotuner's choice to this JSON file")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -563,7 +733,7 @@ def main():
         profile_pass(lgmi, _lib, args.profile_pass)
         return
     if args.config4:
-        print(json.dumps(config4_bench(args, lgmi, orc)), flush=True)
+        print(json.dumps(config4_bench(args, lgmi, orc, tune=True)), flush=True)
         return
     if args.extras:
         print(json.dumps(extras_bench(args, lgmi, _lib, orc)), flush=True)
@@ -574,6 +744,10 @@ def main():
     result = bfs_bench(args, lgmi, _lib, orc)
     if not args.no_bc:
         result["bc"] = bc_bench(args, lgmi, _lib, orc)
+    if not args.quick:
+        result["batch"] = config4_bench(args, lgmi, orc)
+        result["learned_switch"] = config5_bench(args, lgmi, _lib)
+        result["dropin"] = dropin_bench(lgmi, _lib)
     print(json.dumps(result), flush=True)
 
 

@@ -84,7 +84,7 @@ typedef struct {
     int32_t max_levels;        /* deepest BFS level over all groups          */
     int32_t launch;            /* 2: one persistent launch replayed from a   */
                                /* captured hipGraph, 1: one persistent       */
-                               /* cooperative launch, 0: one launch per level */
+                               /* launch, 0: one launch per level           */
     int32_t pad;
     double device_ms;          /* HIP-event time of the whole computation    */
     double forward_ms;         /* sigma passes                                */

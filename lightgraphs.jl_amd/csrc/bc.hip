@@ -1242,8 +1242,11 @@ static std::atomic<int> g_launch_mode{0};
 
 void bc_set_launch(int mode) { g_launch_mode = mode; }
 
-// One launch for all groups (ngroups <= slots): captured in a hipGraph when
-// the runtime captures cooperative launches, else launched cooperatively.
+// One launch for all groups (ngroups <= slots), captured in a hipGraph (a
+// plain launch when capture fails).  The launch is an ordinary one: the
+// kernel itself keeps only the blocks that start together (tickets), so it
+// needs no co-residency promise from a cooperative launch, and the runtime
+// keeps no cooperative-launch state alive until the process ends.
 // Returns false when the launch could not finish (a barrier timed out or a
 // slot ran past the level capacity); the caller then runs the level-per-launch
 // path, which overwrites everything this one wrote.
@@ -1256,10 +1259,9 @@ static bool bc_persistent(lgmi_graph* g, BcArgs a, int32_t ns, const double* end
     hipStream_t st = g->stream;
     int dev = 0;
     LGMI_HIP(hipGetDevice(&dev));
-    int coop = 0, khz = 0, per_cu = 0;
-    LGMI_HIP(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+    int khz = 0, per_cu = 0;
     LGMI_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
-    if (!coop || khz <= 0) return false;
+    if (khz <= 0) return false;
     LGMI_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kBlock, 0));
     const int cu = std::max(1, g->cu_count);
     // the blocks that the occupancy calculation says fit; the kernel itself
@@ -1298,7 +1300,7 @@ static bool bc_persistent(lgmi_graph* g, BcArgs a, int32_t ns, const double* end
             bc_persist_prologue<<<1, 1, 0, st>>>(w->pctl);
             hipError_t el = hipGetLastError();
             if (el == hipSuccess)
-                el = hipLaunchCooperativeKernel(kfn, gd, bd, args, 0, st);
+                el = hipLaunchKernel(kfn, gd, bd, args, 0, st);
             e = hipStreamEndCapture(st, &graph);
             if (el != hipSuccess) e = el;
         }
@@ -1319,7 +1321,7 @@ static bool bc_persistent(lgmi_graph* g, BcArgs a, int32_t ns, const double* end
     } else {
         bc_persist_prologue<<<1, 1, 0, st>>>(w->pctl);
         LGMI_HIP(hipGetLastError());
-        LGMI_HIP(hipLaunchCooperativeKernel(kfn, gd, bd, args, 0, st));
+        LGMI_HIP(hipLaunchKernel(kfn, gd, bd, args, 0, st));
         *launch_kind = 1;
     }
     LGMI_HIP(hipEventRecord(w->ev[2], st));

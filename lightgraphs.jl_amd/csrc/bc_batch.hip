@@ -636,6 +636,8 @@ extern "C" int lgmi_betweenness_batch(int32_t ngraphs, const int32_t* nv, const 
             return p;
         };
         hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t gexec = nullptr;
         try {
             BatchArgs a{};
             a.nv = (const int32_t*)up(nv, sizeof(int32_t) * ngraphs);
@@ -672,15 +674,23 @@ extern "C" int lgmi_betweenness_batch(int32_t ngraphs, const int32_t* nv, const 
             LGMI_HIP(hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn_lds));
             LGMI_HIP(hipEventCreate(&e0));
             LGMI_HIP(hipEventCreate(&e1));
-            LGMI_HIP(hipEventRecord(e0, st));
+            // The persistent launch and the reduction over tasks are captured
+            // in one hipGraph (the task counters are cleared above, outside
+            // the graph) and replayed once.
             void* kargs[] = {&a};
-            LGMI_HIP(hipLaunchKernel(kern, dim3(nblocks), dim3(kBB), kargs, dyn_lds, st));
-            LGMI_HIP(hipGetLastError());
-            int32_t total_out = (int32_t)std::min<int64_t>(out_total, 1 << 30);
-            bc_batch_finish<<<std::max(1, std::min(cu * 8, (int)((out_total + 255) / 256))), 256, 0, st>>>(
-                ngraphs, a.nv, d_bcoff, d_gt0, d_gtn, d_gts, a.task_part, a.part, d_scale, d_out, out_total);
-            (void)total_out;
-            LGMI_HIP(hipGetLastError());
+            LGMI_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+            hipError_t ce = hipLaunchKernel(kern, dim3(nblocks), dim3(kBB), kargs, dyn_lds, st);
+            if (ce == hipSuccess) {
+                bc_batch_finish<<<std::max(1, std::min(cu * 8, (int)((out_total + 255) / 256))), 256, 0, st>>>(
+                    ngraphs, a.nv, d_bcoff, d_gt0, d_gtn, d_gts, a.task_part, a.part, d_scale, d_out, out_total);
+                ce = hipGetLastError();
+            }
+            const hipError_t ee = hipStreamEndCapture(st, &graph);
+            LGMI_HIP(ce);
+            LGMI_HIP(ee);
+            LGMI_HIP(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0));
+            LGMI_HIP(hipEventRecord(e0, st));
+            LGMI_HIP(hipGraphLaunch(gexec, st));
             LGMI_HIP(hipEventRecord(e1, st));
             std::vector<double> hout((size_t)out_total);
             LGMI_HIP(hipMemcpyAsync(hout.data(), d_out, sizeof(double) * (size_t)out_total, hipMemcpyDeviceToHost,
@@ -692,12 +702,17 @@ extern "C" int lgmi_betweenness_batch(int32_t ngraphs, const int32_t* nv, const 
             LGMI_HIP(hipEventElapsedTime(&ms, e0, e1));
             if (device_ms) *device_ms = ms;
         } catch (...) {
+            (void)hipStreamSynchronize(st);
+            if (gexec) (void)hipGraphExecDestroy(gexec);
+            if (graph) (void)hipGraphDestroy(graph);
             if (e0) (void)hipEventDestroy(e0);
             if (e1) (void)hipEventDestroy(e1);
             for (void* p : bufs) dev_free(p);
             (void)hipStreamDestroy(st);
             throw;
         }
+        (void)hipGraphExecDestroy(gexec);
+        (void)hipGraphDestroy(graph);
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         for (void* p : bufs) dev_free(p);

@@ -10,9 +10,27 @@ Vertex ids in the public API are 1-based, as in Julia.
 """
 from __future__ import annotations
 
+import atexit
+import weakref
+
 import numpy as np
 
 from . import _lib
+
+# Graphs that hold a device handle.  At interpreter exit every handle still
+# alive is destroyed while the HIP runtime (and a profiler attached to it)
+# is fully up, instead of whenever a late __del__ or the runtime's own
+# teardown reaches it.
+_live: "weakref.WeakSet" = weakref.WeakSet()
+
+
+@atexit.register
+def _release_all():
+    for g in list(_live):
+        try:
+            g._release()
+        except Exception:
+            pass
 
 
 def _as_i32(a):
@@ -105,6 +123,7 @@ class _GraphBase:
                                       _lib.i32p(icol) if cls.directed else None))
         g = cls.from_csr(rp, col[:m], T=T, in_row_ptr=irp, in_col=icol[:m] if cls.directed else None)
         g._handle = h
+        _live.add(g)
         return g, int(ne[0])
 
     @classmethod
@@ -222,6 +241,7 @@ class _GraphBase:
                                            None, None, 0, _lib.ctypes.byref(h))
             _lib.check(rc)
             self._handle = h
+            _live.add(self)
         return self._handle
 
     def _release(self):

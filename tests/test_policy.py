@@ -132,8 +132,8 @@ def test_probe_records_and_state_restore(oracle):
 def test_learned_policy_outputs_identical_and_gain(oracle):
     """Train on probes of some roots, evaluate on held-out roots: outputs
     are bit-identical to the alpha/beta run, and the per-level decision cost
-    (time of the chosen direction, measured on the identical state) is
-    reported for the heuristic, the network and the best choice."""
+    (time of the chosen direction, measured on the identical state) of the
+    network is below the heuristic's and no better than the best choice."""
     import lgmi
     from lgmi import policy
     graphs = [lgmi.generators.rmat(16, 16, seed=11), lgmi.generators.barabasi_albert(60_000, 8, seed=12),
@@ -156,4 +156,5 @@ def test_learned_policy_outputs_identical_and_gain(oracle):
         assert same, "learned policy changed an output"
     say(f"held-out decision cost (us): {tot}; gain {tot['heuristic'] / tot['learned']:.3f}x")
     assert tot["best"] <= tot["learned"] + 1e-6
-    assert tot["learned"] <= 1.05 * tot["heuristic"]
+    # the network must beat the rule it replaces on roots it never saw
+    assert tot["learned"] < tot["heuristic"]
