@@ -188,6 +188,8 @@ def bfs_bench(args, lgmi, _lib, orc):
         f"({', '.join(f'{t:.2f}' for t in t_runs)}) -> {cpu_gteps:.4f} GTEPS")
 
     traffic = read_traffic()
+    from lgmi import tuning
+    tuning_values = tuning.get()
     result = {
         "metric": METRIC,
         "value": round(gteps, 4),
@@ -209,6 +211,7 @@ def bfs_bench(args, lgmi, _lib, orc):
             "gteps_harmonic_mean_device": round(float(len(per_root) / np.sum(1.0 / per_root)), 3),
             "gteps_median_device": round(float(np.median(per_root)), 3),
             "gteps_learned_switch": round(gteps_learned, 4) if gteps_learned is not None else None,
+            "tuning": tuning_values,
         },
         "roofline": {
             "kernel": "bfs_level",

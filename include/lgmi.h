@@ -340,6 +340,39 @@ int lgmi_betweenness_batch(int32_t ngraphs, const int32_t* nv, const int64_t* ro
 int lgmi_betweenness_batch_tuning(int32_t wave_max_n, double task_work_target, int32_t lds_max_n,
                                   int32_t lane_group, int32_t slots);
 
+/* Launch thresholds of the BFS steps and of the betweenness heavy-row
+ * segments (process-wide).  No counterpart in the reference, whose threaded
+ * BFS has one such knob, queue_segment_size (src/Parallel/traversals/
+ * gdistances.jl:135-139).  The autotuner (lgmi/tuning.py) writes its
+ * choice to lgmi/tuning.json, which the Python package applies before it
+ * creates a graph handle; the Julia shim does the same.  No value changes
+ * a result, only time.  Out-of-range values give LGMI_E_ARG. */
+typedef struct {
+    int32_t wave_row;          /* top-down sweep: rows of at least this many  */
+                               /* entries are expanded by a whole wavefront   */
+                               /* (default 32; 2..257)                        */
+    int32_t lane_rounds;       /* bottom-up: rounds of four entries a         */
+                               /* candidate scans in its own lane before a    */
+                               /* whole wavefront takes its row (default 4;   */
+                               /* 0..64)                                      */
+    int32_t range_front;       /* range mode: frontier vertices at most       */
+                               /* (default 2048; 0..2048, 0 turns it off)     */
+    int32_t queue_front;       /* queue mode: frontier vertices at most       */
+                               /* (default 4096; 0..4096, 0 turns it off)     */
+    int64_t range_light;       /* range mode: frontier edges in rows of at    */
+                               /* most 256 entries, at most (default 32768)   */
+    int64_t queue_min_edges;   /* queue mode: frontier edges at least         */
+    int64_t queue_max_edges;   /* queue mode: frontier edges at most          */
+    int32_t bc_seg_len;        /* betweenness: rows longer than this are cut  */
+                               /* into segments of this many entries (default */
+                               /* 1024; 64..65536; fixed for a graph at its   */
+                               /* first betweenness call)                     */
+    int32_t pad;
+} lgmi_tuning;
+/* t = NULL restores the built-in defaults. */
+int lgmi_set_tuning(const lgmi_tuning* t);
+int lgmi_get_tuning(lgmi_tuning* t);
+
 /* Thread-local message of the last failed call ("" if none). */
 const char* lgmi_last_error(void);
 

@@ -112,6 +112,7 @@ struct BcGraph {
     int32_t nseg_in = 0, nh_in = 0, nseg_out = 0, nh_out = 0;
     int32_t* seg_in[6] = {};          // seg_v, seg_b, seg_e, seg_h, h_first, h_nseg
     int32_t* seg_out[6] = {};
+    int32_t seg_len = 1024;           // segment length the arrays were cut with
 };
 
 void bc_graph_free(BcGraph* b) {
@@ -125,11 +126,10 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 
-// Rows longer than kBcHeavyRow entries are cut into segments of kBcSegLen
+// Rows longer than the segment length (lgmi_tuning.bc_seg_len, fixed for a
+// graph when its segments are built) are cut into segments of that many
 // entries, each a work item of its own; after all segments of a level step,
 // one wavefront per row adds the segments' partial sums in segment order.
-constexpr int32_t kBcHeavyRow = 1024;
-constexpr int32_t kBcSegLen = 1024;
 
 struct BcSegs {
     const int32_t* seg_v;    // [nseg] the row's vertex
@@ -161,6 +161,7 @@ struct BcArgs {
     double* hpart;           // [slot][seg_stride][64] per-segment lane sums
     uint64_t* hgot;          // [slot][seg_stride] per-segment new-lane masks
     int32_t seg_stride;
+    int32_t heavy;           // rows longer than this are heavy (the segment length)
     int32_t fmod;            // levels F keeps: cap (all), or 2 (distances only)
     int32_t fstride;         // slots interleaved in F: F[level][v][slot]
     uint64_t* st;            // distances only: [slot][chunk or heavy row][3][64]
@@ -319,7 +320,7 @@ __device__ bool fwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
         sj = seen[vj];
         bj = a.in_rp[vj];
         ej = a.in_rp[vj + 1];
-        heavy = ej - bj > kBcHeavyRow;
+        heavy = ej - bj > a.heavy;
         cand = (~sj & act) && ej > bj && !heavy;
     }
     uint64_t newF = 0;
@@ -562,7 +563,7 @@ __device__ void bwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
         if (Aj) {
             bj = a.out_rp[vj];
             ej = a.out_rp[vj + 1];
-            if (ej - bj > kBcHeavyRow) Aj = 0;   // a heavy row: its segments do it
+            if (ej - bj > a.heavy) Aj = 0;   // a heavy row: its segments do it
         }
     }
     T bcj = 0;
@@ -1167,6 +1168,8 @@ static BcGraph* bc_graph_get(lgmi_graph* g) {
     if (g->bcg) return g->bcg;
     const int32_t nv = g->nv;
     BcGraph* b = new BcGraph();
+    const int32_t seg_len = tuning_get().bc_seg_len;
+    b->seg_len = seg_len;
     try {
         b->rp_out.resize((size_t)nv + 1);
         LGMI_HIP(hipMemcpy(b->rp_out.data(), g->out.row_ptr, sizeof(int32_t) * ((size_t)nv + 1), hipMemcpyDeviceToHost));
@@ -1178,13 +1181,13 @@ static BcGraph* bc_graph_get(lgmi_graph* g) {
         auto build = [&](const std::vector<int32_t>& rp, std::vector<int32_t> (&arr)[6]) {
             for (int32_t v = 0; v < nv; v++) {
                 const int32_t bb = rp[v], e = rp[v + 1];
-                if (e - bb <= kBcHeavyRow) continue;
+                if (e - bb <= seg_len) continue;
                 const int32_t h = (int32_t)arr[4].size();
                 arr[4].push_back((int32_t)arr[0].size());
-                for (int32_t x = bb; x < e; x += kBcSegLen) {
+                for (int32_t x = bb; x < e; x += seg_len) {
                     arr[0].push_back(v);
                     arr[1].push_back(x);
-                    arr[2].push_back(std::min(e, x + kBcSegLen));
+                    arr[2].push_back(std::min(e, x + seg_len));
                     arr[3].push_back(h);
                 }
                 arr[5].push_back((int32_t)arr[0].size() - arr[4].back());
@@ -1458,6 +1461,7 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
     a.hout = make_segs(bg->seg_out, bg->nseg_out, bg->nh_out);
     a.hpart = w->hpart; a.hgot = w->hgot;
     a.seg_stride = w->seg_stride;
+    a.heavy = bg->seg_len;
     a.fmod = mode == 2 ? 2 : w->cap_levels;
     a.fstride = w->nslots;
     a.st = w->st;

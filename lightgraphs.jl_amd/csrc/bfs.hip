@@ -65,13 +65,12 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kLaneBits = 8;                    // frontier bits per lane in a tile
 constexpr int kTileVerts = 64 * kLaneBits;      // vertices per wavefront tile
 constexpr int kTileWords = kTileVerts / 32;     // bitmap words per tile
-constexpr int32_t kWaveRow = 32;    // top-down: rows this long or longer are expanded by a wavefront
 constexpr int kUnroll = 2;          // top-down: adjacency entries per lane per round
 constexpr int kGraphLevels = 16;    // level launches per captured graph (even: parity is static)
 constexpr int kPartRows = kBlock / kPart;
-constexpr int32_t kRangeFront = 2048;    // range mode: frontier vertices at most
-constexpr int64_t kRangeLight = 32768;   // range mode: light-row frontier edges at most
+constexpr int32_t kRangeFront = 2048;    // range mode: frontier vertices at most (LDS layout)
 constexpr int kEmit = 8;                 // range mode: 1024-vertex slices emitted per round
+constexpr int32_t kQueueFront = 4096;    // queue mode: frontier vertices at most (LDS layout)
 
 struct StepArgs {
     const int32_t* td_rp;     // traversal adjacency (top-down expands it)
@@ -102,7 +101,23 @@ struct StepArgs {
     LevelRec* log;
     int32_t max_log;
     const PolicyWeights* policy;
+    int32_t* done_host;       // host-mapped copy of done_level (nullptr: none)
+    // launch thresholds (lgmi_tuning)
+    int32_t wave_row;         // top-down: rows this long or longer are expanded by a wavefront
+    int32_t lane_rounds;      // bottom-up: per-lane rounds of four entries before whole-wavefront scans
+    int32_t range_front;      // range mode: frontier vertices at most (<= kRangeFront)
+    int32_t queue_front;      // queue mode: frontier vertices at most (<= kQueueFront)
+    int64_t range_light;      // range mode: light-row frontier edges at most
+    int64_t queue_min_edges;  // queue mode: frontier edges at least
+    int64_t queue_max_edges;  // queue mode: frontier edges at most
 };
+
+// done_level and its host-mapped copy (which the host reads after the
+// launches complete, instead of copying the state back).
+__device__ inline void set_done(const StepArgs& a, int32_t v) {
+    a.sh->done_level = v;
+    if (a.done_host) __hip_atomic_store(a.done_host, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
 
 __device__ inline uint32_t* bm_at(const StepArgs& a, int k) {
     return k == 0 ? a.bm0 : (k == 1 ? a.bm1 : a.bm2);
@@ -421,14 +436,14 @@ __device__ __attribute__((always_inline)) inline void td_step(const StepArgs& a,
             }
             k.scan += (unsigned long long)deg;
             // wavefront-wide rows
-            uint64_t m = __ballot(deg >= kWaveRow);
+            uint64_t m = __ballot(deg >= a.wave_row);
             while (m) {
                 const int j = __ffsll((unsigned long long)m) - 1;
                 m &= m - 1;
                 const int32_t wu = __shfl(u, j), wb = __shfl(beg, j), wd = __shfl(deg, j);
                 expand_row<PARENTS>(a, wu, wb, wb + wd, L, FL, Fn, k, s_qn, qout, lane);
             }
-            if (deg >= kWaveRow) deg = 0;
+            if (deg >= a.wave_row) deg = 0;
             // short rows: scan-based gather over the batch, kUnroll entries per lane
             int32_t rtot;
             const int32_t rex = wave_excl_scan(deg, lane, rtot);
@@ -465,12 +480,11 @@ __device__ inline int bu_found(const StepArgs& a, int32_t v, int32_t par, int32_
 
 // Second stage of the parent scan: up to 64 queued candidates, one per lane,
 // scanning four entries per round trip from entry b onwards, for at most
-// kLaneRounds rounds; rows still open after that are scanned by the whole
+// a.lane_rounds rounds; rows still open after that are scanned by the whole
 // wavefront, 64 entries per round trip.
 template <bool PARENTS>
 __device__ __attribute__((always_inline)) inline void bu_drain(const StepArgs& a, const Pending* q, int n, int32_t v0, int32_t L, const uint32_t* FL,
                          Counters& k, uint32_t* s_qn, int32_t* qout, uint32_t* s_fn, int lane) {
-    constexpr int kLaneRounds = 4;
     int32_t v = -1, b = 0, e = 0, b0 = 0;
     // field by field: a whole-struct copy can leave a private copy that the
     // compiler places in LDS at a per-thread address, which needs the
@@ -479,7 +493,7 @@ __device__ __attribute__((always_inline)) inline void bu_drain(const StepArgs& a
     int32_t par = -1;
     int32_t x = b;
     if (v >= 0) {
-        for (int r = 0; r < kLaneRounds && x < e; r++, x += 4) {
+        for (int r = 0; r < a.lane_rounds && x < e; r++, x += 4) {
             k.scan += (unsigned long long)min(4, e - x);
             int32_t uu[4];
             uint32_t bw[4];
@@ -829,6 +843,130 @@ __device__ __attribute__((always_inline)) inline void range_step(const StepArgs&
     }
 }
 
+// Top-down step in queue mode, for a small frontier whose work-group queues
+// are complete (or the single source of level 0).  Every work-group reads
+// the whole frontier list, lays the frontier rows' entries out with a prefix
+// sum in LDS and expands an equal share of them, work-group b the entries
+// [b * T / G, (b + 1) * T / G), four consecutive entries per lane and
+// round; the visit is the sweep's (plain-load filter, atomicMin claim).  No
+// bitmap is swept: the frontier's vertices are folded into the visited
+// bitmap one by one, and F_{k+2} is cleared with plain stores only.
+template <bool PARENTS>
+__device__ __attribute__((always_inline)) inline void queue_step(const StepArgs& a, int32_t L, const uint32_t* FL,
+                                                                 uint32_t* Fn, uint32_t* Fc, Counters& k,
+                                                                 uint32_t* s_qn, int32_t* qout, uint32_t* s_lds,
+                                                                 const int32_t* s_qoff, int32_t nf, int32_t src0,
+                                                                 int lpar, int tid, int lane, int wid) {
+    constexpr int kPer = kQueueFront / kBlock;   // frontier entries per thread
+    int32_t* s_su = reinterpret_cast<int32_t*>(s_lds);   // [kQueueFront] frontier vertex
+    int32_t* s_sb = s_su + kQueueFront;                   // [kQueueFront] first entry of its row
+    int32_t* s_sx = s_sb + kQueueFront;                   // [kQueueFront + 1] entry offsets (prefix)
+    int32_t* s_ws = s_sx + kQueueFront + 1;               // [kWaves] wavefront totals
+    const int32_t* qin = a.qbuf + (size_t)(lpar ^ 1) * a.grid * kQCap;
+    // thread t holds frontier entries kPer * t .. kPer * t + kPer - 1
+    int32_t fu[kPer];
+    #pragma unroll
+    for (int q = 0; q < kPer; q++) {
+        const int32_t j = kPer * tid + q;
+        fu[q] = -1;
+        if (j < nf) {
+            if (src0 >= 0) {
+                fu[q] = src0;
+            } else {
+                int lo = 0, hi = a.grid;                  // largest b with s_qoff[b] <= j
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_qoff[mid] <= j) lo = mid;
+                    else hi = mid;
+                }
+                fu[q] = qin[(size_t)lo * kQCap + (j - s_qoff[lo])];
+            }
+        }
+    }
+    int32_t fb[kPer], len[kPer];
+    #pragma unroll
+    for (int q = 0; q < kPer; q++) {
+        fb[q] = 0;
+        len[q] = 0;
+        if (fu[q] >= 0) {
+            fb[q] = a.td_rp[fu[q]];
+            len[q] = a.td_rp[fu[q] + 1] - fb[q];
+        }
+    }
+    // F_k into the visited bitmap (each frontier vertex by one work-group),
+    // F_{k+2} cleared (stores only); neither is read by this step
+    #pragma unroll
+    for (int q = 0; q < kPer; q++) {
+        const int32_t j = kPer * tid + q;
+        if (fu[q] >= 0 && (j % a.grid) == (int32_t)blockIdx.x)
+            atomicOr(&a.vis[fu[q] >> 5], 1u << (fu[q] & 31));
+    }
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + tid; i < a.nwords / 2; i += (int64_t)a.grid * kBlock)
+        reinterpret_cast<uint2*>(Fc)[i] = make_uint2(0u, 0u);
+    // entry offsets
+    int32_t tsum = 0;
+    #pragma unroll
+    for (int q = 0; q < kPer; q++) tsum += len[q];
+    int32_t wtot;
+    const int32_t ex = wave_excl_scan(tsum, lane, wtot);
+    if (lane == 0) s_ws[wid] = wtot;
+    lds_barrier();
+    if (wid == 0) {
+        const int32_t x = lane < kWaves ? s_ws[lane] : 0;
+        int32_t t2;
+        const int32_t e2 = wave_excl_scan(x, lane, t2);
+        if (lane < kWaves) s_ws[lane] = e2;
+        if (lane == 0) s_sx[nf] = t2;
+    }
+    lds_barrier();
+    {
+        int32_t off = s_ws[wid] + ex;
+        #pragma unroll
+        for (int q = 0; q < kPer; q++) {
+            const int32_t j = kPer * tid + q;
+            if (j < nf) {
+                s_su[j] = fu[q];
+                s_sb[j] = fb[q];
+                s_sx[j] = off;
+            }
+            off += len[q];
+        }
+    }
+    lds_barrier();
+    const int64_t T = s_sx[nf];
+    const int32_t e0 = (int32_t)(T * (int64_t)blockIdx.x / a.grid);
+    const int32_t e1 = (int32_t)(T * (int64_t)(blockIdx.x + 1) / a.grid);
+    for (int32_t base = e0 + wid * 256; base < e1; base += kWaves * 256) {
+        const int32_t e = base + lane * 4;
+        int32_t uu[4], vv[4];
+        int seg = 0;
+        if (e < e1) {
+            int lo = 0, hi = nf;                          // largest segment with s_sx[seg] <= e
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_sx[mid] <= e) lo = mid;
+                else hi = mid;
+            }
+            seg = lo;
+        }
+        int32_t nvalid = 0;
+        #pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int32_t eq = e + q;
+            uu[q] = 0;
+            vv[q] = -1;
+            if (eq < e1) {
+                while (s_sx[seg + 1] <= eq) seg++;
+                uu[q] = s_su[seg];
+                vv[q] = a.td_col[s_sb[seg] + (eq - s_sx[seg])];
+                nvalid++;
+            }
+        }
+        k.scan += (unsigned long long)nvalid;
+        visit_n<PARENTS, 4>(a, uu, vv, L, FL, Fn, k, s_qn, qout, lane);
+    }
+}
+
 // The learned policy's weights as lane j of wavefront 0 needs them (hidden
 // unit j), loaded with the prologue's other loads.  The device copy has the
 // feature standardisation folded into W1 and b1 (done on the host when the
@@ -888,7 +1026,7 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
     __shared__ unsigned long long s_red[kWaves][kPart];
     __shared__ unsigned long long s_sum[kPart];
     __shared__ float s_feat[kFeat];
-    __shared__ int32_t s_p, s_dir, s_exit, s_dl, s_range;
+    __shared__ int32_t s_p, s_dir, s_exit, s_dl, s_mode, s_src0;
     __shared__ uint32_t s_qn[1];                 // the work-group queue's count
     __shared__ uint32_t s_fn[kWaves][kTileWords];
     __shared__ int32_t s_qoff[kMaxGrid + 1];      // per-work-group new-vertex counts, then queue offsets
@@ -900,6 +1038,8 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
     Pending* s_q = reinterpret_cast<Pending*>(s_lds);
     // per-wavefront tile lists of the sweep steps, after the queues
     static_assert(kLdsWords >= kWaves * 128 * 4 + kWaves * kTileVerts, "tile lists exceed the LDS area");
+    static_assert(kLdsWords >= 3 * kQueueFront + 1 + kWaves, "queue-mode lists exceed the LDS area");
+    static_assert(kQueueFront % kBlock == 0, "queue mode: whole frontier entries per thread");
     int32_t* s_lists = reinterpret_cast<int32_t*>(s_lds + kWaves * 128 * 4);
 
     // the wavefront index is uniform over a wavefront; readfirstlane lets the
@@ -943,6 +1083,7 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
     if (tid == 0) {
         s_c = sh->state[lpar ^ 1];   // the first launch reads S_0 from state[1]
         s_dl = sh->done_level;
+        s_src0 = first ? sh->src0 : -1;
     }
     if (tid == 0) s_qn[0] = 0;
     // lanes l, l+16, l+32, l+48 of a wavefront hold the same slot
@@ -1001,7 +1142,7 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
                     ex = 1;
                     if (blockIdx.x == 0) {
                         sh->state[p & 1] = c;
-                        sh->done_level = p;
+                        set_done(a, p);
                     }
                 } else if (may_learn || (blockIdx.x == 0 && k < a.max_log)) {
                     // the features feed the network and work-group 0's level log only
@@ -1024,15 +1165,20 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // queue mode: top-down over a small frontier with few edges whose
+        // work-group queues are complete (level 0: a single source);
         // range mode: top-down over a small frontier that holds heavy rows
         // and few light-row edges, when the range table exists and every
         // work-group queue of the frontier is complete
         const BfsCtrl& c = s_c;
-        const bool range = !s_exit && s_dir == 0 && !first && a.range_size > 0 && c.qovf == 0 &&
-                           c.n_front <= kRangeFront && c.m_light <= kRangeLight &&
+        const bool top = !s_exit && s_dir == 0;
+        const bool queue = top && (first ? s_src0 >= 0 : c.qovf == 0) && c.n_front <= a.queue_front &&
+                           c.m_front >= a.queue_min_edges && c.m_front <= a.queue_max_edges;
+        const bool range = top && !queue && !first && a.range_size > 0 && c.qovf == 0 &&
+                           c.n_front <= a.range_front && c.m_light <= a.range_light &&
                            c.front_hist[5] + c.front_hist[6] > 0;
-        if (lane == 0) s_range = range ? 1 : 0;
-        if (range) {
+        if (lane == 0) s_mode = queue ? 2 : (range ? 1 : 0);
+        if ((range || queue) && !first) {
             // queue offsets: exclusive scan of the per-work-group counts,
             // four consecutive counts per lane (one wavefront scan per 256)
             int32_t carry = 0;
@@ -1062,13 +1208,13 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
     const int32_t p = s_p;
     const int32_t L = s_c.level;
     const int dir = s_dir;
-    const bool range = s_range != 0;
+    const int vmode = s_mode;
 
     if (blockIdx.x == 0 && tid == 0) {
         BfsCtrl c = s_c;
         c.phase = p;
         c.dir = dir;
-        c.variant = range ? 1 : 0;
+        c.variant = vmode;
         if (first) c.t_main = t_now;
         if (!first && L < a.max_log)
             for (int i = 0; i < kFeat; i++) a.log[L].feat[i] = s_feat[i];
@@ -1083,7 +1229,9 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
     Counters k;
     if (dir == 1) bu_step<PARENTS>(a, L, FL, Fn, Fc, k, s_qn, qout, s_fn[wid], s_q + wid * 128,
                                   s_lists + wid * kTileVerts, gw, nw, lane);
-    else if (range) range_step<PARENTS>(a, L, FL, Fn, Fc, k, s_qn, qout, s_lds, s_qoff, s_qoff[a.grid], lpar, tid, lane, wid);
+    else if (vmode == 2) queue_step<PARENTS>(a, L, FL, Fn, Fc, k, s_qn, qout, s_lds, s_qoff, s_c.n_front,
+                                             s_src0, lpar, tid, lane, wid);
+    else if (vmode == 1) range_step<PARENTS>(a, L, FL, Fn, Fc, k, s_qn, qout, s_lds, s_qoff, s_qoff[a.grid], lpar, tid, lane, wid);
     else td_step<PARENTS>(a, s_c, L, FL, Fn, Fc, k, s_qn, qout, s_lists + wid * kTileVerts, gw, nw, lane);
 
     // block reduction -> this block's counters (plain stores)
@@ -1113,9 +1261,9 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
 }
 
 // S_0 of a search from nsrc sources (level-0 counters given).
-__device__ void init_state(const StepArgs& a, int32_t nsrc, unsigned long long m_out, unsigned long long m_in,
-                           const uint32_t* hist, int64_t nnz_scan, int32_t forced, int32_t mode, float alpha,
-                           float beta) {
+__device__ void init_state(const StepArgs& a, int32_t nsrc, int32_t src0, unsigned long long m_out,
+                           unsigned long long m_in, const uint32_t* hist, int64_t nnz_scan, int32_t forced,
+                           int32_t mode, float alpha, float beta) {
     BfsCtrl c;
     c.level = 0;
     c.dir = forced >= 0 ? forced : 0;
@@ -1138,7 +1286,8 @@ __device__ void init_state(const StepArgs& a, int32_t nsrc, unsigned long long m
     c.t_main = wall_clock64();
     a.sh->state[0] = c;
     a.sh->state[1] = c;      // what the first launch reads
-    a.sh->done_level = nsrc == 0 ? 0 : INT_MAX;
+    a.sh->src0 = src0;
+    set_done(a, nsrc == 0 ? 0 : INT_MAX);
     if (a.max_log > 0) {
         float f[kFeat];
         compute_features(&c, f);
@@ -1152,20 +1301,48 @@ __device__ void init_state(const StepArgs& a, int32_t nsrc, unsigned long long m
 __global__ void bfs_init(StepArgs a, int32_t src, int64_t nnz_scan, int32_t forced, int32_t mode, float alpha,
                          float beta) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.nv; i += stride) {
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    // dist / parent in 16-byte stores where both arrays allow it (the
+    // handle's buffers always do; a caller's buffer may not); each element
+    // is written by exactly one thread, the source's with its seed value
+    const bool vec = ((uintptr_t)a.dist & 15) == 0 && ((uintptr_t)a.parent & 15) == 0;
+    const int64_t n4 = vec ? a.nv / 4 : 0;
+    for (int64_t i = t0; i < n4; i += stride) {
+        int4 d = make_int4(-1, -1, -1, -1), p = d;
+        if (src >= 0 && (src >> 2) == i) {
+            const int c = src & 3;
+            if (c == 0) { d.x = 0; p.x = src; }
+            else if (c == 1) { d.y = 0; p.y = src; }
+            else if (c == 2) { d.z = 0; p.z = src; }
+            else { d.w = 0; p.w = src; }
+        }
+        reinterpret_cast<int4*>(a.dist)[i] = d;
+        if (a.parent) reinterpret_cast<int4*>(a.parent)[i] = p;
+    }
+    for (int64_t i = 4 * n4 + t0; i < a.nv; i += stride) {
         a.dist[i] = i == src ? 0 : -1;
         if (a.parent) a.parent[i] = i == src ? (uint32_t)src : 0xffffffffu;
     }
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < a.nwords; i += stride) {
-        a.bm0[i] = (src >= 0 && i == (src >> 5)) ? (1u << (src & 31)) : 0u;
-        a.bm1[i] = 0; a.bm2[i] = 0; a.vis[i] = 0;
+    // bitmaps: nwords is even and every bitmap 8-byte aligned
+    for (int64_t i = t0; i < a.nwords / 2; i += stride) {
+        uint2 f = make_uint2(0u, 0u);
+        if (src >= 0 && (src >> 6) == i) {
+            if ((src >> 5) & 1) f.y = 1u << (src & 31);
+            else f.x = 1u << (src & 31);
+        }
+        const uint2 z = make_uint2(0u, 0u);
+        reinterpret_cast<uint2*>(a.bm0)[i] = f;
+        reinterpret_cast<uint2*>(a.bm1)[i] = z;
+        reinterpret_cast<uint2*>(a.bm2)[i] = z;
+        reinterpret_cast<uint2*>(a.vis)[i] = z;
     }
     if (src >= 0 && blockIdx.x == 0 && threadIdx.x == 0) {
         const int32_t dout = a.td_rp[src + 1] - a.td_rp[src];
         const int32_t din = a.bu_rp[src + 1] - a.bu_rp[src];
         uint32_t hist[kHistBins] = {0, 0, 0, 0, 0, 0, 0, 0};
         hist[hist_bin(dout)] = 1;
-        init_state(a, 1, (unsigned long long)dout, (unsigned long long)din, hist, nnz_scan, forced, mode, alpha, beta);
+        init_state(a, 1, src, (unsigned long long)dout, (unsigned long long)din, hist, nnz_scan, forced, mode, alpha,
+                   beta);
     }
 }
 
@@ -1193,7 +1370,7 @@ __global__ __launch_bounds__(256) void bfs_seed(StepArgs a, const int32_t* srcs,
     atomicAdd(&s_out, mo);
     atomicAdd(&s_in, mi);
     __syncthreads();
-    if (tid == 0) init_state(a, nsrc, s_out, s_in, s_hist, nnz_scan, forced, mode, alpha, beta);
+    if (tid == 0) init_state(a, nsrc, -1, s_out, s_in, s_hist, nnz_scan, forced, mode, alpha, beta);
 }
 
 // Undoes one step: every vertex of F_{k+1} (`Fn`) returns to unvisited and
@@ -1224,6 +1401,7 @@ void bfs_free(BfsWork* w) {
     dev_free(w->bm[0]); dev_free(w->bm[1]); dev_free(w->bm[2]); dev_free(w->vis); dev_free(w->qbuf);
     dev_free(w->sh); dev_free(w->log); dev_free(w->policy);
     if (w->h_sh) (void)hipHostFree(w->h_sh);
+    if (w->h_done) (void)hipHostFree(w->h_done);
     if (w->ev0) (void)hipEventDestroy(w->ev0);
     if (w->ev1) (void)hipEventDestroy(w->ev1);
     for (auto e : w->step_ev) (void)hipEventDestroy(e);
@@ -1260,6 +1438,9 @@ void bfs_prepare(lgmi_graph* g) {
         w->log = dev_alloc_n<LevelRec>(w->max_log);
         w->policy = dev_alloc_n<PolicyWeights>(1);
         LGMI_HIP(hipHostMalloc((void**)&w->h_sh, sizeof(BfsShared), hipHostMallocDefault));
+        LGMI_HIP(hipHostMalloc((void**)&w->h_done, sizeof(int32_t), hipHostMallocMapped | hipHostMallocCoherent));
+        *w->h_done = 0;
+        LGMI_HIP(hipHostGetDevicePointer((void**)&w->d_done, w->h_done, 0));
         LGMI_HIP(hipEventCreate(&w->ev0));
         LGMI_HIP(hipEventCreate(&w->ev1));
         LGMI_HIP(hipMemsetAsync(w->sh, 0, sizeof(BfsShared), g->stream));
@@ -1288,6 +1469,14 @@ static StepArgs graph_args(lgmi_graph* g, int dir_in, int32_t grid) {
     a.nv = g->nv;
     a.nwords = 2 * ((g->nv + 63) / 64);
     a.grid = grid;
+    const lgmi_tuning t = tuning_get();
+    a.wave_row = t.wave_row;
+    a.lane_rounds = t.lane_rounds;
+    a.range_front = std::min(t.range_front, kRangeFront);
+    a.queue_front = std::min(t.queue_front, kQueueFront);
+    a.range_light = t.range_light;
+    a.queue_min_edges = t.queue_min_edges;
+    a.queue_max_edges = t.queue_max_edges;
     return a;
 }
 
@@ -1316,8 +1505,10 @@ static void launch_level(const StepArgs& a, bool parents, int32_t karg, int32_t 
 void policy_upload(lgmi_graph* g) {
     std::lock_guard<std::mutex> lk(policy_mutex());
     PolicyHost& p = policy_host();
-    if (p.have_mlp)
+    if (p.have_mlp && g->bfs->policy_version != p.version) {
         LGMI_HIP(hipMemcpyAsync(g->bfs->policy, &p.w, sizeof(PolicyWeights), hipMemcpyHostToDevice, g->stream));
+        g->bfs->policy_version = p.version;
+    }
 }
 
 void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_src, int dir_in,
@@ -1325,6 +1516,7 @@ void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_
     bfs_prepare(g);
     BfsWork* w = g->bfs;
     if (nsrc > g->nv) throw Error(LGMI_E_ARG, "source list longer than nv (de-duplicate first)");
+    if (nwalls > 0 && single_src < 0) throw Error(LGMI_E_ARG, "walls need a single source");
     int mode;
     float alpha, beta;
     bool have_mlp;
@@ -1338,27 +1530,33 @@ void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_
 
     StepArgs a = make_args(g, dir_in, want_parents != 0);
     if (mode != LGMI_POLICY_LEARNED) a.policy = nullptr;
+    a.done_host = w->d_done;
     const int64_t nnz_scan = g->nnz;  // the reverse CSR has as many entries as the forward one
     hipStream_t s = g->stream;
+    const int init_grid = std::max(1, std::min<int>(w->grid * 8, (g->nv + 255) / 256));
+    static const bool graphs_on = [] {
+        const char* e = std::getenv("LGMI_BFS_GRAPHS");
+        return !(e && e[0] == '0');
+    }();
+    const bool parents = want_parents != 0;
+    const int flags = (parents ? 1 : 0) | (dir_in ? 2 : 0) | (a.policy ? 4 : 0) | ((forced_dir + 1) << 3);
+    // Launches go out in batches (even-sized, so that a captured 16-level
+    // sequence always starts on an even level); a launch after the last
+    // level exits at once.  The first batch is sized by the last search.
+    int batch = std::max(4, std::min(w->last_launches_used + 1, 64));
+    batch = (batch + 1) & ~1;
+    int launched = 0;
+    LGMI_HIP(hipEventRecord(w->ev0, s));
     if (single_src < 0 && nsrc > 0)
         LGMI_HIP(hipMemcpyAsync(w->srcbuf, h_srcs, sizeof(int32_t) * (size_t)nsrc, hipMemcpyHostToDevice, s));
-    LGMI_HIP(hipEventRecord(w->ev0, s));
-    const int init_grid = std::max(1, std::min<int>(w->grid * 8, (g->nv + 255) / 256));
     bfs_init<<<init_grid, 256, 0, s>>>(a, single_src, nnz_scan, forced_dir, mode, alpha, beta);
     if (single_src < 0)
         bfs_seed<<<1, 256, 0, s>>>(a, w->srcbuf, nsrc, nnz_scan, forced_dir, mode, alpha, beta);
     if (nwalls > 0) {
-        if (single_src < 0) throw Error(LGMI_E_ARG, "walls need a single source");
         LGMI_HIP(hipMemcpyAsync(w->srcbuf, h_walls, sizeof(int32_t) * (size_t)nwalls, hipMemcpyHostToDevice, s));
         bfs_walls<<<std::max(1, std::min(w->grid, (nwalls + 255) / 256)), 256, 0, s>>>(a, w->srcbuf, nwalls);
     }
     LGMI_HIP(hipGetLastError());
-    // Enqueue level launches in batches (even-sized, so that a captured
-    // graph always starts on an even level); a launch after the last level
-    // exits at once.
-    int batch = std::max(4, std::min(w->last_launches_used + 1, 64));
-    batch = (batch + 1) & ~1;
-    int launched = 0;
     auto event_at = [&](size_t i) {
         while (w->step_ev.size() <= i) {
             hipEvent_t e;
@@ -1367,16 +1565,19 @@ void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_
         }
         LGMI_HIP(hipEventRecord(w->step_ev[i], s));
     };
-    static const bool graphs_on = [] {
-        const char* e = std::getenv("LGMI_BFS_GRAPHS");
-        return !(e && e[0] == '0');
-    }();
-    const bool parents = want_parents != 0;
+    int32_t done = 0;
     for (;;) {
+        if (launched > 0) {
+            LGMI_HIP(hipEventRecord(w->ev1, s));
+            LGMI_HIP(hipEventSynchronize(w->ev1));
+            done = __atomic_load_n(w->h_done, __ATOMIC_ACQUIRE);
+            if (done < launched) break;
+            if (launched > g->nv + 8) throw Error(LGMI_E_HIP, "BFS did not terminate");
+            batch = std::min(batch * 2, 1024);
+        }
         if (graphs_on && !w->time_steps && launched >= kGraphLevels && (launched & 1) == 0) {
             // deep search: replay captured level sequences
-            const int key = (parents ? 1 : 0) | (dir_in ? 2 : 0) | (a.policy ? 4 : 0) | ((forced_dir + 1) << 3);
-            if (!w->gexec[key]) {
+            if (!w->gexec[flags]) {
                 hipGraph_t gr = nullptr;
                 LGMI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
                 for (int i = 0; i < kGraphLevels; i++) launch_level(a, parents, -1 - (i & 1), forced_dir, s);
@@ -1385,10 +1586,10 @@ void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_
                 const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(gr);
                 LGMI_HIP(e);
-                w->gexec[key] = ex;
+                w->gexec[flags] = ex;
             }
             const int ng = std::max(1, batch / kGraphLevels);
-            for (int i = 0; i < ng; i++) LGMI_HIP(hipGraphLaunch(w->gexec[key], s));
+            for (int i = 0; i < ng; i++) LGMI_HIP(hipGraphLaunch(w->gexec[flags], s));
             launched += ng * kGraphLevels;
         } else {
             for (int i = 0; i < batch; i++) {
@@ -1400,16 +1601,11 @@ void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_
             launched += batch;
         }
         LGMI_HIP(hipGetLastError());
-        LGMI_HIP(hipMemcpyAsync(w->h_sh, w->sh, sizeof(BfsShared), hipMemcpyDeviceToHost, s));
-        LGMI_HIP(hipEventRecord(w->ev1, s));
-        LGMI_HIP(hipEventSynchronize(w->ev1));
-        if (w->h_sh->done_level < launched) break;
-        if (launched > g->nv + 8) throw Error(LGMI_E_HIP, "BFS did not terminate");
-        batch = std::min(batch * 2, 1024);
     }
-    const int levels = w->h_sh->state[w->h_sh->done_level & 1].level;
+    // launch p handles level p; the launch that finds F_p empty stores p
+    const int levels = done;
     w->last_levels = levels;
-    w->last_launches_used = w->h_sh->done_level;
+    w->last_launches_used = done;
     w->last_dir_in = dir_in;
     w->last_parents = want_parents;
     w->last_launches = launched;

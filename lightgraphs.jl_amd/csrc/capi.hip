@@ -91,6 +91,28 @@ int device_cu_count() {
     return cu;
 }
 
+namespace {
+lgmi_tuning tuning_defaults() {
+    lgmi_tuning t{};
+    t.wave_row = 32;
+    t.lane_rounds = 4;
+    t.range_front = 2048;
+    t.queue_front = 4096;
+    t.range_light = 32768;
+    t.queue_min_edges = 4096;
+    t.queue_max_edges = 262144;
+    t.bc_seg_len = 1024;
+    return t;
+}
+std::mutex g_tune_mu;
+lgmi_tuning g_tune = tuning_defaults();
+}  // namespace
+
+lgmi_tuning tuning_get() {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    return g_tune;
+}
+
 PolicyHost& policy_host() {
     static PolicyHost p;
     return p;
@@ -514,6 +536,26 @@ int lgmi_betweenness_set_launch(int mode) {
     LGMI_API_BEGIN
     if (mode != 0 && mode != 1) throw Error(LGMI_E_ARG, "launch mode must be 0 or 1");
     bc_set_launch(mode);
+    LGMI_API_END
+}
+
+int lgmi_set_tuning(const lgmi_tuning* t) {
+    LGMI_API_BEGIN
+    const lgmi_tuning v = t ? *t : tuning_defaults();
+    if (v.wave_row < 2 || v.wave_row > 257 || v.lane_rounds < 0 || v.lane_rounds > 64 || v.range_front < 0 ||
+        v.range_front > 2048 || v.queue_front < 0 || v.queue_front > 4096 || v.range_light < 0 ||
+        v.queue_min_edges < 0 || v.queue_max_edges < 0 || v.bc_seg_len < 64 || v.bc_seg_len > 65536)
+        throw Error(LGMI_E_ARG, "tuning value out of range");
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune = v;
+    g_tune.pad = 0;
+    LGMI_API_END
+}
+
+int lgmi_get_tuning(lgmi_tuning* t) {
+    LGMI_API_BEGIN
+    if (!t) throw Error(LGMI_E_ARG, "bad arguments");
+    *t = tuning_get();
     LGMI_API_END
 }
 

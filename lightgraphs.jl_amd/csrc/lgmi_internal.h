@@ -67,14 +67,15 @@ struct BfsCtrl {
     int64_t nnz_scan;
     int64_t m_light;        // traversal-direction edges of F_k's rows of at most kHeavyRow entries
     int32_t qovf;           // 1: some work-group's queue of F_k overflowed (or there is none)
-    int32_t variant;        // step k: 1 range-mode top-down, 0 otherwise
+    int32_t variant;        // step k: 1 range-mode top-down, 2 queue-mode top-down, 0 otherwise
     uint64_t t_main;        // wall clock at the start of level k's launch
 };
 
 // Words shared between the launches of one search and the host.
 struct BfsShared {
     int32_t done_level;     // level whose launch found F_k empty (INT32_MAX while running)
-    int32_t pad[3];
+    int32_t src0;           // the source of a single-source search (-1: several)
+    int32_t pad[2];
     BfsCtrl state[2];       // S_k by k & 1
 };
 
@@ -85,7 +86,7 @@ struct LevelRec {
     int64_t m_front;
     int64_t m_unvisited;
     int32_t n_next;
-    int32_t variant;        // 1: range-mode top-down
+    int32_t variant;        // 1: range-mode top-down, 2: queue-mode top-down
     int64_t scanned;        // adjacency entries read (bottom-up) / expanded (top-down)
     int64_t cand;           // bottom-up: unvisited vertices examined
     uint64_t t_start;       // first block start (wall clock ticks)
@@ -122,12 +123,15 @@ struct BfsWork {
     int32_t* qbuf = nullptr;     // [2][grid][kQCap]: each work-group's new vertices, by level parity
     BfsShared* sh = nullptr;
     BfsShared* h_sh = nullptr;   // pinned host mirror
+    int32_t* h_done = nullptr;   // pinned host-mapped copy of done_level, written by the kernels
+    int32_t* d_done = nullptr;   // its device address
     LevelRec* log = nullptr;
     int32_t max_log = 0;
     PolicyWeights* policy = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool time_steps = false;              // HIP events around every step launch
     hipGraphExec_t gexec[32] = {};        // captured level sequences (parents, dir_in, learned, forced)
+    uint64_t policy_version = 0;          // version of the weights on the device
     std::vector<hipEvent_t> step_ev;      // launch i runs between step_ev[i] and step_ev[i+1]
     int last_launches = 0;
     double last_steps_event_ms = 0.0;
@@ -228,4 +232,6 @@ struct PolicyHost {
 };
 PolicyHost& policy_host();
 std::mutex& policy_mutex();
+// process-wide launch thresholds (lgmi_set_tuning), read once per call
+lgmi_tuning tuning_get();
 }  // namespace lgmi

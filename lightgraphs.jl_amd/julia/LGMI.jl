@@ -64,7 +64,7 @@ mutable struct Handle
     ptr::Ptr{Cvoid}
 end
 
-function Handle(g::SimpleG)
+function new_handle(g::SimpleG)
     length(g.fadjlist) > 0 && sum(length, g.fadjlist) >= typemax(Int32) && throw(Fallback())
     rp, col = csr(g.fadjlist)
     out = Ref{Ptr{Cvoid}}(C_NULL)
@@ -84,6 +84,80 @@ function Handle(g::SimpleG)
     return h
 end
 
+# Opt-in handle cache.  By default every call converts fadjlist and uploads
+# the graph (a LightGraphs graph is mutable, add_edge! / rem_edge!,
+# src/SimpleGraphs/simplegraph.jl:445-461).  cache!(g) keeps g's device
+# handle for later calls on g; the entry is dropped when nv(g) or ne(g)
+# changes, and uncache!(g) drops it after a mutation that keeps both (for
+# example rem_edge! followed by add_edge!).
+const HANDLES = WeakKeyDict{Any,Tuple{Int,Int,Handle}}()
+const HANDLES_LOCK = ReentrantLock()
+
+function cache!(g::SimpleG)
+    h = new_handle(g)
+    lock(HANDLES_LOCK) do
+        HANDLES[g] = (nv(g), ne(g), h)
+    end
+    return g
+end
+
+uncache!(g::SimpleG) = lock(() -> (delete!(HANDLES, g); g), HANDLES_LOCK)
+
+function handle(g::SimpleG)
+    c = lock(() -> get(HANDLES, g, nothing), HANDLES_LOCK)
+    if c !== nothing
+        c[1] == nv(g) && c[2] == ne(g) && return c[3]
+        uncache!(g)
+    end
+    return new_handle(g)
+end
+
+# Launch thresholds chosen by the autotuner (lgmi/tuning.json, written by
+# lgmi/tuning.py), applied when the module loads.  Layout: lgmi_tuning of
+# include/lgmi.h.
+struct Tuning
+    wave_row::Int32
+    lane_rounds::Int32
+    range_front::Int32
+    queue_front::Int32
+    range_light::Int64
+    queue_min_edges::Int64
+    queue_max_edges::Int64
+    bc_seg_len::Int32
+    pad::Int32
+end
+
+const TUNING_FILE = joinpath(@__DIR__, "..", "lgmi", "tuning.json")
+
+# The file's "best" object is flat: "name": integer.
+function load_tuning(path::AbstractString = TUNING_FILE)
+    isfile(path) || return false
+    txt = read(path, String)
+    i = findfirst("\"best\"", txt)
+    i === nothing && return false
+    j = findnext('}', txt, last(i))
+    vals = Dict(m.captures[1] => parse(Int64, m.captures[2])
+                for m in eachmatch(r"\"(\w+)\"\s*:\s*(-?\d+)", txt[last(i):j]))
+    cur = Ref{Tuning}()
+    check(ccall((:lgmi_get_tuning, liblgmi), Cint, (Ptr{Tuning},), cur))
+    t = cur[]
+    f(name, dflt, T) = T(get(vals, name, dflt))
+    new = Tuning(f("wave_row", t.wave_row, Int32), f("lane_rounds", t.lane_rounds, Int32),
+                 f("range_front", t.range_front, Int32), f("queue_front", t.queue_front, Int32),
+                 f("range_light", t.range_light, Int64), f("queue_min_edges", t.queue_min_edges, Int64),
+                 f("queue_max_edges", t.queue_max_edges, Int64), f("bc_seg_len", t.bc_seg_len, Int32), Int32(0))
+    check(ccall((:lgmi_set_tuning, liblgmi), Cint, (Ref{Tuning},), new))
+    return true
+end
+
+function __init__()
+    try
+        load_tuning()
+    catch err
+        @warn "LGMI: tuning.json not applied" exception = err
+    end
+end
+
 # ---------------------------------------------------------------------
 # gdistances! / gdistances  (src/traversals/bfs.jl:95-141)
 # ---------------------------------------------------------------------
@@ -92,7 +166,7 @@ function device_dist(g::SimpleG, source)
     for s in source
         1 <= s <= nv(g) || throw(BoundsError(vertices(g), s))
     end
-    h = Handle(g)
+    h = handle(g)
     dist = Vector{Int32}(undef, nv(g))
     check(ccall((:lgmi_gdistances, liblgmi), Cint, (Ptr{Cvoid}, Ptr{Int32}, Int32, Ptr{Int32}),
                 h.ptr, srcs, length(srcs), dist))
@@ -122,7 +196,7 @@ function bfs_parents(g::SimpleG{T}, s::Integer; dir = :out) where T
     1 <= s <= nv(g) || throw(BoundsError(vertices(g), s))
     par = Vector{Int32}(undef, nv(g))
     try
-        h = Handle(g)
+        h = handle(g)
         check(ccall((:lgmi_bfs_parents, liblgmi), Cint, (Ptr{Cvoid}, Int32, Cint, Ptr{Int32}),
                     h.ptr, s - 1, dir == :out ? 0 : 1, par))
     catch e
@@ -140,7 +214,7 @@ function device_bc(g::SimpleG, srcs::AbstractVector, k::Integer, normalize::Bool
     for s in srcs
         1 <= s <= nv(g) || throw(BoundsError(vertices(g), s))
     end
-    h = Handle(g)
+    h = handle(g)
     bc = Vector{Float64}(undef, nv(g))
     s0 = Int32[s - 1 for s in srcs]
     check(ccall((:lgmi_betweenness, liblgmi), Cint,
@@ -171,7 +245,7 @@ function device_stress(g::SimpleG, srcs::AbstractVector)
     for s in srcs
         1 <= s <= nv(g) || throw(BoundsError(vertices(g), s))
     end
-    h = Handle(g)
+    h = handle(g)
     out = Vector{Int64}(undef, nv(g))
     s0 = Int32[s - 1 for s in srcs]
     check(ccall((:lgmi_stress, liblgmi), Cint, (Ptr{Cvoid}, Ptr{Int32}, Int32, Ptr{Int64}),
@@ -200,7 +274,7 @@ function device_dstats(g::SimpleG, srcs::AbstractVector)
     for s in srcs
         1 <= s <= nv(g) || throw(BoundsError(vertices(g), s))
     end
-    h = Handle(g)
+    h = handle(g)
     st = Matrix{Int64}(undef, 3, length(srcs))
     s0 = Int32[s - 1 for s in srcs]
     isempty(s0) || check(ccall((:lgmi_distance_stats, liblgmi), Cint, (Ptr{Cvoid}, Ptr{Int32}, Int32, Ptr{Int64}),
@@ -356,7 +430,7 @@ end
 function connected_components!(label::AbstractVector, g::SimpleG{T}) where T
     lab = Vector{Int32}(undef, nv(g))
     nv(g) > 0 && check(ccall((:lgmi_connected_components, liblgmi), Cint, (Ptr{Cvoid}, Ptr{Int32}),
-                             Handle(g).ptr, lab))
+                             handle(g).ptr, lab))
     for v in eachindex(lab)
         label[v] = lab[v] + 1          # the smallest vertex id of v's component
     end
@@ -379,7 +453,7 @@ function has_path(g::SimpleG{T}, u::Integer, v::Integer; exclude_vertices::Abstr
     ex = Int32[x - 1 for x in exclude_vertices]
     res = Ref{Cint}(0)
     check(ccall((:lgmi_has_path, liblgmi), Cint, (Ptr{Cvoid}, Int32, Int32, Ptr{Int32}, Int32, Ptr{Cint}),
-                Handle(g).ptr, u - 1, v - 1, ex, length(ex), res))
+                handle(g).ptr, u - 1, v - 1, ex, length(ex), res))
     return res[] != 0
 end
 
@@ -392,7 +466,7 @@ function neighborhood_dists(g::SimpleG{T}, v::Integer, d, distmx::DefaultDistanc
     cnt = Ref{Int32}(0)
     check(ccall((:lgmi_neighborhood, liblgmi), Cint,
                 (Ptr{Cvoid}, Int32, Int64, Cint, Ptr{Int32}, Ptr{Int32}, Ptr{Int32}),
-                Handle(g).ptr, v - 1, dd, dir == :out ? 0 : 1, verts, dists, cnt))
+                handle(g).ptr, v - 1, dd, dir == :out ? 0 : 1, verts, dists, cnt))
     return [(T(verts[i] + 1), Int(dists[i])) for i in 1:cnt[]]     # FIFO queue order, as _neighborhood
 end
 neighborhood(g::SimpleG, v::Integer, d, distmx::DefaultDistance = weights(g); dir = :out) =
@@ -400,7 +474,7 @@ neighborhood(g::SimpleG, v::Integer, d, distmx::DefaultDistance = weights(g); di
 
 function period(g::SimpleDiGraph)
     p = Ref{Int64}(0)
-    check(ccall((:lgmi_period, liblgmi), Cint, (Ptr{Cvoid}, Ptr{Int64}), Handle(g).ptr, p))   # ArgumentError if not strongly connected
+    check(ccall((:lgmi_period, liblgmi), Cint, (Ptr{Cvoid}, Ptr{Int64}), handle(g).ptr, p))   # ArgumentError if not strongly connected
     return p[]
 end
 

@@ -40,6 +40,7 @@ EXPORTED = [
     "lgmi_betweenness_set_launch", "lgmi_stress", "lgmi_distance_stats",
     "lgmi_connected_components", "lgmi_has_path", "lgmi_period", "lgmi_neighborhood",
     "lgmi_graph_from_edges", "lgmi_graph_size", "lgmi_graph_csr",
+    "lgmi_set_tuning", "lgmi_get_tuning",
     "lgmi_last_error", "lgmi_device_count", "lgmi_synchronize",
     "lgmi_gen_kronecker", "lgmi_gen_erdos_renyi", "lgmi_gen_barabasi_albert",
     "lgmi_gen_watts_strogatz", "lgmi_gen_grid2d", "lgmi_csr_build", "lgmi_csr_transpose",
@@ -81,6 +82,12 @@ class BcStats(ctypes.Structure):
     _fields_ = [("groups", ctypes.c_int32), ("max_levels", ctypes.c_int32), ("launch", ctypes.c_int32),
                 ("pad", ctypes.c_int32), ("device_ms", ctypes.c_double),
                 ("forward_ms", ctypes.c_double), ("backward_ms", ctypes.c_double)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("wave_row", ctypes.c_int32), ("lane_rounds", ctypes.c_int32), ("range_front", ctypes.c_int32),
+                ("queue_front", ctypes.c_int32), ("range_light", ctypes.c_int64), ("queue_min_edges", ctypes.c_int64),
+                ("queue_max_edges", ctypes.c_int64), ("bc_seg_len", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 _i32p = ctypes.POINTER(ctypes.c_int32)
@@ -135,6 +142,8 @@ def load():
         "lgmi_graph_size": ([_vp, _i32p, _i64p], c.c_int),
         "lgmi_graph_csr": ([_vp, _i32p, _i32p, _i32p, _i32p], c.c_int),
         "lgmi_neighborhood": ([_vp, c.c_int32, c.c_int64, c.c_int, _i32p, _i32p, _i32p], c.c_int),
+        "lgmi_set_tuning": ([c.POINTER(Tuning)], c.c_int),
+        "lgmi_get_tuning": ([c.POINTER(Tuning)], c.c_int),
         "lgmi_last_error": ([], c.c_char_p),
         "lgmi_device_count": ([], c.c_int),
         "lgmi_synchronize": ([], c.c_int),

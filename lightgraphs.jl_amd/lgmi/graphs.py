@@ -104,6 +104,8 @@ class _GraphBase:
         dst = np.asarray(dst, dtype=np.int64)
         if one_based:
             src, dst = src - 1, dst - 1
+        from . import tuning
+        tuning.apply_once()
         lib = _lib.load()
         h = _lib.ctypes.c_void_p()
         ne = np.zeros(1, dtype=np.int64)
@@ -228,6 +230,8 @@ class _GraphBase:
         """Device CSR for liblgmi (created once, dropped on mutation)."""
         self._flush()
         if self._handle is None:
+            from . import tuning
+            tuning.apply_once()
             lib = _lib.load()
             h = _lib.ctypes.c_void_p()
             rp, col = self._row_ptr, self._col

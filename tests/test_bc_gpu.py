@@ -223,6 +223,36 @@ def test_heavy_rows_directed_vs_oracle(oracle):
         C.set_launch(0)
 
 
+@pytest.mark.parametrize("seg_len", [64, 4096])
+def test_segment_length_never_changes_result(oracle, seg_len):
+    """The heavy-row segment length (an autotuned threshold, fixed for a
+    graph when its segments are built) changes only time: short segments
+    (64 entries, so that most rows of the graph are heavy) and long ones
+    give the oracle's betweenness and stress, both launch paths."""
+    import lgmi
+    from lgmi import centrality as C
+    from lgmi import tuning
+    g0 = _directed_hubs()
+    rp, col = g0.csr()
+    irp, icol = g0.in_csr()
+    vs0 = np.concatenate([[0, 1, 2, 4999], np.random.default_rng(17).integers(0, 5000, 100)])
+    try:
+        tuning.set_values({"bc_seg_len": seg_len})
+        g = lgmi.SimpleDiGraph.from_csr(rp, col, in_row_ptr=irp, in_col=icol)
+        for mode in (1, 0):
+            C.set_launch(mode)
+            say(f"segment length {seg_len}, launch mode {mode}")
+            got = lgmi.betweenness_centrality(g, vs0 + 1, normalize=False)
+            want = oracle.betweenness(rp, col, irp, True, vs0, normalize=False)
+            assert close(got, want)
+        st = lgmi.stress_centrality(g, vs0 + 1)
+        assert (st == oracle.stress(rp, col, irp, True, vs0)).all()
+    finally:
+        C.set_launch(0)
+        tuning.set_values(None)
+        tuning.load()
+
+
 def test_persistent_launch_matches_level_launches(oracle):
     """One persistent launch (graph-captured or cooperative) against one
     launch per level: bit-identical results, on undirected and directed

@@ -237,35 +237,104 @@ def test_rmat20_full_compare(oracle):
     _check_graph(oracle, g, roots, "rmat20")
 
 
-def test_range_mode_levels_exact(oracle):
-    """Top-down levels over a frontier of hub rows run in range mode (each
-    work-group owns a vertex range and forms parents in LDS); distances and
-    parents stay bit-exact, with and without parents, along out- and
-    in-edges."""
+def _variant_counts(g, roots):
     import ctypes
 
-    import lgmi.generators as gen
     from lgmi import _lib
     lib = _lib.load()
+    buf = (_lib.BfsStep * 20000)()
+    st = _lib.BfsStats()
+    counts = {0: 0, 1: 0, 2: 0}
+    for r in roots:
+        _lib.check(lib.lgmi_bfs_run(g.handle(), r, 0, 1, ctypes.byref(st)))
+        n = lib.lgmi_bfs_steps(g.handle(), 20000, buf)
+        for i in range(n):
+            if buf[i].dir == 0:
+                counts[int(buf[i].variant)] += 1
+    return counts
+
+
+def _hub_neighbour_roots(g, k=3):
+    rp, _ = g.csr()
+    irp, icol = g.in_csr()
+    deg = np.diff(rp)
+    hub = int(np.argmax(deg))
+    # low-degree vertices with an edge to the hub: the hub is in F_1
+    nb = icol[irp[hub]:irp[hub + 1]]
+    nb = nb[deg[nb] > 0]
+    return [int(x) for x in nb[np.argsort(deg[nb], kind="stable")[:k]]]
+
+
+def test_range_mode_levels_exact(oracle):
+    """Top-down levels over a frontier of hub rows run in range mode (each
+    work-group owns a vertex range and forms parents in LDS) when queue mode
+    is off; distances and parents stay bit-exact, with and without parents,
+    along out- and in-edges."""
+    import lgmi.generators as gen
+    from lgmi import tuning
     seen = 0
-    for g, tag in ((gen.rmat(20, 16, seed=2), "rmat20"), (gen.rmat(17, 16, seed=3, directed=True), "rmat17-dir")):
-        rp, col = g.csr()
-        irp, icol = g.in_csr()
-        deg = np.diff(rp)
-        hub = int(np.argmax(deg))
-        # low-degree vertices with an edge to the hub: the hub is in F_1
-        nb = icol[irp[hub]:irp[hub + 1]]
-        nb = nb[deg[nb] > 0]
-        roots = [int(x) for x in nb[np.argsort(deg[nb], kind="stable")[:3]]]
-        _check_graph(oracle, g, roots, tag)
-        buf = (_lib.BfsStep * 256)()
-        st = _lib.BfsStats()
-        for r in roots:
-            _lib.check(lib.lgmi_bfs_run(g.handle(), r, 0, 1, ctypes.byref(st)))
-            n = lib.lgmi_bfs_steps(g.handle(), 256, buf)
-            seen += sum(int(buf[i].variant) for i in range(n))
+    try:
+        tuning.set_values({"queue_front": 0})
+        for g, tag in ((gen.rmat(20, 16, seed=2), "rmat20"), (gen.rmat(17, 16, seed=3, directed=True), "rmat17-dir")):
+            roots = _hub_neighbour_roots(g)
+            _check_graph(oracle, g, roots, tag)
+            seen += _variant_counts(g, roots)[1]
+    finally:
+        tuning.set_values(None)
+        tuning.load()
     say(f"range-mode levels: {seen}")
     assert seen > 0, "no level ran in range mode"
+
+
+def test_queue_mode_levels_exact(oracle):
+    """With queue mode taking every top-down level it may (frontier lists
+    up to 4096 vertices at any edge count), the levels that read the
+    frontier from the work-group queues -- the single source of level 0, hub
+    levels, the small levels of a grid and of a chain -- give bit-exact
+    distances and parents, along out- and in-edges."""
+    import lgmi.generators as gen
+    from lgmi import tuning
+    seen = 0
+    try:
+        tuning.set_values({"queue_front": 4096, "queue_min_edges": 0, "queue_max_edges": 1 << 40,
+                           "range_front": 0})
+        cases = [(gen.rmat(18, 16, seed=6), "rmat18"), (gen.rmat(15, 16, seed=7, directed=True), "rmat15-dir"),
+                 (gen.grid([300, 200]), "grid300x200"), (gen.path_graph(3000), "path3000"),
+                 (gen.barabasi_albert(50_000, 3, seed=8), "ba50k")]
+        for g, tag in cases:
+            rp, _ = g.csr()
+            deg = np.diff(rp)
+            roots = _hub_neighbour_roots(g, 2) + [0, int(np.nonzero(deg > 0)[0][-1])]
+            _check_graph(oracle, g, roots, tag)
+            seen += _variant_counts(g, roots)[2]
+    finally:
+        tuning.set_values(None)
+        tuning.load()
+    say(f"queue-mode levels: {seen}")
+    assert seen > 0, "no level ran in queue mode"
+
+
+@pytest.mark.parametrize("values", [
+    {"wave_row": 2, "lane_rounds": 0},
+    {"wave_row": 257, "lane_rounds": 64},
+    {"queue_front": 0, "range_front": 0},
+    {"queue_front": 256, "queue_min_edges": 0, "queue_max_edges": 2048, "range_light": 1 << 40},
+])
+def test_tuning_never_changes_output(oracle, values):
+    """Every launch threshold setting gives the same distances and parents
+    (the autotuner only trades time)."""
+    import lgmi.generators as gen
+    from lgmi import tuning
+    try:
+        tuning.set_values(values)
+        for g, tag in ((gen.rmat(17, 16, seed=9), "rmat17"), (gen.rmat(14, 8, seed=10, directed=True), "rmat14-dir"),
+                       (gen.grid([64, 64]), "grid64")):
+            rp, _ = g.csr()
+            roots = _hub_neighbour_roots(g, 1) + [int(np.argmax(np.diff(rp)))]
+            _check_graph(oracle, g, roots, f"{tag} {values}")
+    finally:
+        tuning.set_values(None)
+        tuning.load()
 
 
 def test_level_counters_match_oracle(oracle):
