@@ -96,7 +96,10 @@ typedef struct {
  * (outneighbors/inneighbors, src/SimpleGraphs/SimpleGraphs.jl:78-79,92-93;
  * badj, simplegraph.jl:376-377, simpledigraph.jl:381-382) with a device CSR.
  * in_row_ptr/in_col are NULL for an undirected graph (the in-CSR is then the
- * out-CSR); for a directed graph they are the badjlist as a CSR.
+ * out-CSR); for a directed graph they are the badjlist as a CSR.  The host
+ * checks bounds, sorted rows and duplicates; the device then checks that an
+ * undirected CSR is symmetric and that the in-CSR is the transpose of the
+ * out-CSR (LGMI_E_ARG otherwise), an O(nnz log d) pass.
  * ---------------------------------------------------------------------- */
 int lgmi_graph_create(int32_t nv, int64_t nnz, const int32_t* row_ptr, const int32_t* col,
                       int64_t in_nnz, const int32_t* in_row_ptr, const int32_t* in_col,

@@ -50,6 +50,43 @@ __global__ void range_table_kernel(const int32_t* rp, const int32_t* col, const 
     tab[i] = lo;
 }
 
+// Counts the entries (v, u) of a CSR whose mirror is missing: v must be in
+// row u of `orp`/`ocol` (the in-CSR of a directed graph).  A wavefront per
+// row, one entry per lane, a binary search of v in the sorted row u.  For an
+// undirected graph (upper = 1, orp/ocol the CSR itself) only the entries
+// with u > v are searched, and the entries above and below the diagonal are
+// counted (cnt[1], cnt[2]): all upper mirrors present and equal counts make
+// the lower entries exactly those mirrors.  cnt[0] gets the misses.
+__global__ void mirror_check_kernel(const int32_t* rp, const int32_t* col, int32_t nv, const int32_t* orp,
+                                    const int32_t* ocol, int upper, unsigned long long* cnt) {
+    const int lane = threadIdx.x & 63;
+    const int64_t wave = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    unsigned long long miss = 0, above = 0, below = 0;
+    for (int64_t v = wave; v < nv; v += nwaves) {
+        const int32_t b0 = rp[v], e0 = rp[v + 1];
+        for (int32_t k = b0 + lane; k < e0; k += 64) {
+            const int32_t u = col[k];
+            if (upper) {
+                if (u < (int32_t)v) { below++; continue; }
+                if (u == (int32_t)v) continue;
+                above++;
+            }
+            int32_t lo = orp[u], hi = orp[u + 1];
+            const int32_t end = hi;
+            while (lo < hi) {
+                const int32_t m = lo + (hi - lo) / 2;
+                if (ocol[m] < (int32_t)v) lo = m + 1;
+                else hi = m;
+            }
+            if (lo >= end || ocol[lo] != (int32_t)v) miss++;
+        }
+    }
+    if (miss) atomicAdd(&cnt[0], miss);
+    if (above) atomicAdd(&cnt[1], above);
+    if (below) atomicAdd(&cnt[2], below);
+}
+
 int g_device_count = -1;
 std::mutex g_dev_mu;
 }  // namespace
@@ -298,6 +335,33 @@ int lgmi_graph_create(int32_t nv, int64_t nnz, const int32_t* row_ptr, const int
         upload_csr(g, g->out, row_ptr, col, nnz);
         if (directed) upload_csr(g, g->in, in_row_ptr, in_col, in_nnz);
         else g->in = g->out;
+        // an undirected CSR must be symmetric and a directed graph's in-CSR
+        // the transpose of its out-CSR (the bottom-up BFS steps, the range
+        // table and betweenness read the reverse adjacency)
+        if (nnz > 0) {
+            unsigned long long* d_cnt = dev_alloc_n<unsigned long long>(3);
+            unsigned long long cnt[3] = {0, 0, 0};
+            try {
+                LGMI_HIP(hipMemsetAsync(d_cnt, 0, sizeof(cnt), g->stream));
+                const int blocks = (int)std::max<int64_t>(1, std::min<int64_t>(((int64_t)nv + 3) / 4,
+                                                                                (int64_t)g->cu_count * 32));
+                mirror_check_kernel<<<blocks, 256, 0, g->stream>>>(g->out.row_ptr, g->out.col, nv, g->in.row_ptr,
+                                                                   g->in.col, directed ? 0 : 1, d_cnt);
+                LGMI_HIP(hipGetLastError());
+                LGMI_HIP(hipMemcpyAsync(cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, g->stream));
+                LGMI_HIP(hipStreamSynchronize(g->stream));
+            } catch (...) {
+                dev_free(d_cnt);
+                throw;
+            }
+            dev_free(d_cnt);
+            const unsigned long long bad = cnt[0] + (cnt[1] > cnt[2] ? cnt[1] - cnt[2] : cnt[2] - cnt[1]);
+            if (bad)
+                throw Error(LGMI_E_ARG, directed ? "the in-CSR is not the transpose of the out-CSR ("
+                                                       + std::to_string(bad) + " entries unmatched)"
+                                                 : "an undirected graph's CSR must be symmetric ("
+                                                       + std::to_string(bad) + " entries unmatched)");
+        }
         LGMI_HIP(hipStreamSynchronize(g->stream));
     } catch (...) {
         graph_free(g);

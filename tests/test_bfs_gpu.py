@@ -337,10 +337,35 @@ def test_tuning_never_changes_output(oracle, values):
         tuning.load()
 
 
+def _expected_features(d, deg_td, deg_bu, nnz, L, prev_dir):
+    """The 16 policy features of level L's state, from the oracle's
+    distances: log2(1 + x) of the frontier size, frontier edges, unvisited
+    edges, unvisited vertices and the level; the frontier's share of the
+    vertices; edges log-ratio; growth against the previous frontier; the
+    previous direction; the frontier's degree histogram (bins 0, 1-3, 4-15,
+    16-63, 64-255, 256-1023, >= 1024) as fractions."""
+    nv = len(d)
+    f = d == L
+    seen = (d >= 0) & (d <= L)
+    nf, mf = int(f.sum()), int(deg_td[f].sum())
+    mu = int(nnz - deg_bu[seen].sum())
+    nu = int(nv - seen.sum())
+    n_prev = int((d == L - 1).sum()) if L > 0 else 0
+    lg = lambda x: np.log2(1.0 + max(float(x), 0.0))
+    out = [lg(nf), lg(mf), lg(mu), lg(nu), lg(L), nf / nv, lg(mf) - lg(mu), lg(nf) - lg(n_prev), float(prev_dir)]
+    dg = deg_td[f]
+    bins = np.where(dg <= 0, 0, np.minimum((np.floor(np.log2(np.maximum(dg, 1))).astype(int)) // 2 + 1, 6))
+    hist = np.bincount(bins, minlength=7)[:7].astype(np.float64)
+    out += list(hist / hist.sum()) if hist.sum() else [0.0] * 7
+    return np.array(out)
+
+
 def test_level_counters_match_oracle(oracle):
     """The per-level records the kernels reduce on the device (frontier size,
     frontier edges, new vertices) equal the counts from the oracle's
-    distances, level by level, for both directions and for range mode."""
+    distances, level by level, for both directions and for range mode; the
+    16 policy features derived from them on the device (FP32, hardware
+    log2) match the same features computed from the oracle's distances."""
     import ctypes
 
     import lgmi.generators as gen
@@ -349,6 +374,7 @@ def test_level_counters_match_oracle(oracle):
     for g, tag in ((gen.rmat(18, 16, seed=4), "rmat18"), (gen.rmat(15, 16, seed=5, directed=True), "rmat15-dir")):
         rp, col = g.csr()
         deg = np.diff(rp)
+        deg_in = np.diff(g.in_csr()[0])
         hub = int(np.argmax(deg))
         roots = [hub, int(np.nonzero(deg > 0)[0][0])]
         buf = (_lib.BfsStep * 512)()
@@ -366,4 +392,7 @@ def test_level_counters_match_oracle(oracle):
                     assert buf[L].n_front == int(f.sum()), (tag, r, L)
                     assert buf[L].m_front == int(deg[f].sum()), (tag, r, L)
                     assert buf[L].n_next == int((d == L + 1).sum()), (tag, r, L)
+                    want = _expected_features(d, deg, deg_in, len(col), L, buf[L - 1].dir if L > 0 else 0)
+                    got = np.array(buf[L].feat[:16], dtype=np.float64)
+                    assert np.allclose(got, want, rtol=1e-5, atol=1e-5), (tag, r, L, got, want)
         _lib.check(lib.lgmi_bfs_force_direction(g.handle(), -1))

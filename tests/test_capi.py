@@ -175,3 +175,29 @@ def test_tuning_set_get_and_range_checks():
     finally:
         _lib.check(lib.lgmi_set_tuning(None))
     assert tuning.get() == d
+
+
+@pytest.mark.gpu
+def test_create_rejects_unmirrored_csr():
+    """An undirected CSR that is not symmetric, or an in-CSR that is not
+    the transpose of the out-CSR, is rejected on the device (the bottom-up
+    steps and betweenness read the reverse adjacency)."""
+    from lgmi import _lib
+    lib = _lib.load()
+    h = _lib.ctypes.c_void_p()
+    rp = np.array([0, 1, 1, 2], dtype=np.int32)          # 0 -> 1 but not 1 -> 0; 2 -> 0
+    col = np.array([1, 0], dtype=np.int32)
+    rc = lib.lgmi_graph_create(3, 2, _lib.i32p(rp), _lib.i32p(col), 0, None, None, 0, _lib.ctypes.byref(h))
+    assert rc == _lib.LGMI_E_ARG and "symmetric" in _lib.last_error()
+    # directed 0 -> 1, 2 -> 0 with an in-CSR of 0 -> 1, 0 -> 2 (reversed arc)
+    irp = np.array([0, 0, 1, 2], dtype=np.int32)
+    icol = np.array([0, 2], dtype=np.int32)
+    rc = lib.lgmi_graph_create(3, 2, _lib.i32p(rp), _lib.i32p(col), 2, _lib.i32p(irp), _lib.i32p(icol), 1,
+                               _lib.ctypes.byref(h))
+    assert rc == _lib.LGMI_E_ARG and "transpose" in _lib.last_error()
+    # the right in-CSR is accepted: in(0) = [2], in(1) = [0]
+    irp = np.array([0, 1, 2, 2], dtype=np.int32)
+    icol = np.array([2, 0], dtype=np.int32)
+    _lib.check(lib.lgmi_graph_create(3, 2, _lib.i32p(rp), _lib.i32p(col), 2, _lib.i32p(irp), _lib.i32p(icol), 1,
+                                     _lib.ctypes.byref(h)))
+    lib.lgmi_graph_destroy(h)
