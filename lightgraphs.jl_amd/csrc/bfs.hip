@@ -66,7 +66,7 @@ constexpr int kLaneBits = 8;                    // frontier bits per lane in a t
 constexpr int kTileVerts = 64 * kLaneBits;      // vertices per wavefront tile
 constexpr int kTileWords = kTileVerts / 32;     // bitmap words per tile
 constexpr int kUnroll = 2;          // top-down: adjacency entries per lane per round
-constexpr int kGraphLevels = 16;    // level launches per captured graph (even: parity is static)
+constexpr int kGraphLevels = 18;    // level launches per captured graph (a multiple of 6: level mod 6 is static)
 constexpr int kPartRows = kBlock / kPart;
 constexpr int32_t kRangeFront = 2048;    // range mode: frontier vertices at most (LDS layout)
 constexpr int kEmit = 8;                 // range mode: 1024-vertex slices emitted per round
@@ -371,7 +371,7 @@ __device__ inline void expand_row(const StepArgs& a, int32_t u, int32_t b, int32
 template <bool PARENTS>
 __device__ __attribute__((always_inline)) inline void td_step(const StepArgs& a, const BfsCtrl& c, int32_t L, const uint32_t* FL, uint32_t* Fn,
                         uint32_t* Fc, Counters& k, uint32_t* s_qn, int32_t* qout, int32_t* s_list, int gw, int nw,
-                        int lane) {
+                        int lane, uint32_t fl_first) {
     const int32_t ntiles = (a.nwords + kTileWords - 1) / kTileWords;
     const int sh8 = (lane & 3) * kLaneBits;
     // Heavy rows: 256-entry items; lane j of wavefront gw tests item
@@ -399,13 +399,9 @@ __device__ __attribute__((always_inline)) inline void td_step(const StepArgs& a,
         }
     }
     // frontier tiles; the next tile's frontier word is loaded before the
-    // current tile is expanded
+    // current tile is expanded (the first tile's came with the prologue)
     int32_t t = gw;
-    uint32_t fl_next = 0;
-    {
-        const int32_t w = t * kTileWords + (lane >> 2);
-        if (t < ntiles && w < a.nwords) fl_next = FL[w];
-    }
+    uint32_t fl_next = t < ntiles ? fl_first : 0u;
     while (t < ntiles) {
         const int32_t word0 = t * kTileWords;
         const int32_t w = word0 + (lane >> 2);
@@ -618,7 +614,7 @@ __device__ __attribute__((always_inline)) inline Counters bu_tile(const StepArgs
 template <bool PARENTS>
 __device__ __attribute__((always_inline)) inline void bu_step(const StepArgs& a, int32_t L, const uint32_t* FL, uint32_t* Fn, uint32_t* Fc,
                         Counters& k, uint32_t* s_qn, int32_t* qout, uint32_t* s_fn, Pending* s_q, int32_t* s_list,
-                        int gw, int nw, int lane) {
+                        int gw, int nw, int lane, uint32_t fl_first, uint32_t vis_first, uint32_t nz_first) {
     const int32_t ntiles = (a.nwords + kTileWords - 1) / kTileWords;
     const int sh8 = (lane & 3) * kLaneBits;
     // the next tile's words are loaded before the current tile is scanned
@@ -631,7 +627,12 @@ __device__ __attribute__((always_inline)) inline void bu_step(const StepArgs& a,
         vs_n = ok ? a.vis[wn] : 0u;
         nz_n = ok ? a.bu_nz[wn] : 0u;
     };
-    load_tile(t);
+    // the first tile's words came with the prologue
+    if (t < ntiles) {
+        fl_n = fl_first;
+        vs_n = vis_first;
+        nz_n = nz_first;
+    }
     while (t < ntiles) {
         const int32_t word0 = t * kTileWords;
         const int32_t w = word0 + (lane >> 2);
@@ -847,10 +848,13 @@ __device__ __attribute__((always_inline)) inline void range_step(const StepArgs&
 // are complete (or the single source of level 0).  Every work-group reads
 // the whole frontier list, lays the frontier rows' entries out with a prefix
 // sum in LDS and expands an equal share of them, work-group b the entries
-// [b * T / G, (b + 1) * T / G), four consecutive entries per lane and
-// round; the visit is the sweep's (plain-load filter, atomicMin claim).  No
-// bitmap is swept: the frontier's vertices are folded into the visited
-// bitmap one by one, and F_{k+2} is cleared with plain stores only.
+// [b * T / G, (b + 1) * T / G), one entry per lane and round; the visit is
+// the sweep's (plain-load filter, atomicMin claim).  No bitmap is swept: the
+// frontier's vertices are folded into the visited bitmap one by one, and
+// F_{k+2} is cleared with plain stores only.  One entry per lane keeps the
+// inlined code small: with four, the level kernel was 2.5 KB larger and
+// every level, of any variant, slower (a 4096 x 4096 grid 238 against
+// 218 ms for two searches, RMAT-22 10.08 against 9.92 ms for 64).
 template <bool PARENTS>
 __device__ __attribute__((always_inline)) inline void queue_step(const StepArgs& a, int32_t L, const uint32_t* FL,
                                                                  uint32_t* Fn, uint32_t* Fc, Counters& k,
@@ -936,10 +940,9 @@ __device__ __attribute__((always_inline)) inline void queue_step(const StepArgs&
     const int64_t T = s_sx[nf];
     const int32_t e0 = (int32_t)(T * (int64_t)blockIdx.x / a.grid);
     const int32_t e1 = (int32_t)(T * (int64_t)(blockIdx.x + 1) / a.grid);
-    for (int32_t base = e0 + wid * 256; base < e1; base += kWaves * 256) {
-        const int32_t e = base + lane * 4;
-        int32_t uu[4], vv[4];
-        int seg = 0;
+    for (int32_t base = e0 + wid * 64; base < e1; base += kWaves * 64) {
+        const int32_t e = base + lane;
+        int32_t uu[1] = {0}, vv[1] = {-1};
         if (e < e1) {
             int lo = 0, hi = nf;                          // largest segment with s_sx[seg] <= e
             while (hi - lo > 1) {
@@ -947,23 +950,11 @@ __device__ __attribute__((always_inline)) inline void queue_step(const StepArgs&
                 if (s_sx[mid] <= e) lo = mid;
                 else hi = mid;
             }
-            seg = lo;
+            uu[0] = s_su[lo];
+            vv[0] = a.td_col[s_sb[lo] + (e - s_sx[lo])];
+            k.scan += 1ull;
         }
-        int32_t nvalid = 0;
-        #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const int32_t eq = e + q;
-            uu[q] = 0;
-            vv[q] = -1;
-            if (eq < e1) {
-                while (s_sx[seg + 1] <= eq) seg++;
-                uu[q] = s_su[seg];
-                vv[q] = a.td_col[s_sb[seg] + (eq - s_sx[seg])];
-                nvalid++;
-            }
-        }
-        k.scan += (unsigned long long)nvalid;
-        visit_n<PARENTS, 4>(a, uu, vv, L, FL, Fn, k, s_qn, qout, lane);
+        visit_n<PARENTS, 1>(a, uu, vv, L, FL, Fn, k, s_qn, qout, lane);
     }
 }
 
@@ -1014,9 +1005,9 @@ __device__ inline int mlp_direction(const float* xs, const PolicyRegs& pr) {
 }
 
 // One level of the search.  karg >= 0: the level p itself; karg < 0
-// (captured graphs): p is unknown to the host but has parity -1 - karg, and
-// p is read from the state the launch before stored.  forced >= 0 forces
-// the direction.
+// (captured graphs): p is unknown to the host but p mod 6 = -1 - karg (so
+// its parity and the bitmap rotation are static), and p is read from the
+// state the launch before stored.  forced >= 0 forces the direction.
 // Every device function the level kernel calls is inlined (always_inline
 // on the large ones): a call that stays a call makes the kernel pass its
 // arguments through per-lane scratch memory.
@@ -1046,9 +1037,25 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
     // compiler keep it, and the tile loops that depend on it, scalar
     const int tid = threadIdx.x, lane = tid & 63, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
     const uint64_t t_now = wall_clock64();
-    const int lpar = karg >= 0 ? (karg & 1) : (-1 - karg);   // parity of p
+    const int pmod6 = karg >= 0 ? (karg % 6) : (-1 - karg);   // p mod 6
+    const int lpar = pmod6 & 1;                                 // parity of p
     const bool first = karg == 0;
     BfsShared* sh = a.sh;
+    const int gw = blockIdx.x * kWaves + wid, nw = gridDim.x * kWaves;
+
+    // The words of this wavefront's first tile of F_p, the visited bitmap and
+    // the non-empty-row bitmap: the sweep steps read them first whatever the
+    // direction, so their loads go out with the prologue's.
+    uint32_t pf_fl = 0, pf_vis = 0, pf_nz = 0;
+    {
+        const uint32_t* FLp = bm_at(a, pmod6 % 3);
+        const int32_t w = gw * kTileWords + (lane >> 2);
+        if (w < a.nwords) {
+            pf_fl = FLp[w];
+            pf_vis = a.vis[w];
+            pf_nz = a.bu_nz[w];
+        }
+    }
 
     // ---- prologue: the loads below are independent of each other ----
     unsigned long long acc = 0;
@@ -1224,15 +1231,14 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
     const uint32_t* FL = bm_at(a, L % 3);
     uint32_t* Fn = bm_at(a, (L + 1) % 3);
     uint32_t* Fc = bm_at(a, (L + 2) % 3);
-    const int gw = blockIdx.x * kWaves + wid, nw = gridDim.x * kWaves;
     int32_t* qout = a.qbuf + ((size_t)lpar * a.grid + blockIdx.x) * kQCap;
     Counters k;
     if (dir == 1) bu_step<PARENTS>(a, L, FL, Fn, Fc, k, s_qn, qout, s_fn[wid], s_q + wid * 128,
-                                  s_lists + wid * kTileVerts, gw, nw, lane);
+                                  s_lists + wid * kTileVerts, gw, nw, lane, pf_fl, pf_vis, pf_nz);
     else if (vmode == 2) queue_step<PARENTS>(a, L, FL, Fn, Fc, k, s_qn, qout, s_lds, s_qoff, s_c.n_front,
                                              s_src0, lpar, tid, lane, wid);
     else if (vmode == 1) range_step<PARENTS>(a, L, FL, Fn, Fc, k, s_qn, qout, s_lds, s_qoff, s_qoff[a.grid], lpar, tid, lane, wid);
-    else td_step<PARENTS>(a, s_c, L, FL, Fn, Fc, k, s_qn, qout, s_lists + wid * kTileVerts, gw, nw, lane);
+    else td_step<PARENTS>(a, s_c, L, FL, Fn, Fc, k, s_qn, qout, s_lists + wid * kTileVerts, gw, nw, lane, pf_fl);
 
     // block reduction -> this block's counters (plain stores)
     // A work-group's counters of one step fit in 32 bits (at most nnz < 2^31
@@ -1540,11 +1546,11 @@ void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_
     }();
     const bool parents = want_parents != 0;
     const int flags = (parents ? 1 : 0) | (dir_in ? 2 : 0) | (a.policy ? 4 : 0) | ((forced_dir + 1) << 3);
-    // Launches go out in batches (even-sized, so that a captured 16-level
-    // sequence always starts on an even level); a launch after the last
-    // level exits at once.  The first batch is sized by the last search.
-    int batch = std::max(4, std::min(w->last_launches_used + 1, 64));
-    batch = (batch + 1) & ~1;
+    // Launches go out in batches; a launch after the last level exits at
+    // once.  The first batch is sized by the last search with a margin of
+    // one level: an exiting launch costs less than a second round trip to
+    // the host when a search is one level deeper than the last.
+    int batch = std::max(4, std::min(w->last_launches_used + 2, 64));
     int launched = 0;
     LGMI_HIP(hipEventRecord(w->ev0, s));
     if (single_src < 0 && nsrc > 0)
@@ -1575,12 +1581,13 @@ void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_
             if (launched > g->nv + 8) throw Error(LGMI_E_HIP, "BFS did not terminate");
             batch = std::min(batch * 2, 1024);
         }
-        if (graphs_on && !w->time_steps && launched >= kGraphLevels && (launched & 1) == 0) {
-            // deep search: replay captured level sequences
+        if (graphs_on && !w->time_steps && launched >= kGraphLevels && launched % 6 == 0) {
+            // deep search: replay captured level sequences (each starts on a
+            // level that is a multiple of 6)
             if (!w->gexec[flags]) {
                 hipGraph_t gr = nullptr;
                 LGMI_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                for (int i = 0; i < kGraphLevels; i++) launch_level(a, parents, -1 - (i & 1), forced_dir, s);
+                for (int i = 0; i < kGraphLevels; i++) launch_level(a, parents, -1 - (i % 6), forced_dir, s);
                 LGMI_HIP(hipStreamEndCapture(s, &gr));
                 hipGraphExec_t ex = nullptr;
                 const hipError_t e = hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0);
@@ -1592,13 +1599,18 @@ void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_
             for (int i = 0; i < ng; i++) LGMI_HIP(hipGraphLaunch(w->gexec[flags], s));
             launched += ng * kGraphLevels;
         } else {
-            for (int i = 0; i < batch; i++) {
+            // individual launches; a deep search stops at the next multiple
+            // of 6 from which graph replays can start
+            int n = batch;
+            if (graphs_on && !w->time_steps && launched + n > kGraphLevels)
+                n = std::max(1, (std::max(launched, kGraphLevels) + 5) / 6 * 6 - launched);
+            for (int i = 0; i < n; i++) {
                 const size_t li = (size_t)(launched + i);
                 if (w->time_steps) event_at(2 * li);
                 launch_level(a, parents, launched + i, forced_dir, s);
                 if (w->time_steps) event_at(2 * li + 1);
             }
-            launched += batch;
+            launched += n;
         }
         LGMI_HIP(hipGetLastError());
     }
