@@ -173,19 +173,39 @@ def bfs_bench(args, lgmi, _lib, orc):
         f"{tot_bytes / max(launches, 1) / 1e6:.2f} MB/launch, {achieved:.0f} GB/s; "
         f"bottom-up steps {n_bu}: {bu_achieved:.0f} GB/s; top-down steps {n_td}")
 
-    # CPU baseline: serial oracle BFS (distances + parents) on a sample of
-    # roots; one warm-up pass over the sample, then the median of three
+    # CPU baselines on a sample of the roots, each 1 warm-up pass then the
+    # median of three: (a) the reference's threaded path for distances and
+    # parents, Parallel.gdistances + Parallel.bfs_tree, on the host's thread
+    # share (the reported baseline); (b) the serial one-pass BFS with
+    # distances and parents on one core
     sample_roots = roots[: args.cpu_roots]
     e_cpu = float(mcomp[: len(sample_roots)].sum())
+    nthreads = cpu_threads()
+    t_gd = []
 
-    def cpu_pass():
+    def cpu_threaded():
+        t_gd.clear()
+        for r in sample_roots:
+            t0 = time.perf_counter()
+            orc.par_gdistances_threaded(rp, col, [int(r)], nthreads)
+            t_gd.append(time.perf_counter() - t0)
+            orc.par_bfs_tree_threaded(rp, col, int(r), nthreads)
+
+    t_cpu, t_runs = cpu_median(cpu_threaded)
+    cpu_gteps = e_cpu / t_cpu / 1e9
+    gd_gteps = e_cpu / sum(t_gd) / 1e9
+    log(f"cpu baseline (threaded reference path, {nthreads} threads): {len(sample_roots)} roots, median of 3 passes "
+        f"{t_cpu:.2f}s ({', '.join(f'{t:.2f}' for t in t_runs)}) -> {cpu_gteps:.4f} GTEPS; "
+        f"Parallel.gdistances alone {gd_gteps:.4f} GTEPS")
+
+    def cpu_serial():
         for r in sample_roots:
             orc.bfs_dist_parent(rp, col, int(r))
 
-    t_cpu, t_runs = cpu_median(cpu_pass)
-    cpu_gteps = e_cpu / t_cpu / 1e9
-    log(f"cpu baseline: {len(sample_roots)} roots, median of 3 passes {t_cpu:.2f}s "
-        f"({', '.join(f'{t:.2f}' for t in t_runs)}) -> {cpu_gteps:.4f} GTEPS")
+    t_ser, t_ser_runs = cpu_median(cpu_serial)
+    ser_gteps = e_cpu / t_ser / 1e9
+    log(f"cpu serial one-pass BFS: median of 3 passes {t_ser:.2f}s "
+        f"({', '.join(f'{t:.2f}' for t in t_ser_runs)}) -> {ser_gteps:.4f} GTEPS")
 
     traffic = read_traffic()
     from lgmi import tuning
@@ -229,10 +249,15 @@ def bfs_bench(args, lgmi, _lib, orc):
         "cpu_baseline": {
             "value": round(cpu_gteps, 5),
             "unit": "GTEPS",
-            "cores": 1,
+            "cores": nthreads,
             "kind": "port",
-            "sample": f"{len(sample_roots)} of the 64 roots, serial oracle BFS with distances and parents; "
+            "sample": f"{len(sample_roots)} of the 64 roots; the reference's threaded path for distances and parents "
+                      f"(Parallel.gdistances, queue_segment_size 20, then Parallel.bfs_tree) on {nthreads} threads; "
                       f"1 warm-up pass, median of 3 passes ({t_cpu:.2f} s per pass)",
+            "parallel_gdistances_only": round(gd_gteps, 5),
+            "serial_one_pass": {"value": round(ser_gteps, 5), "cores": 1,
+                                "sample": "the same roots, serial BFS writing distances and parents in one pass; "
+                                          f"median of 3 passes ({t_ser:.2f} s per pass)"},
         },
     }
     return result
@@ -389,27 +414,35 @@ def config4_bench(args, lgmi, orc, tune=False):
     l2_bytes = 8.0 * sum_n_nnz
     lds_peak = LDS_BYTES_PER_CLK_CU * ENGINE_CLOCK_GHZ * CU_COUNT          # GB/s
     lds_gbs = lds_bytes / (ms * 1e-3) / 1e9
-    rng = np.random.default_rng(7)
-    idx = rng.choice(len(graphs), size=min(args.cpu_batch_graphs, len(graphs)), replace=False)
+    # CPU sample: systematic sampling with probability proportional to the
+    # work n*m (graphs in random order, one at each of k equally spaced
+    # points of the cumulative work), so the sample follows where the time
+    # goes; estimator: the mean per-work time of the sample times the total
+    # work.  Each sampled graph: one warm-up run, then the median of three.
+    k = min(args.cpu_batch_graphs, len(graphs))
+    order = np.random.default_rng(7).permutation(len(graphs))
+    cum = np.cumsum(np.array([nm[i] for i in order], dtype=np.float64))
+    pts = (np.arange(k) + 0.5) / k * cum[-1]
+    idx = [int(order[j]) for j in np.searchsorted(cum, pts)]
     nthreads = cpu_threads()
-    nm_s = float(sum(nm[i] for i in idx))
-    wants = {}
-
-    def cpu_pass():
-        for i in idx:
-            rp, col = graphs[i].csr()
-            wants[i] = orc.betweenness_threaded(rp, col, rp, False, np.arange(graphs[i].nv()), nthreads=nthreads,
-                                                index_quirk=False)
-
-    t_cpu, t_runs = cpu_median(cpu_pass)
-    maxerr = 0.0
+    per_work, t_cpu, maxerr = [], 0.0, 0.0
     for i in idx:
-        want = wants[i]
+        rp, col = graphs[i].csr()
+        box = {}
+
+        def run_one():
+            box["want"] = orc.betweenness_threaded(rp, col, rp, False, np.arange(graphs[i].nv()), nthreads=nthreads,
+                                                   index_quirk=False)
+
+        t_i, _ = cpu_median(run_one)
+        t_cpu += t_i
+        per_work.append(t_i / nm[i])
+        want = box["want"]
         floor = 1e-15 * max(1.0, float(np.abs(want).max()))
         maxerr = max(maxerr, float((np.abs(res[i] - want) / np.maximum(np.abs(want), floor)).max()))
-    cpu_s = t_cpu * nm_tot / nm_s
-    log(f"cpu oracle: {len(idx)} graphs, {nthreads} threads, median of 3 passes {t_cpu:.2f}s "
-        f"({', '.join(f'{t:.2f}' for t in t_runs)}) -> {cpu_s:.0f}s extrapolated; max rel err {maxerr:.2e}")
+    cpu_s = float(np.mean(per_work)) * nm_tot
+    log(f"cpu oracle: {len(idx)} graphs (work-proportional sample, n {sorted(graphs[i].nv() for i in idx)}), "
+        f"{nthreads} threads, {t_cpu:.2f}s -> {cpu_s:.0f}s estimated; max rel err {maxerr:.2e}")
     out = {
         "workload": "4096 graphs, n log-uniform in [200, 20000], half barabasi_albert(n, 4), half "
                     "watts_strogatz(n, 8, 0.1); all-source exact betweenness, FP64, one persistent launch "
@@ -425,8 +458,9 @@ def config4_bench(args, lgmi, orc, tune=False):
         "sample_max_rel_err": maxerr,
         "cpu_baseline": {"value": round(nm_tot / cpu_s / 1e9, 5), "unit": "G (sum n*m)/s", "cores": nthreads,
                          "kind": "port",
-                         "sample": f"{len(idx)} seeded graphs of the batch, threaded oracle Brandes; 1 warm-up "
-                                   f"pass, median of 3 passes ({t_cpu:.2f} s per pass), time scaled by sum n*m"},
+                         "sample": f"{len(idx)} graphs of the batch drawn with probability proportional to n*m "
+                                   f"(systematic, seeded), threaded oracle Brandes, each 1 warm-up + median of 3 "
+                                   f"({t_cpu:.2f} s in all); total = mean time per n*m x sum n*m"},
         "speedup_vs_cpu": round(cpu_s / (ms * 1e-3), 1),
     }
     if cands is not None:

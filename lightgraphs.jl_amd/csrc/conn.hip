@@ -270,7 +270,7 @@ __global__ __launch_bounds__(kScanBlock) void scan_add(int32_t* out, int32_t n, 
 
 // out[i] = in[0] + ... + in[i-1]; returns the total (device buffers, host
 // synchronisation at the end).
-static int64_t exclusive_scan(hipStream_t s, const int32_t* in, int32_t* out, int32_t n) {
+int64_t exclusive_scan(hipStream_t s, const int32_t* in, int32_t* out, int32_t n) {
     if (n <= 0) return 0;
     const int32_t tiles = (n + kScanTile - 1) / kScanTile;
     int32_t* sums = dev_alloc_n<int32_t>((size_t)tiles);

@@ -11,16 +11,18 @@
 //
 // Each edge becomes one 64-bit key (row << b | column, b = bits of nv) per
 // row it enters; invalid slots get the all-ones key, which sorts last.  A
-// device radix sort over 2b bits orders the keys; a key equal to its
-// predecessor is a duplicate; a scan of the keep flags gives each kept key
-// its column slot; the row pointers come from the row boundaries of the
-// kept keys.  A directed graph sorts a second key set (column << b | row)
-// for the in-neighbour CSR.  The host receives only the row pointers (the
-// handle's derived structures are built from them); columns stay on the
-// device.
+// least-significant-digit radix sort over the low 2b bits orders the keys
+// (8-bit digits; per digit pass: per-tile digit counts, an exclusive scan
+// of the counts in digit-major order, then a stable scatter that ranks each
+// key among the equal digits of its wavefront with eight ballots and among
+// the earlier wavefronts and chunks of its tile through LDS counters); a
+// key equal to its predecessor is a duplicate; an exclusive scan of the
+// keep flags gives each kept key its column slot; the row pointers come
+// from the row boundaries of the kept keys.  A directed graph sorts a
+// second key set (column << b | row) for the in-neighbour CSR.  The host
+// receives only the row pointers (the handle's derived structures are built
+// from them); columns stay on the device.
 #include <cstring>
-
-#include <rocprim/rocprim.hpp>
 
 #include "lgmi_internal.h"
 
@@ -91,6 +93,106 @@ __global__ void scatter_csr(const uint64_t* keys, const int32_t* flag, const int
     }
 }
 
+// ---- radix sort of 64-bit keys, 8 bits per pass ----
+constexpr int kRsBlock = 256;                 // threads per work-group
+constexpr int kRsPer = 16;                    // keys per thread per tile
+constexpr int kRsTile = kRsBlock * kRsPer;    // keys per tile
+constexpr int kRsBins = 256;                  // digit values
+constexpr int kRsWaves = kRsBlock / 64;
+static_assert(kRsBins == kRsBlock, "one digit per thread in the count and carry steps");
+
+// counts[d * tiles + t] = keys of tile t with digit d at `shift`.
+__global__ __launch_bounds__(kRsBlock) void rs_count(const uint64_t* keys, int64_t n, int shift, int32_t tiles,
+                                                     int32_t* counts) {
+    __shared__ int32_t s_h[kRsBins];
+    const int tid = threadIdx.x;
+    s_h[tid] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * kRsTile;
+    #pragma unroll 4
+    for (int j = 0; j < kRsPer; j++) {
+        const int64_t i = base + (int64_t)j * kRsBlock + tid;
+        if (i < n) atomicAdd(&s_h[(int)((keys[i] >> shift) & (kRsBins - 1))], 1);
+    }
+    __syncthreads();
+    counts[(int64_t)tid * tiles + blockIdx.x] = s_h[tid];
+}
+
+// Stable scatter of tile t's keys to their sorted positions: the tile's
+// keys are taken in index order, 256 at a time (one per thread); a key's
+// position is the scanned count of its digit for this tile, plus the keys
+// of that digit in earlier chunks of the tile, in earlier wavefronts of the
+// chunk, and in lower lanes of its wavefront (lanes with an equal digit are
+// found with one ballot per digit bit).
+__global__ __launch_bounds__(kRsBlock) void rs_scatter(const uint64_t* in, uint64_t* out, int64_t n, int shift,
+                                                       int32_t tiles, const int32_t* offs) {
+    __shared__ int32_t s_base[kRsBins];
+    __shared__ int32_t s_wcnt[kRsWaves][kRsBins];
+    const int tid = threadIdx.x, lane = tid & 63, wid = __builtin_amdgcn_readfirstlane(tid >> 6);
+    s_base[tid] = offs[(int64_t)tid * tiles + blockIdx.x];
+    const int64_t base = (int64_t)blockIdx.x * kRsTile;
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int j = 0; j < kRsPer; j++) {
+        #pragma unroll
+        for (int w = 0; w < kRsWaves; w++) s_wcnt[w][tid] = 0;
+        __syncthreads();
+        const int64_t i = base + (int64_t)j * kRsBlock + tid;
+        const bool valid = i < n;
+        const uint64_t k = valid ? in[i] : 0ull;
+        const int d = (int)((k >> shift) & (kRsBins - 1));
+        uint64_t same = __ballot(valid);
+        #pragma unroll
+        for (int bit = 0; bit < 8; bit++) {
+            const uint64_t m = __ballot((d >> bit) & 1);
+            same &= ((d >> bit) & 1) ? m : ~m;
+        }
+        const int rank = __popcll(same & below);
+        if (valid && rank == 0) s_wcnt[wid][d] = __popcll(same);
+        __syncthreads();
+        if (valid) {
+            int32_t pre = 0;
+            for (int w = 0; w < wid; w++) pre += s_wcnt[w][d];
+            out[s_base[d] + pre + rank] = k;
+        }
+        __syncthreads();
+        int32_t add = 0;
+        #pragma unroll
+        for (int w = 0; w < kRsWaves; w++) add += s_wcnt[w][tid];
+        s_base[tid] += add;
+    }
+}
+
+// Sorts n keys by their low end_bit bits; the result is in *keys or *alt
+// (returned); both buffers are overwritten.
+static uint64_t* radix_sort(hipStream_t s, uint64_t* keys, uint64_t* alt, int64_t n, int end_bit) {
+    if (n <= 1) return keys;
+    const int64_t tiles64 = (n + kRsTile - 1) / kRsTile;
+    if (tiles64 * kRsBins > 0x7fffffffLL) throw Error(LGMI_E_TOO_LARGE, "too many keys for the radix sort");
+    const int32_t tiles = (int32_t)tiles64;
+    int32_t* counts = dev_alloc_n<int32_t>((size_t)tiles * kRsBins);
+    int32_t* offs = nullptr;
+    try {
+        offs = dev_alloc_n<int32_t>((size_t)tiles * kRsBins);
+        uint64_t *src = keys, *dst = alt;
+        for (int shift = 0; shift < end_bit; shift += 8) {
+            rs_count<<<tiles, kRsBlock, 0, s>>>(src, n, shift, tiles, counts);
+            LGMI_HIP(hipGetLastError());
+            exclusive_scan(s, counts, offs, tiles * kRsBins);
+            rs_scatter<<<tiles, kRsBlock, 0, s>>>(src, dst, n, shift, tiles, offs);
+            LGMI_HIP(hipGetLastError());
+            std::swap(src, dst);
+        }
+        LGMI_HIP(hipStreamSynchronize(s));
+        dev_free(counts);
+        dev_free(offs);
+        return src;
+    } catch (...) {
+        dev_free(counts);
+        dev_free(offs);
+        throw;
+    }
+}
+
 __global__ void tail_rows(int32_t* row_ptr, int32_t from, int32_t nv, int32_t total) {
     for (int32_t w = from + blockIdx.x * blockDim.x + threadIdx.x; w <= nv; w += gridDim.x * blockDim.x)
         row_ptr[w] = total;
@@ -112,34 +214,18 @@ static int64_t keys_to_csr(lgmi_graph* g, uint64_t* keys, int64_t n, int b, DevC
                            std::vector<int32_t>& rp_host) {
     hipStream_t s = g->stream;
     const int32_t nv = g->nv;
-    uint64_t* sorted = dev_alloc_n<uint64_t>((size_t)std::max<int64_t>(n, 1));
+    uint64_t* alt = dev_alloc_n<uint64_t>((size_t)std::max<int64_t>(n, 1));
+    uint64_t* sorted = nullptr;
     int32_t *flag = nullptr, *pos = nullptr, *lastu = nullptr;
     unsigned long long* dloops = nullptr;
-    void* tmp = nullptr;
     try {
-        size_t tmp_bytes = 0;
-        const unsigned end_bit = (unsigned)std::min(64, 2 * b);
-        LGMI_HIP(rocprim::radix_sort_keys(nullptr, tmp_bytes, keys, sorted, (size_t)n, 0, end_bit, s));
-        tmp = dev_alloc(tmp_bytes);
-        LGMI_HIP(rocprim::radix_sort_keys(tmp, tmp_bytes, keys, sorted, (size_t)n, 0, end_bit, s));
-        dev_free(tmp);
-        tmp = nullptr;
+        sorted = radix_sort(s, keys, alt, n, std::min(64, 2 * b));
         flag = dev_alloc_n<int32_t>((size_t)std::max<int64_t>(n, 1));
         pos = dev_alloc_n<int32_t>((size_t)std::max<int64_t>(n, 1));
         const int grid = std::max(1, (int)std::min<int64_t>(g->cu_count * 8, (n + kIngBlock - 1) / kIngBlock));
         keep_flags<<<grid, kIngBlock, 0, s>>>(sorted, n, flag);
         LGMI_HIP(hipGetLastError());
-        size_t scan_bytes = 0;
-        LGMI_HIP(rocprim::exclusive_scan(nullptr, scan_bytes, flag, pos, 0, (size_t)n, rocprim::plus<int32_t>(), s));
-        tmp = dev_alloc(scan_bytes);
-        LGMI_HIP(rocprim::exclusive_scan(tmp, scan_bytes, flag, pos, 0, (size_t)n, rocprim::plus<int32_t>(), s));
-        int32_t last_pos = 0, last_flag = 0;
-        if (n > 0) {
-            LGMI_HIP(hipMemcpyAsync(&last_pos, pos + (n - 1), sizeof(int32_t), hipMemcpyDeviceToHost, s));
-            LGMI_HIP(hipMemcpyAsync(&last_flag, flag + (n - 1), sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        }
-        LGMI_HIP(hipStreamSynchronize(s));
-        const int64_t total = (int64_t)last_pos + last_flag;
+        const int64_t total = n > 0 ? exclusive_scan(s, flag, pos, (int32_t)n) : 0;
         if (total > 0x7fffffffLL) throw Error(LGMI_E_TOO_LARGE, "more than 2^31-1 adjacency entries");
         d.row_ptr = dev_alloc_n<int32_t>((size_t)nv + 1);
         d.col = dev_alloc_n<int32_t>((size_t)std::max<int64_t>(total, 1));
@@ -165,10 +251,10 @@ static int64_t keys_to_csr(lgmi_graph* g, uint64_t* keys, int64_t n, int b, DevC
         LGMI_HIP(hipMemcpyAsync(rp_host.data(), d.row_ptr, sizeof(int32_t) * ((size_t)nv + 1), hipMemcpyDeviceToHost, s));
         LGMI_HIP(hipStreamSynchronize(s));
         *loops = (int64_t)h_loops;
-        dev_free(sorted); dev_free(flag); dev_free(pos); dev_free(tmp); dev_free(dloops); dev_free(lastu);
+        dev_free(alt); dev_free(flag); dev_free(pos); dev_free(dloops); dev_free(lastu);
         return total;
     } catch (...) {
-        dev_free(sorted); dev_free(flag); dev_free(pos); dev_free(tmp); dev_free(dloops); dev_free(lastu);
+        dev_free(alt); dev_free(flag); dev_free(pos); dev_free(dloops); dev_free(lastu);
         throw;
     }
 }

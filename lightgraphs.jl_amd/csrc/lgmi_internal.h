@@ -220,6 +220,9 @@ lgmi_graph* graph_from_edges(const int32_t* h_src, const int32_t* h_dst, int64_t
                              int64_t* ne_out);
 // conn.hip
 void connected_components(lgmi_graph* g, int32_t* label);
+// out[i] = in[0] + ... + in[i-1] on the device; returns the total (waits for
+// the stream)
+int64_t exclusive_scan(hipStream_t s, const int32_t* in, int32_t* out, int32_t n);
 int64_t period_from_levels(lgmi_graph* g, const int32_t* d_dist);
 int32_t neighborhood(lgmi_graph* g, int32_t v, int64_t d, int dir_in, int32_t* verts, int32_t* dists);
 // global policy state (host copy), set by lgmi_bfs_set_policy

@@ -284,17 +284,27 @@ def load_weights(path: str = WEIGHTS_PATH):
         return z["params"].astype(np.float32), json.loads(str(z["meta"]))
 
 
-def config5(out_dir: str, n_train: int = 2000, n_eval: int = 200, grid_train: int = 40, log=print) -> dict:
+def bench_roots(g, n: int = 64, seed: int = 20260101) -> np.ndarray:
+    """The throughput benchmark's seeded roots (degree >= 1)."""
+    deg = np.diff(g.csr()[0])
+    return np.random.default_rng(seed).choice(np.nonzero(deg > 0)[0], size=n, replace=False)
+
+
+def config5(out_dir: str, n_train: int = 2000, n_eval: int = 200, grid_train: int = 40, rmat22_train: int = 500,
+            log=print) -> dict:
     """The learned-switch workload: directed RMAT-23 (edge factor 16),
-    barabasi_albert(8_000_000, 8) and a 4096 x 4096 grid; seeded training
-    and held-out roots per graph; training on the GPU; evaluation against
-    the alpha/beta heuristic with bit-identical outputs."""
+    barabasi_albert(8_000_000, 8) and a 4096 x 4096 grid, plus the
+    undirected RMAT-22 of the throughput benchmark; seeded training and
+    held-out roots per graph (on RMAT-22 the benchmark's 64 roots are the
+    held-out set and never trained on); training on the GPU; evaluation
+    against the alpha/beta heuristic with bit-identical outputs."""
     from . import generators as gen
     t0 = time.time()
     graphs = [
         ("rmat23_directed", lambda: gen.rmat(23, 16, seed=23, directed=True), n_train),
         ("ba_8M_8", lambda: gen.barabasi_albert(8_000_000, 8, seed=8), n_train),
         ("grid_4096x4096", lambda: gen.grid([4096, 4096]), grid_train),
+        ("rmat22_undirected", lambda: gen.rmat(22, 16, seed=1), rmat22_train),
     ]
     recs, built = {}, {}
     for name, make, ntr in graphs:
@@ -307,6 +317,9 @@ def config5(out_dir: str, n_train: int = 2000, n_eval: int = 200, grid_train: in
         cand = np.nonzero(np.diff(rp) > 0)[0]
         pick = rng.choice(cand, size=min(len(cand), ntr + n_eval), replace=False)
         train_roots, eval_roots = pick[:ntr], pick[ntr:ntr + n_eval]
+        if name == "rmat22_undirected":
+            eval_roots = bench_roots(g)
+            train_roots = np.setdiff1d(pick, eval_roots)[:ntr]
         log(f"{name}: nv={g.nv()} nnz={len(g.csr()[1])} built in {time.time() - t0:.1f}s")
         rec = collect_pipelined(g, train_roots, log=lambda m, n=name: log(f"{n}: {m}"))
         recs[name] = rec

@@ -26,6 +26,11 @@
  *                        reference's own test asserts that equality).
  *   orc_par_bfs_tree     test/parallel/traversals/bfs.jl:26 ([1,1,1,3]); tree
  *                        validity (depth = BFS distance) on random graphs.
+ *   orc_par_gdistances_threaded  equals orc_gdistances on the reference
+ *                        vectors above and on random graphs, for 1 to 16
+ *                        threads and segment sizes 1 to 1000.
+ *   orc_par_bfs_tree_threaded  test/parallel/traversals/bfs.jl:26; tree
+ *                        validity on random graphs for 1 to 16 threads.
  *   orc_dijkstra_allpaths test/shortestpaths/dijkstra.jl:81-97 (pathcounts,
  *                        predecessors, closest_vertices); sigma vs. brute-force
  *                        path enumeration on tiny graphs.
@@ -42,6 +47,7 @@
  *                        direct restatement of the same recursion from the
  *                        enumerated shortest-path DAG on random graphs.
  */
+#define _POSIX_C_SOURCE 200809L   /* pthread_barrier_t */
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -214,6 +220,235 @@ int orc_par_bfs_tree(int32_t nv, const int32_t* row_ptr, const int32_t* col,
     free(q);
     return 0;
 }
+
+/* ------------------------------------------------------------------ */
+/* Threaded Parallel.gdistances! and Parallel.bfs_tree! (CPU baseline)   */
+/* ------------------------------------------------------------------ */
+/* The two functions above give the reference's outputs; these two run the
+ * reference's threaded schedule on `nthreads` POSIX threads so that it can
+ * be timed.  The reference's benign data races (unsynchronised reads and
+ * writes of visited, vert_level, the queue entries and the segment
+ * counters) become relaxed atomic loads and stores of the same locations:
+ * the same schedule, without undefined behaviour in C.  Julia's `@threads`
+ * per level becomes a pool of threads that meet at a barrier twice per
+ * level. */
+#define RLX __ATOMIC_RELAXED
+
+typedef struct {
+    int32_t nv; const int32_t* row_ptr; const int32_t* col;
+    int nt; int32_t seg;
+    int32_t** cur; int64_t* ncur;      /* cur_level_t, length(cur_level_t[t]) */
+    int32_t** nxt; int64_t* nnxt;      /* next_level_t */
+    int64_t* front;                    /* cur_front_t, 16 int64 apart */
+    uint8_t* explored;                 /* queue_explored_t, 64 B apart */
+    uint8_t* visited; int32_t* level;
+    pthread_barrier_t bar; int done; int32_t n_level; int rc;
+} pgd_pool;
+
+typedef struct { pgd_pool* p; int tid; } pgd_arg;
+
+/* One level of the @threads body, gdistances.jl:67-102, for thread `tid`:
+ * its own queue first, then the others' (:71), each one segment claim at a
+ * time (:76-78); a claimed run continues until it meets an entry already
+ * taken (:81-85).  Taken entries are overwritten with -1 (the reference's
+ * zero(T) in 1-based ids). */
+static void pgd_level(pgd_pool* p, int tid) {
+    const int nt = p->nt;
+    const int32_t n_level = p->n_level;
+    int32_t* next = p->nxt[tid];
+    int64_t nn = p->nnxt[tid];
+    for (int k = 0; k < nt; k++) {
+        const int t = (tid + k) % nt;
+        if (__atomic_load_n(&p->explored[(size_t)t * 64], RLX)) continue;   /* :72 */
+        int32_t* cur = p->cur[t];
+        const int64_t len = p->ncur[t];
+        for (;;) {
+            int64_t lf = __atomic_fetch_add(&p->front[(size_t)t * 16], (int64_t)p->seg, RLX);   /* :77-78 */
+            if (lf >= len) break;                                            /* :80 */
+            while (lf < len) {
+                const int32_t v = __atomic_load_n(&cur[lf], RLX);
+                if (v < 0) break;                                            /* :81 */
+                __atomic_store_n(&cur[lf], -1, RLX);                         /* :83 */
+                lf++;
+                if (!(__atomic_load_n(&p->visited[v], RLX) &&
+                      __atomic_load_n(&p->level[v], RLX) == n_level - 1)) continue;   /* :88 */
+                for (int32_t e = p->row_ptr[v]; e < p->row_ptr[v + 1]; e++) {  /* :89-97 */
+                    const int32_t i = p->col[e];
+                    if (!__atomic_load_n(&p->visited[i], RLX)) {
+                        __atomic_store_n(&p->level[i], n_level, RLX);
+                        __atomic_store_n(&p->visited[i], (uint8_t)1, RLX);
+                        next[nn++] = i;
+                    }
+                }
+            }
+        }
+        __atomic_store_n(&p->explored[(size_t)t * 64], (uint8_t)1, RLX);     /* :101 */
+    }
+    p->nnxt[tid] = nn;
+}
+
+/* Between levels (:105-113), on thread 0 while the others wait: swap the
+ * queues, reset the segment counters, mark empty queues explored. */
+static void pgd_swap(pgd_pool* p) {
+    int empty = 1;
+    for (int t = 0; t < p->nt; t++) {
+        int32_t* tmp = p->cur[t]; p->cur[t] = p->nxt[t]; p->nxt[t] = tmp;
+        p->ncur[t] = p->nnxt[t]; p->nnxt[t] = 0;
+        p->front[(size_t)t * 16] = 0;
+        p->explored[(size_t)t * 64] = p->ncur[t] == 0;
+        empty = empty && p->ncur[t] == 0;
+    }
+    p->done = empty;
+    p->n_level++;
+}
+
+static void* pgd_worker(void* arg) {
+    pgd_arg* a = (pgd_arg*)arg;
+    pgd_pool* p = a->p;
+    for (;;) {
+        pthread_barrier_wait(&p->bar);          /* level start: queues ready */
+        if (p->done) break;
+        pgd_level(p, a->tid);
+        if (pthread_barrier_wait(&p->bar) == PTHREAD_BARRIER_SERIAL_THREAD) pgd_swap(p);
+    }
+    return NULL;
+}
+
+/* Parallel.gdistances!(g, sources, vert_level; queue_segment_size) --
+ * src/Parallel/traversals/gdistances.jl:36-116.  Unreached = -1.  Two
+ * threads that find a vertex together both push it (the reference's race,
+ * :91-95), but one thread pushes a vertex at most once in the whole search
+ * (it sees its own store to visited), so each thread's queue holds at most
+ * max(nv, nsrc) entries. */
+int orc_par_gdistances_threaded(int32_t nv, const int32_t* row_ptr, const int32_t* col,
+                                const int32_t* srcs, int32_t nsrc, int nthreads, int32_t seg,
+                                int32_t* vert_level) {
+    if (nthreads < 1) nthreads = 1;
+    if (seg < 1) seg = 1;
+    for (int32_t i = 0; i < nsrc; i++) if (srcs[i] < 0 || srcs[i] >= nv) return -2;
+    pgd_pool p;
+    memset(&p, 0, sizeof(p));
+    p.nv = nv; p.row_ptr = row_ptr; p.col = col; p.nt = nthreads; p.seg = seg;
+    const size_t qcap = (size_t)(nv > nsrc ? nv : nsrc) + 1;
+    p.cur = (int32_t**)calloc((size_t)nthreads, sizeof(int32_t*));
+    p.nxt = (int32_t**)calloc((size_t)nthreads, sizeof(int32_t*));
+    p.ncur = (int64_t*)calloc((size_t)nthreads, sizeof(int64_t));
+    p.nnxt = (int64_t*)calloc((size_t)nthreads, sizeof(int64_t));
+    p.front = (int64_t*)calloc((size_t)nthreads * 16, sizeof(int64_t));
+    p.explored = (uint8_t*)calloc((size_t)nthreads * 64, 1);
+    p.visited = (uint8_t*)calloc((size_t)nv + 1, 1);
+    pgd_arg* args = (pgd_arg*)calloc((size_t)nthreads, sizeof(pgd_arg));
+    pthread_t* th = (pthread_t*)calloc((size_t)nthreads, sizeof(pthread_t));
+    int ok = p.cur && p.nxt && p.ncur && p.nnxt && p.front && p.explored && p.visited && args && th;
+    for (int t = 0; ok && t < nthreads; t++) {
+        p.cur[t] = (int32_t*)malloc(sizeof(int32_t) * qcap);
+        p.nxt[t] = (int32_t*)malloc(sizeof(int32_t) * qcap);
+        ok = p.cur[t] && p.nxt[t];
+    }
+    int rc = ok ? 0 : -1;
+    if (ok) {
+        p.level = vert_level;
+        for (int32_t i = 0; i < nv; i++) vert_level[i] = -1;      /* :46 */
+        for (int32_t i = 0; i < nsrc; i++) { p.visited[srcs[i]] = 1; vert_level[srcs[i]] = 0; }   /* :55-58 */
+        /* partition_sources! (:8-19): contiguous parts of `sources` */
+        for (int t = 0; t < nthreads; t++) {
+            const int64_t lo = (int64_t)nsrc * t / nthreads, hi = (int64_t)nsrc * (t + 1) / nthreads;
+            for (int64_t k = lo; k < hi; k++) p.cur[t][p.ncur[t]++] = srcs[k];
+            p.explored[(size_t)t * 64] = hi == lo;
+        }
+        p.done = nsrc == 0;
+        p.n_level = 1;
+        pthread_barrier_init(&p.bar, NULL, (unsigned)nthreads);
+        for (int t = 1; t < nthreads; t++) {
+            args[t].p = &p; args[t].tid = t;
+            pthread_create(&th[t], NULL, pgd_worker, &args[t]);
+        }
+        args[0].p = &p; args[0].tid = 0;
+        pgd_worker(&args[0]);
+        for (int t = 1; t < nthreads; t++) pthread_join(th[t], NULL);
+        pthread_barrier_destroy(&p.bar);
+    }
+    for (int t = 0; p.cur && p.nxt && t < nthreads; t++) { free(p.cur[t]); free(p.nxt[t]); }
+    free(p.cur); free(p.nxt); free(p.ncur); free(p.nnxt); free(p.front); free(p.explored);
+    free(p.visited); free(args); free(th);
+    return rc;
+}
+
+typedef struct {
+    const int32_t* row_ptr; const int32_t* col; int nt;
+    int32_t* q; int64_t head; int64_t tail;       /* ThreadQueue (bfs.jl:15-19) */
+    int32_t* parents;
+    pthread_barrier_t bar; int done;
+    int64_t lo, hi;                               /* the level being expanded */
+} pbt_pool;
+
+typedef struct { pbt_pool* p; int tid; } pbt_arg;
+
+static void* pbt_worker(void* arg) {
+    pbt_arg* a = (pbt_arg*)arg;
+    pbt_pool* p = a->p;
+    for (;;) {
+        pthread_barrier_wait(&p->bar);
+        if (p->done) break;
+        /* bfskernel (bfs.jl:49-65): `@threads for src in level` gives each
+         * thread one contiguous share of the level (Julia's static schedule) */
+        const int64_t n = p->hi - p->lo;
+        const int64_t a0 = p->lo + n * a->tid / p->nt, a1 = p->lo + n * (a->tid + 1) / p->nt;
+        for (int64_t k = a0; k < a1; k++) {
+            const int32_t src = p->q[k];
+            for (int32_t e = p->row_ptr[src]; e < p->row_ptr[src + 1]; e++) {
+                const int32_t v = p->col[e];
+                int32_t expect = -1;              /* atomic_cas!(parents[v], zero(T), src) :59 */
+                if (__atomic_load_n(&p->parents[v], RLX) == -1 &&
+                    __atomic_compare_exchange_n(&p->parents[v], &expect, src, 0, RLX, RLX)) {
+                    const int64_t off = __atomic_fetch_add(&p->tail, 1, RLX);   /* push! :27-31 */
+                    __atomic_store_n(&p->q[off], v, RLX);
+                }
+            }
+        }
+        if (pthread_barrier_wait(&p->bar) == PTHREAD_BARRIER_SERIAL_THREAD) {
+            /* bfs_tree! loop (:86-90): next level = q[head:tail) */
+            p->lo = p->head = p->hi;
+            p->hi = p->tail;
+            p->done = p->lo == p->hi;
+        }
+    }
+    return NULL;
+}
+
+/* Parallel.bfs_tree!(next, g, source, parents) -- src/Parallel/traversals/
+ * bfs.jl:78-92 on `nthreads` threads.  The parents are one legal outcome of
+ * the reference's first-CAS-wins race; unset = -1.  Each vertex enters the
+ * queue once (the CAS), so nv entries suffice.  The reference's
+ * `isempty` test (:38-40) is the level-empty test here. */
+int orc_par_bfs_tree_threaded(int32_t nv, const int32_t* row_ptr, const int32_t* col,
+                              int32_t source, int nthreads, int32_t* parents) {
+    if (nthreads < 1) nthreads = 1;
+    if (source < 0 || source >= nv) return -2;
+    pbt_pool p;
+    memset(&p, 0, sizeof(p));
+    p.row_ptr = row_ptr; p.col = col; p.nt = nthreads; p.parents = parents;
+    p.q = (int32_t*)malloc(sizeof(int32_t) * ((size_t)nv + 1));
+    pbt_arg* args = (pbt_arg*)calloc((size_t)nthreads, sizeof(pbt_arg));
+    pthread_t* th = (pthread_t*)calloc((size_t)nthreads, sizeof(pthread_t));
+    if (!p.q || !args || !th) { free(p.q); free(args); free(th); return -1; }
+    for (int32_t i = 0; i < nv; i++) parents[i] = -1;
+    parents[source] = source;                       /* :84 */
+    p.q[0] = source; p.tail = 1;                    /* :85 */
+    p.lo = 0; p.hi = 1; p.head = 1;
+    pthread_barrier_init(&p.bar, NULL, (unsigned)nthreads);
+    for (int t = 1; t < nthreads; t++) {
+        args[t].p = &p; args[t].tid = t;
+        pthread_create(&th[t], NULL, pbt_worker, &args[t]);
+    }
+    args[0].p = &p; args[0].tid = 0;
+    pbt_worker(&args[0]);
+    for (int t = 1; t < nthreads; t++) pthread_join(th[t], NULL);
+    pthread_barrier_destroy(&p.bar);
+    free(p.q); free(args); free(th);
+    return 0;
+}
+#undef RLX
 
 /* ------------------------------------------------------------------ */
 /* Binary-heap priority queue with a key index.                         */

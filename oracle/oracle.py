@@ -48,6 +48,9 @@ def _load():
     lib.orc_bfs_parents.argtypes = [ctypes.c_int32, _i32p, _i32p, ctypes.c_int32, _i32p]
     lib.orc_par_gdistances.argtypes = [ctypes.c_int32, _i32p, _i32p, _i32p, ctypes.c_int32, _i32p]
     lib.orc_par_bfs_tree.argtypes = [ctypes.c_int32, _i32p, _i32p, ctypes.c_int32, _i32p]
+    lib.orc_par_gdistances_threaded.argtypes = [ctypes.c_int32, _i32p, _i32p, _i32p, ctypes.c_int32,
+                                                ctypes.c_int, ctypes.c_int32, _i32p]
+    lib.orc_par_bfs_tree_threaded.argtypes = [ctypes.c_int32, _i32p, _i32p, ctypes.c_int32, ctypes.c_int, _i32p]
     lib.orc_dijkstra_allpaths.argtypes = [ctypes.c_int32, _i32p, _i32p, _i32p, ctypes.c_int32,
                                           _i64p, _f64p, _i32p, _i32p, _i32p]
     lib.orc_betweenness.argtypes = [ctypes.c_int32, _i32p, _i32p, _i32p, ctypes.c_int,
@@ -125,6 +128,29 @@ def par_bfs_tree(row_ptr, col, src):
     nv = len(row_ptr) - 1
     out = np.empty(nv, dtype=np.int32)
     _check(lib.orc_par_bfs_tree(nv, _p32(row_ptr), _p32(col), int(src), _p32(out)))
+    return out
+
+
+def par_gdistances_threaded(row_ptr, col, srcs, nthreads, queue_segment_size=20):
+    """Parallel.gdistances on `nthreads` threads (the reference's schedule;
+    timed as the CPU baseline).  Same output as par_gdistances."""
+    lib = _load()
+    row_ptr, col, srcs = _c32(row_ptr), _c32(col), _c32(np.atleast_1d(srcs))
+    nv = len(row_ptr) - 1
+    out = np.empty(nv, dtype=np.int32)
+    _check(lib.orc_par_gdistances_threaded(nv, _p32(row_ptr), _p32(col), _p32(srcs), len(srcs), int(nthreads),
+                                           int(queue_segment_size), _p32(out)))
+    return out
+
+
+def par_bfs_tree_threaded(row_ptr, col, src, nthreads):
+    """Parallel.bfs_tree on `nthreads` threads: one legal outcome of the
+    reference's first-CAS-wins race (timed as the CPU baseline)."""
+    lib = _load()
+    row_ptr, col = _c32(row_ptr), _c32(col)
+    nv = len(row_ptr) - 1
+    out = np.empty(nv, dtype=np.int32)
+    _check(lib.orc_par_bfs_tree_threaded(nv, _p32(row_ptr), _p32(col), int(src), int(nthreads), _p32(out)))
     return out
 
 

@@ -73,3 +73,28 @@ def test_rmat_edge_list_and_bfs(oracle):
     dref = oracle.gdistances(rp, col, [root])
     got = lgmi.gdistances(g, root + 1)
     assert (np.where(dref < 0, np.int64(lgmi.typemax(np.int64)), dref.astype(np.int64)) == got).all()
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_wide_keys_and_skewed_digits(directed):
+    """Keys of 50 bits (nv above 2^24: seven 8-bit passes of the radix
+    sort) and a skewed edge list -- ten source vertices, long runs of equal
+    keys and equal digits, a length that fills no tile evenly -- against
+    the host builder."""
+    import lgmi
+    from lgmi.graphs import csr_from_edges, csr_transpose
+    rng = np.random.default_rng(74)
+    n = (1 << 24) + 3
+    m = 123_457
+    src = rng.integers(0, 10, m)
+    dst = np.where(rng.random(m) < 0.5, rng.integers(0, 64, m), rng.integers(0, n, m))
+    dst[:1000] = n - 1                      # the largest id, repeated
+    say(f"ingest wide keys directed={directed}")
+    cls = lgmi.SimpleDiGraph if directed else lgmi.SimpleGraph
+    g, ne = cls.from_edge_list(src + 1, dst + 1, nv_min=n)
+    rp, col = csr_from_edges(n, src, dst, symmetrize=not directed)
+    assert g.nv() == n
+    assert np.array_equal(g.csr()[0], rp) and np.array_equal(g.csr()[1], col)
+    if directed:
+        irp, icol = csr_transpose(rp, col)
+        assert np.array_equal(g.in_csr()[0], irp) and np.array_equal(g.in_csr()[1], icol)

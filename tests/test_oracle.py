@@ -77,6 +77,71 @@ def test_bfs_parents_reference_vectors(oracle):
     assert par_in.tolist() == [3, 3, 4, 4]
 
 
+def _bfs_tree_ok(rp, col, src, par, dist):
+    """par is a BFS tree of the graph from src: the root is its own parent,
+    every reached vertex's parent is an in-neighbour one level closer, and
+    exactly the reached vertices have a parent."""
+    reached = dist >= 0
+    if par[src] != src or not ((par >= 0) == reached).all():
+        return False
+    for v in np.nonzero(reached)[0]:
+        if v == src:
+            continue
+        u = par[v]
+        if dist[u] != dist[v] - 1 or v not in col[rp[u]:rp[u + 1]]:
+            return False
+    return True
+
+
+def test_threaded_parallel_bfs_reference_vectors(oracle):
+    # test/parallel/traversals/gdistances.jl:7-17 and
+    # test/parallel/traversals/bfs.jl:26, on 1, 3 and 8 threads
+    _, _, (rp5, col5, _, _) = mk(G5)
+    _, _, (rp, col, _, _) = mk(HOUSE)
+    for nt in (1, 3, 8):
+        for srcs in ([0], [0, 2]):
+            assert (oracle.par_gdistances_threaded(rp5, col5, srcs, nt) == oracle.gdistances(rp5, col5, srcs)).all()
+        for srcs in ([1], [0, 1], []):
+            assert (oracle.par_gdistances_threaded(rp, col, srcs, nt) == oracle.gdistances(rp, col, srcs)).all()
+        assert (oracle.par_bfs_tree_threaded(rp5, col5, 0, nt) + 1).tolist() == [1, 1, 1, 3]
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_threaded_parallel_bfs_random(oracle, directed):
+    """The threaded schedules against the serial restatements on graphs large
+    enough for the threads to race: distances equal for every thread count
+    and segment size; the threaded tree is a BFS tree."""
+    rng = np.random.default_rng(7 + directed)
+    n = 30000
+    src = rng.integers(0, n, 240000)
+    dst = (src + rng.geometric(0.002, src.size)) % n          # a long diameter
+    hub = rng.integers(0, n, 4000)
+    src = np.concatenate([src, np.zeros(4000, np.int64)])
+    dst = np.concatenate([dst, hub])
+    rp, col, _, _ = csr_of(n, src, dst, directed, drop_self_loops=True)
+    for roots in ([0], [5, 17, 29999], list(rng.integers(0, n, 64))):
+        ref = oracle.gdistances(rp, col, roots)
+        for nt, seg in ((2, 20), (8, 1), (16, 20), (8, 1000)):
+            assert (oracle.par_gdistances_threaded(rp, col, roots, nt, seg) == ref).all()
+    for root in (0, 123):
+        dist = oracle.gdistances(rp, col, [root])
+        for nt in (1, 4, 16):
+            par = oracle.par_bfs_tree_threaded(rp, col, root, nt)
+            assert _bfs_tree_ok(rp, col, root, par, dist)
+    # the validity check rejects a parent that is adjacent but on the wrong
+    # level, and one on the right level that is not adjacent
+    par = oracle.par_bfs_tree_threaded(rp, col, 0, 4)
+    dist = oracle.gdistances(rp, col, [0])
+    v = next(int(w) for w in np.nonzero(dist == 3)[0] if (dist[col[rp[w]:rp[w + 1]]] >= 3).any())
+    wrong_level = next(int(w) for w in col[rp[v]:rp[v + 1]] if dist[w] >= 3)   # adjacent, level >= 3
+    bad = par.copy(); bad[v] = wrong_level if not directed else int(par[par[v]])
+    assert not _bfs_tree_ok(rp, col, 0, bad, dist)
+    same_level = np.nonzero(dist == 2)[0]
+    u = next(int(w) for w in same_level if v not in col[rp[w]:rp[w + 1]])
+    bad = par.copy(); bad[v] = u
+    assert not _bfs_tree_ok(rp, col, 0, bad, dist)
+
+
 def test_dijkstra_allpaths_reference_vectors(oracle):
     # test/shortestpaths/dijkstra.jl:93-97
     n, _, (rp, col, irp, _) = mk(G615)
