@@ -67,14 +67,18 @@ typedef struct {
     int64_t m_front;           /* traversal-direction edges of F_L           */
     int64_t m_unvisited;       /* scan-direction edges of unvisited vertices */
     int32_t n_next;            /* |F_{L+1}|                                  */
-    int32_t variant;           /* 1: top-down in range mode (each work-group */
-                               /* owns a vertex range), 0: otherwise         */
+    int32_t variant;           /* top-down: 1 range mode (each work-group    */
+                               /* owns a vertex range), 2 queue mode (the    */
+                               /* frontier list is shared out by edges),     */
+                               /* 3 solo mode (one work-group, several       */
+                               /* levels per launch); 0 otherwise            */
     int64_t scanned;           /* bottom-up: adjacency entries read;         */
                                /* top-down: edges expanded                   */
     int64_t cand;              /* bottom-up: unvisited vertices examined     */
     float kernel_us;           /* first block start to last block end        */
     float step_us;             /* previous step end to this step end         */
-    float event_ms;            /* HIP-event time of the launch, 0 if off     */
+    float event_ms;            /* HIP-event time of the launch, shared       */
+                               /* equally among the levels it ran; 0 if off  */
     int32_t pad2;
 } lgmi_bfs_step;
 
@@ -370,7 +374,12 @@ typedef struct {
                                /* into segments of this many entries (default */
                                /* 1024; 64..65536; fixed for a graph at its   */
                                /* first betweenness call)                     */
-    int32_t pad;
+    int32_t solo_front;        /* solo mode: frontier vertices at most        */
+                               /* (default 4096; 0..4096, 0 turns it off)     */
+    int64_t solo_edges;        /* solo mode: frontier edges at most (default  */
+                               /* 2048; 0..262144).  In solo mode one work-   */
+                               /* group runs consecutive small top-down       */
+                               /* levels inside one launch                    */
 } lgmi_tuning;
 /* t = NULL restores the built-in defaults. */
 int lgmi_set_tuning(const lgmi_tuning* t);

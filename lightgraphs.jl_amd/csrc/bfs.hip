@@ -71,6 +71,9 @@ constexpr int kPartRows = kBlock / kPart;
 constexpr int32_t kRangeFront = 2048;    // range mode: frontier vertices at most (LDS layout)
 constexpr int kEmit = 8;                 // range mode: 1024-vertex slices emitted per round
 constexpr int32_t kQueueFront = 4096;    // queue mode: frontier vertices at most (LDS layout)
+constexpr int32_t kSoloFront = 4096;     // solo mode: frontier vertices at most (LDS layout)
+constexpr int kSoloPer = kSoloFront / kBlock;   // solo mode: frontier entries per thread
+constexpr int kSoloGroup = 8;            // solo mode: edges per lane with their loads in flight together
 
 struct StepArgs {
     const int32_t* td_rp;     // traversal adjacency (top-down expands it)
@@ -110,13 +113,20 @@ struct StepArgs {
     int64_t range_light;      // range mode: light-row frontier edges at most
     int64_t queue_min_edges;  // queue mode: frontier edges at least
     int64_t queue_max_edges;  // queue mode: frontier edges at most
+    int64_t solo_edges;       // solo mode: frontier edges at most
+    int32_t solo_front;       // solo mode: frontier vertices at most (<= kSoloFront; 0: off)
 };
 
-// done_level and its host-mapped copy (which the host reads after the
-// launches complete, instead of copying the state back).
-__device__ inline void set_done(const StepArgs& a, int32_t v) {
+// done_level (the launch from which on launches exit at once), the level
+// count of the ended search, and their host-mapped copies (which the host
+// reads after the launches complete, instead of copying the state back).
+__device__ inline void set_done(const StepArgs& a, int32_t v, int32_t levels) {
     a.sh->done_level = v;
-    if (a.done_host) __hip_atomic_store(a.done_host, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    a.sh->done_levels = levels;
+    if (a.done_host) {
+        __hip_atomic_store(a.done_host + 1, levels, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(a.done_host, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
 }
 
 __device__ inline uint32_t* bm_at(const StepArgs& a, int k) {
@@ -958,6 +968,332 @@ __device__ __attribute__((always_inline)) inline void queue_step(const StepArgs&
     }
 }
 
+// Solo-mode visit of N edges u[q] -> v[q] (v[q] < 0: none) by work-group 0
+// alone.  The visited bitmap holds every level up to L (each level's
+// frontier is folded in at its start, and the reads are coherent with
+// those atomics), so a vertex is fresh when its bit is clear.  With parents
+// the atomicMin that finds the parent unset claims the vertex; without, an
+// atomicCAS of its distance.  New vertices go to the LDS list s_next (one
+// LDS atomic per wavefront and call); those past its capacity have their
+// bits set in Fn at once and are counted in `ovf` (per thread, histogram
+// included).  Every lane of the wavefront calls it.
+template <bool PARENTS, int N>
+__device__ __attribute__((always_inline)) inline void visit_solo(const StepArgs& a, const int32_t* u, const int32_t* v,
+                                                                 int32_t L, uint32_t* Fn, int32_t* s_next,
+                                                                 uint32_t* s_nn, Counters& ovf, int lane) {
+    uint32_t seen[N], pre[N];
+    #pragma unroll
+    for (int q = 0; q < N; q++) {
+        seen[q] = 0xffffffffu;
+        pre[q] = 0;
+        if (v[q] >= 0) {
+            seen[q] = __hip_atomic_load(&a.vis[v[q] >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (PARENTS) pre[q] = a.parent[v[q]];
+        }
+    }
+    bool claim[N];
+    #pragma unroll
+    for (int q = 0; q < N; q++) {
+        const bool fresh = v[q] >= 0 && !((seen[q] >> (v[q] & 31)) & 1u);
+        // parent only decreases, so a stale value <= u already rules u out
+        claim[q] = fresh && (PARENTS ? pre[q] > (uint32_t)u[q] : true);
+    }
+    uint32_t old[N];
+    #pragma unroll
+    for (int q = 0; q < N; q++) {
+        old[q] = 0;
+        if (claim[q]) {
+            if (PARENTS) old[q] = atomicMin(&a.parent[v[q]], (uint32_t)u[q]);
+            else old[q] = (uint32_t)atomicCAS(&a.dist[v[q]], -1, L + 1);
+        }
+    }
+    bool won[N];
+    uint64_t m[N];
+    uint32_t tot = 0;
+    #pragma unroll
+    for (int q = 0; q < N; q++) {
+        won[q] = claim[q] && old[q] == 0xffffffffu;
+        if (PARENTS && won[q]) a.dist[v[q]] = L + 1;
+        m[q] = __ballot(won[q]);
+        tot += (uint32_t)__popcll(m[q]);
+    }
+    if (!tot) return;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(s_nn, tot);
+    base = __shfl(base, 0);
+    const uint64_t lt = (1ull << lane) - 1ull;
+    #pragma unroll
+    for (int q = 0; q < N; q++) {
+        if (won[q]) {
+            const uint32_t pos = base + (uint32_t)__popcll(m[q] & lt);
+            if (pos < (uint32_t)kSoloFront) {
+                s_next[pos] = v[q];
+            } else {
+                atomicOr(&Fn[v[q] >> 5], 1u << (v[q] & 31));
+                const int bin = account_new(a, v[q], -1, ovf);
+                #pragma unroll
+                for (int b = 0; b < kHistBins; b++) ovf.hist[b] += bin == b ? 1u : 0u;
+            }
+        }
+        base += (uint32_t)__popcll(m[q]);
+    }
+}
+
+// Solo mode.  Work-group 0 alone runs top-down levels L, L+1, ... while the
+// frontier has at most a.solo_front vertices and a.solo_edges edges;
+// work-group barriers take the place of the launches between them.  The
+// frontier is a list in LDS, expanded with the edges dealt out evenly over
+// the threads (a contiguous run each, found with one binary search over the
+// rows' edge offsets).  No frontier bitmap is written for the levels inside
+// the loop: each level's frontier is folded into the visited bitmap at its
+// start, and the visited bitmap alone tells fresh vertices.  The loop ends
+// at the first level s whose frontier F_s is too large or empty:
+//  * F_s too large (or past the list): F_s's bits go to Fn = F_{p+1}, the
+//    frontier bitmap of launch p + 1, its list to this work-group's queue,
+//    S_{s-1} to state[p & 1] and the counters of step s-1 to `k` for the
+//    epilogue: what the next launch's prologue reads, as if launch p had
+//    run level s-1 alone;
+//  * F_s empty: the search has s levels; S_s and done_level = p + 1 are
+//    stored, as the next launch's prologue would have stored them.
+// Every level run here gets its record in the level log.
+template <bool PARENTS>
+__device__ __attribute__((always_inline)) inline void solo_loop(const StepArgs& a, BfsCtrl* s_cp, int32_t p, int lpar,
+                                                                uint32_t* Fn, int32_t* qout, uint32_t* s_lds,
+                                                                const int32_t* s_qoff, Counters& k, int tid, int lane,
+                                                                int wid) {
+    int32_t* s_fa = reinterpret_cast<int32_t*>(s_lds);    // [kSoloFront] frontier list
+    int32_t* s_fb = s_fa + kSoloFront;                     // [kSoloFront] next list
+    int32_t* s_sb = s_fb + kSoloFront;                     // [kSoloFront] first entry of each frontier row
+    int32_t* s_sx = s_sb + kSoloFront;                     // [kSoloFront + 1] edge offsets (prefix)
+    int32_t* s_ws = s_sx + kSoloFront + 1;                 // [kWaves] wavefront totals
+    uint32_t* s_nn = reinterpret_cast<uint32_t*>(s_ws + kWaves);   // entries appended to the next list
+    uint32_t* s_red = s_nn + 1;                            // [kWaves][12] counter partials
+    uint32_t* s_tot = s_red + kWaves * 12;                 // [12] totals: n, out, in, light, hist[8]
+    int32_t* s_go = reinterpret_cast<int32_t*>(s_tot + 12);   // 0 run the level, 1 hand over, 2 ended
+    static_assert(4 * kSoloFront + 1 + kWaves + 1 + kWaves * 12 + 12 + 1 <=
+                      kRangeMax + kRangeMax / 32 + 3 * kRangeFront + 1 + kWaves,
+                  "solo-mode lists exceed the LDS area");
+    static_assert(kSoloFront <= kQCap, "solo mode hands its frontier over through a work-group queue");
+    static_assert(kSoloFront % kBlock == 0, "solo mode: whole frontier entries per thread");
+
+    int32_t n = s_cp->n_front;
+    const int32_t L0 = s_cp->level;
+    {
+        // F_L0 from the work-group queues of launch p-1
+        const int32_t* qin = a.qbuf + (size_t)(lpar ^ 1) * a.grid * kQCap;
+        for (int32_t j = tid; j < n; j += kBlock) {
+            int lo = 0, hi = a.grid;                      // largest b with s_qoff[b] <= j
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_qoff[mid] <= j) lo = mid;
+                else hi = mid;
+            }
+            s_fa[j] = qin[(size_t)lo * kQCap + (j - s_qoff[lo])];
+        }
+    }
+    if (tid == 0) {
+        s_cp->dir = 0;
+        s_cp->variant = 3;
+        s_cp->phase = p;
+        *s_nn = 0;
+    }
+    __syncthreads();
+    int32_t* cur = s_fa;
+    int32_t* nxt = s_fb;
+    int32_t sl = L0;              // the level being run
+    int64_t t_prev = 0;           // edges expanded by step sl - 1
+    bool over = false;            // F_sl has vertices beyond the list
+    Counters ovf;                 // this thread's share of them
+    for (;;) {
+        // rows of F_sl, F_sl folded into the visited bitmap, F_sl's counters
+        uint32_t cnt[12];
+        cnt[0] = (uint32_t)ovf.n;
+        cnt[1] = (uint32_t)ovf.out;
+        cnt[2] = (uint32_t)ovf.in;
+        cnt[3] = (uint32_t)ovf.light;
+        #pragma unroll
+        for (int b = 0; b < kHistBins; b++) cnt[4 + b] = ovf.hist[b];
+        ovf = Counters();
+        int32_t fb[kSoloPer], len[kSoloPer];
+        #pragma unroll
+        for (int q = 0; q < kSoloPer; q++) {
+            const int32_t j = kSoloPer * tid + q;
+            fb[q] = 0;
+            len[q] = 0;
+            if (j < n) {
+                const int32_t u = cur[j];
+                fb[q] = a.td_rp[u];
+                len[q] = a.td_rp[u + 1] - fb[q];
+                const int32_t din = a.td_rp == a.bu_rp ? len[q] : a.bu_rp[u + 1] - a.bu_rp[u];
+                atomicOr(&a.vis[u >> 5], 1u << (u & 31));
+                cnt[0] += 1u;
+                cnt[1] += (uint32_t)len[q];
+                cnt[2] += (uint32_t)din;
+                if (len[q] <= kHeavyRow) cnt[3] += (uint32_t)len[q];
+                const int bin = hist_bin(len[q]);
+                #pragma unroll
+                for (int b = 0; b < kHistBins; b++) cnt[4 + b] += bin == b ? 1u : 0u;
+            }
+        }
+        int32_t tsum = 0;
+        #pragma unroll
+        for (int q = 0; q < kSoloPer; q++) tsum += len[q];
+        int32_t wtot;
+        const int32_t ex = wave_excl_scan(tsum, lane, wtot);
+        #pragma unroll
+        for (int i = 0; i < 12; i++) cnt[i] = wave_sum(cnt[i]);
+        if (lane == 0) {
+            s_ws[wid] = wtot;
+            #pragma unroll
+            for (int i = 0; i < 12; i++) s_red[wid * 12 + i] = cnt[i];
+        }
+        __syncthreads();   // also completes the visited-bitmap folds
+        if (wid == 0) {
+            const int32_t x = lane < kWaves ? s_ws[lane] : 0;
+            int32_t t2;
+            const int32_t e2 = wave_excl_scan(x, lane, t2);
+            if (lane < kWaves) s_ws[lane] = e2;
+            if (lane < 12) {
+                uint32_t v = 0;
+                for (int w = 0; w < kWaves; w++) v += s_red[w * 12 + lane];
+                s_tot[lane] = v;
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane == 0) {
+                s_sx[n] = t2;
+                int go = 0;
+                if (sl > L0) {
+                    // S_sl from S_{sl-1} and the counters of step sl-1
+                    const uint64_t now = wall_clock64();
+                    const BfsCtrl q = *s_cp;
+                    BfsCtrl c = q;
+                    const int32_t ntot = (int32_t)s_tot[0];
+                    c.level = sl;
+                    c.n_prev = q.n_front;
+                    c.n_front = ntot;
+                    c.prev_dir = 0;
+                    c.m_front = (int64_t)s_tot[1];
+                    c.m_unvisited = q.m_unvisited - (int64_t)s_tot[2];
+                    c.n_unvisited = q.n_unvisited - ntot;
+                    c.m_light = (int64_t)s_tot[3];
+                    for (int b = 0; b < kHistBins; b++) c.front_hist[b] = s_tot[4 + b];
+                    c.qovf = over ? 1 : 0;
+                    c.t_main = now;
+                    if (sl - 1 < a.max_log) {
+                        LevelRec* r = &a.log[sl - 1];
+                        r->dir = 0;
+                        r->variant = 3;
+                        r->n_front = q.n_front;
+                        r->m_front = q.m_front;
+                        r->m_unvisited = q.m_unvisited;
+                        r->n_next = ntot;
+                        r->scanned = t_prev;
+                        r->cand = 0;
+                        r->t_start = q.t_main;
+                        r->t_end = now;
+                        r->t_prev_end = q.t_main;
+                        r->launch = p;
+                    }
+                    if (ntot == 0) {
+                        go = 2;
+                        a.sh->state[lpar] = c;
+                        set_done(a, p + 1, sl);
+                    } else if (over || ntot > a.solo_front || c.m_front > a.solo_edges ||
+                               c.m_front > 8 * (int64_t)ntot || ntot > 2 * q.n_front + 64) {
+                        go = 1;
+                        a.sh->state[lpar] = q;
+                    } else {
+                        if (sl < a.max_log) {
+                            float f[kFeat];
+                            compute_features(&c, f);
+                            for (int i = 0; i < kFeat; i++) a.log[sl].feat[i] = f[i];
+                        }
+                        *s_cp = c;
+                    }
+                }
+                *s_go = go;
+            }
+        }
+        __syncthreads();
+        const int go = *s_go;
+        if (go == 2) return;          // k stays zero
+        if (go == 1) {
+            // hand F_sl over to launch p + 1
+            for (int32_t j = tid; j < n; j += kBlock) {
+                const int32_t u = cur[j];
+                atomicOr(&Fn[u >> 5], 1u << (u & 31));
+                qout[j] = u;
+            }
+            if (wid == 0) {
+                if (lane == 0) {
+                    k.n = s_tot[0];
+                    k.out = s_tot[1];
+                    k.in = s_tot[2];
+                    k.light = s_tot[3];
+                    k.scan = (unsigned long long)t_prev;
+                }
+                #pragma unroll
+                for (int b = 0; b < kHistBins; b++) k.hist[b] = s_tot[4 + b];
+            }
+            return;
+        }
+        if (tid == 0) *s_nn = 0;
+        // the rows' first entries and edge offsets
+        {
+            int32_t off = s_ws[wid] + ex;
+            #pragma unroll
+            for (int q = 0; q < kSoloPer; q++) {
+                const int32_t j = kSoloPer * tid + q;
+                if (j < n) {
+                    s_sb[j] = fb[q];
+                    s_sx[j] = off;
+                }
+                off += len[q];
+            }
+        }
+        __syncthreads();
+        // step sl -> sl + 1: a contiguous run of edges per thread
+        const int32_t T = s_sx[n];
+        const int32_t per = (T + kBlock - 1) / kBlock;
+        const int32_t e0 = min(T, tid * per), e1 = min(T, e0 + per);
+        int lo = 0;
+        if (e0 < e1) {
+            int hi = n;                                   // largest row with s_sx[row] <= e0
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_sx[mid] <= e0) lo = mid;
+                else hi = mid;
+            }
+        }
+        for (int32_t base = 0; base < per; base += kSoloGroup) {
+            int32_t uu[kSoloGroup], vv[kSoloGroup];
+            #pragma unroll
+            for (int q = 0; q < kSoloGroup; q++) {
+                const int32_t e = e0 + base + q;
+                uu[q] = 0;
+                vv[q] = -1;
+                if (e < e1) {
+                    while (s_sx[lo + 1] <= e) lo++;
+                    uu[q] = cur[lo];
+                    vv[q] = a.td_col[s_sb[lo] + (e - s_sx[lo])];
+                }
+            }
+            visit_solo<PARENTS, kSoloGroup>(a, uu, vv, sl, Fn, nxt, s_nn, ovf, lane);
+        }
+        __syncthreads();
+        const uint32_t nn = *s_nn;
+        over = nn > (uint32_t)kSoloFront;
+        n = over ? kSoloFront : (int32_t)nn;
+        t_prev = T;
+        int32_t* t = cur;
+        cur = nxt;
+        nxt = t;
+        sl++;
+    }
+}
+
 // The learned policy's weights as lane j of wavefront 0 needs them (hidden
 // unit j), loaded with the prologue's other loads.  The device copy has the
 // feature standardisation folded into W1 and b1 (done on the host when the
@@ -1017,7 +1353,7 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
     __shared__ unsigned long long s_red[kWaves][kPart];
     __shared__ unsigned long long s_sum[kPart];
     __shared__ float s_feat[kFeat];
-    __shared__ int32_t s_p, s_dir, s_exit, s_dl, s_mode, s_src0;
+    __shared__ int32_t s_p, s_dir, s_exit, s_dl, s_mode, s_src0, s_solo;
     __shared__ uint32_t s_qn[1];                 // the work-group queue's count
     __shared__ uint32_t s_fn[kWaves][kTileWords];
     __shared__ int32_t s_qoff[kMaxGrid + 1];      // per-work-group new-vertex counts, then queue offsets
@@ -1144,12 +1480,13 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
                     r->t_start = q.t_main;
                     r->t_end = t_now;
                     r->t_prev_end = q.t_main;
+                    r->launch = q.phase;
                 }
                 if (n_next == 0) {
                     ex = 1;
                     if (blockIdx.x == 0) {
                         sh->state[p & 1] = c;
-                        set_done(a, p);
+                        set_done(a, p, k);
                     }
                 } else if (may_learn || (blockIdx.x == 0 && k < a.max_log)) {
                     // the features feed the network and work-group 0's level log only
@@ -1159,13 +1496,28 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
             }
             s_p = p;
             s_exit = ex;
-            if (!ex && !may_learn)
-                s_dir = forced >= 0 ? forced : (first ? s_c.dir : heuristic_direction(s_c));
+            // solo mode: a run of small levels -- this frontier and the
+            // last one small, this one of low-degree rows (at most 8
+            // entries per vertex on average), its list complete, neither
+            // growing nor shrinking fast: a chain, a grid -- whatever
+            // direction the policy would pick.  A solo level costs more
+            // than an ordinary one when the run ends after it (the next
+            // frontier's rows are read before the hand-over), so a lone
+            // small level (the last levels of a search shrink fast), a
+            // frontier of hub rows that explodes next, and level 0 stay
+            // with the other modes.
+            const bool solo = !ex && !first && forced < 0 && a.solo_front > 0 && s_c.qovf == 0 &&
+                              s_c.n_front <= a.solo_front && s_c.n_prev <= a.solo_front &&
+                              s_c.m_front <= a.solo_edges && s_c.m_front <= 8 * (int64_t)s_c.n_front &&
+                              s_c.n_front <= 2 * s_c.n_prev + 64 && 2 * s_c.n_front >= s_c.n_prev;
+            s_solo = solo ? 1 : 0;
+            if (!ex && (!may_learn || solo))
+                s_dir = solo ? 0 : (forced >= 0 ? forced : (first ? s_c.dir : heuristic_direction(s_c)));
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (may_learn && !s_exit) {
+        if (may_learn && !s_exit && !s_solo) {
             const int d = mlp_direction(s_feat, pr);
             if (lane == 0) s_dir = d;
         }
@@ -1178,14 +1530,15 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
         // and few light-row edges, when the range table exists and every
         // work-group queue of the frontier is complete
         const BfsCtrl& c = s_c;
-        const bool top = !s_exit && s_dir == 0;
+        const bool solo = !s_exit && s_solo;
+        const bool top = !s_exit && !solo && s_dir == 0;
         const bool queue = top && (first ? s_src0 >= 0 : c.qovf == 0) && c.n_front <= a.queue_front &&
                            c.m_front >= a.queue_min_edges && c.m_front <= a.queue_max_edges;
         const bool range = top && !queue && !first && a.range_size > 0 && c.qovf == 0 &&
                            c.n_front <= a.range_front && c.m_light <= a.range_light &&
                            c.front_hist[5] + c.front_hist[6] > 0;
-        if (lane == 0) s_mode = queue ? 2 : (range ? 1 : 0);
-        if ((range || queue) && !first) {
+        if (lane == 0) s_mode = solo ? 3 : (queue ? 2 : (range ? 1 : 0));
+        if ((range || queue || solo) && !first) {
             // queue offsets: exclusive scan of the per-work-group counts,
             // four consecutive counts per lane (one wavefront scan per 256)
             int32_t carry = 0;
@@ -1228,12 +1581,20 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
         sh->state[p & 1] = c;
     }
 
-    const uint32_t* FL = bm_at(a, L % 3);
-    uint32_t* Fn = bm_at(a, (L + 1) % 3);
-    uint32_t* Fc = bm_at(a, (L + 2) % 3);
+    // launch p reads F from bitmap p mod 3, writes the next frontier to
+    // p + 1 and clears p + 2 (a solo launch runs several levels, so the
+    // rotation follows the launch, not the level)
+    const uint32_t* FL = bm_at(a, pmod6 % 3);
+    uint32_t* Fn = bm_at(a, (pmod6 + 1) % 3);
+    uint32_t* Fc = bm_at(a, (pmod6 + 2) % 3);
     int32_t* qout = a.qbuf + ((size_t)lpar * a.grid + blockIdx.x) * kQCap;
     Counters k;
-    if (dir == 1) bu_step<PARENTS>(a, L, FL, Fn, Fc, k, s_qn, qout, s_fn[wid], s_q + wid * 128,
+    if (vmode == 3) {
+        for (int64_t i = (int64_t)blockIdx.x * kBlock + tid; i < a.nwords / 2; i += (int64_t)a.grid * kBlock)
+            reinterpret_cast<uint2*>(Fc)[i] = make_uint2(0u, 0u);
+        if (blockIdx.x == 0)
+            solo_loop<PARENTS>(a, &s_c, p, lpar, Fn, qout, s_lds, s_qoff, k, tid, lane, wid);
+    } else if (dir == 1) bu_step<PARENTS>(a, L, FL, Fn, Fc, k, s_qn, qout, s_fn[wid], s_q + wid * 128,
                                   s_lists + wid * kTileVerts, gw, nw, lane, pf_fl, pf_vis, pf_nz);
     else if (vmode == 2) queue_step<PARENTS>(a, L, FL, Fn, Fc, k, s_qn, qout, s_lds, s_qoff, s_c.n_front,
                                              s_src0, lpar, tid, lane, wid);
@@ -1293,7 +1654,7 @@ __device__ void init_state(const StepArgs& a, int32_t nsrc, int32_t src0, unsign
     a.sh->state[0] = c;
     a.sh->state[1] = c;      // what the first launch reads
     a.sh->src0 = src0;
-    set_done(a, nsrc == 0 ? 0 : INT_MAX);
+    set_done(a, nsrc == 0 ? 0 : INT_MAX, 0);
     if (a.max_log > 0) {
         float f[kFeat];
         compute_features(&c, f);
@@ -1444,8 +1805,9 @@ void bfs_prepare(lgmi_graph* g) {
         w->log = dev_alloc_n<LevelRec>(w->max_log);
         w->policy = dev_alloc_n<PolicyWeights>(1);
         LGMI_HIP(hipHostMalloc((void**)&w->h_sh, sizeof(BfsShared), hipHostMallocDefault));
-        LGMI_HIP(hipHostMalloc((void**)&w->h_done, sizeof(int32_t), hipHostMallocMapped | hipHostMallocCoherent));
-        *w->h_done = 0;
+        LGMI_HIP(hipHostMalloc((void**)&w->h_done, 2 * sizeof(int32_t), hipHostMallocMapped | hipHostMallocCoherent));
+        w->h_done[0] = 0;
+        w->h_done[1] = 0;
         LGMI_HIP(hipHostGetDevicePointer((void**)&w->d_done, w->h_done, 0));
         LGMI_HIP(hipEventCreate(&w->ev0));
         LGMI_HIP(hipEventCreate(&w->ev1));
@@ -1483,6 +1845,8 @@ static StepArgs graph_args(lgmi_graph* g, int dir_in, int32_t grid) {
     a.range_light = t.range_light;
     a.queue_min_edges = t.queue_min_edges;
     a.queue_max_edges = t.queue_max_edges;
+    a.solo_front = std::min(t.solo_front, kSoloFront);
+    a.solo_edges = t.solo_edges;
     return a;
 }
 
@@ -1614,8 +1978,9 @@ void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_
         }
         LGMI_HIP(hipGetLastError());
     }
-    // launch p handles level p; the launch that finds F_p empty stores p
-    const int levels = done;
+    // the launch from which on launches exit (done) and the level count
+    // (equal unless solo launches ran several levels each)
+    const int levels = __atomic_load_n(w->h_done + 1, __ATOMIC_ACQUIRE);
     w->last_levels = levels;
     w->last_launches_used = done;
     w->last_dir_in = dir_in;
@@ -1803,6 +2168,12 @@ extern "C" int lgmi_bfs_steps(lgmi_graph_t g, int32_t max_steps, lgmi_bfs_step* 
         LGMI_HIP(hipGetDevice(&dev));
         LGMI_HIP(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
         const double us_per_tick = khz > 0 ? 1000.0 / (double)khz : 0.01;
+        // levels per launch: a launch's event time is shared equally among them
+        std::vector<int> per_launch(w->last_launch_ms.size(), 0);
+        for (int i = 0; i < n; i++) {
+            const int li = recs[(size_t)i].launch;
+            if (li >= 0 && (size_t)li < per_launch.size()) per_launch[(size_t)li]++;
+        }
         for (int i = 0; i < n; i++) {
             const LevelRec& r = recs[(size_t)i];
             lgmi_bfs_step& o = out[i];
@@ -1818,7 +2189,10 @@ extern "C" int lgmi_bfs_steps(lgmi_graph_t g, int32_t max_steps, lgmi_bfs_step* 
             o.cand = r.cand;
             o.kernel_us = (float)((double)(r.t_end - r.t_start) * us_per_tick);
             o.step_us = (float)((double)(r.t_end - r.t_prev_end) * us_per_tick);
-            o.event_ms = (size_t)i < w->last_launch_ms.size() ? w->last_launch_ms[(size_t)i] : 0.f;
+            const int li = r.launch;
+            o.event_ms = (li >= 0 && (size_t)li < per_launch.size() && per_launch[(size_t)li] > 0)
+                             ? w->last_launch_ms[(size_t)li] / (float)per_launch[(size_t)li]
+                             : 0.f;
         }
         return n;
     } catch (const Error& e) {
@@ -1867,6 +2241,7 @@ extern "C" int lgmi_bfs_probe(lgmi_graph_t g, int32_t src, int dir_in, const int
         BfsWork* w = g->bfs;
         StepArgs a = make_args(g, dir_in, true);
         a.policy = nullptr;
+        a.solo_front = 0;   // one level per launch: launch index = level
         hipStream_t s = g->stream;
         const int init_grid = std::max(1, std::min<int>(w->grid * 8, (g->nv + 255) / 256));
         bfs_init<<<init_grid, 256, 0, s>>>(a, src, g->nnz, -1, LGMI_POLICY_HEURISTIC, 15.0f, 18.0f);

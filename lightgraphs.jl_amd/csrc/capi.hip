@@ -139,6 +139,8 @@ lgmi_tuning tuning_defaults() {
     t.queue_min_edges = 4096;
     t.queue_max_edges = 262144;
     t.bc_seg_len = 1024;
+    t.solo_front = 4096;
+    t.solo_edges = 2048;
     return t;
 }
 std::mutex g_tune_mu;
@@ -608,11 +610,11 @@ int lgmi_set_tuning(const lgmi_tuning* t) {
     const lgmi_tuning v = t ? *t : tuning_defaults();
     if (v.wave_row < 2 || v.wave_row > 257 || v.lane_rounds < 0 || v.lane_rounds > 64 || v.range_front < 0 ||
         v.range_front > 2048 || v.queue_front < 0 || v.queue_front > 4096 || v.range_light < 0 ||
-        v.queue_min_edges < 0 || v.queue_max_edges < 0 || v.bc_seg_len < 64 || v.bc_seg_len > 65536)
+        v.queue_min_edges < 0 || v.queue_max_edges < 0 || v.bc_seg_len < 64 || v.bc_seg_len > 65536 ||
+        v.solo_front < 0 || v.solo_front > 4096 || v.solo_edges < 0 || v.solo_edges > 262144)
         throw Error(LGMI_E_ARG, "tuning value out of range");
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune = v;
-    g_tune.pad = 0;
     LGMI_API_END
 }
 

@@ -41,10 +41,11 @@ T* dev_alloc_n(size_t n) {
 }
 
 // ---------------------------------------------------------------------
-// BFS level state (device resident).  One launch of the level kernel per
-// BFS level k: every work-group sums the per-block counters of step k-1,
-// derives the state S_k from S_{k-1}, picks the direction of step k and
-// runs it; work-group 0 also stores S_k (state[k & 1]) and the level log.
+// BFS level state (device resident).  Launch p of the level kernel runs
+// level k (or, in solo mode, levels k, k+1, ... in work-group 0): every
+// work-group sums the per-block counters launch p-1 wrote, derives the state
+// S_k from S_{k-1}, picks the direction of step k and runs it; work-group 0
+// also stores the state (state[p & 1]) and the level log.
 // ---------------------------------------------------------------------
 constexpr int kFeat = LGMI_POLICY_FEATURES;  // per-level policy features
 constexpr int kHidden = LGMI_POLICY_HIDDEN;
@@ -67,16 +68,17 @@ struct BfsCtrl {
     int64_t nnz_scan;
     int64_t m_light;        // traversal-direction edges of F_k's rows of at most kHeavyRow entries
     int32_t qovf;           // 1: some work-group's queue of F_k overflowed (or there is none)
-    int32_t variant;        // step k: 1 range-mode top-down, 2 queue-mode top-down, 0 otherwise
+    int32_t variant;        // step k: top-down in 1 range, 2 queue, 3 solo mode; 0 otherwise
     uint64_t t_main;        // wall clock at the start of level k's launch
 };
 
 // Words shared between the launches of one search and the host.
 struct BfsShared {
-    int32_t done_level;     // level whose launch found F_k empty (INT32_MAX while running)
+    int32_t done_level;     // launch that found the search ended (INT32_MAX while running)
     int32_t src0;           // the source of a single-source search (-1: several)
-    int32_t pad[2];
-    BfsCtrl state[2];       // S_k by k & 1
+    int32_t done_levels;    // levels of the ended search (the empty frontier's k)
+    int32_t pad;
+    BfsCtrl state[2];       // the state launch p stored, by p & 1
 };
 
 struct LevelRec {
@@ -86,12 +88,14 @@ struct LevelRec {
     int64_t m_front;
     int64_t m_unvisited;
     int32_t n_next;
-    int32_t variant;        // 1: range-mode top-down, 2: queue-mode top-down
+    int32_t variant;        // top-down: 1 range mode, 2 queue mode, 3 solo mode
     int64_t scanned;        // adjacency entries read (bottom-up) / expanded (top-down)
     int64_t cand;           // bottom-up: unvisited vertices examined
     uint64_t t_start;       // first block start (wall clock ticks)
     uint64_t t_end;         // last block end
     uint64_t t_prev_end;    // end of the previous step (launch gap included)
+    int32_t launch;         // index of the launch that ran the step
+    int32_t pad;
 };
 
 struct PolicyWeights {      // FP32 MLP 16 -> 64 (ReLU) -> 2
@@ -123,7 +127,7 @@ struct BfsWork {
     int32_t* qbuf = nullptr;     // [2][grid][kQCap]: each work-group's new vertices, by level parity
     BfsShared* sh = nullptr;
     BfsShared* h_sh = nullptr;   // pinned host mirror
-    int32_t* h_done = nullptr;   // pinned host-mapped copy of done_level, written by the kernels
+    int32_t* h_done = nullptr;   // pinned host-mapped [done_level, done_levels], written by the kernels
     int32_t* d_done = nullptr;   // its device address
     LevelRec* log = nullptr;
     int32_t max_log = 0;

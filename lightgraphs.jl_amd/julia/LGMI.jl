@@ -124,7 +124,8 @@ struct Tuning
     queue_min_edges::Int64
     queue_max_edges::Int64
     bc_seg_len::Int32
-    pad::Int32
+    solo_front::Int32
+    solo_edges::Int64
 end
 
 const TUNING_FILE = joinpath(@__DIR__, "..", "lgmi", "tuning.json")
@@ -145,7 +146,8 @@ function load_tuning(path::AbstractString = TUNING_FILE)
     new = Tuning(f("wave_row", t.wave_row, Int32), f("lane_rounds", t.lane_rounds, Int32),
                  f("range_front", t.range_front, Int32), f("queue_front", t.queue_front, Int32),
                  f("range_light", t.range_light, Int64), f("queue_min_edges", t.queue_min_edges, Int64),
-                 f("queue_max_edges", t.queue_max_edges, Int64), f("bc_seg_len", t.bc_seg_len, Int32), Int32(0))
+                 f("queue_max_edges", t.queue_max_edges, Int64), f("bc_seg_len", t.bc_seg_len, Int32),
+                 f("solo_front", t.solo_front, Int32), f("solo_edges", t.solo_edges, Int64))
     check(ccall((:lgmi_set_tuning, liblgmi), Cint, (Ref{Tuning},), new))
     return true
 end

@@ -87,7 +87,8 @@ class BcStats(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("wave_row", ctypes.c_int32), ("lane_rounds", ctypes.c_int32), ("range_front", ctypes.c_int32),
                 ("queue_front", ctypes.c_int32), ("range_light", ctypes.c_int64), ("queue_min_edges", ctypes.c_int64),
-                ("queue_max_edges", ctypes.c_int64), ("bc_seg_len", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("queue_max_edges", ctypes.c_int64), ("bc_seg_len", ctypes.c_int32), ("solo_front", ctypes.c_int32),
+                ("solo_edges", ctypes.c_int64)]
 
 
 _i32p = ctypes.POINTER(ctypes.c_int32)

@@ -20,6 +20,10 @@ changes a result (the tests run every step variant against the oracle):
   are expanded by a whole wavefront, shorter ones packed into the lanes;
 * ``lane_rounds`` -- a bottom-up candidate scans this many rounds of four
   entries in its own lane before a whole wavefront takes its row;
+* ``solo_front`` / ``solo_edges`` -- a level whose frontier has at most
+  this many vertices and edges runs in solo mode: one work-group expands it
+  and goes on to the next level inside the same launch while the frontier
+  stays that small (0 turns solo mode off);
 * ``bc_seg_len`` -- betweenness cuts rows longer than this into segments of
   this many entries (fixed for a graph at its first betweenness call).
 
@@ -43,10 +47,12 @@ from . import _lib
 TUNING_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuning.json")
 
 FIELDS = ["wave_row", "lane_rounds", "range_front", "queue_front", "range_light", "queue_min_edges",
-          "queue_max_edges", "bc_seg_len"]
+          "queue_max_edges", "bc_seg_len", "solo_front", "solo_edges"]
 
 # Candidate values of the coordinate search, in the order they are searched.
 CANDIDATES = {
+    "solo_edges": [0, 2048, 4096, 16384, 32768],
+    "solo_front": [512, 1024, 2048],
     "queue_min_edges": [0, 256, 1024, 4096, 16384, 65536, 1 << 40],
     "queue_max_edges": [16384, 65536, 262144, 1048576],
     "queue_front": [256, 1024, 2048, 4096],

@@ -244,7 +244,7 @@ def _variant_counts(g, roots):
     lib = _lib.load()
     buf = (_lib.BfsStep * 20000)()
     st = _lib.BfsStats()
-    counts = {0: 0, 1: 0, 2: 0}
+    counts = {0: 0, 1: 0, 2: 0, 3: 0}
     for r in roots:
         _lib.check(lib.lgmi_bfs_run(g.handle(), r, 0, 1, ctypes.byref(st)))
         n = lib.lgmi_bfs_steps(g.handle(), 20000, buf)
@@ -274,7 +274,7 @@ def test_range_mode_levels_exact(oracle):
     from lgmi import tuning
     seen = 0
     try:
-        tuning.set_values({"queue_front": 0})
+        tuning.set_values({"queue_front": 0, "solo_front": 0})
         for g, tag in ((gen.rmat(20, 16, seed=2), "rmat20"), (gen.rmat(17, 16, seed=3, directed=True), "rmat17-dir")):
             roots = _hub_neighbour_roots(g)
             _check_graph(oracle, g, roots, tag)
@@ -297,7 +297,7 @@ def test_queue_mode_levels_exact(oracle):
     seen = 0
     try:
         tuning.set_values({"queue_front": 4096, "queue_min_edges": 0, "queue_max_edges": 1 << 40,
-                           "range_front": 0})
+                           "range_front": 0, "solo_front": 0})
         cases = [(gen.rmat(18, 16, seed=6), "rmat18"), (gen.rmat(15, 16, seed=7, directed=True), "rmat15-dir"),
                  (gen.grid([300, 200]), "grid300x200"), (gen.path_graph(3000), "path3000"),
                  (gen.barabasi_albert(50_000, 3, seed=8), "ba50k")]
@@ -319,6 +319,9 @@ def test_queue_mode_levels_exact(oracle):
     {"wave_row": 257, "lane_rounds": 64},
     {"queue_front": 0, "range_front": 0},
     {"queue_front": 256, "queue_min_edges": 0, "queue_max_edges": 2048, "range_light": 1 << 40},
+    {"solo_front": 0},
+    {"solo_front": 64, "solo_edges": 256},
+    {"solo_front": 4096, "solo_edges": 262144},
 ])
 def test_tuning_never_changes_output(oracle, values):
     """Every launch threshold setting gives the same distances and parents
@@ -371,7 +374,8 @@ def test_level_counters_match_oracle(oracle):
     import lgmi.generators as gen
     from lgmi import _lib
     lib = _lib.load()
-    for g, tag in ((gen.rmat(18, 16, seed=4), "rmat18"), (gen.rmat(15, 16, seed=5, directed=True), "rmat15-dir")):
+    for g, tag in ((gen.rmat(18, 16, seed=4), "rmat18"), (gen.rmat(15, 16, seed=5, directed=True), "rmat15-dir"),
+                   (gen.grid([128, 96]), "grid128x96")):
         rp, col = g.csr()
         deg = np.diff(rp)
         deg_in = np.diff(g.in_csr()[0])
@@ -396,3 +400,68 @@ def test_level_counters_match_oracle(oracle):
                     got = np.array(buf[L].feat[:16], dtype=np.float64)
                     assert np.allclose(got, want, rtol=1e-5, atol=1e-5), (tag, r, L, got, want)
         _lib.check(lib.lgmi_bfs_force_direction(g.handle(), -1))
+
+
+def test_solo_mode_levels_exact(oracle):
+    """Small top-down levels run in solo mode: work-group 0 alone expands
+    them and goes on to the next level inside the same launch.  Distances
+    and parents stay bit-exact along out- and in-edges, with and without
+    parents, for chains and grids (thousands of levels in few launches),
+    hub levels, multi-source searches, and frontiers that outgrow the LDS
+    list inside a solo level (a star: its hub's 30000 leaves are claimed by
+    the one work-group, 4096 of them listed, the rest handed over through
+    the frontier bitmap)."""
+    import ctypes
+
+    import lgmi
+    import lgmi.generators as gen
+    from lgmi import _lib, tuning
+    lib = _lib.load()
+    seen = 0
+    try:
+        for values in ({}, {"solo_front": 4096, "solo_edges": 262144}):
+            tuning.set_values(None)
+            tuning.set_values(values)
+            star = gen.star_graph(30001)
+            cases = [(gen.grid([300, 200]), "grid300x200"), (gen.path_graph(5000), "path5000"),
+                     (gen.rmat(18, 16, seed=6), "rmat18"), (gen.rmat(15, 16, seed=7, directed=True), "rmat15-dir"),
+                     (gen.barabasi_albert(50_000, 3, seed=8), "ba50k"), (star, "star30001")]
+            for g, tag in cases:
+                rp, _ = g.csr()
+                deg = np.diff(rp)
+                roots = [0, int(np.nonzero(deg > 0)[0][-1])] + _hub_neighbour_roots(g, 1)
+                _check_graph(oracle, g, roots, f"{tag} {values}")
+                seen += _variant_counts(g, roots)[3]
+                # multi-source distances (solo from level 1 on)
+                srcs = np.array(sorted(set(roots)), dtype=np.int32)
+                from lgmi import traversals
+                want = oracle.gdistances(rp, g.csr()[1], srcs)
+                assert (traversals._device_dist(g, srcs) == want).all(), f"{tag} multi-source"
+            # a chain runs its levels in a handful of launches
+            g = gen.path_graph(5000)
+            st = _lib.BfsStats()
+            _lib.check(lib.lgmi_bfs_run(g.handle(), 0, 0, 1, ctypes.byref(st)))
+            say(f"path5000 {values}: {st.levels} levels in {st.launches} launches")
+            assert st.levels == 5000
+    finally:
+        tuning.set_values(None)
+        tuning.load()
+    say(f"solo-mode levels: {seen}")
+    assert seen > 0, "no level ran in solo mode"
+
+
+def test_solo_mode_has_path_walls(oracle):
+    """has_path's excluded vertices are marked visited before the search;
+    solo levels (which tell fresh vertices by the visited bitmap alone)
+    never cross them.  The wall is the set of grid vertices at distance 60
+    from vertex 1, once whole and once with a gap."""
+    import lgmi
+    import lgmi.generators as gen
+    g = gen.grid([120, 80])
+    rp, col = g.csr()
+    d = oracle.gdistances(rp, col, [0])
+    far = int(np.argmax(d))
+    ring = np.nonzero(d == 60)[0]
+    for wall, want in ((ring[ring != ring[len(ring) // 2]], True), (ring, False)):
+        assert oracle.has_path(rp, col, 0, far, exclude=[int(w) for w in wall]) == want
+        assert lgmi.has_path(g, 1, far + 1, exclude_vertices=[int(w) + 1 for w in wall]) == want

@@ -162,13 +162,15 @@ def test_tuning_set_get_and_range_checks():
     _lib.check(lib.lgmi_set_tuning(None))
     d = tuning.get()
     assert d == {"wave_row": 32, "lane_rounds": 4, "range_front": 2048, "queue_front": 4096, "range_light": 32768,
-                 "queue_min_edges": 4096, "queue_max_edges": 262144, "bc_seg_len": 1024}
+                 "queue_min_edges": 4096, "queue_max_edges": 262144, "bc_seg_len": 1024, "solo_front": 4096,
+                 "solo_edges": 2048}
     try:
         tuning.set_values({"wave_row": 64, "queue_min_edges": 0, "bc_seg_len": 512})
         got = tuning.get()
         assert got["wave_row"] == 64 and got["queue_min_edges"] == 0 and got["bc_seg_len"] == 512
         for bad in ({"wave_row": 1}, {"lane_rounds": 65}, {"range_front": 4096}, {"queue_front": 5000},
-                    {"bc_seg_len": 32}, {"queue_max_edges": -1}):
+                    {"bc_seg_len": 32}, {"queue_max_edges": -1}, {"solo_front": 4097}, {"solo_edges": -1},
+                    {"solo_edges": 262145}):
             with pytest.raises(ValueError):
                 tuning.set_values(bad)
         assert tuning.get() == got        # a rejected setting changes nothing
