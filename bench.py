@@ -556,10 +556,13 @@ def dropin_bench(lgmi, _lib):
     return out
 
 
-def profile_pass(lgmi, _lib, nbfs):
-    """Workload for rocprofv3 counter runs: one 256 MiB device-to-device
-    copy (the runtime's copy kernel, a known byte count for calibrating
-    FETCH_SIZE / WRITE_SIZE) followed by `nbfs` BFS runs on RMAT-22."""
+def profile_pass(lgmi, _lib, nbfs, levels_out=""):
+    """Workload for rocprofv3 runs: one 256 MiB device-to-device copy (the
+    runtime's copy kernel, a known byte count for calibrating FETCH_SIZE /
+    WRITE_SIZE) followed by `nbfs` BFS runs on RMAT-22.  With `levels_out`
+    the level records of those runs (direction, mode, launch index,
+    counters) go to that JSON file, in launch order, for
+    --direction-summary."""
     import ctypes
     hip = ctypes.CDLL("libamdhip64.so")
     nbytes = 256 << 20
@@ -578,10 +581,74 @@ def profile_pass(lgmi, _lib, nbfs):
     h = g.handle()
     rng = np.random.default_rng(20260101)
     roots = rng.choice(np.nonzero(deg > 0)[0], size=64, replace=False).astype(np.int32)
+    st = _lib.BfsStats()
+    buf = (_lib.BfsStep * 4096)()
+    runs = []
     for i in range(nbfs):
-        _lib.check(lib.lgmi_bfs_run(h, int(roots[i % 64]), 0, 1, None))
+        _lib.check(lib.lgmi_bfs_run(h, int(roots[i % 64]), 0, 1, ctypes.byref(st)))
+        if levels_out:
+            n = lib.lgmi_bfs_steps(h, 4096, buf)
+            if n < 0:
+                _lib.check(n)
+            runs.append({"launches": int(st.launches), "levels": int(st.levels),
+                         "recs": [{"dir": buf[k].dir, "variant": buf[k].variant, "launch": buf[k].launch,
+                                   "n_front": buf[k].n_front, "m_front": buf[k].m_front,
+                                   "n_next": buf[k].n_next, "scanned": buf[k].scanned, "cand": buf[k].cand}
+                                  for k in range(n)]})
     _lib.check(lib.lgmi_synchronize())
+    if levels_out:
+        with open(levels_out, "w") as f:
+            json.dump({"nv": int(g.nv()), "runs": runs}, f)
     log(f"profile pass done: copy {nbytes} B x3, {nbfs} BFS")
+
+
+def direction_summary(trace_csv, levels_json):
+    """Per-direction table of the bfs_level dispatches of a profiled
+    --profile-pass run: each dispatch of rocprofv3's kernel trace (in
+    start order) is labelled from the level records of the same run (the
+    launches of one search are consecutive; a launch that ran no level is
+    one issued after the search ended; a solo launch runs several), and its
+    duration and the algorithmic bytes of its levels are summed per label."""
+    import csv
+    from types import SimpleNamespace
+    rows = [r for r in csv.DictReader(open(trace_csv)) if "bfs_level" in r["Kernel_Name"]]
+    rows.sort(key=lambda r: int(r["Start_Timestamp"]))
+    dur = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3 for r in rows]   # us
+    lv = json.load(open(levels_json))
+    nv = lv["nv"]
+    labels, nbytes = [], []
+    for run in lv["runs"]:
+        by_launch = {}
+        for rec in run["recs"]:
+            by_launch.setdefault(rec["launch"], []).append(rec)
+        for li in range(run["launches"]):
+            recs = by_launch.get(li, [])
+            if not recs:
+                lab = "after the last level"
+            elif len(recs) > 1 or recs[0]["variant"] == 3:
+                lab = "top-down, solo"
+            elif recs[0]["dir"] == 1:
+                lab = "bottom-up"
+            else:
+                lab = "top-down, " + {0: "sweep", 1: "range", 2: "queue"}[recs[0]["variant"]]
+            labels.append(lab)
+            nbytes.append(sum(step_bytes(SimpleNamespace(**r), nv) for r in recs))
+    if len(labels) != len(dur):
+        raise SystemExit(f"{len(dur)} bfs_level dispatches in the trace, {len(labels)} launches in the level log")
+    agg = {}
+    for lab, t, b in zip(labels, dur, nbytes):
+        a = agg.setdefault(lab, [0, 0.0, 0.0])
+        a[0] += 1
+        a[1] += t
+        a[2] += b
+    tot = sum(a[1] for a in agg.values())
+    lines = ["| dispatches | count | total ms | avg us | share | algorithmic GB/s | fraction of 8 TB/s |",
+             "|---|---|---|---|---|---|---|"]
+    for lab, (n, t, b) in sorted(agg.items(), key=lambda kv: -kv[1][1]):
+        gbs = b / (t * 1e-6) / 1e9 if t > 0 else 0.0
+        lines.append(f"| {lab} | {n} | {t / 1e3:.2f} | {t / n:.2f} | {100 * t / tot:.1f} % | {gbs:.0f} | "
+                     f"{gbs / HBM_PEAK_GBS:.4f} |")
+    print("\n".join(lines))
 
 
 def lg_load_bench(lgmi, g):
@@ -738,6 +805,9 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--profile-pass", type=int, default=0,
                     help="run only the counter-profiling workload with this many BFS runs")
+    ap.add_argument("--levels-out", default="", help="with --profile-pass: write the runs' level records here")
+    ap.add_argument("--direction-summary", nargs=2, metavar=("TRACE_CSV", "LEVELS_JSON"),
+                    help="print the per-direction table of a profiled --profile-pass run and exit")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--warmup", type=int, default=8)
@@ -756,6 +826,9 @@ def main():
     ap.add_argument("--config5-roots", type=int, default=20)
     ap.add_argument("--save-tuning", default="", help="write the batch autotuner's choice to this JSON file")
     args = ap.parse_args()
+    if args.direction_summary:
+        direction_summary(*args.direction_summary)
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (every box here has one GPU)",
               file=sys.stderr)
@@ -767,7 +840,7 @@ def main():
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
     if args.profile_pass:
-        profile_pass(lgmi, _lib, args.profile_pass)
+        profile_pass(lgmi, _lib, args.profile_pass, args.levels_out)
         return
     if args.config4:
         print(json.dumps(config4_bench(args, lgmi, orc, tune=True)), flush=True)

@@ -79,7 +79,8 @@ typedef struct {
     float step_us;             /* previous step end to this step end         */
     float event_ms;            /* HIP-event time of the launch, shared       */
                                /* equally among the levels it ran; 0 if off  */
-    int32_t pad2;
+    int32_t launch;            /* index of the level-kernel launch that ran  */
+                               /* the step (solo launches run several)       */
 } lgmi_bfs_step;
 
 /* Statistics of the last betweenness run. */

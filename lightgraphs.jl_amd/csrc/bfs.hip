@@ -2193,6 +2193,7 @@ extern "C" int lgmi_bfs_steps(lgmi_graph_t g, int32_t max_steps, lgmi_bfs_step* 
             o.event_ms = (li >= 0 && (size_t)li < per_launch.size() && per_launch[(size_t)li] > 0)
                              ? w->last_launch_ms[(size_t)li] / (float)per_launch[(size_t)li]
                              : 0.f;
+            o.launch = li;
         }
         return n;
     } catch (const Error& e) {

@@ -70,7 +70,7 @@ class BfsStep(ctypes.Structure):
                 ("m_front", ctypes.c_int64), ("m_unvisited", ctypes.c_int64), ("n_next", ctypes.c_int32),
                 ("variant", ctypes.c_int32), ("scanned", ctypes.c_int64), ("cand", ctypes.c_int64),
                 ("kernel_us", ctypes.c_float), ("step_us", ctypes.c_float), ("event_ms", ctypes.c_float),
-                ("pad2", ctypes.c_int32)]
+                ("launch", ctypes.c_int32)]
 
 
 class ProbeRec(ctypes.Structure):
