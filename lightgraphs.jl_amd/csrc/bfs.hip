@@ -51,6 +51,7 @@
 #include "lgmi_internal.h"
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cstdlib>
 #include <cstring>
@@ -123,10 +124,10 @@ struct StepArgs {
 __device__ inline void set_done(const StepArgs& a, int32_t v, int32_t levels) {
     a.sh->done_level = v;
     a.sh->done_levels = levels;
-    if (a.done_host) {
-        __hip_atomic_store(a.done_host + 1, levels, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-        __hip_atomic_store(a.done_host, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
-    }
+    if (a.done_host)   // one 64-bit store: the host never sees one half without the other
+        __hip_atomic_store(reinterpret_cast<unsigned long long*>(a.done_host),
+                           ((unsigned long long)(uint32_t)levels << 32) | (uint32_t)v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_SYSTEM);
 }
 
 __device__ inline uint32_t* bm_at(const StepArgs& a, int k) {
@@ -1916,6 +1917,23 @@ void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_
     // the host when a search is one level deeper than the last.
     int batch = std::max(4, std::min(w->last_launches_used + 2, 64));
     int launched = 0;
+    // Completion.  The launch that ends the search stores its index (and the
+    // level count) into host-mapped memory.  Without statistics or per-step
+    // timing the host polls that word and returns as soon as the search has
+    // ended: the batch's spare launches, which exit at once, then finish
+    // behind the caller's next work on the same stream, and every later read
+    // of the results is ordered after them on that stream.  With either, the
+    // host waits for the whole batch.  The word is reset here, before
+    // anything is enqueued; the kernels of the last search wrote it before
+    // that search returned.
+    const bool poll = stats == nullptr && !w->time_steps;
+    __atomic_store_n(reinterpret_cast<unsigned long long*>(w->h_done), (unsigned long long)INT_MAX, __ATOMIC_RELEASE);
+    auto read_done = [&](int32_t& lev) {
+        const unsigned long long x = __atomic_load_n(reinterpret_cast<unsigned long long*>(w->h_done), __ATOMIC_ACQUIRE);
+        lev = (int32_t)(x >> 32);
+        return (int32_t)(uint32_t)x;
+    };
+    int32_t levels = 0;
     LGMI_HIP(hipEventRecord(w->ev0, s));
     if (single_src < 0 && nsrc > 0)
         LGMI_HIP(hipMemcpyAsync(w->srcbuf, h_srcs, sizeof(int32_t) * (size_t)nsrc, hipMemcpyHostToDevice, s));
@@ -1939,8 +1957,32 @@ void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_
     for (;;) {
         if (launched > 0) {
             LGMI_HIP(hipEventRecord(w->ev1, s));
-            LGMI_HIP(hipEventSynchronize(w->ev1));
-            done = __atomic_load_n(w->h_done, __ATOMIC_ACQUIRE);
+            if (poll) {
+                // spin for up to 0.5 ms (a whole RMAT-22 search takes about
+                // 0.16 ms), then wait for the batch without spinning
+                const auto t0 = std::chrono::steady_clock::now();
+                for (unsigned it = 0;; it++) {
+                    done = read_done(levels);
+                    if (done < launched) break;
+                    if ((it & 255) == 255) {   // the batch may end without ending the search
+                        if (std::chrono::steady_clock::now() - t0 > std::chrono::microseconds(500)) {
+                            LGMI_HIP(hipEventSynchronize(w->ev1));
+                            done = read_done(levels);
+                            break;
+                        }
+                        const hipError_t q = hipEventQuery(w->ev1);
+                        if (q == hipSuccess) {
+                            done = read_done(levels);
+                            break;
+                        }
+                        if (q != hipErrorNotReady) LGMI_HIP(q);
+                    }
+                    __builtin_ia32_pause();
+                }
+            } else {
+                LGMI_HIP(hipEventSynchronize(w->ev1));
+                done = read_done(levels);
+            }
             if (done < launched) break;
             if (launched > g->nv + 8) throw Error(LGMI_E_HIP, "BFS did not terminate");
             batch = std::min(batch * 2, 1024);
@@ -1978,9 +2020,8 @@ void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_
         }
         LGMI_HIP(hipGetLastError());
     }
-    // the launch from which on launches exit (done) and the level count
+    // done: the launch from which on launches exit; levels: the level count
     // (equal unless solo launches ran several levels each)
-    const int levels = __atomic_load_n(w->h_done + 1, __ATOMIC_ACQUIRE);
     w->last_levels = levels;
     w->last_launches_used = done;
     w->last_dir_in = dir_in;
@@ -2162,6 +2203,7 @@ extern "C" int lgmi_bfs_steps(lgmi_graph_t g, int32_t max_steps, lgmi_bfs_step* 
         if (!w) return 0;
         const int n = std::min({max_steps, w->last_levels, w->max_log});
         if (n <= 0) return 0;
+        LGMI_HIP(hipStreamSynchronize(g->stream));   // the search may have returned before its last launches ended
         std::vector<LevelRec> recs((size_t)n);
         LGMI_HIP(hipMemcpy(recs.data(), w->log, sizeof(LevelRec) * (size_t)n, hipMemcpyDeviceToHost));
         int dev = 0, khz = 0;

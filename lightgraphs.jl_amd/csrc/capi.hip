@@ -548,7 +548,8 @@ int lgmi_has_path(lgmi_graph_t g, int32_t u, int32_t v, const int32_t* exclude, 
     }
     bfs_run(g, nullptr, 1, u, 0, 0, -1, nullptr, ex.data(), (int32_t)ex.size());
     int32_t dv = -1;
-    LGMI_HIP(hipMemcpy(&dv, g->bfs->dist + v, sizeof(int32_t), hipMemcpyDeviceToHost));
+    LGMI_HIP(hipMemcpyAsync(&dv, g->bfs->dist + v, sizeof(int32_t), hipMemcpyDeviceToHost, g->stream));
+    LGMI_HIP(hipStreamSynchronize(g->stream));
     *result = dv >= 0 ? 1 : 0;
     LGMI_API_END
 }
