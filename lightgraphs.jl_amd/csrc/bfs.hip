@@ -1684,8 +1684,11 @@ __global__ void bfs_init(StepArgs a, int32_t src, int64_t nnz_scan, int32_t forc
             else if (c == 2) { d.z = 0; p.z = src; }
             else { d.w = 0; p.w = src; }
         }
-        reinterpret_cast<int4*>(a.dist)[i] = d;
-        if (a.parent) reinterpret_cast<int4*>(a.parent)[i] = p;
+        // streaming stores: the whole arrays are written once, nothing
+        // here reads them back
+        typedef int v4i __attribute__((ext_vector_type(4)));
+        __builtin_nontemporal_store(v4i{d.x, d.y, d.z, d.w}, reinterpret_cast<v4i*>(a.dist) + i);
+        if (a.parent) __builtin_nontemporal_store(v4i{p.x, p.y, p.z, p.w}, reinterpret_cast<v4i*>(a.parent) + i);
     }
     for (int64_t i = 4 * n4 + t0; i < a.nv; i += stride) {
         a.dist[i] = i == src ? 0 : -1;
