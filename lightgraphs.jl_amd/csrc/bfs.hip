@@ -1744,6 +1744,73 @@ __global__ __launch_bounds__(256) void bfs_seed(StepArgs a, const int32_t* srcs,
     if (tid == 0) init_state(a, nsrc, -1, s_out, s_in, s_hist, nnz_scan, forced, mode, alpha, beta);
 }
 
+// Level 0 of a single-source search, as launch 0.  Every entry of the
+// source's row is a new vertex at distance 1 whose parent is the source
+// (rows hold no duplicates; a self-loop is skipped), so no vertex needs a
+// claim: plain stores of distance and parent, the F_1 bits, and what launch
+// 1's prologue reads -- each work-group's queue of new vertices and its
+// per-block counters (work-group b of a.grid takes the b-th share of the
+// row) and the state launch 0 stores.  Where an ordinary launch sweeps every
+// bitmap tile to find the source and expands its row with one wavefront,
+// this takes two dependent round trips.  bfs_init has stored S_0, seeded
+// the source and cleared the bitmaps and the distances and parents.
+constexpr int kFirstBlock = 256;
+__global__ __launch_bounds__(kFirstBlock) void bfs_first(StepArgs a, int32_t src) {
+    __shared__ uint32_t s_red[kFirstBlock / 64][6 + kHistBins];
+    __shared__ uint32_t s_qn[1];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    const int32_t nb = a.grid;
+    const int32_t b0 = a.td_rp[src], deg = a.td_rp[src + 1] - b0;
+    const int32_t lo = b0 + (int32_t)((int64_t)deg * blockIdx.x / nb);
+    const int32_t hi = b0 + (int32_t)((int64_t)deg * (blockIdx.x + 1) / nb);
+    uint32_t* Fn = a.bm1;                                      // launch 0 writes bitmap 1
+    int32_t* qout = a.qbuf + (size_t)blockIdx.x * kQCap;       // launch parity 0
+    if (tid == 0) s_qn[0] = 0;
+    __syncthreads();
+    Counters k;
+    for (int32_t base = lo; base < hi; base += kFirstBlock) {  // uniform over the work-group
+        const int32_t e = base + tid;
+        int32_t v = e < hi ? a.td_col[e] : -1;
+        if (v == src) v = -1;
+        k.scan += e < hi ? 1ull : 0ull;
+        int bin = 0;
+        if (v >= 0) {
+            a.dist[v] = 1;
+            if (a.parent) a.parent[v] = (uint32_t)src;
+            atomicOr(&Fn[v >> 5], 1u << (v & 31));
+            bin = account_new(a, v, -1, k);
+        }
+        push_new(v >= 0, v, bin, k, s_qn, qout, lane);
+    }
+    uint32_t r[6] = {(uint32_t)k.n, (uint32_t)k.out, (uint32_t)k.in, (uint32_t)k.scan, (uint32_t)k.cand,
+                     (uint32_t)k.light};
+    #pragma unroll
+    for (int i = 0; i < 6; i++) r[i] = wave_sum(r[i]);
+    if (lane == 0) {
+        #pragma unroll
+        for (int i = 0; i < 6; i++) s_red[wid][i] = r[i];
+        #pragma unroll
+        for (int b = 0; b < kHistBins; b++) s_red[wid][6 + b] = k.hist[b];
+    }
+    __syncthreads();
+    unsigned long long* out = a.part + (size_t)blockIdx.x * kPart;   // launch parity 0
+    if (tid < 6 + kHistBins) {
+        unsigned long long sum = 0;
+        for (int q = 0; q < kFirstBlock / 64; q++) sum += s_red[q][tid];
+        if (tid < 6) out[tid < 5 ? tid : P_LIGHT] = sum;
+        else out[P_HIST + tid - 6] = sum;
+        if (tid == P_NEXT) out[P_QOVF] = sum > (unsigned long long)kQCap ? 1ull : 0ull;
+    }
+    if (blockIdx.x == 0 && tid == 0) {
+        atomicOr(&a.vis[src >> 5], 1u << (src & 31));          // F_0 folded into the visited bitmap
+        BfsCtrl* c = &a.sh->state[0];    // field by field: no private copy of the struct
+        c->phase = 0;
+        c->dir = 0;
+        c->variant = 2;
+        c->t_main = wall_clock64();
+    }
+}
+
 // Undoes one step: every vertex of F_{k+1} (`Fn`) returns to unvisited and
 // Fn is cleared.  Used by the direction probe to run the other direction on
 // the identical state.
@@ -1956,9 +2023,19 @@ void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_
         }
         LGMI_HIP(hipEventRecord(w->step_ev[i], s));
     };
+    // Level 0 of a single source is its own small kernel (launch 0) when the
+    // policy decides the directions; a forced direction (the direction
+    // probe's labels, the tests that force one) and has_path's excluded
+    // vertices keep the ordinary launch.
+    if (single_src >= 0 && forced_dir < 0 && nwalls == 0) {
+        if (w->time_steps) event_at(0);
+        bfs_first<<<a.grid, kFirstBlock, 0, s>>>(a, single_src);
+        if (w->time_steps) event_at(1);
+        launched = 1;
+    }
     int32_t done = 0;
-    for (;;) {
-        if (launched > 0) {
+    for (bool first_batch = true;; first_batch = false) {
+        if (!first_batch) {
             LGMI_HIP(hipEventRecord(w->ev1, s));
             if (poll) {
                 // spin for up to 0.5 ms (a whole RMAT-22 search takes about
@@ -2010,7 +2087,8 @@ void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_
         } else {
             // individual launches; a deep search stops at the next multiple
             // of 6 from which graph replays can start
-            int n = batch;
+            // the first batch counts the level-0 kernel among its launches
+            int n = first_batch ? std::max(1, batch - launched) : batch;
             if (graphs_on && !w->time_steps && launched + n > kGraphLevels)
                 n = std::max(1, (std::max(launched, kGraphLevels) + 5) / 6 * 6 - launched);
             for (int i = 0; i < n; i++) {
