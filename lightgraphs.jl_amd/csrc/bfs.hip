@@ -1327,6 +1327,15 @@ __device__ inline int heuristic_direction(const BfsCtrl& c) {
     return (shrinking && small) ? 0 : 1;
 }
 
+// The network decides contested levels only: after a bottom-up step, or
+// when the frontier's edges exceed 1/64 of the unvisited edges.  Below that
+// a top-down step costs a small fraction of a bottom-up sweep and the
+// alpha/beta rule decides (top-down for any alpha up to 64); skipping the
+// network there saves its features and inference in the prologue.
+__device__ inline bool policy_contested(const BfsCtrl& c) {
+    return c.prev_dir == 1 || (double)c.m_front * 64.0 > (double)c.m_unvisited;
+}
+
 // MLP logits by wavefront 0 (lane j = hidden unit j); returns the
 // direction in every lane.
 __device__ inline int mlp_direction(const float* xs, const PolicyRegs& pr) {
@@ -1354,7 +1363,7 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
     __shared__ unsigned long long s_red[kWaves][kPart];
     __shared__ unsigned long long s_sum[kPart];
     __shared__ float s_feat[kFeat];
-    __shared__ int32_t s_p, s_dir, s_exit, s_dl, s_mode, s_src0, s_solo;
+    __shared__ int32_t s_p, s_dir, s_exit, s_dl, s_mode, s_src0, s_solo, s_learn;
     __shared__ uint32_t s_qn[1];                 // the work-group queue's count
     __shared__ uint32_t s_fn[kWaves][kTileWords];
     __shared__ int32_t s_qoff[kMaxGrid + 1];      // per-work-group new-vertex counts, then queue offsets
@@ -1489,7 +1498,7 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
                         sh->state[p & 1] = c;
                         set_done(a, p, k);
                     }
-                } else if (may_learn || (blockIdx.x == 0 && k < a.max_log)) {
+                } else if ((may_learn && policy_contested(c)) || (blockIdx.x == 0 && k < a.max_log)) {
                     // the features feed the network and work-group 0's level log only
                     compute_features(&c, s_feat);
                 }
@@ -1512,13 +1521,15 @@ __global__ __launch_bounds__(kBlock) void bfs_level(StepArgs a, int32_t karg, in
                               s_c.m_front <= a.solo_edges && s_c.m_front <= 8 * (int64_t)s_c.n_front &&
                               s_c.n_front <= 2 * s_c.n_prev + 64 && 2 * s_c.n_front >= s_c.n_prev;
             s_solo = solo ? 1 : 0;
-            if (!ex && (!may_learn || solo))
+            const bool learn = !ex && may_learn && !solo && policy_contested(s_c);
+            s_learn = learn ? 1 : 0;
+            if (!ex && !learn)
                 s_dir = solo ? 0 : (forced >= 0 ? forced : (first ? s_c.dir : heuristic_direction(s_c)));
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (may_learn && !s_exit && !s_solo) {
+        if (s_learn) {
             const int d = mlp_direction(s_feat, pr);
             if (lane == 0) s_dir = d;
         }
