@@ -603,7 +603,8 @@ def profile_pass(lgmi, _lib, nbfs, levels_out=""):
 
 
 def direction_summary(trace_csv, levels_json):
-    """Per-direction table of the bfs_level dispatches of a profiled
+    """Per-direction table of the level-kernel dispatches (bfs_first for
+    level 0 of a single source, then bfs_level) of a profiled
     --profile-pass run: each dispatch of rocprofv3's kernel trace (in
     start order) is labelled from the level records of the same run (the
     launches of one search are consecutive; a launch that ran no level is
@@ -611,7 +612,8 @@ def direction_summary(trace_csv, levels_json):
     duration and the algorithmic bytes of its levels are summed per label."""
     import csv
     from types import SimpleNamespace
-    rows = [r for r in csv.DictReader(open(trace_csv)) if "bfs_level" in r["Kernel_Name"]]
+    rows = [r for r in csv.DictReader(open(trace_csv)) if "bfs_level" in r["Kernel_Name"] or
+            "bfs_first" in r["Kernel_Name"]]
     rows.sort(key=lambda r: int(r["Start_Timestamp"]))
     dur = [(int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3 for r in rows]   # us
     lv = json.load(open(levels_json))
@@ -623,7 +625,9 @@ def direction_summary(trace_csv, levels_json):
             by_launch.setdefault(rec["launch"], []).append(rec)
         for li in range(run["launches"]):
             recs = by_launch.get(li, [])
-            if not recs:
+            if li == 0 and recs:      # single-source searches: launch 0 is bfs_first
+                lab = "level 0 (bfs_first)"
+            elif not recs:
                 lab = "after the last level"
             elif len(recs) > 1 or recs[0]["variant"] == 3:
                 lab = "top-down, solo"
