@@ -347,6 +347,7 @@ def bc_bench(args, lgmi, _lib, orc):
         "forward_ms": round(stt["forward_ms"], 2),
         "backward_ms": round(stt["backward_ms"], 2),
         "launch": launch,
+        "persistent_work_groups": {"launched": int(stats[-1]["blocks"]), "participants": int(stats[-1]["participants"])},
         "roofline": {"bound": "hbm", "kernel": "bc_persist" if stats[-1]["launch"] else "bc_forward/bc_backward",
                      "achieved": round(bc_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(bc_gbs / HBM_PEAK_GBS, 4),

@@ -90,10 +90,14 @@ typedef struct {
     int32_t launch;            /* 2: one persistent launch replayed from a   */
                                /* captured hipGraph, 1: one persistent       */
                                /* launch, 0: one launch per level           */
-    int32_t pad;
+    int32_t participants;      /* persistent launch: work-groups that took   */
+                               /* part (those resident together at its       */
+                               /* start); 0 otherwise                        */
     double device_ms;          /* HIP-event time of the whole computation    */
     double forward_ms;         /* sigma passes                                */
     double backward_ms;        /* delta passes                                */
+    int32_t blocks;            /* persistent launch: work-groups launched    */
+    int32_t pad;
 } lgmi_bc_stats;
 
 /* ----------------------------------------------------------------------

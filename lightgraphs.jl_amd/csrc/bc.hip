@@ -1254,7 +1254,8 @@ void bc_set_launch(int mode) { g_launch_mode = mode; }
 // slot ran past the level capacity); the caller then runs the level-per-launch
 // path, which overwrites everything this one wrote.
 static bool bc_persistent(lgmi_graph* g, BcArgs a, int32_t ns, const double* endp, int do_scale, double scale,
-                          int mode, int32_t* launch_kind, float* fwd_ms, float* bwd_ms, int32_t* max_levels) {
+                          int mode, int32_t* launch_kind, float* fwd_ms, float* bwd_ms, int32_t* max_levels,
+                          int32_t* participants, int32_t* blocks) {
     const void* kfn = mode == 2 ? reinterpret_cast<const void*>(bc_persist<2>)
                       : mode == 1 ? reinterpret_cast<const void*>(bc_persist<1>)
                                   : reinterpret_cast<const void*>(bc_persist<0>);
@@ -1338,6 +1339,8 @@ static bool bc_persistent(lgmi_graph* g, BcArgs a, int32_t ns, const double* end
         return false;
     }
     const double tick_ms = 1.0 / (double)khz;
+    *participants = c.closed > 0 ? (int32_t)c.closed - 1 : 0;
+    *blocks = grid;
     *fwd_ms = (float)((double)(c.t_bwd - c.t_fwd) * tick_ms);
     *bwd_ms = (float)((double)(c.t_end - c.t_bwd) * tick_ms);
     int32_t lm = 0;
@@ -1534,11 +1537,14 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
     float fwd_ms = 0.f, bwd_ms = 0.f;
     int32_t max_levels = 0;
     int32_t launch_kind = 0;
+    int32_t participants = 0, blocks = 0;
     const int64_t max_items = (int64_t)ngroups * ((int64_t)seg_stride + a.nchunks);
     if (g_launch_mode == 0 && ngroups >= 1 && ngroups <= nslots && nv > 0 && max_items < ((int64_t)1 << 31)) {
         fill_groups(0, ngroups);
-        if (!bc_persistent(g, a, ngroups, endp, do_scale, scale, mode, &launch_kind, &fwd_ms, &bwd_ms, &max_levels)) {
+        if (!bc_persistent(g, a, ngroups, endp, do_scale, scale, mode, &launch_kind, &fwd_ms, &bwd_ms, &max_levels,
+                           &participants, &blocks)) {
             launch_kind = 0;
+            participants = blocks = 0;
             fwd_ms = bwd_ms = 0.f;
             max_levels = 0;
         } else if (mode == 2) {
@@ -1644,6 +1650,9 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
         stats->groups = ngroups;
         stats->max_levels = max_levels;
         stats->launch = launch_kind;
+        stats->participants = participants;
+        stats->blocks = blocks;
+        stats->pad = 0;
         stats->device_ms = tot;
         stats->forward_ms = fwd_ms;
         stats->backward_ms = bwd_ms;

@@ -80,8 +80,9 @@ class ProbeRec(ctypes.Structure):
 
 class BcStats(ctypes.Structure):
     _fields_ = [("groups", ctypes.c_int32), ("max_levels", ctypes.c_int32), ("launch", ctypes.c_int32),
-                ("pad", ctypes.c_int32), ("device_ms", ctypes.c_double),
-                ("forward_ms", ctypes.c_double), ("backward_ms", ctypes.c_double)]
+                ("participants", ctypes.c_int32), ("device_ms", ctypes.c_double),
+                ("forward_ms", ctypes.c_double), ("backward_ms", ctypes.c_double), ("blocks", ctypes.c_int32),
+                ("pad", ctypes.c_int32)]
 
 
 class Tuning(ctypes.Structure):

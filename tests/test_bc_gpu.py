@@ -279,6 +279,12 @@ def test_persistent_launch_matches_level_launches(oracle):
                     got = lgmi.betweenness_centrality(g, vs, normalize=False, endpoints=endpoints)
                     st = C.last_stats(g)
                     assert st["launch"] in (1, 2), st
+                    # the work-groups that took part: resident together,
+                    # at least half of those launched (6 of the 7 per CU the
+                    # occupancy query reports fit), and a multiple that the
+                    # lane groups divide evenly (4 wavefronts each)
+                    assert st["blocks"] // 2 <= st["participants"] <= st["blocks"], st
+                    assert (st["participants"] * 4) % st["groups"] == 0, st
                     assert (_bits(got) == _bits(ref)).all()
         # a replay of the same call comes from the captured graph when the
         # runtime captured it, and gives the same bits
