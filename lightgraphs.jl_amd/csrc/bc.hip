@@ -15,8 +15,14 @@
 // shortest-path counts.  Here 64 sources advance together: for every vertex
 // a uint64 holds one bit per source lane (seen[v]; F_L[v] = lanes for which
 // v is at distance L).  A wavefront takes one vertex at a time and its 64
-// lanes are the 64 sources, so the FP64 rows sig[v][0..63] are read and
-// written as 512-byte coalesced rows.
+// lanes are the 64 sources; sig[v][0..63] holds sigma by lane.
+// The gathers read a neighbour's values for the lanes of one level only, so
+// those are kept a second time, rank-packed: stage[L & 1][v][r] holds the
+// value of the r-th lane (in lane order) of F_L[v].  The lanes one hit needs
+// then sit in the first popcount(F_L[v]) entries of a 512-byte row instead
+// of spread over the whole row (about 1.6 instead of 2.5 lines per hit on
+// RMAT graphs with hub-ordered sources).  Two planes suffice: a step reads
+// the level before (forward) or after (backward) and writes its own.
 //
 // Forward (level L -> L+1, pull over in-neighbours): for every vertex with
 // an unseen lane, OR the frontier masks of its in-neighbours and, per lane,
@@ -81,6 +87,7 @@ struct BcWork {
     uint64_t* seen = nullptr;    // [slot][nv]
     uint64_t* F = nullptr;       // [slot][cap][nv]
     double* sig = nullptr;       // [slot][nv][64]
+    double* stage = nullptr;     // [slot][2][nv][64] rank-packed values of one level
     double* bc_acc = nullptr;    // [slot][nv]
     double* bc_out = nullptr;    // [nv]
     double* endp = nullptr;      // [nv] endpoints term
@@ -153,6 +160,7 @@ struct BcArgs {
     uint64_t* seen;
     uint64_t* F;
     double* sig;
+    double* stage;           // [slot][2][nv][64]: plane L & 1 holds level L rank-packed
     double* bc_acc;
     const uint64_t* active;
     BcCtrl* ctrl;
@@ -180,36 +188,49 @@ struct BcArgs {
 template <int MODE> struct BwdAcc { using T = double; };
 template <> struct BwdAcc<1> { using T = uint64_t; };
 
+// Position of this lane among the set bits of the wave-uniform mask m below
+// it (the rank-packed index of the lane's entry).
+__device__ inline int lane_rank(uint64_t m) {
+    return (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ inline uint64_t readlane_u64(uint64_t x, int t) {
+    return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(x >> 32), t) << 32) |
+           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)x, t);
+}
+
 // Up to eight of the hits in `hits` (lanes whose adjacency entry uk[t] has
-// a non-empty lane mask mk[t]): lane i adds to acc entry i of the 64-entry
-// rows (of type T) of the hits whose mask has bit i -- or 1 per such hit
-// when LOAD is false -- one after another in ascending hit order (the order
-// of the row), with the eight loads in flight together.  Removes the hits it
-// took and ORs their masks into `got`.  Every lane calls it with the same
-// `hits`.
+// a non-empty lane mask mk[t] = fk[t] & wanted lanes, fk[t] being the
+// neighbour's lanes on the level read): lane i adds, for every hit whose
+// mask has bit i, the neighbour's value for lane i -- entry rank(i in fk[t])
+// of its rank-packed row in `stage` -- or 1 when LOAD is false, one after
+// another in ascending hit order (the order of the row), with the eight
+// loads in flight together.  Removes the hits it took and ORs their masks
+// into `got`.  Every lane calls it with the same `hits`.
 template <typename T, bool LOAD>
-__device__ inline void gather_hits(uint64_t& hits, int32_t uk, uint64_t mk, const T* sig, int lane,
+__device__ inline void gather_hits(uint64_t& hits, int32_t uk, uint64_t mk, uint64_t fk, const T* stage, int lane,
                                    T& acc, uint64_t& got) {
     constexpr int kH = 8;
     int32_t uu[kH];
-    uint64_t mm[kH];
+    uint64_t mm[kH], ff[kH];
     #pragma unroll
     for (int q = 0; q < kH; q++) {
         uu[q] = 0;
         mm[q] = 0;
+        ff[q] = 0;
         if (hits) {
             // t is the same in every lane, so the reads are scalar
             const int t = __builtin_amdgcn_readfirstlane(__ffsll((unsigned long long)hits) - 1);
             hits &= hits - 1;
             uu[q] = __builtin_amdgcn_readlane(uk, t);
-            mm[q] = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(mk >> 32), t) << 32) |
-                    (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)mk, t);
+            mm[q] = readlane_u64(mk, t);
+            if constexpr (LOAD) ff[q] = readlane_u64(fk, t);
         }
     }
     T x[kH];
     #pragma unroll
     for (int q = 0; q < kH; q++) {
-        if constexpr (LOAD) x[q] = ((mm[q] >> lane) & 1) ? sig[(size_t)uu[q] * 64 + lane] : T(0);
+        if constexpr (LOAD) x[q] = ((mm[q] >> lane) & 1) ? stage[(size_t)uu[q] * 64 + lane_rank(ff[q])] : T(0);
         else x[q] = ((mm[q] >> lane) & 1) ? T(1) : T(0);
     }
     #pragma unroll
@@ -289,6 +310,12 @@ __device__ inline size_t fidx(const BcArgs& a, int slot, int32_t L, int32_t v) {
     return ((size_t)lev * a.nv + v) * a.fstride + slot;
 }
 
+// Plane of the rank-packed values of level L of one slot ([nv][64]).
+template <typename T = double>
+__device__ inline T* stage_plane(const BcArgs& a, int slot, int32_t L) {
+    return reinterpret_cast<T*>(a.stage) + ((size_t)slot * 2 + (size_t)(L & 1)) * (size_t)a.nv * 64;
+}
+
 __device__ inline uint64_t wave_or_u64(uint64_t m) {
     for (int o = 32; o >= 1; o >>= 1) m |= __shfl_xor(m, o);
     return m;
@@ -311,6 +338,11 @@ __device__ bool fwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
     const uint64_t act = a.active[slot];
     uint64_t* seen = a.seen + (size_t)slot * nv;
     double* sig = MODE == 2 ? nullptr : a.sig + (size_t)slot * nv * 64;
+    // sigma on levels 0 and 1 is 1 in every lane (one path from the source),
+    // so those steps count the hits; later ones read the level's plane
+    const bool ld = MODE == 0 && L >= 2;
+    const double* rd = ld ? stage_plane(a, slot, L) : nullptr;
+    double* wr = MODE == 0 && L >= 1 ? stage_plane(a, slot, L + 1) : nullptr;
     const int32_t vj = ch * 64 + lane;
     uint64_t sj = 0;
     int32_t bj = 0, ej = 0;
@@ -335,18 +367,20 @@ __device__ bool fwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
         uint64_t got = 0;
         for (int32_t off = b; off < e; off += 64) {
             const int32_t k = off + lane;
-            uint64_t mk = 0;
+            uint64_t mk = 0, fk = 0;
             int32_t uk = 0;
             if (k < e) {
                 uk = a.in_col[k];
-                mk = a.F[fidx(a, slot, L, uk)] & unseen;
+                fk = a.F[fidx(a, slot, L, uk)];
+                mk = fk & unseen;
             }
             if constexpr (MODE == 2) {
                 got |= wave_or_u64(mk);
                 if (got == unseen) break;    // every unseen lane found
             } else {
                 uint64_t hits = __ballot(mk != 0);
-                while (hits) gather_hits<double, MODE == 0>(hits, uk, mk, sig, lane, acc, got);
+                if (ld) while (hits) gather_hits<double, true>(hits, uk, mk, fk, rd, lane, acc, got);
+                else while (hits) gather_hits<double, false>(hits, uk, mk, fk, rd, lane, acc, got);
             }
         }
         if constexpr (MODE == 2) {
@@ -356,7 +390,10 @@ __device__ bool fwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
                 st_e = (uint64_t)(L + 1);
             }
         } else {
-            if ((got >> lane) & 1) sig[(size_t)v * 64 + lane] = acc;
+            if ((got >> lane) & 1) {
+                sig[(size_t)v * 64 + lane] = acc;
+                if (wr) wr[(size_t)v * 64 + lane_rank(got)] = acc;
+            }
         }
         if (lane == j) newF = got;
     }
@@ -388,7 +425,8 @@ __device__ bool fwd_seg(const BcArgs& a, int slot, int32_t sg, int32_t L, int la
     const int32_t nv = a.nv;
     const int32_t v = a.hin.seg_v[sg];
     const uint64_t act = a.active[slot];
-    const double* sig = MODE == 2 ? nullptr : a.sig + (size_t)slot * nv * 64;
+    const bool ld = MODE == 0 && L >= 2;   // as in fwd_chunk
+    const double* rd = ld ? stage_plane(a, slot, L) : nullptr;
     const uint64_t unseen = ~a.seen[(size_t)slot * nv + v] & act;
     double acc = 0.0;
     uint64_t got = 0;
@@ -396,18 +434,20 @@ __device__ bool fwd_seg(const BcArgs& a, int slot, int32_t sg, int32_t L, int la
         const int32_t b = a.hin.seg_b[sg], e = a.hin.seg_e[sg];
         for (int32_t off = b; off < e; off += 64) {
             const int32_t k = off + lane;
-            uint64_t mk = 0;
+            uint64_t mk = 0, fk = 0;
             int32_t uk = 0;
             if (k < e) {
                 uk = a.in_col[k];
-                mk = a.F[fidx(a, slot, L, uk)] & unseen;
+                fk = a.F[fidx(a, slot, L, uk)];
+                mk = fk & unseen;
             }
             if constexpr (MODE == 2) {
                 got |= wave_or_u64(mk);
                 if (got == unseen) break;
             } else {
                 uint64_t hits = __ballot(mk != 0);
-                while (hits) gather_hits<double, MODE == 0>(hits, uk, mk, sig, lane, acc, got);
+                if (ld) while (hits) gather_hits<double, true>(hits, uk, mk, fk, rd, lane, acc, got);
+                else while (hits) gather_hits<double, false>(hits, uk, mk, fk, rd, lane, acc, got);
             }
         }
     }
@@ -442,7 +482,10 @@ __device__ void fwd_combine(const BcArgs& a, int slot, int32_t h, int32_t L, int
             tot += part[(size_t)q * 64 + lane];
             gm |= pgot[q];
         }
-        if ((gm >> lane) & 1) a.sig[((size_t)slot * nv + v) * 64 + lane] = tot;
+        if ((gm >> lane) & 1) {
+            a.sig[((size_t)slot * nv + v) * 64 + lane] = tot;
+            if (MODE == 0 && L >= 1) stage_plane(a, slot, L + 1)[(size_t)v * 64 + lane_rank(gm)] = tot;
+        }
     }
     if (lane == 0) {
         *Fn = gm;
@@ -524,22 +567,22 @@ __global__ __launch_bounds__(kBlock) void bc_forward_fin(BcArgs a) {
 // its successors' coefficients, the vertex's coefficient (1 + delta) / sigma
 // into its sigma slot, and the lane sum of delta into the slot's partial
 // betweenness.
-// One lane of a vertex on level L once its successor sum acc is complete:
-// betweenness turns sigma into the coefficient (1 + delta) / sigma and
+// One lane of a vertex on level L once its successor sum acc is complete
+// (sv: the lane's entry of sig): betweenness writes the coefficient
+// (1 + delta) / sigma to *outp (the lane's entry of the level's plane) and
 // returns delta = sigma * acc; stress (_stress_accumulate_basic!,
 // stress.jl:55-77) returns acc times the predecessor count the forward pass
-// left in the slot, and leaves acc + 1 there for the vertex's predecessors.
+// left in sig, and writes acc + 1 for the vertex's predecessors.
 template <int MODE>
-__device__ inline typename BwdAcc<MODE>::T bwd_finish_lane(double* slotp, typename BwdAcc<MODE>::T acc) {
+__device__ inline typename BwdAcc<MODE>::T bwd_finish_lane(double sv, typename BwdAcc<MODE>::T* outp,
+                                                           typename BwdAcc<MODE>::T acc) {
     if constexpr (MODE == 0) {
-        const double sv = *slotp;
         const double d = sv * acc;
-        *slotp = (1.0 + d) / sv;
+        *outp = (1.0 + d) / sv;
         return d;
     } else {
-        const uint64_t npred = (uint64_t)*slotp;
-        *reinterpret_cast<uint64_t*>(slotp) = acc + 1;
-        return acc * npred;
+        *outp = acc + 1;
+        return acc * (uint64_t)sv;
     }
 }
 
@@ -553,7 +596,9 @@ template <int MODE>
 __device__ void bwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int lane) {
     using T = typename BwdAcc<MODE>::T;
     const int32_t nv = a.nv;
-    double* sig = a.sig + (size_t)slot * nv * 64;
+    const double* sig = a.sig + (size_t)slot * nv * 64;
+    const T* rd = stage_plane<T>(a, slot, L + 1);
+    T* wr = stage_plane<T>(a, slot, L);
     T* acc_bc = reinterpret_cast<T*>(a.bc_acc) + (size_t)slot * nv;
     const int32_t vj = ch * 64 + lane;
     uint64_t Aj = 0;
@@ -577,18 +622,20 @@ __device__ void bwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
         T acc = 0;
         for (int32_t off = b; off < e; off += 64) {
             const int32_t kk = off + lane;
-            uint64_t mk = 0;
+            uint64_t mk = 0, fk = 0;
             int32_t wk = 0;
             if (kk < e) {
                 wk = a.out_col[kk];
-                mk = a.F[fidx(a, slot, L + 1, wk)] & A;
+                fk = a.F[fidx(a, slot, L + 1, wk)];
+                mk = fk & A;
             }
             uint64_t hits = __ballot(mk != 0);
             uint64_t unused = 0;
-            while (hits) gather_hits<T, true>(hits, wk, mk, reinterpret_cast<const T*>(sig), lane, acc, unused);
+            while (hits) gather_hits<T, true>(hits, wk, mk, fk, rd, lane, acc, unused);
         }
         T contrib = 0;
-        if ((A >> lane) & 1) contrib = bwd_finish_lane<MODE>(&sig[(size_t)v * 64 + lane], acc);
+        if ((A >> lane) & 1)
+            contrib = bwd_finish_lane<MODE>(sig[(size_t)v * 64 + lane], &wr[(size_t)v * 64 + lane_rank(A)], acc);
         contrib = lane_sum(contrib);
         if (lane == j) bcj = contrib;
     }
@@ -605,20 +652,22 @@ __device__ void bwd_seg(const BcArgs& a, int slot, int32_t sg, int32_t L, int la
     const int32_t v = a.hout.seg_v[sg];
     const uint64_t A = a.F[fidx(a, slot, L, v)];
     if (!A) return;
-    const T* sig = reinterpret_cast<const T*>(a.sig) + (size_t)slot * nv * 64;
+    (void)nv;
+    const T* rd = stage_plane<T>(a, slot, L + 1);
     const int32_t b = a.hout.seg_b[sg], e = a.hout.seg_e[sg];
     T acc = 0;
     for (int32_t off = b; off < e; off += 64) {
         const int32_t kk = off + lane;
-        uint64_t mk = 0;
+        uint64_t mk = 0, fk = 0;
         int32_t wk = 0;
         if (kk < e) {
             wk = a.out_col[kk];
-            mk = a.F[fidx(a, slot, L + 1, wk)] & A;
+            fk = a.F[fidx(a, slot, L + 1, wk)];
+            mk = fk & A;
         }
         uint64_t hits = __ballot(mk != 0);
         uint64_t unused = 0;
-        while (hits) gather_hits<T, true>(hits, wk, mk, sig, lane, acc, unused);
+        while (hits) gather_hits<T, true>(hits, wk, mk, fk, rd, lane, acc, unused);
     }
     reinterpret_cast<T*>(a.hpart)[((size_t)slot * a.seg_stride + sg) * 64 + lane] = acc;
 }
@@ -636,9 +685,11 @@ __device__ void bwd_combine(const BcArgs& a, int slot, int32_t h, int32_t L, int
     const T* part = reinterpret_cast<const T*>(a.hpart) + ((size_t)slot * a.seg_stride + f0) * 64;
     T tot = 0;
     for (int32_t q = 0; q < nsg; q++) tot += part[(size_t)q * 64 + lane];
-    double* sig = a.sig + (size_t)slot * nv * 64;
+    const double* sig = a.sig + (size_t)slot * nv * 64;
     T contrib = 0;
-    if ((A >> lane) & 1) contrib = bwd_finish_lane<MODE>(&sig[(size_t)v * 64 + lane], tot);
+    if ((A >> lane) & 1)
+        contrib = bwd_finish_lane<MODE>(sig[(size_t)v * 64 + lane], &stage_plane<T>(a, slot, L)[(size_t)v * 64 + lane_rank(A)],
+                                        tot);
     contrib = lane_sum(contrib);
     if (lane == 0) reinterpret_cast<T*>(a.bc_acc)[(size_t)slot * nv + v] += contrib;
 }
@@ -1088,7 +1139,7 @@ __global__ void bc_finish(const double* acc, int32_t nslots, int32_t nv, const d
 
 void bc_free(BcWork* w) {
     if (!w) return;
-    dev_free(w->seen); dev_free(w->F); dev_free(w->sig); dev_free(w->bc_acc); dev_free(w->bc_out);
+    dev_free(w->seen); dev_free(w->F); dev_free(w->sig); dev_free(w->stage); dev_free(w->bc_acc); dev_free(w->bc_out);
     dev_free(w->endp); dev_free(w->active); dev_free(w->lane_src); dev_free(w->ctrl);
     if (w->h_ctrl) (void)hipHostFree(w->h_ctrl);
     for (auto& e : w->ev) if (e) (void)hipEventDestroy(e);
@@ -1123,6 +1174,7 @@ static void bc_alloc(lgmi_graph* g, int32_t nslots, int32_t cap, int32_t seg_str
         w->F = dev_alloc_n<uint64_t>(nv * nslots * (size_t)cap);
         if (need_sig) {
             w->sig = dev_alloc_n<double>(nv * nslots * 64);
+            w->stage = dev_alloc_n<double>(nv * nslots * 2 * 64);
             w->bc_acc = dev_alloc_n<double>(nv * nslots);
             w->has_sig = true;
         }
@@ -1431,7 +1483,7 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
     const size_t per_slot = mode == 2
         ? (size_t)std::max(nv, 1) * (8 + 8 * (size_t)cap0 + 8) + (size_t)seg_stride * 8 +
               ((size_t)nchunks + bg->nh_in) * 192 * 8
-        : (size_t)std::max(nv, 1) * (8 + 8 * (size_t)cap0 + 512 + 8) + (size_t)seg_stride * 520;
+        : (size_t)std::max(nv, 1) * (8 + 8 * (size_t)cap0 + 3 * 512 + 8) + (size_t)seg_stride * 520;
     size_t budget = free_device_bytes() / 3;
     if (budget == 0) budget = (size_t)4 << 30;
     int32_t nslots = (int32_t)std::max<size_t>(1, std::min<size_t>((size_t)std::max(ngroups, 1), budget / per_slot));
@@ -1458,7 +1510,7 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
     a.out_rp = g->out.row_ptr; a.out_col = g->out.col;
     a.nv = nv;
     a.nchunks = (nv + 63) / 64;
-    a.seen = w->seen; a.F = w->F; a.sig = w->sig; a.bc_acc = w->bc_acc;
+    a.seen = w->seen; a.F = w->F; a.sig = w->sig; a.stage = w->stage; a.bc_acc = w->bc_acc;
     a.active = w->active; a.ctrl = w->ctrl; a.cap = w->cap_levels;
     a.hin = make_segs(bg->seg_in, bg->nseg_in, bg->nh_in);
     a.hout = make_segs(bg->seg_out, bg->nseg_out, bg->nh_out);
@@ -1468,7 +1520,7 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
     a.fmod = mode == 2 ? 2 : w->cap_levels;
     a.fstride = w->nslots;
     a.st = w->st;
-    if (mode == 2) { a.sig = nullptr; a.bc_acc = nullptr; }
+    if (mode == 2) { a.sig = nullptr; a.stage = nullptr; a.bc_acc = nullptr; }
 
     const int cu = std::max(1, g->cu_count);
     const int target_blocks = cu * 8;
