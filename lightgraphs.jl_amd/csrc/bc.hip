@@ -1484,7 +1484,9 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
         ? (size_t)std::max(nv, 1) * (8 + 8 * (size_t)cap0 + 8) + (size_t)seg_stride * 8 +
               ((size_t)nchunks + bg->nh_in) * 192 * 8
         : (size_t)std::max(nv, 1) * (8 + 8 * (size_t)cap0 + 3 * 512 + 8) + (size_t)seg_stride * 520;
-    size_t budget = free_device_bytes() / 3;
+    // half the free memory: 16 slots of RMAT-22 (7.6 GB each with the
+    // stage planes) then fit one persistent launch
+    size_t budget = free_device_bytes() / 2;
     if (budget == 0) budget = (size_t)4 << 30;
     int32_t nslots = (int32_t)std::max<size_t>(1, std::min<size_t>((size_t)std::max(ngroups, 1), budget / per_slot));
     nslots = std::min(nslots, kMaxSlotsP);

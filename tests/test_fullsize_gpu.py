@@ -103,3 +103,27 @@ def test_betweenness_rmat20_1024_distance_identity(oracle):
     want = 0.5 * n / 1024 * inner
     got = float(np.sum(bc))
     assert abs(got - want) <= 1e-12 * abs(want), (got, want)
+
+
+def test_betweenness_rmat22_1024_one_launch_identity(oracle, rmat22):
+    """Exact betweenness on RMAT-22 with 1024 sources: the 16 lane groups
+    fit one persistent launch (the state is about 7.6 GB per group), and the
+    result meets the same dependency-sum identity as on RMAT-20."""
+    import lgmi
+    from lgmi.distances import distance_stats
+    g = rmat22
+    rp, col = g.csr()
+    n = g.nv()
+    srcs = _bench_roots(np.diff(rp), 1024, 4242)
+    say("betweenness on rmat22, 1024 sources, normalize=false")
+    bc = lgmi.betweenness_centrality(g, srcs + 1, normalize=False)
+    assert lgmi.centrality.last_stats(g)["launch"] == 2, "expected one persistent, graph-captured launch"
+    st = distance_stats(g, srcs + 1)
+    for i in (0, 511, 1023):
+        d_or, _, _ = oracle.bfs_dist_parent(rp, col, int(srcs[i]))
+        reach = d_or >= 0
+        assert int(st[i, 0]) == int(d_or[reach].sum()) and int(st[i, 1]) == int(reach.sum())
+    inner = int(np.sum(st[:, 0] - st[:, 1] + 1))
+    want = 0.5 * n / 1024 * inner
+    got = float(np.sum(bc))
+    assert abs(got - want) <= 1e-12 * abs(want), (got, want)
