@@ -248,7 +248,9 @@ int lgmi_bfs_set_policy(int mode, float alpha, float beta, const float* mlp, int
  * positions findall returns (Parallel/centrality/betweenness.jl:53-56).
  * k_scale is the k of _rescale! (the reference's length(vs)).  Results
  * match the reference within 1e-12 relative error (only the floating-point
- * summation order differs). */
+ * summation order differs).  LGMI_E_TOO_LARGE when the search state of one
+ * 64-source lane group (about 1.8 KB per vertex) does not fit in the free
+ * device memory. */
 int lgmi_betweenness(lgmi_graph_t g, const int32_t* srcs, int32_t nsrc, int64_t k_scale,
                      int skip_degree0, int normalize, int endpoints, double* bc);
 /* Replaces, for unweighted graphs:

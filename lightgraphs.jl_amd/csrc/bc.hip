@@ -1486,8 +1486,14 @@ void bc_run(lgmi_graph* g, const int32_t* srcs, int32_t nsrc, int64_t k_scale, i
         : (size_t)std::max(nv, 1) * (8 + 8 * (size_t)cap0 + 3 * 512 + 8) + (size_t)seg_stride * 520;
     // half the free memory: 16 slots of RMAT-22 (7.6 GB each with the
     // stage planes) then fit one persistent launch
-    size_t budget = free_device_bytes() / 2;
+    const size_t free_b = free_device_bytes();
+    size_t budget = free_b / 2;
     if (budget == 0) budget = (size_t)4 << 30;
+    // a graph whose single lane group does not fit is refused as too large
+    // (the Julia shim then runs the reference method)
+    if (free_b > 0 && per_slot > free_b - free_b / 8)
+        throw Error(LGMI_E_TOO_LARGE, "the state of one 64-source lane group (" + std::to_string(per_slot) +
+                                          " bytes) exceeds the free device memory");
     int32_t nslots = (int32_t)std::max<size_t>(1, std::min<size_t>((size_t)std::max(ngroups, 1), budget / per_slot));
     nslots = std::min(nslots, kMaxSlotsP);
     bc_alloc(g, nslots, cap0, seg_stride, mode != 2);
