@@ -166,8 +166,25 @@ def autotune(log=print) -> dict:
                 best, best_score, best_per = trial, sc, per
         log(f"{field} -> {best[field]} (score {best_score:.4f})")
     set_values(best)
-    # betweenness segment length: fresh graph handles, so that each
-    # candidate builds its own segments
+    bc_best, bc_hist = tune_bc_segments(best["bc_seg_len"], log)
+    best["bc_seg_len"] = bc_best
+    set_values(best)
+    for name, g, roots in work:
+        g._release()
+    return {"best": best, "score": round(best_score, 4), "defaults_ms": {k: round(v, 3) for k, v in base.items()},
+            "best_ms": {k: round(v, 3) for k, v in best_per.items()}, "bfs_candidates": history,
+            "bc_candidates": bc_hist,
+            "workloads": {"rmat22": "symmetrised RMAT-22, 64 seeded roots", "grid2048": "2048 x 2048 grid, 2 roots",
+                          "ba2M": "barabasi_albert(2_000_000, 8), 16 roots",
+                          "bc": "RMAT-20 betweenness, 1024 seeded sources"}}
+
+
+def tune_bc_segments(current: int, log=print):
+    """The betweenness segment length: each candidate on a fresh graph
+    handle (so that it builds its own segments) of the betweenness workload
+    (RMAT-20, 1024 seeded sources), the median of three device times.
+    Returns the choice (a candidate must beat the best so far by 1 %) and
+    the candidates' times."""
     from . import generators as gen
     from .centrality import betweenness_centrality, last_stats
     from .graphs import SimpleGraph
@@ -175,7 +192,7 @@ def autotune(log=print) -> dict:
     rng = np.random.default_rng(4242)
     srcs = rng.choice(np.nonzero(np.diff(g20.csr()[0]) > 0)[0], size=1024, replace=False) + 1
     bc_hist = []
-    bc_best, bc_best_ms = best["bc_seg_len"], None
+    bc_best, bc_best_ms = current, None
     for L in BC_CANDIDATES:
         set_values({"bc_seg_len": L})
         g = SimpleGraph.from_csr(*g20.csr())
@@ -190,16 +207,8 @@ def autotune(log=print) -> dict:
         log(f"bc_seg_len={L}: {m:.2f} ms")
         if bc_best_ms is None or m < bc_best_ms * 0.99:
             bc_best, bc_best_ms = L, m
-    best["bc_seg_len"] = bc_best
-    set_values(best)
-    for name, g, roots in work:
-        g._release()
-    return {"best": best, "score": round(best_score, 4), "defaults_ms": {k: round(v, 3) for k, v in base.items()},
-            "best_ms": {k: round(v, 3) for k, v in best_per.items()}, "bfs_candidates": history,
-            "bc_candidates": bc_hist,
-            "workloads": {"rmat22": "symmetrised RMAT-22, 64 seeded roots", "grid2048": "2048 x 2048 grid, 2 roots",
-                          "ba2M": "barabasi_albert(2_000_000, 8), 16 roots",
-                          "bc": "RMAT-20 betweenness, 1024 seeded sources"}}
+    set_values({"bc_seg_len": current})
+    return bc_best, bc_hist
 
 
 def save(result: dict, path: str = TUNING_PATH) -> None:
@@ -212,7 +221,16 @@ if __name__ == "__main__":
     import sys
     ap = argparse.ArgumentParser(description="autotune the BFS and betweenness launch thresholds")
     ap.add_argument("--out", default=TUNING_PATH)
+    ap.add_argument("--bc-only", action="store_true",
+                    help="time the betweenness segment lengths with the other thresholds from tuning.json; "
+                         "print the candidates, write nothing")
     a = ap.parse_args()
-    res = autotune(log=lambda m: print(m, file=sys.stderr, flush=True))
-    save(res, a.out)
-    print(json.dumps({"best": res["best"], "score": res["score"]}))
+    say = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    if a.bc_only:
+        load()
+        choice, hist = tune_bc_segments(get()["bc_seg_len"], say)
+        print(json.dumps({"bc_seg_len": choice, "bc_candidates": hist}))
+    else:
+        res = autotune(log=say)
+        save(res, a.out)
+        print(json.dumps({"best": res["best"], "score": res["score"]}))
