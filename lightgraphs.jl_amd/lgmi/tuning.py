@@ -182,27 +182,32 @@ def autotune(log=print) -> dict:
 def tune_bc_segments(current: int, log=print):
     """The betweenness segment length: each candidate on a fresh graph
     handle (so that it builds its own segments) of the betweenness workload
-    (RMAT-20, 1024 seeded sources), the median of three device times.
-    Returns the choice (a candidate must beat the best so far by 1 %) and
-    the candidates' times."""
+    (RMAT-20, 1024 seeded sources), three timed runs after a warm-up, in two
+    passes over the candidates in opposite orders (a single pass in one
+    order let the GPU's warming over the run decide between close
+    candidates); a candidate's time is the median of its six runs.  Returns
+    the choice (a candidate must beat the best so far by 1 %) and the
+    candidates' times."""
     from . import generators as gen
     from .centrality import betweenness_centrality, last_stats
     from .graphs import SimpleGraph
     g20 = gen.rmat(20, 16, seed=1)
     rng = np.random.default_rng(4242)
     srcs = rng.choice(np.nonzero(np.diff(g20.csr()[0]) > 0)[0], size=1024, replace=False) + 1
+    runs = {L: [] for L in BC_CANDIDATES}
+    for order in (list(BC_CANDIDATES), list(reversed(BC_CANDIDATES))):
+        for L in order:
+            set_values({"bc_seg_len": L})
+            g = SimpleGraph.from_csr(*g20.csr())
+            betweenness_centrality(g, srcs)
+            for _ in range(3):
+                betweenness_centrality(g, srcs)
+                runs[L].append(last_stats(g)["device_ms"])
+            g._release()
     bc_hist = []
     bc_best, bc_best_ms = current, None
     for L in BC_CANDIDATES:
-        set_values({"bc_seg_len": L})
-        g = SimpleGraph.from_csr(*g20.csr())
-        betweenness_centrality(g, srcs)
-        ms = []
-        for _ in range(3):
-            betweenness_centrality(g, srcs)
-            ms.append(last_stats(g)["device_ms"])
-        g._release()
-        m = float(np.median(ms))
+        m = float(np.median(runs[L]))
         bc_hist.append({"bc_seg_len": L, "device_ms": round(m, 3)})
         log(f"bc_seg_len={L}: {m:.2f} ms")
         if bc_best_ms is None or m < bc_best_ms * 0.99:
