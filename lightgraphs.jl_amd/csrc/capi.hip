@@ -2,6 +2,8 @@
 // of liblgmi.so (declared in include/lgmi.h).
 #include "lgmi_internal.h"
 
+#include <thread>
+
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
@@ -161,17 +163,60 @@ std::mutex& policy_mutex() {
     return m;
 }
 
+// Host check of a CSR: row_ptr from 0 to nnz and monotone, neighbour ids in
+// range, rows sorted ascending without duplicates.  Large graphs are checked
+// by up to 16 threads over vertex ranges; the violation reported is the one
+// at the smallest vertex, as a single pass would report it.
 static void check_csr(int32_t nv, int64_t nnz, const int32_t* rp, const int32_t* col, const char* what) {
     if (!rp || (nnz > 0 && !col)) throw Error(LGMI_E_ARG, std::string(what) + ": null array");
     if (rp[0] != 0 || (int64_t)rp[nv] != nnz) throw Error(LGMI_E_ARG, std::string(what) + ": row_ptr[0] must be 0 and row_ptr[nv] == nnz");
-    for (int32_t v = 0; v < nv; v++) {
-        const int32_t b = rp[v], e = rp[v + 1];
-        if (e < b) throw Error(LGMI_E_ARG, std::string(what) + ": row_ptr not monotone");
-        for (int32_t k = b; k < e; k++) {
-            const int32_t u = col[k];
-            if (u < 0 || u >= nv) throw Error(LGMI_E_BOUNDS, std::string(what) + ": neighbour id out of range");
-            if (k > b && col[k - 1] >= u) throw Error(LGMI_E_ARG, std::string(what) + ": rows must be sorted ascending without duplicates");
+    // 0 ok, 1 not monotone, 2 id out of range, 3 unsorted or duplicate; the first violation of [v0, v1)
+    auto scan = [&](int32_t v0, int32_t v1, int32_t& at) -> int {
+        for (int32_t v = v0; v < v1; v++) {
+            const int32_t b = rp[v], e = rp[v + 1];
+            at = v;
+            if (e < b || b < 0 || (int64_t)e > nnz) return 1;
+            for (int32_t k = b; k < e; k++) {
+                const int32_t u = col[k];
+                if (u < 0 || u >= nv) return 2;
+                if (k > b && col[k - 1] >= u) return 3;
+            }
         }
+        return 0;
+    };
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const int nt = (int)std::min<int64_t>({16, (int64_t)hw, nnz / (1 << 20) + 1, (int64_t)std::max(nv, 1)});
+    std::vector<int> code((size_t)nt, 0);
+    std::vector<int32_t> at((size_t)nt, 0);
+    if (nt <= 1) {
+        code[0] = scan(0, nv, at[0]);
+    } else {
+        // vertex ranges of about equal entry counts: thread t starts at the
+        // first vertex whose row begins at or after t * nnz / nt (row_ptr is
+        // only trusted after the monotonicity check, so the split points are
+        // clamped into order; a wrong split costs balance, not correctness)
+        std::vector<int32_t> cut((size_t)nt + 1, 0);
+        cut[(size_t)nt] = nv;
+        for (int t = 1; t < nt; t++) {
+            const int64_t target = nnz * t / nt;
+            int32_t lo = cut[(size_t)t - 1], hi = nv;
+            while (lo < hi) {
+                const int32_t mid = lo + (hi - lo) / 2;
+                if ((int64_t)rp[mid] < target) lo = mid + 1;
+                else hi = mid;
+            }
+            cut[(size_t)t] = lo;
+        }
+        std::vector<std::thread> th;
+        th.reserve((size_t)nt);
+        for (int t = 0; t < nt; t++)
+            th.emplace_back([&, t]() { code[(size_t)t] = scan(cut[(size_t)t], cut[(size_t)t + 1], at[(size_t)t]); });
+        for (auto& x : th) x.join();
+    }
+    for (int t = 0; t < nt; t++) {
+        if (code[(size_t)t] == 1) throw Error(LGMI_E_ARG, std::string(what) + ": row_ptr not monotone");
+        if (code[(size_t)t] == 2) throw Error(LGMI_E_BOUNDS, std::string(what) + ": neighbour id out of range");
+        if (code[(size_t)t] == 3) throw Error(LGMI_E_ARG, std::string(what) + ": rows must be sorted ascending without duplicates");
     }
 }
 

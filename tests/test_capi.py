@@ -203,3 +203,43 @@ def test_create_rejects_unmirrored_csr():
     _lib.check(lib.lgmi_graph_create(3, 2, _lib.i32p(rp), _lib.i32p(col), 2, _lib.i32p(irp), _lib.i32p(icol), 1,
                                      _lib.ctypes.byref(h)))
     lib.lgmi_graph_destroy(h)
+
+
+def test_create_rejects_bad_csr_threaded_check():
+    """A CSR of 4M entries is checked by several threads over vertex ranges:
+    every violation is found wherever it lies, and when there are several
+    the one at the smallest vertex is reported, as one pass would."""
+    from lgmi import _lib
+    lib = _lib.load()
+    nv, d = 200_000, 20
+    rp = np.arange(0, (nv + 1) * d, d, dtype=np.int32)
+    col = ((np.arange(nv, dtype=np.int64)[:, None] + 1 + np.arange(d)[None, :] * 7) % nv).astype(np.int32)
+    col.sort(axis=1)
+    base = np.ascontiguousarray(col.reshape(-1))
+
+    def create(c):
+        h = _lib.ctypes.c_void_p()
+        return lib.lgmi_graph_create(nv, len(c), _lib.i32p(rp), _lib.i32p(c), 0, None, None, 0, _lib.ctypes.byref(h))
+
+    def with_unsorted(c, v):
+        c[v * d], c[v * d + 1] = c[v * d + 1], c[v * d]
+
+    c = base.copy()
+    with_unsorted(c, 150_000)
+    assert create(c) == -1
+    c = base.copy()
+    c[190_000 * d + 3] = nv + 5
+    assert create(c) == -2
+    c = base.copy()                       # unsorted row first, out-of-range id later
+    with_unsorted(c, 50_000)
+    c[190_000 * d + 3] = nv + 5
+    assert create(c) == -1
+    c = base.copy()                       # out-of-range id first, unsorted row later
+    c[30_000 * d + 3] = -7
+    with_unsorted(c, 150_000)
+    assert create(c) == -2
+    rp_bad = rp.copy()                    # a row pointer past nnz in a late range
+    rp_bad[180_000] = rp_bad[-1] + 100
+    h = _lib.ctypes.c_void_p()
+    rc = lib.lgmi_graph_create(nv, len(base), _lib.i32p(rp_bad), _lib.i32p(base), 0, None, None, 0, _lib.ctypes.byref(h))
+    assert rc == -1
