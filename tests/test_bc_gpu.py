@@ -318,3 +318,32 @@ def test_persistent_level_overflow_falls_back(oracle):
     rp, col = g.csr()
     assert close(a, oracle.betweenness(rp, col, rp, False, vs - 1))
 
+
+
+def test_path_counts_beyond_fp32(oracle):
+    """Path counts above 2^24 (the layered graph: 15^7 paths by layer 8, up
+    to about 1.1e14), where a count narrowed to FP32 anywhere on the way
+    would be rounded: against the oracle at 1e-12 with all sources, with
+    sources whose lane groups mix small and large counts, and the persistent
+    launch against one launch per level, bit for bit."""
+    import lgmi
+    from lgmi import centrality as C
+    from conftest import layered_edges
+    n, s, d = layered_edges()
+    g = lgmi.SimpleGraph.from_edges(n, s, d)
+    rp, col = g.csr()
+    rng = np.random.default_rng(21)
+    cases = [np.arange(1, n + 1), np.sort(rng.choice(n, 70, replace=False)) + 1,
+             np.arange(n - 39, n + 1), np.array([1])]
+    try:
+        for vs in cases:
+            say(f"layered graph, {len(vs)} sources")
+            C.set_launch(0)
+            got = lgmi.betweenness_centrality(g, vs, normalize=False)
+            want = oracle.betweenness(rp, col, rp, False, vs - 1, normalize=False)
+            assert close(got, want)
+            C.set_launch(1)
+            per_level = lgmi.betweenness_centrality(g, vs, normalize=False)
+            assert (_bits(got) == _bits(per_level)).all()
+    finally:
+        C.set_launch(0)

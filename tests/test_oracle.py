@@ -668,3 +668,51 @@ def test_neighborhood_order_vs_networkx(oracle):
                 if depth[b] <= dd:
                     want.append((b, depth[b]))
             assert oracle.neighborhood_dists(rp, col, v, dd) == want
+
+
+def _brandes_py(n, rp, col, round_sigma_f32=False):
+    """Textbook Brandes (all sources, undirected, normalize=false halving),
+    optionally rounding each predecessor's path count to FP32 where it is
+    added -- the mistake a narrowed path-count store would make."""
+    from collections import deque
+    bc = [0.0] * n
+    for s in range(n):
+        d = [-1] * n
+        sig = [0.0] * n
+        d[s], sig[s] = 0, 1.0
+        order, q = [], deque([s])
+        while q:
+            v = q.popleft()
+            order.append(v)
+            for w in col[rp[v]:rp[v + 1]]:
+                if d[w] < 0:
+                    d[w] = d[v] + 1
+                    q.append(w)
+                if d[w] == d[v] + 1:
+                    sv = float(np.float32(sig[v])) if round_sigma_f32 else sig[v]
+                    sig[w] += sv
+        delta = [0.0] * n
+        for w in reversed(order):
+            for v in col[rp[w]:rp[w + 1]]:
+                if d[v] == d[w] - 1:
+                    delta[v] += sig[v] / sig[w] * (1.0 + delta[w])
+            if w != s:
+                bc[w] += delta[w]
+    return np.array(bc) * 0.5
+
+
+def test_oracle_betweenness_on_layered_counts_beyond_fp32(oracle):
+    """The layered graph's path counts pass 2^24: the oracle agrees with a
+    textbook Brandes at 1e-12, and the same Brandes with the counts rounded
+    to FP32 where they are summed misses by far more than that bar, so a GPU
+    test on this graph catches a narrowed path count."""
+    from conftest import layered_edges
+    n, s, d = layered_edges()
+    rp, col, _, _ = csr_of(n, s, d, False)
+    rp_l, col_l = rp.tolist(), col.tolist()
+    exact = _brandes_py(n, rp_l, col_l)
+    got = oracle.betweenness(rp, col, rp, False, np.arange(n), normalize=False)
+    assert np.allclose(got, exact, rtol=1e-12, atol=0)
+    narrowed = _brandes_py(n, rp_l, col_l, round_sigma_f32=True)
+    rel = np.abs(narrowed - exact) / np.maximum(np.abs(exact), 1e-300)
+    assert rel.max() > 1e-9, rel.max()
