@@ -97,23 +97,55 @@ def bfs_bench(args, lgmi, _lib, orc):
         mcomp[i] = deg[dist >= 0].sum() / 2.0
     log(f"m_comp per root: min {mcomp.min():.3e} max {mcomp.max():.3e}")
 
-    # warmup
-    for i in range(args.warmup):
-        _lib.check(lib.lgmi_bfs_run(h, int(roots[i % len(roots)]), 0, 1, None))
-    _lib.check(lib.lgmi_synchronize())
-
-    # timed region
+    # The K timed searches go through lgmi_bfs_run_many: eight searches at a
+    # time, each on its own stream and search state, each writing distances
+    # and parents into its own rows of caller-owned device arrays (up to 256
+    # searches per call, the arrays reused from call to call).
     K = args.steps
-    edges = 0.0
+    order = np.array([roots[i % len(roots)] for i in range(max(K, args.warmup))], dtype=np.int32)
+    chunk = max(1, min(K, 256))
+    dbuf = _lib.DeviceBuffer(4 * nv * chunk)
+    pbuf = _lib.DeviceBuffer(4 * nv * chunk)
+
+    def run_many(n):
+        for s0 in range(0, n, chunk):
+            part = np.ascontiguousarray(order[s0:min(n, s0 + chunk)])
+            _lib.check(lib.lgmi_bfs_run_many(h, _lib.i32p(part), len(part), 0, dbuf.ptr, pbuf.ptr))
+
+    run_many(args.warmup)
     _lib.check(lib.lgmi_synchronize())
+    edges = float(sum(mcomp[i % len(roots)] for i in range(K)))
     t_start = time.perf_counter()
-    for i in range(K):
-        _lib.check(lib.lgmi_bfs_run(h, int(roots[i % len(roots)]), 0, 1, None))
-        edges += mcomp[i % len(roots)]
+    run_many(K)
     _lib.check(lib.lgmi_synchronize())
     elapsed = time.perf_counter() - t_start
     gteps = edges / elapsed / 1e9
-    log(f"timed {K} BFS: {elapsed * 1e3:.2f} ms total, {gteps:.2f} GTEPS")
+    log(f"timed {K} BFS (lgmi_bfs_run_many): {elapsed * 1e3:.2f} ms total, {gteps:.2f} GTEPS")
+    # the rows of the last call against the per-call search (untimed)
+    n_last = K - (K - 1) // chunk * chunk
+    rows_d = dbuf.to_host(np.int32, (n_last, nv))
+    rows_p = pbuf.to_host(np.int32, (n_last, nv))
+    par_one = np.empty(nv, dtype=np.int32)
+    for j in sorted({0, n_last - 1}):
+        _lib.check(lib.lgmi_bfs_run(h, int(order[K - n_last + j]), 0, 1, None))
+        _lib.check(lib.lgmi_bfs_fetch(h, _lib.i32p(dist), _lib.i32p(par_one)))
+        if not (np.array_equal(rows_d[j], dist) and np.array_equal(rows_p[j], par_one)):
+            raise RuntimeError("lgmi_bfs_run_many row differs from the per-call search")
+    del rows_d, rows_p
+    dbuf.free()
+    pbuf.free()
+
+    # the same searches one call at a time (lgmi_bfs_run per root)
+    for i in range(args.warmup):
+        _lib.check(lib.lgmi_bfs_run(h, int(roots[i % len(roots)]), 0, 1, None))
+    _lib.check(lib.lgmi_synchronize())
+    t_pc = time.perf_counter()
+    for i in range(K):
+        _lib.check(lib.lgmi_bfs_run(h, int(roots[i % len(roots)]), 0, 1, None))
+    _lib.check(lib.lgmi_synchronize())
+    elapsed_pc = time.perf_counter() - t_pc
+    gteps_per_call = edges / elapsed_pc / 1e9
+    log(f"timed {K} BFS one call each: {elapsed_pc * 1e3:.2f} ms total, {gteps_per_call:.2f} GTEPS")
 
     # the same timed runs with the learned direction switch (outputs are the
     # same by construction; the GPU tests check it)
@@ -131,7 +163,7 @@ def bfs_bench(args, lgmi, _lib, orc):
                 _lib.check(lib.lgmi_bfs_run(h, int(roots[i % len(roots)]), 0, 1, None))
             _lib.check(lib.lgmi_synchronize())
             gteps_learned = edges / (time.perf_counter() - t_l) / 1e9
-            log(f"timed {K} BFS with the learned switch: {gteps_learned:.2f} GTEPS")
+            log(f"timed {K} BFS with the learned switch, one call each: {gteps_learned:.2f} GTEPS")
         finally:
             policy.set_heuristic()
 
@@ -228,9 +260,11 @@ def bfs_bench(args, lgmi, _lib, orc):
                         "a/b/c = 0.57/0.19/0.19, self-loops and duplicates removed, Int32 CSR; "
                         "64 seeded roots of degree >= 1",
             "nv": nv, "nnz": nnz, "roots": len(roots), "policy": "alpha=15/beta=18 heuristic",
+            "entry_point": "lgmi_bfs_run_many (8 searches at a time; rows checked against lgmi_bfs_run)",
+            "gteps_per_call": round(gteps_per_call, 4),
             "gteps_harmonic_mean_device": round(float(len(per_root) / np.sum(1.0 / per_root)), 3),
             "gteps_median_device": round(float(np.median(per_root)), 3),
-            "gteps_learned_switch": round(gteps_learned, 4) if gteps_learned is not None else None,
+            "gteps_learned_switch_per_call": round(gteps_learned, 4) if gteps_learned is not None else None,
             "tuning": tuning_values,
         },
         "roofline": {

@@ -162,6 +162,23 @@ int lgmi_bfs_run(lgmi_graph_t g, int32_t src, int dir_in, int want_parents, lgmi
 /* Copies the last run's results to the host; either pointer may be NULL. */
 int lgmi_bfs_fetch(lgmi_graph_t g, int32_t* dist, int32_t* parent);
 
+/* Many single-source searches in one call: the loop a caller writes over
+ * gdistances / bfs_parents (src/traversals/bfs.jl:35-66, 95-141), one
+ * search per root, each with the results of lgmi_bfs for that root.  Up to
+ * eight searches run at once, each on a search state, HIP stream and host
+ * thread of its own, so that one search's level launches fill the time that
+ * another's launch boundaries and small levels leave the GPU.  dist and parent are
+ * caller-owned DEVICE arrays of nroots x nv Int32: row i holds the
+ * distances (-1 unreached) and parents (parent[root] = root, -1
+ * unreached) of roots[i]; parent NULL: distances only.  The call returns
+ * when every row is written; work the caller queued on dist / parent must
+ * be complete before it.  LGMI_E_BOUNDS when a root is outside [0, nv).
+ * The first call on a handle allocates the eight search states (about
+ * nv / 2 bytes each plus 8 MB of queues); lgmi_bfs_fetch does not see
+ * these searches. */
+int lgmi_bfs_run_many(lgmi_graph_t g, const int32_t* roots, int32_t nroots, int dir_in, int32_t* dist,
+                      int32_t* parent);
+
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code that must not call hipMalloc: tight serving loops, harnesses
  * that forbid it): the single-source BFS of lgmi_bfs_run / lgmi_bfs with the

@@ -51,10 +51,13 @@
 #include "lgmi_internal.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <climits>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
+#include <thread>
 #include <vector>
 
 namespace lgmi {
@@ -1856,6 +1859,7 @@ void bfs_free(BfsWork* w) {
     for (auto e : w->step_ev) (void)hipEventDestroy(e);
     for (auto& x : w->gexec)
         if (x) (void)hipGraphExecDestroy(x);
+    if (w->own_stream) (void)hipStreamDestroy(w->own_stream);
     delete w;
 }
 
@@ -1868,16 +1872,20 @@ __global__ void bfs_walls(StepArgs a, const int32_t* walls, int32_t n) {
     }
 }
 
-void bfs_prepare(lgmi_graph* g) {
-    if (g->bfs) return;
+// A search workspace on stream s.  With per_call false it holds no
+// distance or parent array and no source list (the concurrent single-source
+// searches write the caller's arrays).
+static BfsWork* bfs_work_new(lgmi_graph* g, hipStream_t s, bool per_call) {
     BfsWork* w = new BfsWork();
     try {
         w->nv = g->nv;
         w->nwords = 2 * ((g->nv + 63) / 64);
         w->grid = std::min(std::max(1, g->cu_count), kMaxGrid);
-        w->dist = dev_alloc_n<int32_t>(g->nv);
-        w->parent = dev_alloc_n<int32_t>(g->nv);
-        w->srcbuf = dev_alloc_n<int32_t>((size_t)g->nv + 1);
+        if (per_call) {
+            w->dist = dev_alloc_n<int32_t>(g->nv);
+            w->parent = dev_alloc_n<int32_t>(g->nv);
+            w->srcbuf = dev_alloc_n<int32_t>((size_t)g->nv + 1);
+        }
         w->part = dev_alloc_n<unsigned long long>((size_t)2 * w->grid * kPart);
         for (int k = 0; k < 3; k++) w->bm[k] = dev_alloc_n<uint32_t>(std::max(w->nwords, 2));
         w->vis = dev_alloc_n<uint32_t>(std::max(w->nwords, 2));
@@ -1893,14 +1901,19 @@ void bfs_prepare(lgmi_graph* g) {
         LGMI_HIP(hipHostGetDevicePointer((void**)&w->d_done, w->h_done, 0));
         LGMI_HIP(hipEventCreate(&w->ev0));
         LGMI_HIP(hipEventCreate(&w->ev1));
-        LGMI_HIP(hipMemsetAsync(w->sh, 0, sizeof(BfsShared), g->stream));
-        LGMI_HIP(hipMemsetAsync(w->log, 0, sizeof(LevelRec) * w->max_log, g->stream));
-        LGMI_HIP(hipMemsetAsync(w->part, 0, sizeof(unsigned long long) * 2 * (size_t)w->grid * kPart, g->stream));
+        LGMI_HIP(hipMemsetAsync(w->sh, 0, sizeof(BfsShared), s));
+        LGMI_HIP(hipMemsetAsync(w->log, 0, sizeof(LevelRec) * w->max_log, s));
+        LGMI_HIP(hipMemsetAsync(w->part, 0, sizeof(unsigned long long) * 2 * (size_t)w->grid * kPart, s));
     } catch (...) {
         bfs_free(w);
         throw;
     }
-    g->bfs = w;
+    return w;
+}
+
+void bfs_prepare(lgmi_graph* g) {
+    if (g->bfs) return;
+    g->bfs = bfs_work_new(g, g->stream, true);
 }
 
 // The adjacency half of the step arguments (traversal and reverse CSR with
@@ -1932,8 +1945,7 @@ static StepArgs graph_args(lgmi_graph* g, int dir_in, int32_t grid) {
     return a;
 }
 
-static StepArgs make_args(lgmi_graph* g, int dir_in, bool parents) {
-    BfsWork* w = g->bfs;
+static StepArgs make_args(lgmi_graph* g, BfsWork* w, int dir_in, bool parents) {
     StepArgs a = graph_args(g, dir_in, w->grid);
     a.dist = w->dist;
     a.parent = parents ? reinterpret_cast<uint32_t*>(w->parent) : nullptr;
@@ -1954,19 +1966,24 @@ static void launch_level(const StepArgs& a, bool parents, int32_t karg, int32_t 
     else bfs_level<false><<<a.grid, kBlock, 0, s>>>(a, karg, forced);
 }
 
-void policy_upload(lgmi_graph* g) {
+// The learned policy's weights into workspace w, on stream s, when they
+// changed since its last search.
+static void policy_upload(BfsWork* w, hipStream_t s) {
     std::lock_guard<std::mutex> lk(policy_mutex());
     PolicyHost& p = policy_host();
-    if (p.have_mlp && g->bfs->policy_version != p.version) {
-        LGMI_HIP(hipMemcpyAsync(g->bfs->policy, &p.w, sizeof(PolicyWeights), hipMemcpyHostToDevice, g->stream));
-        g->bfs->policy_version = p.version;
+    if (p.have_mlp && w->policy_version != p.version) {
+        LGMI_HIP(hipMemcpyAsync(w->policy, &p.w, sizeof(PolicyWeights), hipMemcpyHostToDevice, s));
+        w->policy_version = p.version;
     }
 }
 
-void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_src, int dir_in,
-             int want_parents, int forced_dir, lgmi_bfs_stats* stats, const int32_t* h_walls, int32_t nwalls) {
-    bfs_prepare(g);
-    BfsWork* w = g->bfs;
+// One search on workspace w and stream s.  out_dist / out_parent, when
+// out_dist is not NULL, replace the workspace's own result arrays (the
+// concurrent searches; captured level sequences, whose arguments hold the
+// arrays, are then not used).
+static void bfs_search(lgmi_graph* g, BfsWork* w, hipStream_t s, int32_t* out_dist, int32_t* out_parent,
+                       const int32_t* h_srcs, int32_t nsrc, int32_t single_src, int dir_in, int want_parents,
+                       int forced_dir, lgmi_bfs_stats* stats, const int32_t* h_walls, int32_t nwalls) {
     if (nsrc > g->nv) throw Error(LGMI_E_ARG, "source list longer than nv (de-duplicate first)");
     if (nwalls > 0 && single_src < 0) throw Error(LGMI_E_ARG, "walls need a single source");
     int mode;
@@ -1977,19 +1994,23 @@ void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_
         PolicyHost& p = policy_host();
         mode = p.mode; alpha = p.alpha; beta = p.beta; have_mlp = p.have_mlp;
     }
-    if (mode == LGMI_POLICY_LEARNED && have_mlp) policy_upload(g);
+    if (mode == LGMI_POLICY_LEARNED && have_mlp) policy_upload(w, s);
     if (mode == LGMI_POLICY_LEARNED && !have_mlp) mode = LGMI_POLICY_HEURISTIC;
 
-    StepArgs a = make_args(g, dir_in, want_parents != 0);
+    StepArgs a = make_args(g, w, dir_in, want_parents != 0);
+    if (out_dist) {
+        a.dist = out_dist;
+        a.parent = want_parents ? reinterpret_cast<uint32_t*>(out_parent) : nullptr;
+    }
     if (mode != LGMI_POLICY_LEARNED) a.policy = nullptr;
     a.done_host = w->d_done;
     const int64_t nnz_scan = g->nnz;  // the reverse CSR has as many entries as the forward one
-    hipStream_t s = g->stream;
     const int init_grid = std::max(1, std::min<int>(w->grid * 8, (g->nv + 255) / 256));
-    static const bool graphs_on = [] {
+    static const bool graphs_env = [] {
         const char* e = std::getenv("LGMI_BFS_GRAPHS");
         return !(e && e[0] == '0');
     }();
+    const bool graphs_on = graphs_env && out_dist == nullptr;
     const bool parents = want_parents != 0;
     const int flags = (parents ? 1 : 0) | (dir_in ? 2 : 0) | (a.policy ? 4 : 0) | ((forced_dir + 1) << 3);
     // Launches go out in batches; a launch after the last level exits at
@@ -2149,6 +2170,91 @@ void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_
             stats->reached = (int32_t)reached;
         }
     }
+}
+
+void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_src, int dir_in,
+             int want_parents, int forced_dir, lgmi_bfs_stats* stats, const int32_t* h_walls, int32_t nwalls) {
+    bfs_prepare(g);
+    bfs_search(g, g->bfs, g->stream, nullptr, nullptr, h_srcs, nsrc, single_src, dir_in, want_parents, forced_dir,
+               stats, h_walls, nwalls);
+}
+
+// Independent single-source searches from roots[0..nroots), up to
+// kManyStreams at a time, each on a workspace and stream of its own driven by
+// a host thread of its own; a thread takes the next root when its search
+// ends.  Row i of dist / parent (device arrays of nroots x nv) receives the
+// results of roots[i].  A level launch of one search fills the gaps that
+// another search's launch boundaries and small levels leave.  RMAT-22,
+// 64 roots (GTEPS, medians of four runs): 2 at a time 468, 4 475, 8 485,
+// 12 481, 16 476, against 403 for one search per call.
+constexpr int kManyStreams = 8;
+
+void bfs_run_many(lgmi_graph* g, const int32_t* roots, int32_t nroots, int dir_in, int32_t* dist, int32_t* parent) {
+    if (nroots <= 0) return;
+    const int K = std::min<int>(kManyStreams, nroots);
+    while ((int)g->bfs_pool.size() < K) {
+        hipStream_t st = nullptr;
+        LGMI_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        BfsWork* w = nullptr;
+        try {
+            w = bfs_work_new(g, st, false);
+        } catch (...) {
+            (void)hipStreamDestroy(st);
+            throw;
+        }
+        w->own_stream = st;
+        g->bfs_pool.push_back(w);
+    }
+    // the searches start after the handle's queued work (the CSR upload and
+    // any earlier search) is complete
+    hipEvent_t ready = nullptr;
+    LGMI_HIP(hipEventCreateWithFlags(&ready, hipEventDisableTiming));
+    try {
+        LGMI_HIP(hipEventRecord(ready, g->stream));
+        for (int k = 0; k < K; k++) LGMI_HIP(hipStreamWaitEvent(g->bfs_pool[(size_t)k]->own_stream, ready, 0));
+    } catch (...) {
+        (void)hipEventDestroy(ready);
+        throw;
+    }
+    int dev = 0;
+    LGMI_HIP(hipGetDevice(&dev));
+    const int forced = g->bfs ? g->bfs_forced : -1;
+    const int want_parents = parent != nullptr;
+    const size_t nv = (size_t)g->nv;
+    std::atomic<int32_t> next{0};
+    std::atomic<bool> failed{false};
+    std::vector<std::exception_ptr> err((size_t)K);
+    auto worker = [&](int k) {
+        BfsWork* w = g->bfs_pool[(size_t)k];
+        try {
+            LGMI_HIP(hipSetDevice(dev));
+            for (;;) {
+                const int32_t i = next.fetch_add(1);
+                if (i >= nroots || failed.load()) break;
+                bfs_search(g, w, w->own_stream, dist + (size_t)i * nv, want_parents ? parent + (size_t)i * nv : nullptr,
+                           nullptr, 1, roots[i], dir_in, want_parents, forced, nullptr, nullptr, 0);
+            }
+        } catch (...) {
+            err[(size_t)k] = std::current_exception();
+            failed.store(true);
+        }
+        (void)hipStreamSynchronize(w->own_stream);
+    };
+    std::vector<std::thread> th;
+    th.reserve((size_t)K - 1);
+    try {
+        for (int k = 1; k < K; k++) th.emplace_back(worker, k);
+    } catch (...) {   // no thread for the rest: the started ones finish their searches
+        failed.store(true);
+        for (auto& t : th) t.join();
+        (void)hipEventDestroy(ready);
+        throw;
+    }
+    worker(0);
+    for (auto& t : th) t.join();
+    (void)hipEventDestroy(ready);
+    for (auto& e : err)
+        if (e) std::rethrow_exception(e);
 }
 
 void bfs_fetch(lgmi_graph* g, int32_t* dist, int32_t* parent) {
@@ -2374,7 +2480,7 @@ extern "C" int lgmi_bfs_probe(lgmi_graph_t g, int32_t src, int dir_in, const int
             if (levels[i] <= levels[i - 1]) throw Error(LGMI_E_ARG, "levels must be strictly increasing");
         bfs_prepare(g);
         BfsWork* w = g->bfs;
-        StepArgs a = make_args(g, dir_in, true);
+        StepArgs a = make_args(g, w, dir_in, true);
         a.policy = nullptr;
         a.solo_front = 0;   // one level per launch: launch index = level
         hipStream_t s = g->stream;

@@ -302,6 +302,11 @@ static void graph_free(lgmi_graph* g) {
     if (!g) return;
     if (g->stream) (void)hipStreamSynchronize(g->stream);
     bfs_free(g->bfs);
+    for (BfsWork* w : g->bfs_pool) {
+        if (w->own_stream) (void)hipStreamSynchronize(w->own_stream);
+        bfs_free(w);
+    }
+    g->bfs_pool.clear();
     bc_free(g->bc);
     bc_graph_free(g->bcg);
     if (g->in.row_ptr != g->out.row_ptr) {
@@ -496,6 +501,16 @@ int lgmi_bfs_run(lgmi_graph_t g, int32_t src, int dir_in, int want_parents, lgmi
     check_src(g, src);
     const int forced = g->bfs ? g->bfs_forced : -1;
     bfs_run(g, nullptr, 1, src, dir_in ? 1 : 0, want_parents ? 1 : 0, forced, stats);
+    LGMI_API_END
+}
+
+int lgmi_bfs_run_many(lgmi_graph_t g, const int32_t* roots, int32_t nroots, int dir_in, int32_t* dist,
+                      int32_t* parent) {
+    LGMI_API_BEGIN
+    if (!g || nroots < 0 || (nroots > 0 && (!roots || !dist))) throw Error(LGMI_E_ARG, "bad arguments");
+    std::lock_guard<std::mutex> lk(g->mu);
+    for (int32_t i = 0; i < nroots; i++) check_src(g, roots[i]);
+    bfs_run_many(g, roots, nroots, dir_in ? 1 : 0, dist, parent);
     LGMI_API_END
 }
 

@@ -144,6 +144,7 @@ struct BfsWork {
     int last_launches_used = 0;        // launch index that finished the last search
     int last_dir_in = 0;
     int last_parents = 0;
+    hipStream_t own_stream = nullptr;     // the stream of a concurrent-search workspace (destroyed with it)
     ~BfsWork();
 };
 
@@ -189,6 +190,7 @@ struct lgmi_graph {
     hipStream_t stream = nullptr;
     std::mutex mu;
     lgmi::BfsWork* bfs = nullptr;
+    std::vector<lgmi::BfsWork*> bfs_pool;   // workspaces of lgmi_bfs_run_many, one stream each
     lgmi::BcWork* bc = nullptr;
     lgmi::BcGraph* bcg = nullptr;   // betweenness: host row pointers, heavy-row segments
     int cu_count = 0;
@@ -203,8 +205,8 @@ void bfs_run(lgmi_graph* g, const int32_t* h_srcs, int32_t nsrc, int32_t single_
              int want_parents, int forced_dir, lgmi_bfs_stats* stats, const int32_t* h_walls = nullptr,
              int32_t nwalls = 0);
 void bfs_fetch(lgmi_graph* g, int32_t* dist, int32_t* parent);
+void bfs_run_many(lgmi_graph* g, const int32_t* roots, int32_t nroots, int dir_in, int32_t* dist, int32_t* parent);
 void bfs_free(BfsWork* w);
-void policy_upload(lgmi_graph* g);
 // bc.hip
 // mode 0: betweenness into bc; mode 1: stress centrality, Int64 results
 // written into the same nv * 8 bytes; mode 2: per-source distance

@@ -33,7 +33,7 @@ LGMI_NOALLOC_SCRATCH_BYTES = 17 << 20
 # Every symbol include/lgmi.h and include/lgmi_gen.h declare.
 EXPORTED = [
     "lgmi_graph_create", "lgmi_graph_destroy", "lgmi_gdistances", "lgmi_bfs_parents", "lgmi_bfs",
-    "lgmi_bfs_run", "lgmi_bfs_fetch", "lgmi_bfs_run_noalloc", "lgmi_bfs_steps", "lgmi_bfs_time_steps", "lgmi_bfs_force_direction",
+    "lgmi_bfs_run", "lgmi_bfs_fetch", "lgmi_bfs_run_many", "lgmi_bfs_run_noalloc", "lgmi_bfs_steps", "lgmi_bfs_time_steps", "lgmi_bfs_force_direction",
     "lgmi_bfs_probe", "lgmi_mlp_loss_grad", "lgmi_mlp_train", "lgmi_mlp_forward",
     "lgmi_bfs_set_policy", "lgmi_betweenness", "lgmi_betweenness_stats", "lgmi_betweenness_batch",
     "lgmi_betweenness_batch_tuning",
@@ -120,6 +120,7 @@ def load():
         "lgmi_bfs": ([_vp, c.c_int32, c.c_int, _i32p, _i32p], c.c_int),
         "lgmi_bfs_run": ([_vp, c.c_int32, c.c_int, c.c_int, c.POINTER(BfsStats)], c.c_int),
         "lgmi_bfs_fetch": ([_vp, _i32p, _i32p], c.c_int),
+        "lgmi_bfs_run_many": ([_vp, _i32p, c.c_int32, c.c_int, _vp, _vp], c.c_int),
         "lgmi_bfs_run_noalloc": ([_vp, c.c_int32, c.c_int, _vp, _vp], c.c_int),
         "lgmi_bfs_steps": ([_vp, c.c_int32, c.POINTER(BfsStep)], c.c_int),
         "lgmi_bfs_time_steps": ([_vp, c.c_int], c.c_int),
@@ -205,3 +206,45 @@ def f32p(a: np.ndarray):
 def f64p(a: np.ndarray):
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(_f64p)
+
+
+class DeviceBuffer:
+    """Device memory from the HIP runtime for the entry points that write
+    caller-owned device arrays (lgmi_bfs_run_many, lgmi_bfs_run_noalloc)."""
+
+    _hip = None
+
+    @classmethod
+    def _rt(cls):
+        if cls._hip is None:
+            h = ctypes.CDLL("libamdhip64.so")
+            h.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+            h.hipFree.argtypes = [ctypes.c_void_p]
+            h.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+            cls._hip = h
+        return cls._hip
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        self.ptr = ctypes.c_void_p()
+        if self._rt().hipMalloc(ctypes.byref(self.ptr), ctypes.c_size_t(max(self.nbytes, 4))) != 0:
+            raise LgmiError(LGMI_E_ALLOC, f"hipMalloc of {self.nbytes} bytes failed")
+
+    def to_host(self, dtype, shape) -> np.ndarray:
+        out = np.empty(shape, dtype=dtype)
+        if out.nbytes > self.nbytes:
+            raise ValueError("copy larger than the buffer")
+        if self._rt().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), self.ptr, ctypes.c_size_t(out.nbytes), 2) != 0:
+            raise LgmiError(LGMI_E_HIP, "hipMemcpy device to host failed")
+        return out
+
+    def free(self) -> None:
+        if self.ptr:
+            self._rt().hipFree(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

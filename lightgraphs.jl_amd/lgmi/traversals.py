@@ -74,6 +74,43 @@ def bfs_parents(g: _GraphBase, s: int, dir="out") -> np.ndarray:
     return np.where(par >= 0, par + 1, 0).astype(g.T)
 
 
+def _device_many(g: _GraphBase, roots0: np.ndarray, dir_in: int, parents: bool):
+    """lgmi_bfs_run_many: 0-based distances and parents (-1 unreached) of
+    one single-source search per root, rows in root order."""
+    roots0 = np.ascontiguousarray(roots0, dtype=np.int32)
+    n, k = g.nv(), len(roots0)
+    lib = _lib.load()
+    d = _lib.DeviceBuffer(4 * n * k)
+    p = _lib.DeviceBuffer(4 * n * k) if parents else None
+    try:
+        _lib.check(lib.lgmi_bfs_run_many(g.handle(), _lib.i32p(roots0), k, int(dir_in), d.ptr,
+                                         p.ptr if p is not None else None))
+        dist = d.to_host(np.int32, (k, n))
+        par = p.to_host(np.int32, (k, n)) if p is not None else None
+    finally:
+        d.free()
+        if p is not None:
+            p.free()
+    return dist, par
+
+
+def bfs_many(g: _GraphBase, sources, dir="out"):
+    """The loop over sources of gdistances(g, s) and bfs_parents(g, s; dir)
+    (bfs.jl:35-66, 95-141) in one call, several searches at a time on the
+    GPU.  Returns (dists, parents), one row per source: distances as
+    gdistances gives them (typemax(T) unreached) and parents as bfs_parents
+    does (0 unreached)."""
+    srcs = np.atleast_1d(np.asarray(sources, dtype=np.int64))
+    n = g.nv()
+    if srcs.size and (srcs.min() < 1 or srcs.max() > n):
+        raise IndexError(f"source vertex out of range 1:{n}")
+    dist, par = _device_many(g, srcs - 1, 0 if dir in ("out", ":out") else 1, True)
+    dists = dist.astype(g.T)
+    dists[dist < 0] = typemax(g.T)
+    parents = np.where(par >= 0, par.astype(g.T) + 1, 0).astype(g.T)
+    return dists, parents
+
+
 def tree(parents: np.ndarray) -> SimpleDiGraph:
     """tree(parents) -- bfs.jl:10-19: arcs u -> v for every parent u > 0,
     u != v."""
