@@ -240,6 +240,74 @@ __device__ inline void gather_hits(uint64_t& hits, int32_t uk, uint64_t mk, uint
     }
 }
 
+// As gather_hits over a pass that holds the entries of several rows, entry
+// t belonging to the row of chunk lane rk[t] (rows are contiguous and in
+// order).  The running sum acc and mask got belong to row `cur`; when a hit
+// of another row comes, flush(cur, acc, got) finishes that row and both
+// restart from zero, so every row adds its values in the order of its
+// entries, as one row at a time does.
+template <typename T, bool LOAD, typename Flush>
+__device__ inline void gather_rows(uint64_t& hits, int32_t uk, uint64_t mk, uint64_t fk, int32_t rk, const T* stage,
+                                   int lane, T& acc, uint64_t& got, int32_t& cur, Flush& flush) {
+    constexpr int kH = 8;
+    int32_t uu[kH], rr[kH];
+    uint64_t mm[kH], ff[kH];
+    #pragma unroll
+    for (int q = 0; q < kH; q++) {
+        uu[q] = 0;
+        rr[q] = -1;
+        mm[q] = 0;
+        ff[q] = 0;
+        if (hits) {
+            const int t = __builtin_amdgcn_readfirstlane(__ffsll((unsigned long long)hits) - 1);
+            hits &= hits - 1;
+            uu[q] = __builtin_amdgcn_readlane(uk, t);
+            rr[q] = __builtin_amdgcn_readlane(rk, t);
+            mm[q] = readlane_u64(mk, t);
+            if constexpr (LOAD) ff[q] = readlane_u64(fk, t);
+        }
+    }
+    T x[kH];
+    #pragma unroll
+    for (int q = 0; q < kH; q++) {
+        if constexpr (LOAD) x[q] = ((mm[q] >> lane) & 1) ? stage[(size_t)uu[q] * 64 + lane_rank(ff[q])] : T(0);
+        else x[q] = ((mm[q] >> lane) & 1) ? T(1) : T(0);
+    }
+    #pragma unroll
+    for (int q = 0; q < kH; q++) {
+        if (rr[q] >= 0) {
+            if (rr[q] != cur) {
+                if (cur >= 0) flush(cur, acc, got);
+                cur = rr[q];
+                acc = T(0);
+                got = 0;
+            }
+            acc += x[q];
+            got |= mm[q];
+        }
+    }
+}
+
+// Inclusive prefix sum of x over the lanes of the wavefront.
+__device__ inline int32_t wave_incl_sum(int32_t x, int lane) {
+    #pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+// The first lane j whose value e (nondecreasing over the lanes) exceeds k,
+// for a k below the last lane's value.
+__device__ inline int first_lane_above(int32_t e, int32_t k) {
+    int lo = 0;
+    #pragma unroll
+    for (int step = 32; step >= 1; step >>= 1)
+        if (__shfl(e, lo + step - 1) <= k) lo += step;
+    return lo;
+}
+
 // For source i: its out-neighbour of largest out-degree (the smallest id
 // among ties), -1 for an empty row.  One wavefront per source.
 __global__ void bc_source_hubs(const int32_t* rp, const int32_t* col, const int32_t* srcs, int32_t n, int32_t* hub) {
@@ -356,11 +424,28 @@ __device__ bool fwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
         cand = (~sj & act) && ej > bj && !heavy;
     }
     uint64_t newF = 0;
-    uint64_t cm = __ballot(cand);
+    // a finished row j of the chunk: its sums and mask
+    auto finish = [&](int32_t j, double acc, uint64_t got) {
+        if constexpr (MODE == 2) {
+            if ((got >> lane) & 1) {
+                st_d += (uint64_t)(L + 1);
+                st_c += 1;
+                st_e = (uint64_t)(L + 1);
+            }
+        } else {
+            if ((got >> lane) & 1) {
+                const int32_t v = ch * 64 + j;
+                sig[(size_t)v * 64 + lane] = acc;
+                if (wr) wr[(size_t)v * 64 + lane_rank(got)] = acc;
+            }
+        }
+        if (lane == j) newF = got;
+    };
+    // rows of more than 64 entries: one row at a time, 64 entries a pass
+    uint64_t cm = __ballot(cand && ej - bj > 64);
     while (cm) {
         const int j = __ffsll((unsigned long long)cm) - 1;
         cm &= cm - 1;
-        const int32_t v = ch * 64 + j;
         const uint64_t unseen = ~__shfl(sj, j) & act;
         const int32_t b = __shfl(bj, j), e = __shfl(ej, j);
         double acc = 0.0;
@@ -383,19 +468,45 @@ __device__ bool fwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
                 else while (hits) gather_hits<double, false>(hits, uk, mk, fk, rd, lane, acc, got);
             }
         }
-        if constexpr (MODE == 2) {
-            if ((got >> lane) & 1) {
-                st_d += (uint64_t)(L + 1);
-                st_c += 1;
-                st_e = (uint64_t)(L + 1);
-            }
-        } else {
-            if ((got >> lane) & 1) {
-                sig[(size_t)v * 64 + lane] = acc;
-                if (wr) wr[(size_t)v * 64 + lane_rank(got)] = acc;
-            }
+        finish(j, acc, got);
+    }
+    // rows of at most 64 entries, packed: their entries laid end to end in
+    // lane order, each 64-entry pass holding whole rows (row j ends at
+    // entry E_j of the packed list)
+    const bool shortr = cand && ej - bj <= 64;
+    const int32_t lenj = shortr ? ej - bj : 0;
+    const int32_t E = wave_incl_sum(lenj, lane);
+    const int32_t S = E - lenj;                 // where row j starts in the packed list
+    const int32_t total = __builtin_amdgcn_readlane(E, 63);
+    const uint64_t unseen_j = ~sj & act;
+    for (int32_t base = 0; base < total;) {
+        // the pass ends at the last row end within 64 entries of base
+        const uint64_t endm = __ballot(lenj > 0 && E > base && E <= base + 64);
+        if (!endm) break;   // not reached: the next row starts at base and has at most 64 entries
+        const int lastj = 63 - __builtin_clzll(endm);
+        const int32_t pend = __builtin_amdgcn_readlane(E, lastj);
+        // this lane's entry of the packed list (lanes past the pass repeat
+        // its last entry: every lane takes part in the shuffles) and the row
+        // it belongs to
+        const int32_t kk = min(base + lane, pend - 1);
+        const int32_t rk = first_lane_above(E, kk);
+        const int32_t idx = __shfl(bj, rk) + (kk - __shfl(S, rk));
+        const uint64_t unseen = __shfl(unseen_j, rk);
+        int32_t uk = 0;
+        uint64_t fk = 0, mk = 0;
+        if (base + lane < pend) {
+            uk = a.in_col[idx];
+            fk = a.F[fidx(a, slot, L, uk)];
+            mk = fk & unseen;
         }
-        if (lane == j) newF = got;
+        uint64_t hits = __ballot(mk != 0);
+        double acc = 0.0;
+        uint64_t got = 0;
+        int32_t cur = -1;
+        if (MODE != 0 || !ld) while (hits) gather_rows<double, false>(hits, uk, mk, fk, rk, rd, lane, acc, got, cur, finish);
+        else while (hits) gather_rows<double, true>(hits, uk, mk, fk, rk, rd, lane, acc, got, cur, finish);
+        if (cur >= 0) finish(cur, acc, got);
+        base = pend;
     }
     if (vj < nv && !heavy) {
         a.F[fidx(a, slot, L + 1, vj)] = newF;
@@ -612,11 +723,23 @@ __device__ void bwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
         }
     }
     T bcj = 0;
-    uint64_t am = __ballot(Aj != 0);
-    while (am) {
-        const int j = __ffsll((unsigned long long)am) - 1;
-        am &= am - 1;
+    // row j of the chunk done: its successor sum acc gives the lanes'
+    // coefficients and the vertex's contribution
+    auto finish = [&](int32_t j, T acc) {
+        const uint64_t A = readlane_u64(Aj, j);
         const int32_t v = ch * 64 + j;
+        T contrib = 0;
+        if ((A >> lane) & 1)
+            contrib = bwd_finish_lane<MODE>(sig[(size_t)v * 64 + lane], &wr[(size_t)v * 64 + lane_rank(A)], acc);
+        contrib = lane_sum(contrib);
+        if (lane == j) bcj = contrib;
+    };
+    const uint64_t am = __ballot(Aj != 0);
+    // rows of more than 64 entries: one row at a time, 64 entries a pass
+    uint64_t lm = am & __ballot(ej - bj > 64);
+    while (lm) {
+        const int j = __ffsll((unsigned long long)lm) - 1;
+        lm &= lm - 1;
         const uint64_t A = __shfl(Aj, j);
         const int32_t b = __shfl(bj, j), e = __shfl(ej, j);
         T acc = 0;
@@ -633,11 +756,61 @@ __device__ void bwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
             uint64_t unused = 0;
             while (hits) gather_hits<T, true>(hits, wk, mk, fk, rd, lane, acc, unused);
         }
-        T contrib = 0;
-        if ((A >> lane) & 1)
-            contrib = bwd_finish_lane<MODE>(sig[(size_t)v * 64 + lane], &wr[(size_t)v * 64 + lane_rank(A)], acc);
-        contrib = lane_sum(contrib);
-        if (lane == j) bcj = contrib;
+        finish(j, acc);
+    }
+    // rows without entries have no successors
+    uint64_t zm = am & __ballot(ej == bj);
+    while (zm) {
+        const int j = __ffsll((unsigned long long)zm) - 1;
+        zm &= zm - 1;
+        finish(j, T(0));
+    }
+    // rows of 1 to 64 entries, packed as in fwd_chunk; a row of a pass with
+    // no hit is finished with a zero sum, in row order with the others
+    const int32_t lenj = (Aj != 0 && ej - bj <= 64) ? ej - bj : 0;
+    const int32_t E = wave_incl_sum(lenj, lane);
+    const int32_t S = E - lenj;
+    const int32_t total = __builtin_amdgcn_readlane(E, 63);
+    uint64_t pending = 0;   // rows of the current pass not finished yet
+    auto flush = [&](int32_t j, T acc, uint64_t) {
+        uint64_t below = pending & ((1ull << j) - 1);
+        while (below) {
+            const int r = __ffsll((unsigned long long)below) - 1;
+            below &= below - 1;
+            finish(r, T(0));
+        }
+        finish(j, acc);
+        pending &= j == 63 ? 0ull : ~((2ull << j) - 1);
+    };
+    for (int32_t base = 0; base < total;) {
+        const uint64_t endm = __ballot(lenj > 0 && E > base && E <= base + 64);
+        if (!endm) break;   // not reached: the next row starts at base and has at most 64 entries
+        const int lastj = 63 - __builtin_clzll(endm);
+        const int32_t pend = __builtin_amdgcn_readlane(E, lastj);
+        pending = endm;
+        const int32_t kk = min(base + lane, pend - 1);
+        const int32_t rk = first_lane_above(E, kk);
+        const int32_t idx = __shfl(bj, rk) + (kk - __shfl(S, rk));
+        const uint64_t A = __shfl(Aj, rk);
+        int32_t wk = 0;
+        uint64_t fk = 0, mk = 0;
+        if (base + lane < pend) {
+            wk = a.out_col[idx];
+            fk = a.F[fidx(a, slot, L + 1, wk)];
+            mk = fk & A;
+        }
+        uint64_t hits = __ballot(mk != 0);
+        T acc = 0;
+        uint64_t got = 0;
+        int32_t cur = -1;
+        while (hits) gather_rows<T, true>(hits, wk, mk, fk, rk, rd, lane, acc, got, cur, flush);
+        if (cur >= 0) flush(cur, acc, got);
+        while (pending) {
+            const int r = __ffsll((unsigned long long)pending) - 1;
+            pending &= pending - 1;
+            finish(r, T(0));
+        }
+        base = pend;
     }
     if (Aj) acc_bc[vj] += bcj;
 }

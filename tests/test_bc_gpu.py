@@ -347,3 +347,49 @@ def test_path_counts_beyond_fp32(oracle):
             assert (_bits(got) == _bits(per_level)).all()
     finally:
         C.set_launch(0)
+
+
+def _row_length_graph(directed):
+    """Vertex v has out-degree (7 v) mod 71, so every chunk of 64 vertices
+    holds rows of 0 to 70 entries: rows that share a 64-entry pass, rows of
+    exactly 64, rows that take a pass of their own (65-70), empty rows, and
+    vertices without out-edges reached in a digraph."""
+    import lgmi
+    n = 900
+    rng = np.random.default_rng(41)
+    src, dst = [], []
+    for v in range(n):
+        d = (7 * v) % 71
+        nb = rng.choice(n, size=d, replace=False)
+        src.extend([v] * d)
+        dst.extend(nb.tolist())
+    cls = lgmi.SimpleDiGraph if directed else lgmi.SimpleGraph
+    return cls.from_edges(n, np.array(src), np.array(dst), drop_self_loops=True)
+
+
+@pytest.mark.parametrize("directed", [False, True])
+def test_packed_row_lengths_vs_oracle(oracle, directed):
+    """Betweenness, stress and closeness on graphs whose rows span the
+    packed and one-row-per-pass forms of both sweeps, against the oracle;
+    the persistent launch and one launch per level bit for bit."""
+    import lgmi
+    from lgmi import centrality as C
+    g = _row_length_graph(directed)
+    rp, col = g.csr()
+    irp = g.in_csr()[0] if directed else rp
+    n = g.nv()
+    vs = np.arange(1, n + 1)
+    try:
+        say(f"row lengths, directed={directed}")
+        C.set_launch(0)
+        got = lgmi.betweenness_centrality(g, vs, normalize=False)
+        assert close(got, oracle.betweenness(rp, col, irp, directed, vs - 1, normalize=False))
+        C.set_launch(1)
+        assert (_bits(lgmi.betweenness_centrality(g, vs, normalize=False)) == _bits(got)).all()
+        C.set_launch(0)
+        st = lgmi.stress_centrality(g)
+        assert st.tolist() == oracle.stress(rp, col, irp, directed, np.arange(n), nthreads=8).tolist()
+        cl = lgmi.closeness_centrality(g)
+        assert cl.tolist() == oracle.closeness(rp, col, irp, directed, normalize=True).tolist()
+    finally:
+        C.set_launch(0)
