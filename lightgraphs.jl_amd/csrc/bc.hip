@@ -288,26 +288,6 @@ __device__ inline void gather_rows(uint64_t& hits, int32_t uk, uint64_t mk, uint
     }
 }
 
-// Inclusive prefix sum of x over the lanes of the wavefront.
-__device__ inline int32_t wave_incl_sum(int32_t x, int lane) {
-    #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int32_t y = __shfl_up(x, o);
-        if (lane >= o) x += y;
-    }
-    return x;
-}
-
-// The first lane j whose value e (nondecreasing over the lanes) exceeds k,
-// for a k below the last lane's value.
-__device__ inline int first_lane_above(int32_t e, int32_t k) {
-    int lo = 0;
-    #pragma unroll
-    for (int step = 32; step >= 1; step >>= 1)
-        if (__shfl(e, lo + step - 1) <= k) lo += step;
-    return lo;
-}
-
 // For source i: its out-neighbour of largest out-degree (the smallest id
 // among ties), -1 for an empty row.  One wavefront per source.
 __global__ void bc_source_hubs(const int32_t* rp, const int32_t* col, const int32_t* srcs, int32_t n, int32_t* hub) {

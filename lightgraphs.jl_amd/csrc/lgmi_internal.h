@@ -40,6 +40,28 @@ T* dev_alloc_n(size_t n) {
     return static_cast<T*>(dev_alloc(sizeof(T) * (n ? n : 1)));
 }
 
+// Wavefront helpers for packing short adjacency rows end to end over the 64
+// lanes.  Every lane of the wavefront calls them.
+// Inclusive prefix sum of x over the lanes.
+__device__ inline int32_t wave_incl_sum(int32_t x, int lane) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+// The first lane j whose value e (nondecreasing over the lanes) exceeds k,
+// for a k below the last lane's value.
+__device__ inline int first_lane_above(int32_t e, int32_t k) {
+    int lo = 0;
+#pragma unroll
+    for (int step = 32; step >= 1; step >>= 1)
+        if (__shfl(e, lo + step - 1) <= k) lo += step;
+    return lo;
+}
+
 // ---------------------------------------------------------------------
 // BFS level state (device resident).  Launch p of the level kernel runs
 // level k (or, in solo mode, levels k, k+1, ... in work-group 0): every
