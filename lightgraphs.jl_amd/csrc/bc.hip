@@ -59,10 +59,10 @@ struct BcCtrl {
 
 // Barrier and result words of the persistent launch.  arrive and closed are
 // cleared before every launch, bar_count is zero between launches; bar_gen
-// only grows.  abort is set when a barrier wait
-// times out (the blocks of a launch were not all resident), overflow when a
-// slot needs more levels than F holds; either makes the host redo the
-// computation level by level.
+// (the last barrier generation of the last complete launch) only grows.
+// abort is set when a barrier wait times out (the blocks of a launch were
+// not all resident), overflow when a slot needs more levels than F holds;
+// either makes the host redo the computation level by level.
 // Each word that blocks update concurrently has a 128-byte line of its own:
 // measured on gfx950, stores into a line that other work-groups update with
 // atomics can lose those updates.
@@ -943,13 +943,17 @@ __device__ bool grid_sync(BcPersistCtl* c, uint32_t* rel, uint32_t vb, uint32_t 
                           uint64_t timeout, int* s_ok) {
     __shared__ int s_last;
     __syncthreads();
+    gen++;   // in every thread: the generation this barrier releases
     if (threadIdx.x == 0) {
         int ok = __hip_atomic_load(&c->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0;
         int last = 0;
         if (ok) {
-            const uint32_t prev = __hip_atomic_fetch_add(&c->bar_count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+            // the arrival releases this block's writes; only the last
+            // arrival acquires the others' before it releases everyone
+            const uint32_t prev = __hip_atomic_fetch_add(&c->bar_count, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
             if (prev == nblocks - 1) {
                 atomicExch(&c->bar_count, 0u);
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
                 last = 1;
             }
         }
@@ -958,13 +962,21 @@ __device__ bool grid_sync(BcPersistCtl* c, uint32_t* rel, uint32_t vb, uint32_t 
     }
     __syncthreads();
     if (s_last) {
-        for (uint32_t b = threadIdx.x; b < nblocks; b += blockDim.x)
-            __hip_atomic_store(&rel[(size_t)b * 32], gen + 1, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        // one release fence in the first wavefront, then its lanes store
+        // every block's flag: each flag store follows the fence in its own
+        // lane (a release store per flag wrote the L2 back once per flag,
+        // and kept this block late for the next barrier)
+        if (threadIdx.x < 64) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            for (uint32_t b = threadIdx.x; b < nblocks; b += 64)
+                __hip_atomic_store(&rel[(size_t)b * 32], gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
     } else if (threadIdx.x == 0 && *s_ok) {
         const uint64_t t0 = wall_clock64();
         int nap = 0;
-        // relaxed polls: an acquiring load would invalidate the L2 each time
-        while (__hip_atomic_load(&rel[(size_t)vb * 32], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != gen + 1) {
+        // relaxed polls: an acquiring load would invalidate the L2 each time;
+        // flags only grow, so a flag at or past gen has released this barrier
+        while ((int32_t)(__hip_atomic_load(&rel[(size_t)vb * 32], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - gen) < 0) {
             if ((nap & 15) == 15 && __hip_atomic_load(&c->abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
                 *s_ok = 0;
                 break;
@@ -980,7 +992,6 @@ __device__ bool grid_sync(BcPersistCtl* c, uint32_t* rel, uint32_t vb, uint32_t 
         }
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // one L2 invalidate
     }
-    gen++;
     __syncthreads();
     return *s_ok != 0;
 }
@@ -1071,8 +1082,9 @@ __global__ __launch_bounds__(kBlock) void bc_persist(BcArgs a, PersistArgs p) {
     const uint32_t nrows_f = (uint32_t)p.ns * (uint32_t)a.hin.nh, nrows_b = (uint32_t)p.ns * (uint32_t)a.hout.nh;
     const int64_t nv = a.nv;
     const int ns = p.ns, W = p.words;
-    uint32_t gen = 0;
-    if (tid == 0) gen = __hip_atomic_load(&c->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // barrier generations continue from the last launch's, so that no flag
+    // left by it can release a barrier of this one
+    uint32_t gen = __hip_atomic_load(&c->bar_gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (vb == 0 && tid == 0) {
         c->overflow = 0;
         c->levels = 0;
@@ -1213,6 +1225,7 @@ __global__ __launch_bounds__(kBlock) void bc_persist(BcArgs a, PersistArgs p) {
         if (vb == 0 && tid == 0) {
             c->t_end = wall_clock64();
             c->barriers = nbar;
+            c->bar_gen = gen;
         }
         for (int sl = (int)vb; sl < ns; sl += (int)G) st_reduce(a, sl, p.st_out);
         return;
@@ -1241,6 +1254,7 @@ __global__ __launch_bounds__(kBlock) void bc_persist(BcArgs a, PersistArgs p) {
     if (vb == 0 && tid == 0) {
         c->t_end = wall_clock64();
         c->barriers = nbar;
+        c->bar_gen = gen;
     }
     for (int64_t v = (int64_t)vb * kBlock + tid; v < nv; v += (int64_t)G * kBlock) {
         if constexpr (MODE == 0) {
@@ -1257,10 +1271,13 @@ __global__ __launch_bounds__(kBlock) void bc_persist(BcArgs a, PersistArgs p) {
     }
 }
 
+// After a launch that stopped early: its blocks may have left flags up to
+// some generations past bar_gen, so the next launch starts far beyond them.
 __global__ void bc_ctl_reset(BcPersistCtl* c) {
     c->bar_count = 0;
     c->abort = 0;
     c->overflow = 0;
+    c->bar_gen += 1u << 20;
 }
 
 // Runs before every persistent launch (in the same captured graph).
