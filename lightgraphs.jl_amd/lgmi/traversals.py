@@ -74,19 +74,30 @@ def bfs_parents(g: _GraphBase, s: int, dir="out") -> np.ndarray:
     return np.where(par >= 0, par + 1, 0).astype(g.T)
 
 
-def _device_many(g: _GraphBase, roots0: np.ndarray, dir_in: int, parents: bool):
+def _device_many(g: _GraphBase, roots0: np.ndarray, dir_in: int, parents: bool, chunk: int = 256):
     """lgmi_bfs_run_many: 0-based distances and parents (-1 unreached) of
-    one single-source search per root, rows in root order."""
+    one single-source search per root, rows in root order.  Up to `chunk`
+    roots per call, into device arrays reused from call to call."""
     roots0 = np.ascontiguousarray(roots0, dtype=np.int32)
     n, k = g.nv(), len(roots0)
     lib = _lib.load()
-    d = _lib.DeviceBuffer(4 * n * k)
-    p = _lib.DeviceBuffer(4 * n * k) if parents else None
+    h = g.handle()
+    if k == 0:
+        _lib.check(lib.lgmi_bfs_run_many(h, None, 0, int(dir_in), None, None))
+        return np.empty((0, n), np.int32), (np.empty((0, n), np.int32) if parents else None)
+    m = max(1, min(k, chunk))
+    d = _lib.DeviceBuffer(4 * n * m)
+    p = _lib.DeviceBuffer(4 * n * m) if parents else None
+    dist = np.empty((k, n), np.int32)
+    par = np.empty((k, n), np.int32) if parents else None
     try:
-        _lib.check(lib.lgmi_bfs_run_many(g.handle(), _lib.i32p(roots0), k, int(dir_in), d.ptr,
-                                         p.ptr if p is not None else None))
-        dist = d.to_host(np.int32, (k, n))
-        par = p.to_host(np.int32, (k, n)) if p is not None else None
+        for s0 in range(0, k, m):
+            part = np.ascontiguousarray(roots0[s0:s0 + m])
+            _lib.check(lib.lgmi_bfs_run_many(h, _lib.i32p(part), len(part), int(dir_in), d.ptr,
+                                             p.ptr if p is not None else None))
+            dist[s0:s0 + len(part)] = d.to_host(np.int32, (len(part), n))
+            if p is not None:
+                par[s0:s0 + len(part)] = p.to_host(np.int32, (len(part), n))
     finally:
         d.free()
         if p is not None:

@@ -59,6 +59,9 @@ def test_many_rmat_undirected(oracle):
     assert p2 is None
     d3, _ = traversals._device_many(g, roots, 0, True)
     assert np.array_equal(d2, d3)
+    # in calls of 5 roots each (the device arrays reused between calls)
+    d4, p4 = traversals._device_many(g, roots, 0, True, chunk=5)
+    assert np.array_equal(d4, d3) and np.array_equal(p4, traversals._device_many(g, roots, 0, True)[1])
 
 
 def test_many_directed_both_directions(oracle):
