@@ -288,6 +288,43 @@ __device__ inline void gather_rows(uint64_t& hits, int32_t uk, uint64_t mk, uint
     }
 }
 
+// One 64-entry pass over the packed short rows of a chunk (row j's entries
+// end at entry E_j of the packed list, start at S_j = E_j - len_j): the
+// whole rows that fit from entry `base` on, each lane's entry, its row and
+// that row's lane mask `key`, and the lane's column entry, whose load is
+// issued here so that it runs while the previous pass gathers.  Every lane
+// calls it.  pend < 0: no row starts at base (not reached, as the next row
+// starts at base and holds at most 64 entries).
+struct PackedPass {
+    uint64_t rows;   // the rows of the pass (chunk lanes)
+    uint64_t key;    // this lane's row's lane mask
+    int32_t pend;    // where the pass ends in the packed list
+    int32_t rk;      // this lane's row
+    int32_t uk;      // this lane's column entry (lanes past the pass: 0)
+    bool on;         // this lane holds an entry
+};
+
+__device__ inline PackedPass packed_pass(int32_t base, int32_t E, int32_t lenj, int32_t bj, uint64_t keyj,
+                                         const int32_t* col, int lane) {
+    PackedPass ps;
+    ps.rows = __ballot(lenj > 0 && E > base && E <= base + 64);
+    ps.pend = -1;
+    ps.key = 0;
+    ps.rk = 0;
+    ps.uk = 0;
+    ps.on = false;
+    if (!ps.rows) return ps;
+    ps.pend = __builtin_amdgcn_readlane(E, 63 - __builtin_clzll(ps.rows));
+    // lanes past the pass repeat its last entry: every lane takes part in the shuffles
+    const int32_t kk = min(base + lane, ps.pend - 1);
+    ps.rk = first_lane_above(E, kk);
+    const int32_t idx = __shfl(bj, ps.rk) + (kk - __shfl(E - lenj, ps.rk));
+    ps.key = __shfl(keyj, ps.rk);
+    ps.on = base + lane < ps.pend;
+    if (ps.on) ps.uk = col[idx];
+    return ps;
+}
+
 // For source i: its out-neighbour of largest out-degree (the smallest id
 // among ties), -1 for an empty row.  One wavefront per source.
 __global__ void bc_source_hubs(const int32_t* rp, const int32_t* col, const int32_t* srcs, int32_t n, int32_t* hub) {
@@ -456,37 +493,27 @@ __device__ bool fwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
     const bool shortr = cand && ej - bj <= 64;
     const int32_t lenj = shortr ? ej - bj : 0;
     const int32_t E = wave_incl_sum(lenj, lane);
-    const int32_t S = E - lenj;                 // where row j starts in the packed list
     const int32_t total = __builtin_amdgcn_readlane(E, 63);
     const uint64_t unseen_j = ~sj & act;
-    for (int32_t base = 0; base < total;) {
-        // the pass ends at the last row end within 64 entries of base
-        const uint64_t endm = __ballot(lenj > 0 && E > base && E <= base + 64);
-        if (!endm) break;   // not reached: the next row starts at base and has at most 64 entries
-        const int lastj = 63 - __builtin_clzll(endm);
-        const int32_t pend = __builtin_amdgcn_readlane(E, lastj);
-        // this lane's entry of the packed list (lanes past the pass repeat
-        // its last entry: every lane takes part in the shuffles) and the row
-        // it belongs to
-        const int32_t kk = min(base + lane, pend - 1);
-        const int32_t rk = first_lane_above(E, kk);
-        const int32_t idx = __shfl(bj, rk) + (kk - __shfl(S, rk));
-        const uint64_t unseen = __shfl(unseen_j, rk);
-        int32_t uk = 0;
+    // each pass's column loads go out before the previous pass gathers
+    PackedPass pc = packed_pass(0, E, lenj, bj, unseen_j, a.in_col, lane);
+    while (pc.pend > 0) {
+        PackedPass pn;
+        pn.pend = -1;
+        if (pc.pend < total) pn = packed_pass(pc.pend, E, lenj, bj, unseen_j, a.in_col, lane);
         uint64_t fk = 0, mk = 0;
-        if (base + lane < pend) {
-            uk = a.in_col[idx];
-            fk = a.F[fidx(a, slot, L, uk)];
-            mk = fk & unseen;
+        if (pc.on) {
+            fk = a.F[fidx(a, slot, L, pc.uk)];
+            mk = fk & pc.key;
         }
         uint64_t hits = __ballot(mk != 0);
         double acc = 0.0;
         uint64_t got = 0;
         int32_t cur = -1;
-        if (MODE != 0 || !ld) while (hits) gather_rows<double, false>(hits, uk, mk, fk, rk, rd, lane, acc, got, cur, finish);
-        else while (hits) gather_rows<double, true>(hits, uk, mk, fk, rk, rd, lane, acc, got, cur, finish);
+        if (MODE != 0 || !ld) while (hits) gather_rows<double, false>(hits, pc.uk, mk, fk, pc.rk, rd, lane, acc, got, cur, finish);
+        else while (hits) gather_rows<double, true>(hits, pc.uk, mk, fk, pc.rk, rd, lane, acc, got, cur, finish);
         if (cur >= 0) finish(cur, acc, got);
-        base = pend;
+        pc = pn;
     }
     if (vj < nv && !heavy) {
         a.F[fidx(a, slot, L + 1, vj)] = newF;
@@ -749,7 +776,6 @@ __device__ void bwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
     // no hit is finished with a zero sum, in row order with the others
     const int32_t lenj = (Aj != 0 && ej - bj <= 64) ? ej - bj : 0;
     const int32_t E = wave_incl_sum(lenj, lane);
-    const int32_t S = E - lenj;
     const int32_t total = __builtin_amdgcn_readlane(E, 63);
     uint64_t pending = 0;   // rows of the current pass not finished yet
     auto flush = [&](int32_t j, T acc, uint64_t) {
@@ -762,35 +788,29 @@ __device__ void bwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
         finish(j, acc);
         pending &= j == 63 ? 0ull : ~((2ull << j) - 1);
     };
-    for (int32_t base = 0; base < total;) {
-        const uint64_t endm = __ballot(lenj > 0 && E > base && E <= base + 64);
-        if (!endm) break;   // not reached: the next row starts at base and has at most 64 entries
-        const int lastj = 63 - __builtin_clzll(endm);
-        const int32_t pend = __builtin_amdgcn_readlane(E, lastj);
-        pending = endm;
-        const int32_t kk = min(base + lane, pend - 1);
-        const int32_t rk = first_lane_above(E, kk);
-        const int32_t idx = __shfl(bj, rk) + (kk - __shfl(S, rk));
-        const uint64_t A = __shfl(Aj, rk);
-        int32_t wk = 0;
+    PackedPass pc = packed_pass(0, E, lenj, bj, Aj, a.out_col, lane);
+    while (pc.pend > 0) {
+        PackedPass pn;
+        pn.pend = -1;
+        if (pc.pend < total) pn = packed_pass(pc.pend, E, lenj, bj, Aj, a.out_col, lane);
+        pending = pc.rows;
         uint64_t fk = 0, mk = 0;
-        if (base + lane < pend) {
-            wk = a.out_col[idx];
-            fk = a.F[fidx(a, slot, L + 1, wk)];
-            mk = fk & A;
+        if (pc.on) {
+            fk = a.F[fidx(a, slot, L + 1, pc.uk)];
+            mk = fk & pc.key;
         }
         uint64_t hits = __ballot(mk != 0);
         T acc = 0;
         uint64_t got = 0;
         int32_t cur = -1;
-        while (hits) gather_rows<T, true>(hits, wk, mk, fk, rk, rd, lane, acc, got, cur, flush);
+        while (hits) gather_rows<T, true>(hits, pc.uk, mk, fk, pc.rk, rd, lane, acc, got, cur, flush);
         if (cur >= 0) flush(cur, acc, got);
         while (pending) {
             const int r = __ffsll((unsigned long long)pending) - 1;
             pending &= pending - 1;
             finish(r, T(0));
         }
-        base = pend;
+        pc = pn;
     }
     if (Aj) acc_bc[vj] += bcj;
 }
