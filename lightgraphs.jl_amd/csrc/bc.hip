@@ -467,12 +467,16 @@ __device__ bool fwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
         const int32_t b = __shfl(bj, j), e = __shfl(ej, j);
         double acc = 0.0;
         uint64_t got = 0;
+        // each 64-entry piece's column loads go out before the piece
+        // before it gathers (not for the distances, which may end a row
+        // early and keep the registers for a seventh wavefront per SIMD)
+        int32_t un = MODE != 2 && b + lane < e ? a.in_col[b + lane] : 0;
         for (int32_t off = b; off < e; off += 64) {
             const int32_t k = off + lane;
+            const int32_t uk = MODE != 2 ? un : (k < e ? a.in_col[k] : 0);
+            if (MODE != 2 && off + 64 < e) un = off + 64 + lane < e ? a.in_col[off + 64 + lane] : 0;
             uint64_t mk = 0, fk = 0;
-            int32_t uk = 0;
             if (k < e) {
-                uk = a.in_col[k];
                 fk = a.F[fidx(a, slot, L, uk)];
                 mk = fk & unseen;
             }
@@ -550,12 +554,16 @@ __device__ bool fwd_seg(const BcArgs& a, int slot, int32_t sg, int32_t L, int la
     uint64_t got = 0;
     if (unseen) {
         const int32_t b = a.hin.seg_b[sg], e = a.hin.seg_e[sg];
+        // each 64-entry piece's column loads go out before the piece
+        // before it gathers (not for the distances, which may end a row
+        // early and keep the registers for a seventh wavefront per SIMD)
+        int32_t un = MODE != 2 && b + lane < e ? a.in_col[b + lane] : 0;
         for (int32_t off = b; off < e; off += 64) {
             const int32_t k = off + lane;
+            const int32_t uk = MODE != 2 ? un : (k < e ? a.in_col[k] : 0);
+            if (MODE != 2 && off + 64 < e) un = off + 64 + lane < e ? a.in_col[off + 64 + lane] : 0;
             uint64_t mk = 0, fk = 0;
-            int32_t uk = 0;
             if (k < e) {
-                uk = a.in_col[k];
                 fk = a.F[fidx(a, slot, L, uk)];
                 mk = fk & unseen;
             }
@@ -750,12 +758,13 @@ __device__ void bwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
         const uint64_t A = __shfl(Aj, j);
         const int32_t b = __shfl(bj, j), e = __shfl(ej, j);
         T acc = 0;
+        int32_t wn = b + lane < e ? a.out_col[b + lane] : 0;   // as in fwd_chunk
         for (int32_t off = b; off < e; off += 64) {
             const int32_t kk = off + lane;
+            const int32_t wk = wn;
+            if (off + 64 < e) wn = off + 64 + lane < e ? a.out_col[off + 64 + lane] : 0;
             uint64_t mk = 0, fk = 0;
-            int32_t wk = 0;
             if (kk < e) {
-                wk = a.out_col[kk];
                 fk = a.F[fidx(a, slot, L + 1, wk)];
                 mk = fk & A;
             }
@@ -829,12 +838,13 @@ __device__ void bwd_seg(const BcArgs& a, int slot, int32_t sg, int32_t L, int la
     const T* rd = stage_plane<T>(a, slot, L + 1);
     const int32_t b = a.hout.seg_b[sg], e = a.hout.seg_e[sg];
     T acc = 0;
+    int32_t wn = b + lane < e ? a.out_col[b + lane] : 0;   // as in fwd_chunk
     for (int32_t off = b; off < e; off += 64) {
         const int32_t kk = off + lane;
+        const int32_t wk = wn;
+        if (off + 64 < e) wn = off + 64 + lane < e ? a.out_col[off + 64 + lane] : 0;
         uint64_t mk = 0, fk = 0;
-        int32_t wk = 0;
         if (kk < e) {
-            wk = a.out_col[kk];
             fk = a.F[fidx(a, slot, L + 1, wk)];
             mk = fk & A;
         }
