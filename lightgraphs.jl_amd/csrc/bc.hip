@@ -452,7 +452,7 @@ __device__ bool fwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
         } else {
             if ((got >> lane) & 1) {
                 const int32_t v = ch * 64 + j;
-                sig[(size_t)v * 64 + lane] = acc;
+                __builtin_nontemporal_store(acc, &sig[(size_t)v * 64 + lane]);
                 if (wr) wr[(size_t)v * 64 + lane_rank(got)] = acc;
             }
         }
@@ -609,7 +609,7 @@ __device__ void fwd_combine(const BcArgs& a, int slot, int32_t h, int32_t L, int
             gm |= pgot[q];
         }
         if ((gm >> lane) & 1) {
-            a.sig[((size_t)slot * nv + v) * 64 + lane] = tot;
+            __builtin_nontemporal_store(tot, &a.sig[((size_t)slot * nv + v) * 64 + lane]);
             if (MODE == 0 && L >= 1) stage_plane(a, slot, L + 1)[(size_t)v * 64 + lane_rank(gm)] = tot;
         }
     }
@@ -745,7 +745,7 @@ __device__ void bwd_chunk(const BcArgs& a, int slot, int32_t ch, int32_t L, int 
         const int32_t v = ch * 64 + j;
         T contrib = 0;
         if ((A >> lane) & 1)
-            contrib = bwd_finish_lane<MODE>(sig[(size_t)v * 64 + lane], &wr[(size_t)v * 64 + lane_rank(A)], acc);
+            contrib = bwd_finish_lane<MODE>(__builtin_nontemporal_load(&sig[(size_t)v * 64 + lane]), &wr[(size_t)v * 64 + lane_rank(A)], acc);
         contrib = lane_sum(contrib);
         if (lane == j) bcj = contrib;
     };
@@ -871,7 +871,7 @@ __device__ void bwd_combine(const BcArgs& a, int slot, int32_t h, int32_t L, int
     const double* sig = a.sig + (size_t)slot * nv * 64;
     T contrib = 0;
     if ((A >> lane) & 1)
-        contrib = bwd_finish_lane<MODE>(sig[(size_t)v * 64 + lane], &stage_plane<T>(a, slot, L)[(size_t)v * 64 + lane_rank(A)],
+        contrib = bwd_finish_lane<MODE>(__builtin_nontemporal_load(&sig[(size_t)v * 64 + lane]), &stage_plane<T>(a, slot, L)[(size_t)v * 64 + lane_rank(A)],
                                         tot);
     contrib = lane_sum(contrib);
     if (lane == 0) reinterpret_cast<T*>(a.bc_acc)[(size_t)slot * nv + v] += contrib;
