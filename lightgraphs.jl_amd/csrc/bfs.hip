@@ -356,7 +356,7 @@ __device__ inline void visit_n(const StepArgs& a, const int32_t* u, const int32_
         bin[q] = 0;
         if (won[q]) {
             if (PARENTS) atomicOr(&Fn[v[q] >> 5], 1u << (v[q] & 31));
-            a.dist[v[q]] = L + 1;
+            __builtin_nontemporal_store(L + 1, &a.dist[v[q]]);
             bin[q] = account_new(a, v[q], -1, k);
         }
     }
@@ -482,8 +482,8 @@ struct Pending {
 template <bool PARENTS>
 __device__ inline int bu_found(const StepArgs& a, int32_t v, int32_t par, int32_t off, int32_t b0, int32_t e,
                                int32_t L, Counters& k, uint32_t* s_fn) {
-    a.dist[v] = L + 1;
-    if (PARENTS) a.parent[v] = (uint32_t)par;
+    __builtin_nontemporal_store(L + 1, &a.dist[v]);
+    if (PARENTS) __builtin_nontemporal_store((uint32_t)par, &a.parent[v]);
     atomicOr(&s_fn[off >> 5], 1u << (off & 31));
     return account_new(a, v, e - b0, k);
 }
@@ -843,7 +843,7 @@ __device__ __attribute__((always_inline)) inline void range_step(const StepArgs&
             }
             bin[q] = 0;
             if (fresh[q]) {
-                a.dist[v[q]] = L + 1;
+                __builtin_nontemporal_store(L + 1, &a.dist[v[q]]);
                 if (PARENTS) a.parent[v[q]] = par[q];
                 const int32_t dout = d1[q] - d0[q];
                 const int32_t din = (a.td_rp == a.bu_rp) ? dout : (a.bu_rp[v[q] + 1] - a.bu_rp[v[q]]);
@@ -1017,7 +1017,7 @@ __device__ __attribute__((always_inline)) inline void visit_solo(const StepArgs&
     #pragma unroll
     for (int q = 0; q < N; q++) {
         won[q] = claim[q] && old[q] == 0xffffffffu;
-        if (PARENTS && won[q]) a.dist[v[q]] = L + 1;
+        if (PARENTS && won[q]) __builtin_nontemporal_store(L + 1, &a.dist[v[q]]);
         m[q] = __ballot(won[q]);
         tot += (uint32_t)__popcll(m[q]);
     }
