@@ -4,7 +4,9 @@
   throughput workload: distances and parents of four of its 64 seeded roots
   against the oracle's serial BFS, bit for bit, plus the invariants every
   BFS result has (edges join levels at most one apart; the parent is a
-  neighbour one level up and the smallest such id) on a further root.
+  neighbour one level up and the smallest such id) on a further root.  All
+  64 roots also go through lgmi_bfs_run_many, five of its rows against the
+  oracle bit for bit.
 * Exact betweenness on RMAT-20 with the 1024 seeded sources of the
   betweenness workload, through a size-independent identity: for a source s,
   the dependencies sum to sum_t (d(s, t) - 1) over the vertices t != s that s
@@ -53,6 +55,31 @@ def test_bfs_rmat22_vs_oracle(oracle, rmat22):
         p_gpu = traversals._parents0(g, r + 1, 0)
         assert np.array_equal(d_gpu, d_or), f"dist root {r}"
         assert np.array_equal(p_gpu, p_or), f"parents root {r}"
+
+
+def test_bfs_many_rmat22_vs_oracle(oracle, rmat22):
+    """All 64 roots of the throughput workload in one lgmi_bfs_run_many call
+    (eight searches at a time), the entry point the throughput is measured
+    on: five rows bit for bit against the oracle, and every row a search of
+    its own root over the same component."""
+    from lgmi import traversals
+    g = rmat22
+    rp, col = g.csr()
+    roots = _bench_roots(np.diff(rp), 64, 20260101)
+    say("rmat22 run_many, 64 roots")
+    dist, par = traversals._device_many(g, roots, 0, True)
+    assert dist.shape == (64, g.nv()) and par.shape == (64, g.nv())
+    for i in (0, 1, 2, 3, 63):
+        r = int(roots[i])
+        say(f"rmat22 run_many row {i}, root {r}")
+        d_or, p_or, _ = oracle.bfs_dist_parent(rp, col, r)
+        assert np.array_equal(dist[i], d_or), f"dist row {i} root {r}"
+        assert np.array_equal(par[i], p_or), f"parents row {i} root {r}"
+    idx = np.arange(64)
+    assert np.all(dist[idx, roots] == 0) and np.all(par[idx, roots] == roots)
+    reached = dist >= 0
+    assert np.all(reached == reached[0]), "the roots share one component"
+    assert np.array_equal(par >= 0, reached)
 
 
 def test_bfs_rmat22_invariants(rmat22):
